@@ -1,0 +1,729 @@
+// Host driver of the MI355X AM sparse-PCA solver and the C ABI (include/spca.h).
+//
+// The batched AM loop mirrors the reference GPU driver
+// (c++/src/gpugpower/gpu_sparse_PCA_solver.h:29-211) and the CPU batch driver
+// (c++/src/gpower/sparse_PCA_solver.h:170-243), re-planned for one MI355X:
+//   - the matrix stays resident in HBM in the layouts the products want (row major and
+//     column major, FP32 and/or bf16 hi+lo and/or fp16), keyed so jobs can share it;
+//   - every per-SP reduction stays on the device; the host only reads a convergence flag
+//     every `check_every` iterations (kernels enqueued after the batch has finished see
+//     the device flag and return at once, so the iteration count is exact);
+//   - per-phase kernel time is taken with hipEvents on the solver's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/spca.h"
+#include "spca_internal.h"
+
+namespace spca {
+
+struct Matrix {
+    int m = 0, n = 0, mp = 0, np = 0;
+    uint64_t key = 0;
+    unsigned storage = 0;
+    float* a32_cm = nullptr;  // mp x np column major: FP32 master, A^T*Z operand
+    float* a32_rm = nullptr;  // mp x np row major (ld np): A*X operand
+    uint16_t* hi_cm = nullptr;
+    uint16_t* lo_cm = nullptr;
+    uint16_t* hi_rm = nullptr;
+    uint16_t* lo_rm = nullptr;
+    uint16_t* h_cm = nullptr;
+    uint16_t* h_rm = nullptr;
+    uint64_t bytes = 0;
+    int refs = 0;
+};
+
+struct Workspace {
+    int np = 0, mp = 0, bp = 0, splits = 0;
+    float* X = nullptr;
+    uint16_t* Xhi = nullptr;
+    uint16_t* Xlo = nullptr;
+    float* Zpart = nullptr;
+    float* Z = nullptr;
+    uint16_t* Zhi = nullptr;
+    uint16_t* Zlo = nullptr;
+    float* V = nullptr;
+    float* fstate = nullptr;  // val, fz, err, errmax  (4 x bp floats)
+    int* istate = nullptr;    // iters, nnz, active, done (3 x bp + 4 ints)
+    double* ev = nullptr;
+    uint64_t bytes = 0;
+};
+
+}  // namespace spca
+
+struct spca_ctx_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t used = 0;
+    std::mutex mu;
+    std::string err;
+    std::map<uint64_t, spca::Matrix*> mats;
+    uint64_t next_handle = 1;
+    spca::Workspace ws;
+    int* h_done = nullptr;  // pinned
+    std::vector<hipEvent_t> events;  // pool of (start, stop) pairs for phase timing
+    hipEvent_t loop_ev[2] = {nullptr, nullptr};
+};
+
+using namespace spca;
+
+namespace {
+
+const char* kVersion = "spca_mi355x 0.1 (gfx950)";
+
+int fail(spca_ctx ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                                         \
+    do {                                                                                           \
+        hipError_t e_ = (expr);                                                                    \
+        if (e_ != hipSuccess)                                                                      \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? SPCA_ENOMEM : SPCA_EDEVICE,               \
+                        std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+    } while (0)
+
+template <class T>
+int dev_alloc(spca_ctx ctx, T** p, size_t count, uint64_t* acct) {
+    const uint64_t b = (uint64_t)count * sizeof(T);
+    if (ctx->used + b > ctx->budget)
+        return fail(ctx, SPCA_EBUDGET, "allocation of " + std::to_string(b) + " bytes exceeds the HBM budget");
+    hipError_t e = hipMalloc((void**)p, b ? b : 16);
+    if (e != hipSuccess) return fail(ctx, SPCA_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    ctx->used += b;
+    *acct += b;
+    return SPCA_OK;
+}
+template <class T>
+void dev_free(spca_ctx ctx, T*& p, uint64_t bytes_of_p) {
+    if (p) {
+        (void)hipFree(p);
+        ctx->used -= bytes_of_p;
+        p = nullptr;
+    }
+}
+
+bool valid_formulation(int f) { return f >= 0 && f <= 7; }
+
+void free_matrix(spca_ctx ctx, Matrix* M) {
+    (void)hipStreamSynchronize(ctx->stream);
+    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    ctx->used -= M->bytes;
+    delete M;
+}
+
+// Build the operand layouts for `storage` from the FP32 column-major master.
+int add_storage(spca_ctx ctx, Matrix* M, unsigned storage) {
+    const size_t cnt = (size_t)M->mp * M->np;
+    hipStream_t st = ctx->stream;
+    unsigned want = storage & ~M->storage;
+    if (!want) return SPCA_OK;
+    float* rm_tmp = nullptr;
+    uint64_t tmp_bytes = 0;
+    int rc;
+    auto need_rm = [&]() -> int {
+        if (M->a32_rm) return SPCA_OK;
+        if (rm_tmp) return SPCA_OK;
+        if ((rc = dev_alloc(ctx, &rm_tmp, cnt, &tmp_bytes))) return rc;
+        HIP_TRY(ctx, launch_transpose_f32(M->a32_cm, M->mp, M->np, M->mp, rm_tmp, M->np, st));
+        return SPCA_OK;
+    };
+    if (want & SPCA_STORE_FP32) {
+        if ((rc = dev_alloc(ctx, &M->a32_rm, cnt, &M->bytes))) return rc;
+        HIP_TRY(ctx, launch_transpose_f32(M->a32_cm, M->mp, M->np, M->mp, M->a32_rm, M->np, st));
+        M->storage |= SPCA_STORE_FP32;
+    }
+    if (want & SPCA_STORE_BF16X3) {
+        if ((rc = need_rm())) return rc;
+        const float* rm = M->a32_rm ? M->a32_rm : rm_tmp;
+        if ((rc = dev_alloc(ctx, &M->hi_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_cm, cnt, &M->bytes)) ||
+            (rc = dev_alloc(ctx, &M->hi_rm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_rm, cnt, &M->bytes)))
+            return rc;
+        HIP_TRY(ctx, launch_split_bf16(M->a32_cm, cnt, M->hi_cm, M->lo_cm, st));
+        HIP_TRY(ctx, launch_split_bf16(rm, cnt, M->hi_rm, M->lo_rm, st));
+        M->storage |= SPCA_STORE_BF16X3;
+    }
+    if (want & SPCA_STORE_FP16) {
+        if ((rc = need_rm())) return rc;
+        const float* rm = M->a32_rm ? M->a32_rm : rm_tmp;
+        if ((rc = dev_alloc(ctx, &M->h_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->h_rm, cnt, &M->bytes)))
+            return rc;
+        HIP_TRY(ctx, launch_to_fp16(M->a32_cm, cnt, M->h_cm, st));
+        HIP_TRY(ctx, launch_to_fp16(rm, cnt, M->h_rm, st));
+        M->storage |= SPCA_STORE_FP16;
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(st));
+    if (rm_tmp) dev_free(ctx, rm_tmp, tmp_bytes);
+    return SPCA_OK;
+}
+
+int make_matrix(spca_ctx ctx, const float* src, bool src_on_device, int m, int n, int ld, unsigned storage,
+                uint64_t key, spca_matrix* out) {
+    if (!ctx || !src || !out || m <= 0 || n <= 0 || ld < m) return fail(ctx, SPCA_EINVAL, "bad matrix arguments");
+    if (key) {
+        for (auto& kv : ctx->mats) {
+            Matrix* M = kv.second;
+            if (M->key == key && M->m == m && M->n == n) {
+                int rc = add_storage(ctx, M, storage);
+                if (rc) return rc;
+                M->refs++;
+                *out = kv.first;
+                return SPCA_OK;
+            }
+        }
+    }
+    Matrix* M = new Matrix();
+    M->m = m;
+    M->n = n;
+    M->mp = round_up(m, kPadMN);
+    M->np = round_up(n, kPadMN);
+    M->key = key;
+    M->refs = 1;
+    const size_t cnt = (size_t)M->mp * M->np;
+    int rc = dev_alloc(ctx, &M->a32_cm, cnt, &M->bytes);
+    if (rc) {
+        delete M;
+        return rc;
+    }
+    float* staging = nullptr;
+    uint64_t sbytes = 0;
+    const float* dsrc = src;
+    if (!src_on_device) {
+        rc = dev_alloc(ctx, &staging, (size_t)ld * n, &sbytes);
+        if (rc) {
+            free_matrix(ctx, M);
+            return rc;
+        }
+        hipError_t e = hipMemcpyAsync(staging, src, (size_t)ld * n * sizeof(float), hipMemcpyHostToDevice, ctx->stream);
+        if (e != hipSuccess) {
+            dev_free(ctx, staging, sbytes);
+            free_matrix(ctx, M);
+            return fail(ctx, SPCA_EDEVICE, std::string("H2D copy: ") + hipGetErrorString(e));
+        }
+        dsrc = staging;
+    }
+    hipError_t e = launch_pad_copy_f32(dsrc, m, n, ld, M->a32_cm, M->mp, M->mp, M->np, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (staging) dev_free(ctx, staging, sbytes);
+    if (e != hipSuccess) {
+        free_matrix(ctx, M);
+        return fail(ctx, SPCA_EDEVICE, std::string("pad copy: ") + hipGetErrorString(e));
+    }
+    rc = add_storage(ctx, M, storage);
+    if (rc) {
+        free_matrix(ctx, M);
+        return rc;
+    }
+    const uint64_t h = ctx->next_handle++;
+    ctx->mats[h] = M;
+    *out = h;
+    return SPCA_OK;
+}
+
+void free_workspace(spca_ctx ctx) {
+    Workspace& w = ctx->ws;
+    (void)hipStreamSynchronize(ctx->stream);
+    void* ptrs[] = {w.X, w.Xhi, w.Xlo, w.Zpart, w.Z, w.Zhi, w.Zlo, w.V, w.fstate, w.istate, w.ev};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    ctx->used -= w.bytes;
+    w = Workspace();
+}
+
+int ensure_workspace(spca_ctx ctx, int np, int mp, int bp, int splits) {
+    Workspace& w = ctx->ws;
+    if (w.X && w.np == np && w.mp == mp && w.bp == bp && w.splits >= splits) return SPCA_OK;
+    free_workspace(ctx);
+    w.np = np;
+    w.mp = mp;
+    w.bp = bp;
+    w.splits = splits;
+    const size_t xs = (size_t)np * bp, zs = (size_t)mp * bp;
+    int rc;
+    if ((rc = dev_alloc(ctx, &w.X, xs, &w.bytes)) || (rc = dev_alloc(ctx, &w.Xhi, xs, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.Xlo, xs, &w.bytes)) || (rc = dev_alloc(ctx, &w.Zpart, zs * splits, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.Z, zs, &w.bytes)) || (rc = dev_alloc(ctx, &w.Zhi, zs, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.Zlo, zs, &w.bytes)) || (rc = dev_alloc(ctx, &w.V, xs, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.fstate, (size_t)4 * bp, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + 4, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes))) {
+        free_workspace(ctx);
+        return rc;
+    }
+    HIP_TRY(ctx, hipMemsetAsync(w.Z, 0, zs * sizeof(float), ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(w.Zhi, 0, zs * sizeof(uint16_t), ctx->stream));
+    HIP_TRY(ctx, hipMemsetAsync(w.Zlo, 0, zs * sizeof(uint16_t), ctx->stream));
+    return SPCA_OK;
+}
+
+SpState state_of(Workspace& w) {
+    SpState s;
+    s.val = w.fstate;
+    s.fz = w.fstate + w.bp;
+    s.err = w.fstate + 2 * w.bp;
+    s.errmax = w.fstate + 3 * w.bp;
+    s.iters = w.istate;
+    s.nnz = w.istate + w.bp;
+    s.active = w.istate + 2 * w.bp;
+    s.done = w.istate + 3 * w.bp;
+    return s;
+}
+
+// reference getSignleStartingPoint<F> (c++/src/gpower/gpower_commons.h:21-47) with
+// myseed = the global SP index k, evaluated in F (the index floor(n * r / RAND_MAX)
+// depends on F's rounding, so the float and double instantiations draw different SPs),
+// then stored as float.
+template <class F>
+void reference_starting_point(float* out, unsigned n, unsigned s, bool constrained, unsigned k) {
+    std::vector<F> V(n, (F)0);
+    unsigned seed = k;
+    if (constrained) {
+        F tmp_norm = 0;
+        for (unsigned i = 0; i < s; i++) {
+            unsigned idx = (unsigned)(int)((F)n * (F)rand_r(&seed) / (F)RAND_MAX);
+            if (idx == n) idx--;
+            F tmp = (F)rand_r(&seed) / (F)RAND_MAX;
+            V[idx] = tmp;
+            tmp_norm += tmp * tmp;
+        }
+        const F sc = (F)1 / std::sqrt(tmp_norm);
+        for (unsigned i = 0; i < n; i++) V[i] = sc * V[i];
+    } else {
+        for (unsigned i = 0; i < n; i++) {
+            F tmp = (F)rand_r(&seed) / (F)RAND_MAX;
+            V[i] = -1 + 2 * tmp;
+        }
+    }
+    for (unsigned i = 0; i < n; i++) out[i] = (float)V[i];
+}
+
+int choose_splits(int mp, int np, int bp) {
+    const int tiles = (mp / kGemmBM) * (bp / kGemmBN);
+    int splits = 1;
+    while (tiles * splits < 2048 && np % (kGemmBK * splits * 2) == 0 && np / (splits * 2) >= 512) splits *= 2;
+    return splits;
+}
+
+// Kernel time per phase (0: A*X, 1: A^T*Z, 2: epilogue/step/convergence), from
+// hipEvent pairs recorded on the solver stream and folded in after each host sync.
+struct PhaseTimer {
+    spca_ctx ctx;
+    size_t used = 0;  // events handed out since the last flush (always pairs)
+    double ms[3] = {0, 0, 0};
+    std::vector<int> phase;
+    hipError_t pair(hipEvent_t* a, hipEvent_t* b, int ph) {
+        while (ctx->events.size() < used + 2) {
+            hipEvent_t e;
+            hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) return r;
+            ctx->events.push_back(e);
+        }
+        *a = ctx->events[used];
+        *b = ctx->events[used + 1];
+        used += 2;
+        phase.push_back(ph);
+        return hipSuccess;
+    }
+    void flush() {
+        for (size_t i = 0; i < phase.size(); i++) {
+            float t = 0;
+            if (hipEventElapsedTime(&t, ctx->events[2 * i], ctx->events[2 * i + 1]) == hipSuccess) ms[phase[i]] += t;
+        }
+        phase.clear();
+        used = 0;
+    }
+};
+
+int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float* X0, float* x_out,
+               spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
+    const auto t_wall0 = std::chrono::steady_clock::now();
+    spca_settings st = *st_in;
+    if (!valid_formulation(st.formulation)) return fail(ctx, SPCA_EINVAL, "formulation must be 0..7");
+    if (st.precision < 0 || st.precision > 2) return fail(ctx, SPCA_EINVAL, "precision must be 0..2");
+    if (st.max_iter < 1 || st.total_sp < 1 || st.batch < 0) return fail(ctx, SPCA_EINVAL, "bad iteration/SP counts");
+    if (st.use_otf) return fail(ctx, SPCA_EINVAL, "on-the-fly refill is not supported by this build");
+    if (st.start < SPCA_START_EXPLICIT || st.start > SPCA_START_REFERENCE_F64)
+        return fail(ctx, SPCA_EINVAL, "start must be a spca_start value");
+    if (st.start == SPCA_START_EXPLICIT && !X0) return fail(ctx, SPCA_EINVAL, "X0 is required for explicit starts");
+    if (!x_out) return fail(ctx, SPCA_EINVAL, "x_out is required");
+    const unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
+    int rc = add_storage(ctx, M, need);
+    if (rc) return rc;
+    const int n = M->n, m = M->m;
+    const int total_batches = spca_settings_check(&st, (unsigned)n);
+    if (st.constraint < 1) st.constraint = 1;
+    const int batch = st.batch;
+    const int bp = round_up(batch, kPadB);
+    const int splits = choose_splits(M->mp, M->np, bp);
+    if ((rc = ensure_workspace(ctx, M->np, M->mp, bp, splits))) return rc;
+    Workspace& w = ctx->ws;
+    hipStream_t s = ctx->stream;
+    SpState sp = state_of(w);
+    const bool constrained = st.formulation < 4;
+    const int check_every = st.check_every > 0 ? st.check_every : 1;
+
+    Operand xop{w.X, st.precision ? w.Xhi : nullptr, st.precision == 1 ? w.Xlo : nullptr};
+    Operand zop{w.Z, st.precision ? w.Zhi : nullptr, st.precision == 1 ? w.Zlo : nullptr};
+    const float* P1_32 = M->a32_rm;
+    const uint16_t* P1_hi = st.precision == 1 ? M->hi_rm : M->h_rm;
+    const uint16_t* P1_lo = M->lo_rm;
+    const float* P2_32 = M->a32_cm;
+    const uint16_t* P2_hi = st.precision == 1 ? M->hi_cm : M->h_cm;
+    const uint16_t* P2_lo = M->lo_cm;
+    const long zsplit = (long)M->mp * bp;
+
+    StepArgs sa;
+    sa.formulation = st.formulation;
+    sa.precision = st.precision;
+    sa.n = n;
+    sa.ldv = M->np;
+    sa.s = st.constraint;
+    sa.gamma = st.penalty;
+    sa.batch = batch;
+
+    std::vector<float> hX((size_t)n * batch);
+    std::vector<float> hval(batch), herr(batch);
+    std::vector<int> hit(batch), hnnz(batch);
+    std::vector<double> hev(batch);
+    double best = -1;
+    int best_sp = -1;
+    unsigned total_it = 0;
+    unsigned long long sp_its = 0;
+    double loop_ms = 0, last_err = 0;
+    PhaseTimer pt{ctx};
+    std::vector<float> x_best(n, 0.f);
+    hipEvent_t* loop_ev = ctx->loop_ev;
+
+    for (int bt = 0; bt < total_batches; bt++) {
+        const int shift = bt * batch;
+        // ---- starting points
+        for (int jj = 0; jj < batch; jj++) {
+            float* col = hX.data() + (size_t)jj * n;
+            if (st.start == SPCA_START_EXPLICIT)
+                std::memcpy(col, X0 + (size_t)(shift + jj) * n, sizeof(float) * n);
+            else if (st.start == SPCA_START_REFERENCE_F64)
+                reference_starting_point<double>(col, (unsigned)n, st.constraint, constrained, (unsigned)(shift + jj));
+            else
+                reference_starting_point<float>(col, (unsigned)n, st.constraint, constrained, (unsigned)(shift + jj));
+        }
+        HIP_TRY(ctx, hipMemsetAsync(w.X, 0, sizeof(float) * (size_t)w.np * w.bp, s));
+        HIP_TRY(ctx, hipMemcpy2DAsync(w.X, sizeof(float) * w.np, hX.data(), sizeof(float) * n, sizeof(float) * n,
+                                      batch, hipMemcpyHostToDevice, s));
+        if (st.precision) {
+            HIP_TRY(ctx, hipMemsetAsync(w.Xhi, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
+            HIP_TRY(ctx, hipMemsetAsync(w.Xlo, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
+            HIP_TRY(ctx, launch_x_operand(xop, st.precision, w.np, batch, s));
+        }
+        // ---- per-SP state: val = err = 0, iters = nnz = 0, active = (j < batch), done = 0
+        HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
+        HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 4), s));
+        {
+            std::vector<int> act(w.bp, 0);
+            std::fill(act.begin(), act.begin() + batch, 1);
+            HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
+            HIP_TRY(ctx, hipStreamSynchronize(s));
+        }
+        HIP_TRY(ctx, hipEventRecord(loop_ev[0], s));
+        for (int it = 0; it < st.max_iter; it++) {
+            hipEvent_t e0, e1;
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
+            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, M->np, w.X, w.Xhi, w.Xlo, M->np, w.Zpart,
+                                        M->mp, zsplit, M->mp, bp, M->np, splits, sp.done, s));
+            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
+            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, st.precision, batch,
+                                       sp, s));
+            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
+            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, M->mp, w.Z, w.Zhi, w.Zlo, M->mp, w.V,
+                                        M->np, 0, M->np, bp, M->mp, 1, sp.done, s));
+            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
+            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
+            HIP_TRY(ctx, launch_converge(sp, batch, it, st.tolerance, st.max_iter,
+                                         st.termination == SPCA_TERM_PER_SP, s));
+            HIP_TRY(ctx, hipEventRecord(e1, s));
+            if ((it + 1) % check_every == 0 || it + 1 == st.max_iter) {
+                HIP_TRY(ctx, hipMemcpyAsync(ctx->h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+                HIP_TRY(ctx, hipStreamSynchronize(s));
+                pt.flush();
+                if (ctx->h_done[0]) break;
+            }
+        }
+        HIP_TRY(ctx, hipEventRecord(loop_ev[1], s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        pt.flush();
+        {
+            float t = 0;
+            HIP_TRY(ctx, hipEventElapsedTime(&t, loop_ev[0], loop_ev[1]));
+            loop_ms += t;
+        }
+        // ---- batch results
+        int done_h[2];
+        float emax;
+        HIP_TRY(ctx, hipMemcpyAsync(done_h, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(hval.data(), sp.val, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(herr.data(), sp.err, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(hit.data(), sp.iters, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(hnnz.data(), sp.nnz, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(&emax, sp.errmax, sizeof(float), hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        total_it += (unsigned)done_h[1];
+        last_err = emax;
+        for (int jj = 0; jj < batch; jj++) sp_its += (unsigned long long)hit[jj];
+        // best of batch: first maximum, as sparse_PCA_solver.h:212-231
+        int sel = 0;
+        float bv = hval[0];
+        for (int jj = 1; jj < batch; jj++)
+            if (hval[jj] > bv) {
+                bv = hval[jj];
+                sel = jj;
+            }
+        const bool want_ev = per_sp != nullptr || stats != nullptr;
+        if (want_ev) {
+            HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, m, n, w.X, w.np, batch, spca_is_l1_variance(st.formulation),
+                                        w.ev, s));
+            HIP_TRY(ctx, hipMemcpyAsync(hev.data(), w.ev, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+            HIP_TRY(ctx, hipStreamSynchronize(s));
+        }
+        if (per_sp) {
+            for (int jj = 0; jj < batch; jj++) {
+                spca_sp_result& r = per_sp[shift + jj];
+                r.value = hval[jj];
+                r.iters = hit[jj];
+                r.nnz = hnnz[jj];
+                r.converged = (double)herr[jj] < st.tolerance;
+                r.ev_fp64 = hev[jj];
+            }
+        }
+        if (X_final)
+            HIP_TRY(ctx, hipMemcpy2DAsync(X_final + (size_t)shift * n, sizeof(float) * n, w.X, sizeof(float) * w.np,
+                                          sizeof(float) * n, batch, hipMemcpyDeviceToHost, s));
+        if (best < (double)bv || best_sp < 0) {
+            best = bv;
+            best_sp = shift + sel;
+            HIP_TRY(ctx, hipMemcpyAsync(x_best.data(), w.X + (size_t)sel * w.np, sizeof(float) * n,
+                                        hipMemcpyDeviceToHost, s));
+            if (stats) stats->ev_fp64 = hev[sel];
+        }
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+    }
+    // final x <- x/|x|_2 (sparse_PCA_solver.h:236-237)
+    double nx = 0;
+    for (int i = 0; i < n; i++) nx += (double)x_best[i] * (double)x_best[i];
+    const float inv = 1.0f / (float)std::sqrt(nx);
+    for (int i = 0; i < n; i++) x_out[i] = x_best[i] * inv;
+    if (stats) {
+        stats->it = total_it;
+        stats->fval = best;
+        stats->error = last_err;
+        stats->loop_ms = loop_ms;
+        stats->wall_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall0).count();
+        stats->n = (unsigned)n;
+        stats->best_sp = best_sp;
+        stats->total_sp = st.total_sp;
+        stats->constraint = st.constraint;
+        stats->gemm_ax_ms = pt.ms[0];
+        stats->gemm_atz_ms = pt.ms[1];
+        stats->step_ms = pt.ms[2];
+        stats->sp_iterations = sp_its;
+    }
+    return SPCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* spca_version(void) { return kVersion; }
+
+void spca_settings_default(spca_settings* st) {
+    if (!st) return;
+    std::memset(st, 0, sizeof(*st));
+    st->formulation = L0_constrained_L2_PCA;
+    st->constraint = 10;
+    st->penalty = 0;
+    st->tolerance = 0.01;
+    st->max_iter = 20;
+    st->total_sp = 64;
+    st->batch = 64;
+    st->use_otf = 0;
+    st->termination = SPCA_TERM_BATCH_MAX;
+    st->precision = SPCA_PREC_FP32;
+    st->start = SPCA_START_EXPLICIT;
+    st->check_every = 1;
+}
+
+int spca_settings_check(spca_settings* st, unsigned n) {
+    if (!st) return 0;
+    if (st->constraint > n) st->constraint = n;
+    if (st->batch == 0) st->batch = st->total_sp;
+    int tb = st->total_sp / st->batch;
+    if (tb * st->batch < st->total_sp) tb++;
+    st->total_sp = tb * st->batch;
+    return tb;
+}
+
+int spca_is_constrained(int f) { return f >= 0 && f < 4; }
+int spca_is_l1_constrained(int f) { return !(f == L0_constrained_L1_PCA || f == L0_constrained_L2_PCA); }
+int spca_is_l1_penalized(int f) { return !(f == L0_penalized_L1_PCA || f == L0_penalized_L2_PCA); }
+int spca_is_l1_variance(int f) { return f == 1 || f == 3 || f == 5 || f == 7; }
+
+int spca_ctx_create(int device, uint64_t hbm_budget_bytes, spca_ctx* out) {
+    if (!out) return SPCA_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return SPCA_EDEVICE;
+    if (device < 0 || device >= count) return SPCA_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return SPCA_EDEVICE;
+    spca_ctx c = new spca_ctx_s();
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return SPCA_EDEVICE;
+    }
+    size_t fr = 0, tot = 0;
+    (void)hipMemGetInfo(&fr, &tot);
+    c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(0.9 * (double)fr);
+    if (hipHostMalloc((void**)&c->h_done, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess ||
+        hipEventCreate(&c->loop_ev[0]) != hipSuccess || hipEventCreate(&c->loop_ev[1]) != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return SPCA_ENOMEM;
+    }
+    *out = c;
+    return SPCA_OK;
+}
+
+int spca_ctx_destroy(spca_ctx ctx) {
+    if (!ctx) return SPCA_EINVAL;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        (void)hipSetDevice(ctx->device);
+        for (auto& kv : ctx->mats) free_matrix(ctx, kv.second);
+        ctx->mats.clear();
+        free_workspace(ctx);
+        for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+        for (hipEvent_t e : ctx->loop_ev) (void)hipEventDestroy(e);
+        (void)hipHostFree(ctx->h_done);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return SPCA_OK;
+}
+
+const char* spca_last_error(spca_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int spca_matrix_upload(spca_ctx ctx, const float* A, int m, int n, int ld, unsigned storage, uint64_t key,
+                       spca_matrix* out) {
+    if (!ctx) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    return make_matrix(ctx, A, false, m, n, ld, storage, key, out);
+}
+
+int spca_matrix_from_device(spca_ctx ctx, const float* d_A, int m, int n, int ld, unsigned storage, uint64_t key,
+                            spca_matrix* out) {
+    if (!ctx) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    return make_matrix(ctx, d_A, true, m, n, ld, storage, key, out);
+}
+
+int spca_matrix_add_storage(spca_ctx ctx, spca_matrix mat, unsigned storage) {
+    if (!ctx) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto itm = ctx->mats.find(mat);
+    if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+    (void)hipSetDevice(ctx->device);
+    return add_storage(ctx, itm->second, storage);
+}
+
+int spca_matrix_release(spca_ctx ctx, spca_matrix mat) {
+    if (!ctx) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto itm = ctx->mats.find(mat);
+    if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+    if (--itm->second->refs <= 0) {
+        (void)hipSetDevice(ctx->device);
+        free_matrix(ctx, itm->second);
+        ctx->mats.erase(itm);
+    }
+    return SPCA_OK;
+}
+
+int spca_matrix_bytes(spca_ctx ctx, spca_matrix mat, uint64_t* bytes) {
+    if (!ctx || !bytes) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto itm = ctx->mats.find(mat);
+    if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+    *bytes = itm->second->bytes;
+    return SPCA_OK;
+}
+
+int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, float* x_out,
+               spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
+    if (!ctx || !settings) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto itm = ctx->mats.find(mat);
+    if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+    (void)hipSetDevice(ctx->device);
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    return solve_impl(ctx, itm->second, settings, X0, x_out, stats, per_sp, X_final);
+}
+
+int spca_dense_solver_device(spca_ctx ctx, unsigned m, unsigned n, const float* d_B, float* h_x,
+                             const spca_settings* settings, spca_stats* stats, unsigned LD_M, unsigned LD_N) {
+    (void)LD_N;
+    if (!ctx || !settings) return SPCA_EINVAL;
+    spca_matrix h = 0;
+    const unsigned need = settings->precision == 0 ? SPCA_STORE_FP32
+                          : settings->precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
+    int rc = spca_matrix_from_device(ctx, d_B, (int)m, (int)n, (int)LD_M, need, 0, &h);
+    if (rc) return rc;
+    // the reference GPU solver draws its own starting points
+    spca_settings st = *settings;
+    if (st.start == SPCA_START_EXPLICIT) st.start = SPCA_START_REFERENCE;
+    rc = spca_solve(ctx, h, &st, nullptr, h_x, stats, nullptr, nullptr);
+    spca_matrix_release(ctx, h);
+    return rc;
+}
+
+int spca_dense_solver_host(spca_ctx ctx, const float* B, int ldB, float* x, unsigned m, unsigned n,
+                           const spca_settings* settings, spca_stats* stats, double* fval_out) {
+    if (!ctx || !settings) return SPCA_EINVAL;
+    spca_matrix h = 0;
+    const unsigned need = settings->precision == 0 ? SPCA_STORE_FP32
+                          : settings->precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
+    int rc = spca_matrix_upload(ctx, B, (int)m, (int)n, ldB, need, 0, &h);
+    if (rc) return rc;
+    spca_stats local;
+    spca_stats* sp = stats ? stats : &local;
+    spca_settings st = *settings;
+    if (st.start == SPCA_START_EXPLICIT) st.start = SPCA_START_REFERENCE;
+    rc = spca_solve(ctx, h, &st, nullptr, x, sp, nullptr, nullptr);
+    if (fval_out) *fval_out = sp->fval;
+    spca_matrix_release(ctx, h);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,91 @@
+// Internal interface between the host driver (spca_host.cpp) and the HIP kernels
+// (spca_kernels.hip).  Not part of the public C ABI (include/spca.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace spca {
+
+// Padding of the resident operand layouts: rows and columns of A are padded with zeros
+// to multiples of kPadMN so every product tile is full; iterate batches are padded to
+// multiples of kPadB starting points (padded columns stay zero and are never active).
+constexpr int kPadMN = 256;
+constexpr int kPadB = 128;
+
+// GEMM tile: 128x128 output tile, 32-deep K stage, 256 threads (2x2 waves of 64x64).
+constexpr int kGemmBM = 128;
+constexpr int kGemmBN = 128;
+constexpr int kGemmBK = 32;
+
+// Select/formulation-step kernel: one workgroup of kSelThreads per starting point.
+constexpr int kSelThreads = 1024;
+constexpr int kSelMaxEPT = 64;  // elements per thread held in registers (n <= 65536)
+
+inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+
+// Per-starting-point device state (structure of arrays, length = padded batch).
+struct SpState {
+    float* val;      // value of the last completed iteration (reference vals[i].val)
+    float* fz;       // objective taken from z in this iteration (|z|_2 or |z|_1)
+    float* err;      // current_error
+    int* iters;      // iterations performed
+    int* nnz;        // nonzeros of the iterate
+    int* active;     // 1 while the SP iterates
+    int* done;       // [0]: batch finished flag, [1]: iterations counted, [2]: spare
+    float* errmax;   // [0]: batch max error of the last iteration
+};
+
+// Operand buffers of one iterate matrix (n or m rows, padded batch columns).
+struct Operand {
+    float* f32;          // FP32 values (also the master copy of iterates)
+    uint16_t* hi;        // bf16 hi part, or fp16 values
+    uint16_t* lo;        // bf16 lo part (bf16x3 only)
+};
+
+struct StepArgs {
+    int formulation;
+    int precision;       // spca_precision
+    int n;               // true length of x (columns of A)
+    int ldv;             // leading dimension of V and X (padded n)
+    unsigned s;          // constraint after clamp
+    double gamma;
+    int batch;           // true number of SPs in this batch
+};
+
+// ---- launchers (spca_kernels.hip); every one enqueues on `st` and returns a hip status
+hipError_t launch_pad_copy_f32(const float* src, int m, int n, int lds, float* dst, int ldd, int mp,
+                               int np, hipStream_t st);
+hipError_t launch_transpose_f32(const float* src, int rows, int cols, int lds, float* dst, int ldd,
+                                hipStream_t st);
+hipError_t launch_split_bf16(const float* src, size_t count, uint16_t* hi, uint16_t* lo, hipStream_t st);
+hipError_t launch_to_fp16(const float* src, size_t count, uint16_t* h, hipStream_t st);
+
+// C[r + c*ldc + split*split_stride] = sum_{k in split} P[r*ldp + k] * Q[c*ldq + k]
+// M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.
+hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo,
+                          long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq,
+                          float* C, long ldc, long split_stride, int M, int N, int K, int splits,
+                          const int* done, hipStream_t st);
+
+// Sum split partials of Z, per-column statistic and map (sign / normalise), write the
+// operand of the second product.
+hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int m, int ldz,
+                         Operand zop, int formulation, int precision, int batch, SpState sp,
+                         hipStream_t st);
+
+// Formulation step on V (A^T Z) for each active SP, writes X (+operand), updates state.
+hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st);
+
+// Batch convergence: batch max error / per-SP freezing, done flag.
+hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_iter, int per_sp,
+                           hipStream_t st);
+
+// Write the operand forms of X from its FP32 values (for starting points).
+hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st);
+
+// FP64 explained variance of each normalised column of X: |A x|_2^2 or |A x|_1.
+hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* X, int ldx, int batch,
+                          int l1, double* ev, hipStream_t st);
+
+}  // namespace spca

@@ -1,0 +1,857 @@
+// HIP kernels of the MI355X AM sparse-PCA solver (gfx950 / CDNA4).
+//
+// One batched AM iteration over B starting points is, on one stream:
+//   gemm_tn  (Zpart = A * X, split over K)           reference cublasSgemm(N,N),
+//                                                    c++/src/gpugpower/gpu_sparse_PCA_solver.h:111-112
+//   prep_z   (sum splits, |z|_2 or |z|_1, sign / normalise map, operand write)
+//                                                    :105-121,158-164 (nrm2/asum/scal/sgn)
+//   gemm_tn  (V = A^T * Z)                           :125-126,168-169
+//   step     (top-s select | lambda soft threshold | gamma threshold, norm, error)
+//                                                    c++/src/gpugpower/my_cublas_wrapper.h:571-710
+//   converge (batch max error, termination)         gpu_sparse_PCA_solver.h:176-186
+// Iteration semantics follow the reference CPU method (c++/src/gpower/gpower_commons.h:67-177);
+// the thresholds follow the reference GPU path (sorted tau, lambda scan).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "spca_internal.h"
+
+namespace spca {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef uint32_t u4 __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------- conversions
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    // round to nearest even; inputs here are finite
+    uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ uint16_t f2h(float f) {
+    _Float16 h = (_Float16)f;
+    return __builtin_bit_cast(uint16_t, h);
+}
+
+// Buffer resources: a bounded view of one column.  Loads past num_records return 0 and
+// stores past it are dropped, which handles ragged column lengths without per-element
+// address registers (offsets go in the instruction's scalar offset).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t col_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+
+// ---------------------------------------------------------------- reductions
+// Deterministic: fixed shuffle tree inside a wave, fixed order across waves.
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_down(v, o, 64));
+    return v;
+}
+template <int NT>
+__device__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double r = 0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) r += red[i];
+    return r;
+}
+template <int NT>
+__device__ float block_max(float v, double* red) {
+    v = wave_max(v);
+    const int w = threadIdx.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = (double)v;
+    __syncthreads();
+    float r = 0.f;
+#pragma unroll
+    for (int i = 0; i < NT / 64; i++) r = fmaxf(r, (float)red[i]);
+    return r;
+}
+__device__ __forceinline__ uint64_t shfl_down_u64(uint64_t v, int o) {
+    uint32_t lo = __shfl_down((uint32_t)v, o, 64);
+    uint32_t hi = __shfl_down((uint32_t)(v >> 32), o, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint32_t shfl_down_u32(uint32_t v, int o) { return __shfl_down(v, o, 64); }
+// inclusive suffix sum over lanes l..63 (lane 63 holds the top)
+__device__ __forceinline__ uint32_t wave_suffix_u32(uint32_t v) {
+    const int l = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t t = shfl_down_u32(v, o);
+        if (l + o < 64) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ uint64_t wave_suffix_u64(uint64_t v) {
+    const int l = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+        uint64_t t = shfl_down_u64(v, o);
+        if (l + o < 64) v += t;
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_down(v, o, 64));
+    return __shfl(v, 0, 64);
+}
+
+// ---------------------------------------------------------------- data preparation
+__global__ void pad_copy_kernel(const float* __restrict__ src, int m, int n, int lds, float* __restrict__ dst,
+                                int ldd, int mp, int np) {
+    const long total = (long)mp * np;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % mp), k = (int)(e / mp);
+        dst[(long)k * ldd + i] = (i < m && k < n) ? src[(long)k * lds + i] : 0.f;
+    }
+}
+hipError_t launch_pad_copy_f32(const float* src, int m, int n, int lds, float* dst, int ldd, int mp, int np,
+                               hipStream_t st) {
+    pad_copy_kernel<<<4096, 256, 0, st>>>(src, m, n, lds, dst, ldd, mp, np);
+    return hipGetLastError();
+}
+
+// dst[c*ldd + r] = src[r + c*lds] transposed: src is rows x cols column major, dst is
+// cols x rows column major (i.e. src row-major).  rows, cols multiples of 64.
+__global__ void transpose_kernel(const float* __restrict__ src, int rows, int cols, int lds,
+                                 float* __restrict__ dst, int ldd) {
+    __shared__ float tile[64][65];
+    const int r0 = blockIdx.x * 64, c0 = blockIdx.y * 64;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;  // 256 threads: 4 rows per pass
+    for (int c = ty; c < 64; c += 4) tile[c][tx] = src[(long)(c0 + c) * lds + r0 + tx];
+    __syncthreads();
+    for (int r = ty; r < 64; r += 4) dst[(long)(r0 + r) * ldd + c0 + tx] = tile[tx][r];
+}
+hipError_t launch_transpose_f32(const float* src, int rows, int cols, int lds, float* dst, int ldd,
+                                hipStream_t st) {
+    dim3 g(rows / 64, cols / 64);
+    transpose_kernel<<<g, 256, 0, st>>>(src, rows, cols, lds, dst, ldd);
+    return hipGetLastError();
+}
+
+__global__ void split_bf16_kernel(const float* __restrict__ src, size_t count, uint16_t* __restrict__ hi,
+                                  uint16_t* __restrict__ lo) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x) {
+        const float x = src[e];
+        const uint16_t h = f2bf(x);
+        hi[e] = h;
+        lo[e] = f2bf(x - bf2f(h));
+    }
+}
+hipError_t launch_split_bf16(const float* src, size_t count, uint16_t* hi, uint16_t* lo, hipStream_t st) {
+    split_bf16_kernel<<<8192, 256, 0, st>>>(src, count, hi, lo);
+    return hipGetLastError();
+}
+__global__ void to_fp16_kernel(const float* __restrict__ src, size_t count, uint16_t* __restrict__ h) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
+        h[e] = f2h(src[e]);
+}
+hipError_t launch_to_fp16(const float* src, size_t count, uint16_t* h, hipStream_t st) {
+    to_fp16_kernel<<<8192, 256, 0, st>>>(src, count, h);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- products
+// C[r + c*ldc] = sum_k P[r*ldp + k] * Q[c*ldq + k]: both operands are contiguous along
+// the reduction, because the matrix is resident in both orientations (row major for
+// A*X, column major for A^T*Z).  One 128x128 tile per workgroup, 2x2 waves of 64x64,
+// 16x16 MFMA blocks.  blockIdx is remapped so that consecutive tiles (same P panel,
+// neighbouring N tiles) land on the same XCD (dispatch is round-robin over 8 XCDs),
+// which keeps each P panel in that XCD's L2.
+__device__ __forceinline__ int xcd_remap(int bid, int total) {
+    return (total & 7) ? bid : (bid & 7) * (total >> 3) + (bid >> 3);
+}
+
+__global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restrict__ P, long ldp,
+                                                          const float* __restrict__ Q, long ldq,
+                                                          float* __restrict__ C, long ldc, long split_stride,
+                                                          int tiles_m, int tiles_n, int splits, int ksplit,
+                                                          const int* __restrict__ done) {
+    if (done && *done) return;
+    const int total = tiles_m * tiles_n * splits;
+    const int t = xcd_remap(blockIdx.x, total);
+    const int tn = t % tiles_n;
+    const int rest = t / tiles_n;
+    const int sp = rest % splits;
+    const int tm = rest / splits;
+    const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
+
+    __shared__ __attribute__((aligned(16))) float Ps[kGemmBM][kGemmBK + 4];
+    __shared__ __attribute__((aligned(16))) float Qs[kGemmBN][kGemmBK + 4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int lrow = tid >> 3, lc4 = (tid & 7) * 4;
+    const float* Pg = P + (m0 + lrow) * ldp + k0 + lc4;
+    const float* Qg = Q + (n0 + lrow) * ldq + k0 + lc4;
+    f4 pr[4], qr[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        pr[i] = *(const f4*)(Pg + (long)(32 * i) * ldp);
+        qr[i] = *(const f4*)(Qg + (long)(32 * i) * ldq);
+    }
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    for (int kt = 0; kt < ksplit; kt += kGemmBK) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            *(f4*)&Ps[lrow + 32 * i][lc4] = pr[i];
+            *(f4*)&Qs[lrow + 32 * i][lc4] = qr[i];
+        }
+        __syncthreads();
+        if (kt + kGemmBK < ksplit) {
+            Pg += kGemmBK;
+            Qg += kGemmBK;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                pr[i] = *(const f4*)(Pg + (long)(32 * i) * ldp);
+                qr[i] = *(const f4*)(Qg + (long)(32 * i) * ldq);
+            }
+        }
+#pragma unroll
+        for (int kk = 0; kk < kGemmBK; kk += 4) {
+            float a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[i] = Ps[wm * 64 + i * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = Qs[wn * 64 + j * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    float* Cb = C + (long)sp * split_stride;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+            const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+            *(f4*)(Cb + row + col * ldc) = acc[i][j];
+        }
+}
+
+// 16-bit operand products.  PREC 1: bf16x3 (Phi*Qhi + Phi*Qlo + Plo*Qhi), PREC 2: fp16.
+// One 16x16x32 MFMA covers a whole 32-deep K stage; lane l supplies row l%16 and
+// k = 8*(l/16) .. +7 of each operand.
+template <int PREC>
+__global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restrict__ Phi,
+                                                         const uint16_t* __restrict__ Plo, long ldp,
+                                                         const uint16_t* __restrict__ Qhi,
+                                                         const uint16_t* __restrict__ Qlo, long ldq,
+                                                         float* __restrict__ C, long ldc, long split_stride,
+                                                         int tiles_m, int tiles_n, int splits, int ksplit,
+                                                         const int* __restrict__ done) {
+    if (done && *done) return;
+    constexpr int NT = (PREC == 1) ? 2 : 1;  // number of parts per operand
+    constexpr int LDK = kGemmBK + 8;         // 80-byte rows: 16-byte aligned
+    const int total = tiles_m * tiles_n * splits;
+    const int t = xcd_remap(blockIdx.x, total);
+    const int tn = t % tiles_n;
+    const int rest = t / tiles_n;
+    const int sp = rest % splits;
+    const int tm = rest / splits;
+    const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
+
+    __shared__ __attribute__((aligned(16))) uint16_t Ps[NT][kGemmBM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Qs[NT][kGemmBN][LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // a 128 x 32 tile of 16-bit values is 512 chunks of 16 bytes: 2 per thread
+    const int lrow = tid >> 2, lc8 = (tid & 3) * 8;
+    const uint16_t* Pg[NT];
+    const uint16_t* Qg[NT];
+    Pg[0] = Phi + (m0 + lrow) * ldp + k0 + lc8;
+    Qg[0] = Qhi + (n0 + lrow) * ldq + k0 + lc8;
+    if constexpr (NT == 2) {
+        Pg[1] = Plo + (m0 + lrow) * ldp + k0 + lc8;
+        Qg[1] = Qlo + (n0 + lrow) * ldq + k0 + lc8;
+    }
+    u4 pr[NT][2], qr[NT][2];
+#pragma unroll
+    for (int p = 0; p < NT; p++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            pr[p][i] = *(const u4*)(Pg[p] + (long)(64 * i) * ldp);
+            qr[p][i] = *(const u4*)(Qg[p] + (long)(64 * i) * ldq);
+        }
+    f4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    for (int kt = 0; kt < ksplit; kt += kGemmBK) {
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < NT; p++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                *(u4*)&Ps[p][lrow + 64 * i][lc8] = pr[p][i];
+                *(u4*)&Qs[p][lrow + 64 * i][lc8] = qr[p][i];
+            }
+        __syncthreads();
+        if (kt + kGemmBK < ksplit) {
+#pragma unroll
+            for (int p = 0; p < NT; p++) {
+                Pg[p] += kGemmBK;
+                Qg[p] += kGemmBK;
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    pr[p][i] = *(const u4*)(Pg[p] + (long)(64 * i) * ldp);
+                    qr[p][i] = *(const u4*)(Qg[p] + (long)(64 * i) * ldq);
+                }
+            }
+        }
+        const int kof = 8 * (lane >> 4);
+        u4 a[NT][4], b[NT][4];
+#pragma unroll
+        for (int p = 0; p < NT; p++) {
+#pragma unroll
+            for (int i = 0; i < 4; i++) a[p][i] = *(const u4*)&Ps[p][wm * 64 + i * 16 + (lane & 15)][kof];
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[p][j] = *(const u4*)&Qs[p][wn * 64 + j * 16 + (lane & 15)][kof];
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if constexpr (PREC == 1) {
+                    // small terms first, then the leading product
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[0][i]), __builtin_bit_cast(bf8, b[1][j]), acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[1][i]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[0][i]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+                } else {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        __builtin_bit_cast(h8, a[0][i]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
+                }
+            }
+    }
+    float* Cb = C + (long)sp * split_stride;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+            const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+            *(f4*)(Cb + row + col * ldc) = acc[i][j];
+        }
+}
+
+hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo, long ldp,
+                          const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq, float* C,
+                          long ldc, long split_stride, int M, int N, int K, int splits, const int* done,
+                          hipStream_t st) {
+    if (M % kGemmBM || N % kGemmBN || splits < 1 || K % (kGemmBK * splits)) return hipErrorInvalidValue;
+    const int tiles_m = M / kGemmBM, tiles_n = N / kGemmBN, ksplit = K / splits;
+    const int grid = tiles_m * tiles_n * splits;
+    if (precision == 0) {
+        gemm_tn_f32_kernel<<<grid, 256, 0, st>>>(P32, ldp, Q32, ldq, C, ldc, split_stride, tiles_m, tiles_n, splits,
+                                                 ksplit, done);
+    } else if (precision == 1) {
+        gemm_tn_16_kernel<1><<<grid, 256, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
+                                                   tiles_n, splits, ksplit, done);
+    } else {
+        gemm_tn_16_kernel<2><<<grid, 256, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
+                                                   tiles_n, splits, ksplit, done);
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- z epilogue
+__device__ __forceinline__ bool is_constrained(int f) { return f < 4; }
+__device__ __forceinline__ bool is_l1_variance(int f) { return f == 1 || f == 3 || f == 5 || f == 7; }
+__device__ __forceinline__ float sgnf(float v) { return (float)((v > 0.f) - (v < 0.f)); }
+
+__device__ __forceinline__ void write_operand(Operand op, int precision, long idx, float v) {
+    op.f32[idx] = v;
+    if (precision == 1) {
+        const uint16_t h = f2bf(v);
+        op.hi[idx] = h;
+        op.lo[idx] = f2bf(v - bf2f(h));
+    } else if (precision == 2) {
+        op.hi[idx] = f2h(v);
+    }
+}
+
+// One workgroup per SP column.  z = sum of split partials (fixed order).  Constrained:
+// L1 variance f = |z|_1 then z = sgn z; L2 variance f = |z|_2 (z unchanged)
+// (c++/src/gpower/gpower_commons.h:75-101).  Penalised: z = sgn z or z/|z|_2 (:150-166).
+__global__ __launch_bounds__(256) void prep_z_kernel(const float* __restrict__ Zpart, int splits,
+                                                     long split_stride, int m, int ldz, Operand zop,
+                                                     int formulation, int precision, SpState sp) {
+    const int j = blockIdx.x;
+    if (*sp.done || !sp.active[j]) return;
+    __shared__ double red[4];
+    const float* zc = Zpart + (long)j * ldz;
+    const bool l1v = is_l1_variance(formulation);
+    double acc = 0;
+    for (int i = threadIdx.x; i < m; i += 256) {
+        float z = zc[i];
+        for (int s = 1; s < splits; s++) z += zc[(long)s * split_stride + i];
+        zop.f32[(long)j * ldz + i] = z;
+        acc += l1v ? (double)fabsf(z) : (double)z * (double)z;
+    }
+    const double tot = block_sum<256>(acc, red);
+    float stat = l1v ? (float)tot : (float)sqrt(tot);
+    if (threadIdx.x == 0) sp.fz[j] = stat;
+    const bool constrained = is_constrained(formulation);
+    const float inv = 1.0f / stat;
+    for (int i = threadIdx.x; i < m; i += 256) {
+        const long idx = (long)j * ldz + i;
+        float z = zop.f32[idx];
+        if (l1v) z = sgnf(z);
+        else if (!constrained) z = z * inv;
+        write_operand(zop, precision, idx, z);
+    }
+}
+hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int m, int ldz, Operand zop,
+                         int formulation, int precision, int batch, SpState sp, hipStream_t st) {
+    prep_z_kernel<<<batch, 256, 0, st>>>(Zpart, splits, split_stride, m, ldz, zop, formulation, precision, sp);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- formulation step
+// Exact selection without sorting.  A first pass bins |v| linearly over [0, max] (2048
+// bins, LDS histogram), which spreads the LDS atomics; the bin holding the answer is then
+// resolved exactly by four 8-bit radix passes over the IEEE bits of |v| (monotone for
+// non-negative floats), restricted to that bin's elements.  Counts are integers and the
+// sums the lambda search needs are accumulated in 2^-40-scaled fixed point, so every
+// decision is independent of atomic order: results are bitwise reproducible.
+// The column is re-read per pass through a bounded buffer view (coalesced; it stays in
+// L2/MALL), so any n works with one code path.
+constexpr int kBins = 2048;
+constexpr double kFix = 1099511627776.0;  // 2^40: n * 2^40 < 2^63 for n < 2^23
+
+struct SelShared {
+    uint32_t cnt[kBins];
+    uint64_t s1[kBins];
+    uint64_t s2[kBins];
+    double red[kSelThreads / 64];
+    int ires[4];
+    uint64_t ures[2];
+};
+
+__device__ __forceinline__ int lin_bin(float a, float scale) {
+    int b = (int)(a * scale);
+    return b < kBins - 1 ? b : kBins - 1;
+}
+
+// lambda of the reference GPU scan (c++/src/gpugpower/my_cublas_wrapper.h:335-375):
+// with r(l) = (S1 - c l)/sqrt(S2 - 2 l S1 + c l^2) over elements > l, the survivors are
+// the elements a with r(a) < w (w = sqrt(s)); lambda is the "-" root of
+// c(c-w^2) l^2 + (2 S1 w^2 - 2 c S1) l + (S1^2 - w^2 S2) = 0 over those survivors.
+__device__ __forceinline__ bool pred_r(double c, double S1, double S2, double lam, double w) {
+    const double r = (S1 - c * lam) / sqrt(S2 - 2.0 * lam * S1 + c * lam * lam);
+    return r >= w;  // NaN (no element above lam) is false
+}
+
+// One wave (lanes own NB/64 consecutive bins) scans a histogram from the top.  L0 mode
+// (l1c false): find the bin where the count from the top reaches `rank`.  Lambda mode:
+// find the largest bin whose lower boundary satisfies pred_r.  Writes ires[0] = bin (or
+// -1), ires[1] = count above it, ures = fixed-point sums above it.
+template <int NB, class Boundary>
+__device__ void scan_hist(SelShared& sh, bool l1c, uint32_t rank, uint32_t cA, uint64_t s1A, uint64_t s2A,
+                          double M, double wd, Boundary boundary) {
+    const int lane = threadIdx.x & 63;
+    constexpr int PER = NB / 64;
+    uint32_t lc = 0;
+    uint64_t l1 = 0, l2 = 0;
+    for (int b = lane * PER; b < lane * PER + PER; b++) {
+        lc += sh.cnt[b];
+        if (l1c) { l1 += sh.s1[b]; l2 += sh.s2[b]; }
+    }
+    const uint32_t ic = wave_suffix_u32(lc);
+    uint64_t i1 = 0, i2 = 0;
+    if (l1c) { i1 = wave_suffix_u64(l1); i2 = wave_suffix_u64(l2); }
+    uint32_t c = ic - lc;
+    uint64_t S1 = i1 - l1, S2 = i2 - l2;
+    int found = -1;
+    uint32_t fc = 0;
+    uint64_t f1 = 0, f2 = 0;
+    for (int b = lane * PER + PER - 1; b >= lane * PER; b--) {
+        const uint32_t cb = sh.cnt[b];
+        if (!l1c) {
+            if (found < 0 && c < rank && c + cb >= rank) { found = b; fc = c; }
+        } else {
+            const uint64_t b1 = sh.s1[b], b2 = sh.s2[b];
+            double lam;
+            const int forced = boundary(b, lam);  // 1: true, -1: false, 0: evaluate
+            const bool p = forced > 0 || (forced == 0 && pred_r((double)(cA + c + cb), (double)(s1A + S1 + b1) * M / kFix,
+                                                                 (double)(s2A + S2 + b2) * M * M / kFix, lam, wd));
+            if (found < 0 && p) { found = b; fc = c; f1 = S1; f2 = S2; }
+            S1 += b1;
+            S2 += b2;
+        }
+        c += cb;
+    }
+    const int fmax = wave_max_i(found);
+    if (found == fmax && found >= 0) {
+        sh.ires[0] = found;
+        sh.ires[1] = (int)fc;
+        sh.ures[0] = f1;
+        sh.ures[1] = f2;
+    }
+    if (lane == 0 && fmax < 0) sh.ires[0] = -1;
+}
+
+__global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restrict__ V, Operand xop, StepArgs a,
+                                                           SpState sp) {
+    const int j = blockIdx.x;
+    if (*sp.done || !sp.active[j]) return;
+    __shared__ SelShared sh;
+    const int tid = threadIdx.x;
+    const int nq = (a.n + kSelThreads - 1) / kSelThreads;
+    const __amdgpu_buffer_rsrc_t vr = col_rsrc(V + (long)j * a.ldv, (uint32_t)a.n * 4u);
+#define LDV(q) __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vr, tid * 4, (q) * kSelThreads * 4, 0))
+    const int f = a.formulation;
+    float fval;
+    // element transform applied when the iterate is written: keep |v| >= tau (L0),
+    // soft map with lam (L1), penalised maps; then multiply by `scale_out`
+    float tau = 0.f, lam = 0.f, scale_out = 1.f;
+
+    if (f < 4) {
+        // ---- constrained: select, then x <- x/|x|_2
+        float mloc = 0.f;
+        for (int q = 0; q < nq; q++) mloc = fmaxf(mloc, fabsf(LDV(q)));
+        const float M = block_max<kSelThreads>(mloc, sh.red);
+        const bool l1c = (f == 2 || f == 3);
+        const float scale = M > 0.f ? (float)kBins / M : 0.f;
+        const double invM = M > 0.f ? 1.0 / (double)M : 0.0;
+        const double Md = (double)M;
+        for (int b = tid; b < kBins; b += kSelThreads) {
+            sh.cnt[b] = 0;
+            sh.s1[b] = 0;
+            sh.s2[b] = 0;
+        }
+        __syncthreads();
+        for (int q = 0; q < nq; q++) {
+            const float av = fabsf(LDV(q));
+            const int b = lin_bin(av, scale);
+            atomicAdd(&sh.cnt[b], 1u);
+            if (l1c && av > 0.f) {
+                const double u = (double)av * invM;
+                atomicAdd((unsigned long long*)&sh.s1[b], (unsigned long long)(u * kFix + 0.5));
+                atomicAdd((unsigned long long*)&sh.s2[b], (unsigned long long)(u * u * kFix + 0.5));
+            }
+        }
+        __syncthreads();
+        const double wd = (double)(float)sqrt((double)a.s);
+        if (tid < 64)
+            scan_hist<kBins>(sh, l1c, a.s, 0u, 0ull, 0ull, Md, wd, [&](int b, double& l) {
+                l = (double)b * Md / (double)kBins;
+                return 0;
+            });
+        __syncthreads();
+        const int bstar = sh.ires[0];
+        uint32_t prefix = 0;  // key of the answer within bin bstar
+        uint32_t cA = 0, rank = a.s;
+        uint64_t s1A = 0, s2A = 0;
+        if (bstar >= 0) {
+            cA = (uint32_t)sh.ires[1];
+            s1A = sh.ures[0];
+            s2A = sh.ures[1];
+            rank = a.s - cA;
+            const double lam_b = (double)bstar * Md / (double)kBins;
+            for (int pass = 0; pass < 4; pass++) {
+                const int shift = 24 - 8 * pass;
+                __syncthreads();
+                if (tid < 256) {
+                    sh.cnt[tid] = 0;
+                    sh.s1[tid] = 0;
+                    sh.s2[tid] = 0;
+                }
+                __syncthreads();
+                for (int q = 0; q < nq; q++) {
+                    const float av = fabsf(LDV(q));
+                    const uint32_t key = __float_as_uint(av);
+                    if (lin_bin(av, scale) == bstar && (pass == 0 || (key >> (shift + 8)) == prefix)) {
+                        const int d = (key >> shift) & 255;
+                        atomicAdd(&sh.cnt[d], 1u);
+                        if (l1c && av > 0.f) {
+                            const double u = (double)av * invM;
+                            atomicAdd((unsigned long long*)&sh.s1[d], (unsigned long long)(u * kFix + 0.5));
+                            atomicAdd((unsigned long long*)&sh.s2[d], (unsigned long long)(u * u * kFix + 0.5));
+                        }
+                    }
+                }
+                __syncthreads();
+                if (tid < 64)
+                    scan_hist<256>(sh, l1c, rank, cA, s1A, s2A, Md, wd, [&](int d, double& l) {
+                        const uint32_t K = (pass == 0 ? 0u : (prefix << (shift + 8))) | ((uint32_t)d << shift);
+                        // keys from +inf up (inf, NaN, sign bit set) bound no finite |v|: false.
+                        // A boundary at or below the bin's lower edge is true by the choice of bstar.
+                        if (K >= 0x7F800000u) return -1;
+                        l = (double)__uint_as_float(K);
+                        return l <= lam_b ? 1 : 0;
+                    });
+                __syncthreads();
+                const int d = sh.ires[0];
+                if (d < 0) break;  // cannot happen: digit 0 of the current range always qualifies
+                cA += (uint32_t)sh.ires[1];
+                s1A += sh.ures[0];
+                s2A += sh.ures[1];
+                rank -= (uint32_t)sh.ires[1];
+                prefix = (pass == 0 ? 0u : prefix << 8) | (uint32_t)d;
+            }
+        }
+        __syncthreads();
+        double norm2 = 0;
+        if (!l1c) {
+            // ---- L0 constrained: tau = s-th largest |v| (counting ties), keep |v| >= tau
+            // (my_cublas_wrapper.h:240-255,623-626)
+            tau = bstar >= 0 ? __uint_as_float(prefix) : 0.f;
+            for (int q = 0; q < nq; q++) {
+                const float v = LDV(q);
+                if (!(fabsf(v) < tau)) norm2 += (double)v * (double)v;
+            }
+        } else {
+            // ---- L1 constrained: survivors are the elements above the answer key, or all
+            // n elements when no element triggers the scan
+            double c = 0, S1 = 0, S2 = 0;
+            for (int q = 0; q < nq; q++) {
+                const float av = fabsf(LDV(q));
+                bool in;
+                if (bstar < 0) {
+                    in = (tid + q * kSelThreads) < a.n;
+                } else {
+                    const int b = lin_bin(av, scale);
+                    in = b > bstar || (b == bstar && __float_as_uint(av) > prefix);
+                }
+                if (in) {
+                    c += 1.0;
+                    S1 += (double)av;
+                    S2 += (double)av * (double)av;
+                }
+            }
+            c = block_sum<kSelThreads>(c, sh.red);
+            S1 = block_sum<kSelThreads>(S1, sh.red);
+            S2 = block_sum<kSelThreads>(S2, sh.red);
+            const double w2 = wd * wd;
+            const double qa = c * (c - w2);
+            const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
+            const double qc = S1 * S1 - w2 * S2;
+            lam = (float)((-qb - sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa));
+            for (int q = 0; q < nq; q++) {
+                const float t2 = fabsf(LDV(q)) - lam;
+                if (t2 > 0.f) norm2 += (double)t2 * (double)t2;
+            }
+        }
+        norm2 = block_sum<kSelThreads>(norm2, sh.red);
+        const float nf = (float)sqrt(norm2);
+        scale_out = 1.0f / nf;
+        fval = sp.fz[j];
+    } else {
+        // ---- penalised (my_cublas_wrapper.h:170-238,636-710), gamma in the iterate's precision
+        const float g = (float)a.gamma;
+        double acc = 0;
+        if (f == 4 || f == 5) {
+            for (int q = 0; q < nq; q++) {
+                const float v = LDV(q);
+                const float t = v * v - g;
+                if (t > 0.f) acc += (double)t;
+            }
+            fval = (float)block_sum<kSelThreads>(acc, sh.red);
+        } else {
+            for (int q = 0; q < nq; q++) {
+                const float t = fabsf(LDV(q)) - g;
+                if (t > 0.f) acc += (double)t * (double)t;
+            }
+            fval = (float)sqrt(block_sum<kSelThreads>(acc, sh.red));
+        }
+        lam = g;
+    }
+    // ---- write the iterate (+ operand forms), count nonzeros, update the SP record
+    double nz = 0;
+    {
+        const long base = (long)j * a.ldv;
+        const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + base, (uint32_t)a.n * 4u);
+        const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)(xop.hi + base) : (const void*)xop.f32, a.precision ? (uint32_t)a.n * 2u : 0u);
+        const __amdgpu_buffer_rsrc_t lr = col_rsrc(xop.lo ? (const void*)(xop.lo + base) : (const void*)xop.f32, a.precision == 1 ? (uint32_t)a.n * 2u : 0u);
+        for (int q = 0; q < nq; q++) {
+            const float v = LDV(q);
+            float x;
+            if (f == 0 || f == 1) x = fabsf(v) < tau ? 0.f : v;
+            else if (f == 2 || f == 3 || f == 6 || f == 7) {
+                const float t2 = fabsf(v) - lam;
+                x = t2 > 0.f ? t2 * sgnf(v) : 0.f;
+            } else {
+                x = (v * v - lam) <= 0.f ? 0.f : v;  // L0 penalised: lam holds gamma
+            }
+            if (f < 4) x = x * scale_out;
+            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, tid * 4, q * kSelThreads * 4, 0);
+            nz += (x != 0.f && (tid + q * kSelThreads) < a.n) ? 1.0 : 0.0;
+            if (a.precision == 1) {
+                const uint16_t h = f2bf(x);
+                __builtin_amdgcn_raw_buffer_store_b16(h, hr, tid * 2, q * kSelThreads * 2, 0);
+                __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, tid * 2, q * kSelThreads * 2, 0);
+            } else if (a.precision == 2) {
+                __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, tid * 2, q * kSelThreads * 2, 0);
+            }
+        }
+    }
+#undef LDV
+    nz = block_sum<kSelThreads>(nz, sh.red);
+    if (tid == 0) {
+        const float prev = sp.val[j];
+        sp.err[j] = fabsf(fval - prev) / prev;  // computeTheError (termination_criteria.h:29-36)
+        sp.val[j] = fval;
+        sp.nnz[j] = (int)nz;
+    }
+}
+
+hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st) {
+    if (a.n <= 0 || (long)a.n * 4 > 0x7fffffffL) return hipErrorInvalidValue;
+    step_kernel<<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- convergence
+// Batch rule (gpu_sparse_PCA_solver.h:176-186, sparse_PCA_solver.h:203-207): error =
+// max_j err_j with NaN skipped by the '<' update; stop iff it > 0 and error < tol.
+// Per-SP rule: each active SP stops on its own error or at max_iter.
+__global__ __launch_bounds__(256) void converge_kernel(SpState sp, int batch, int it, double tol, int max_iter,
+                                                       int per_sp) {
+    if (*sp.done) return;
+    __shared__ float wmax[4];
+    __shared__ int wany[4];
+    float e = 0.f;
+    int any = 0;
+    for (int j = threadIdx.x; j < batch; j += 256) {
+        if (!sp.active[j]) continue;
+        const float ej = sp.err[j];
+        if (e < ej) e = ej;
+        sp.iters[j] += 1;
+        if (per_sp) {
+            if ((it > 0 && (double)ej < tol) || sp.iters[j] >= max_iter) sp.active[j] = 0;
+            else any = 1;
+        }
+    }
+    // max with the same NaN-skipping semantics: a NaN never wins a '<' comparison
+    for (int o = 32; o > 0; o >>= 1) {
+        const float t = __shfl_down(e, o, 64);
+        if (e < t) e = t;
+        any |= __shfl_down(any, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) { wmax[threadIdx.x >> 6] = e; wany[threadIdx.x >> 6] = any; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = 0.f;
+        int a = 0;
+        for (int w = 0; w < 4; w++) {
+            if (m < wmax[w]) m = wmax[w];
+            a |= wany[w];
+        }
+        sp.errmax[0] = m;
+        sp.done[1] += 1;
+        int stop;
+        if (per_sp) stop = !a;
+        else stop = it > 0 && (double)m < tol;
+        if (stop || it + 1 >= max_iter) sp.done[0] = 1;
+    }
+}
+hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_iter, int per_sp, hipStream_t st) {
+    converge_kernel<<<1, 256, 0, st>>>(sp, batch, it, tol, max_iter, per_sp);
+    return hipGetLastError();
+}
+
+__global__ void x_operand_kernel(Operand xop, int precision, int ldx, int batch) {
+    const long total = (long)ldx * batch;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
+        write_operand(xop, precision, e, xop.f32[e]);
+}
+hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st) {
+    if (precision == 0) return hipSuccess;
+    x_operand_kernel<<<2048, 256, 0, st>>>(xop, precision, ldx, batch);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- FP64 explained variance
+// ev_j = |A xhat_j|_2^2 (L2 variance) or |A xhat_j|_1 (L1 variance), xhat = x/|x|_2, all
+// in FP64 from the FP32 master copy of A.  Nonzeros of x are compacted in index order
+// (ballot prefix counts), so the summation order is fixed.
+constexpr int kEvThreads = 256;
+constexpr int kEvList = 2048;
+__global__ __launch_bounds__(kEvThreads) void ev_fp64_kernel(const float* __restrict__ A, int lda, int m, int n,
+                                                              const float* __restrict__ X, int ldx, int l1,
+                                                              double* __restrict__ ev) {
+    const int j = blockIdx.x;
+    const float* x = X + (long)j * ldx;
+    __shared__ double red[kEvThreads / 64];
+    __shared__ int idxs[kEvList];
+    __shared__ double vals[kEvList];
+    __shared__ int wcount[kEvThreads / 64];
+    double nx = 0;
+    for (int k = threadIdx.x; k < n; k += kEvThreads) nx += (double)x[k] * (double)x[k];
+    nx = block_sum<kEvThreads>(nx, red);
+    const double inv = 1.0 / sqrt(nx);
+    constexpr int RPT = 64;  // rows per thread held in registers (m <= 16384)
+    double acc[RPT];
+#pragma unroll
+    for (int r = 0; r < RPT; r++) acc[r] = 0;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int kbase = 0;
+    while (kbase < n) {
+        // compact up to kEvList nonzeros from [kbase, ...) in index order
+        int listed = 0;
+        int k0 = kbase;
+        while (k0 < n && listed + kEvThreads <= kEvList) {
+            const int k = k0 + threadIdx.x;
+            const float xv = k < n ? x[k] : 0.f;
+            const bool nzf = xv != 0.f;
+            const uint64_t bal = __ballot(nzf);
+            const int before = __popcll(bal & ((1ull << lane) - 1ull));
+            if (lane == 0) wcount[w] = __popcll(bal);
+            __syncthreads();
+            int off = listed;
+            for (int i = 0; i < w; i++) off += wcount[i];
+            if (nzf) { idxs[off + before] = k; vals[off + before] = (double)xv * inv; }
+            int tot = 0;
+            for (int i = 0; i < kEvThreads / 64; i++) tot += wcount[i];
+            __syncthreads();
+            listed += tot;
+            k0 += kEvThreads;
+        }
+        kbase = k0;
+        for (int t = 0; t < listed; t++) {
+            const float* col = A + (long)idxs[t] * lda;
+            const double xv = vals[t];
+#pragma unroll
+            for (int r = 0; r < RPT; r++) {
+                const int i = threadIdx.x + r * kEvThreads;
+                if (i < m) acc[r] += (double)col[i] * xv;
+            }
+        }
+        __syncthreads();
+    }
+    double s = 0;
+#pragma unroll
+    for (int r = 0; r < RPT; r++) s += l1 ? fabs(acc[r]) : acc[r] * acc[r];
+    s = block_sum<kEvThreads>(s, red);
+    if (threadIdx.x == 0) ev[j] = s;
+}
+hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* X, int ldx, int batch, int l1,
+                          double* ev, hipStream_t st) {
+    if (m > 64 * kEvThreads) return hipErrorInvalidValue;
+    ev_fp64_kernel<<<batch, kEvThreads, 0, st>>>(Acm, lda, m, n, X, ldx, l1, ev);
+    return hipGetLastError();
+}
+
+}  // namespace spca

@@ -1,0 +1,316 @@
+"""Python mirror of the reference solver interface over the C ABI of libspca_mi355x.so.
+
+The names follow the reference's C++ API so that code written against it reads the same:
+
+* ``SPCA_Formulation``        c++/src/class/optimization_settings.h:27-37
+* ``OptimizationSettings``    c++/src/class/optimization_settings.h:71-168
+* ``OptimizationStatistics``  c++/src/class/optimization_statistics.h:27-46
+* ``GPUSolver.denseDataSolver``  c++/src/gpugpower/gpu_sparse_PCA_solver.h:29-211
+
+Every solve runs on the MI355X through HIP kernels; there is no CPU fallback.  Loading
+raises if the library has not been built (``python 24am_amd/spca_mi355x/build.py``).
+"""
+import ctypes
+import enum
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libspca_mi355x.so")
+
+SPCA_OK, SPCA_EINVAL, SPCA_ENOMEM, SPCA_EBUDGET, SPCA_EDEVICE, SPCA_ECANCELLED, SPCA_ENOTFOUND, SPCA_EAGAIN = range(8)
+STATUS_NAMES = {0: "SPCA_OK", 1: "SPCA_EINVAL", 2: "SPCA_ENOMEM", 3: "SPCA_EBUDGET", 4: "SPCA_EDEVICE",
+                5: "SPCA_ECANCELLED", 6: "SPCA_ENOTFOUND", 7: "SPCA_EAGAIN"}
+
+STORE_FP32, STORE_BF16X3, STORE_FP16 = 1, 2, 4
+PREC_FP32, PREC_BF16X3, PREC_FP16 = 0, 1, 2
+TERM_BATCH_MAX, TERM_PER_SP = 0, 1
+START_EXPLICIT, START_REFERENCE, START_REFERENCE_F64 = 0, 1, 2
+
+# every symbol include/spca.h declares
+EXPORTS = [
+    "spca_ctx_create", "spca_ctx_destroy", "spca_last_error", "spca_version", "spca_matrix_upload",
+    "spca_matrix_from_device", "spca_matrix_add_storage", "spca_matrix_release", "spca_matrix_bytes", "spca_solve",
+    "spca_dense_solver_device", "spca_dense_solver_host", "spca_settings_default", "spca_settings_check",
+    "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
+]
+
+
+class SPCA_Formulation(enum.IntEnum):
+    L0_constrained_L2_PCA = 0
+    L0_constrained_L1_PCA = 1
+    L1_constrained_L2_PCA = 2
+    L1_constrained_L1_PCA = 3
+    L0_penalized_L2_PCA = 4
+    L0_penalized_L1_PCA = 5
+    L1_penalized_L2_PCA = 6
+    L1_penalized_L1_PCA = 7
+
+
+class SpcaError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _Settings(ctypes.Structure):
+    _fields_ = [
+        ("formulation", ctypes.c_int), ("constraint", ctypes.c_uint), ("penalty", ctypes.c_double),
+        ("tolerance", ctypes.c_double), ("max_iter", ctypes.c_int), ("total_sp", ctypes.c_int),
+        ("batch", ctypes.c_int), ("use_otf", ctypes.c_int), ("termination", ctypes.c_int),
+        ("precision", ctypes.c_int), ("start", ctypes.c_int), ("check_every", ctypes.c_int),
+    ]
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [
+        ("it", ctypes.c_uint), ("fval", ctypes.c_double), ("error", ctypes.c_double),
+        ("loop_ms", ctypes.c_double), ("wall_ms", ctypes.c_double), ("n", ctypes.c_uint),
+        ("best_sp", ctypes.c_int), ("total_sp", ctypes.c_int), ("constraint", ctypes.c_uint),
+        ("ev_fp64", ctypes.c_double), ("gemm_ax_ms", ctypes.c_double), ("gemm_atz_ms", ctypes.c_double),
+        ("step_ms", ctypes.c_double), ("sp_iterations", ctypes.c_ulonglong),
+    ]
+
+
+class _SpResult(ctypes.Structure):
+    _fields_ = [("value", ctypes.c_float), ("iters", ctypes.c_int), ("nnz", ctypes.c_int),
+                ("converged", ctypes.c_int), ("ev_fp64", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libspca_mi355x.so; raises if it is missing (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run python 24am_amd/spca_mi355x/build.py")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, U, U64 = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint, ctypes.c_uint64
+        sig = {
+            "spca_ctx_create": (I, [I, U64, ctypes.POINTER(P)]),
+            "spca_ctx_destroy": (I, [P]),
+            "spca_last_error": (ctypes.c_char_p, [P]),
+            "spca_version": (ctypes.c_char_p, []),
+            "spca_matrix_upload": (I, [P, P, I, I, I, U, U64, ctypes.POINTER(U64)]),
+            "spca_matrix_from_device": (I, [P, P, I, I, I, U, U64, ctypes.POINTER(U64)]),
+            "spca_matrix_add_storage": (I, [P, U64, U]),
+            "spca_matrix_release": (I, [P, U64]),
+            "spca_matrix_bytes": (I, [P, U64, ctypes.POINTER(U64)]),
+            "spca_solve": (I, [P, U64, ctypes.POINTER(_Settings), P, P, ctypes.POINTER(_Stats), P, P]),
+            "spca_dense_solver_device": (I, [P, U, U, P, P, ctypes.POINTER(_Settings), ctypes.POINTER(_Stats), U, U]),
+            "spca_dense_solver_host": (I, [P, P, I, P, U, U, ctypes.POINTER(_Settings), ctypes.POINTER(_Stats),
+                                           ctypes.POINTER(ctypes.c_double)]),
+            "spca_settings_default": (None, [ctypes.POINTER(_Settings)]),
+            "spca_settings_check": (I, [ctypes.POINTER(_Settings), U]),
+            "spca_is_constrained": (I, [I]),
+            "spca_is_l1_constrained": (I, [I]),
+            "spca_is_l1_penalized": (I, [I]),
+            "spca_is_l1_variance": (I, [I]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def version():
+    return lib().spca_version().decode()
+
+
+class OptimizationSettings:
+    """Reference OptimizationSettings (defaults from optimization_settings.h:101-123) plus
+    the MI355X knobs: precision, termination rule, starting-point source, check cadence."""
+
+    def __init__(self, **kw):
+        self.tolerance = 0.01
+        self.constraintParameter = 10
+        self.penaltyParameter = 0.0
+        self.totalStartingPoints = 64
+        self.batchSize = 64
+        self.useOTF = False
+        self.maximumIterations = 20
+        self.formulation = SPCA_Formulation.L0_constrained_L2_PCA
+        self.precision = PREC_FP32
+        self.termination = TERM_BATCH_MAX
+        self.start = START_EXPLICIT
+        self.check_every = 1
+        for k, v in kw.items():
+            if not hasattr(self, k):
+                raise AttributeError(k)
+            setattr(self, k, v)
+
+    def isConstrainedProblem(self):
+        return bool(lib().spca_is_constrained(int(self.formulation)))
+
+    def isL1ConstrainedProblem(self):
+        return bool(lib().spca_is_l1_constrained(int(self.formulation)))
+
+    def isL1PenalizedProblem(self):
+        return bool(lib().spca_is_l1_penalized(int(self.formulation)))
+
+    def chceckInputAndModifyIt(self, n):
+        """Reference clamp: s <= n, total SPs rounded up to a batch multiple; returns batches."""
+        c = self._c()
+        tb = lib().spca_settings_check(ctypes.byref(c), n)
+        self.constraintParameter, self.batchSize, self.totalStartingPoints = c.constraint, c.batch, c.total_sp
+        return tb
+
+    def _c(self):
+        return _Settings(int(self.formulation), int(self.constraintParameter), float(self.penaltyParameter),
+                         float(self.tolerance), int(self.maximumIterations), int(self.totalStartingPoints),
+                         int(self.batchSize), int(bool(self.useOTF)), int(self.termination), int(self.precision),
+                         int(self.start), int(self.check_every))
+
+
+class OptimizationStatistics:
+    """Reference OptimizationStatistics plus timing and FP64 explained variance."""
+
+    def __init__(self):
+        self.it = 0
+        self.fval = 0.0
+        self.error = 0.0
+        self.totalTrueComputationTime = 0.0  # seconds, iteration loop (hipEvents)
+        self.totalElapsedTime = 0.0          # seconds, whole call
+        self.n = 0
+        self.values = []
+        self.iters = []
+        self.cardinalities = []
+        self.ev_fp64 = None
+        self.sp_ev_fp64 = []
+        self.converged = []
+        self.best_sp = -1
+        self.gemm_ax_ms = 0.0
+        self.gemm_atz_ms = 0.0
+        self.step_ms = 0.0
+        self.sp_iterations = 0
+
+    def _fill(self, s, per_sp=None):
+        self.it, self.fval, self.error = s.it, s.fval, s.error
+        self.totalTrueComputationTime = s.loop_ms / 1e3
+        self.totalElapsedTime = s.wall_ms / 1e3
+        self.n, self.best_sp, self.ev_fp64 = s.n, s.best_sp, s.ev_fp64
+        self.gemm_ax_ms, self.gemm_atz_ms, self.step_ms = s.gemm_ax_ms, s.gemm_atz_ms, s.step_ms
+        self.sp_iterations = s.sp_iterations
+        if per_sp is not None:
+            self.values = np.array([r.value for r in per_sp], dtype=np.float32)
+            self.iters = np.array([r.iters for r in per_sp], dtype=np.int32)
+            self.cardinalities = np.array([r.nnz for r in per_sp], dtype=np.int32)
+            self.converged = np.array([r.converged for r in per_sp], dtype=bool)
+            self.sp_ev_fp64 = np.array([r.ev_fp64 for r in per_sp], dtype=np.float64)
+
+
+def _check(ctx_ptr, rc):
+    if rc != SPCA_OK:
+        msg = lib().spca_last_error(ctx_ptr).decode() if ctx_ptr else ""
+        raise SpcaError(rc, msg)
+
+
+class Context:
+    """Owns the device, the solver stream, resident matrices and the HBM budget."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        self._p = ctypes.c_void_p()
+        rc = lib().spca_ctx_create(device, hbm_budget_bytes, ctypes.byref(self._p))
+        if rc != SPCA_OK:
+            raise SpcaError(rc, "spca_ctx_create failed")
+
+    @property
+    def ptr(self):
+        return self._p
+
+    def close(self):
+        if self._p:
+            lib().spca_ctx_destroy(self._p)
+            self._p = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, A, storage=STORE_FP32, key=0):
+        """A: (m x n) float32 array (any order); copied column major to the device."""
+        A = np.asfortranarray(A, dtype=np.float32)
+        m, n = A.shape
+        h = ctypes.c_uint64()
+        _check(self._p, lib().spca_matrix_upload(self._p, A.ctypes.data_as(ctypes.c_void_p), m, n, m, storage, key,
+                                                 ctypes.byref(h)))
+        return Matrix(self, h.value, m, n)
+
+
+class Matrix:
+    def __init__(self, ctx, handle, m, n):
+        self.ctx, self.handle, self.m, self.n = ctx, handle, m, n
+
+    def nbytes(self):
+        b = ctypes.c_uint64()
+        _check(self.ctx.ptr, lib().spca_matrix_bytes(self.ctx.ptr, self.handle, ctypes.byref(b)))
+        return b.value
+
+    def add_storage(self, storage):
+        _check(self.ctx.ptr, lib().spca_matrix_add_storage(self.ctx.ptr, self.handle, storage))
+
+    def release(self):
+        if self.handle:
+            _check(self.ctx.ptr, lib().spca_matrix_release(self.ctx.ptr, self.handle))
+            self.handle = 0
+
+    def solve(self, settings, X0=None, stats=None, want_X=False, per_sp=True):
+        """Returns x (unit norm, length n); fills `stats`; with want_X also returns X (n x total)."""
+        c = settings._c()
+        total = _total_after_check(c, self.n)
+        x = np.zeros(self.n, dtype=np.float32)
+        X0p = None
+        if c.start == START_EXPLICIT:
+            if X0 is None:
+                raise ValueError("X0 required for START_EXPLICIT")
+            X0 = np.asfortranarray(X0, dtype=np.float32)
+            if X0.shape != (self.n, total):
+                raise ValueError(f"X0 must be {self.n} x {total}")
+            X0p = X0.ctypes.data_as(ctypes.c_void_p)
+        res = (_SpResult * total)() if per_sp else None
+        Xf = np.zeros((self.n, total), dtype=np.float32, order="F") if want_X else None
+        s = _Stats()
+        _check(self.ctx.ptr, lib().spca_solve(self.ctx.ptr, self.handle, ctypes.byref(c), X0p,
+                                              x.ctypes.data_as(ctypes.c_void_p), ctypes.byref(s),
+                                              ctypes.cast(res, ctypes.c_void_p) if res is not None else None,
+                                              Xf.ctypes.data_as(ctypes.c_void_p) if want_X else None))
+        if stats is not None:
+            stats._fill(s, res)
+        return (x, Xf) if want_X else x
+
+
+def _total_after_check(c, n):
+    c2 = _Settings.from_buffer_copy(c)
+    lib().spca_settings_check(ctypes.byref(c2), n)
+    return c2.total_sp
+
+
+class GPUSolver:
+    """Mirror of SPCASolver::GPUSolver (c++/src/gpugpower/gpu_sparse_PCA_solver.h)."""
+
+    @staticmethod
+    def denseDataSolver(ctx, A, settings, stats, X0=None):
+        """Solve on host matrix A (m x n); returns the unit-norm x of length n.  With X0=None
+        the reference starting-point generator is used (SP k seeded with k)."""
+        if X0 is None and settings.start == START_EXPLICIT:
+            settings = OptimizationSettings(**{k: v for k, v in vars(settings).items()})
+            settings.start = START_REFERENCE
+        need = {PREC_FP32: STORE_FP32, PREC_BF16X3: STORE_BF16X3, PREC_FP16: STORE_FP16}[settings.precision]
+        M = ctx.upload(A, storage=need)
+        try:
+            return M.solve(settings, X0=X0, stats=stats)
+        finally:
+            M.release()

@@ -1,0 +1,200 @@
+/*
+ * spca.h -- C ABI of the MI355X alternating-maximization (AM) sparse-PCA solver.
+ *
+ * One library, libspca_mi355x.so, built with hipcc for gfx950.  Every entry point
+ * returns an int status (SPCA_OK on success); no C++ exception crosses this boundary.
+ * Buffers passed in are caller-owned host memory unless a name says "device";
+ * device memory the solver uses is owned by the library.
+ *
+ * Matrix convention is the reference's: A has m rows (samples) and n columns
+ * (variables), column major, leading dimension ld >= m.  An iterate x has length n.
+ *
+ * Reference interfaces this boundary replaces (optml/24am):
+ *   - SPCASolver::GPUSolver::denseDataSolver<F>(cublasHandle_t&, m, n, d_B, h_x,
+ *       OptimizationSettings*, OptimizationStatistics*, LD_M, LD_N)
+ *       c++/src/gpugpower/gpu_sparse_PCA_solver.h:29-35      -> spca_dense_solver_device,
+ *                                                                spca_solve
+ *   - SPCASolver::MulticoreSolver::denseDataSolver<F>(B, ldB, x, m, n, settings, stats)
+ *       c++/src/gpower/sparse_PCA_solver.h:36-40             -> spca_dense_solver_host
+ *   - SolverStructures::OptimizationSettings / OptimizationStatistics
+ *       c++/src/class/optimization_settings.h:71-168,
+ *       c++/src/class/optimization_statistics.h:27-46        -> spca_settings / spca_stats
+ *   - SPCA_Formulation  c++/src/class/optimization_settings.h:27-37 -> spca_formulation
+ */
+#ifndef SPCA_MI355X_H_
+#define SPCA_MI355X_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define SPCA_OK 0
+#define SPCA_EINVAL 1     /* bad argument (null pointer, bad enum, bad size) */
+#define SPCA_ENOMEM 2     /* device allocation failed */
+#define SPCA_EBUDGET 3    /* request exceeds the context's HBM budget */
+#define SPCA_EDEVICE 4    /* HIP runtime error */
+#define SPCA_ECANCELLED 5 /* job cancelled before completion */
+#define SPCA_ENOTFOUND 6  /* unknown job or matrix handle */
+#define SPCA_EAGAIN 7     /* job not finished yet (spca_poll) */
+
+/* ---- enums -------------------------------------------------------------- */
+/* reference SPCA_Formulation, same values (c++/src/class/optimization_settings.h:27-37) */
+typedef enum spca_formulation {
+    L0_constrained_L2_PCA = 0,
+    L0_constrained_L1_PCA = 1,
+    L1_constrained_L2_PCA = 2,
+    L1_constrained_L1_PCA = 3,
+    L0_penalized_L2_PCA = 4,
+    L0_penalized_L1_PCA = 5,
+    L1_penalized_L2_PCA = 6,
+    L1_penalized_L1_PCA = 7
+} spca_formulation;
+
+/* arithmetic of the two products A*X and A^T*Z; everything else is FP32/FP64 */
+typedef enum spca_precision {
+    SPCA_PREC_FP32 = 0,   /* FP32 MFMA, FP32 accumulate */
+    SPCA_PREC_BF16X3 = 1, /* A and iterates split hi+lo in bf16, three MFMA products, FP32 accumulate */
+    SPCA_PREC_FP16 = 2    /* A and iterates in fp16, FP32 accumulate */
+} spca_precision;
+
+/* when a batch stops (reference batch mode: c++/src/gpower/sparse_PCA_solver.h:184-208) */
+typedef enum spca_termination {
+    SPCA_TERM_BATCH_MAX = 0, /* stop the batch when max_j err_j < tol (and it > 0) */
+    SPCA_TERM_PER_SP = 1     /* freeze each starting point once its own err < tol */
+} spca_termination;
+
+/* where starting points come from */
+typedef enum spca_start {
+    SPCA_START_EXPLICIT = 0,     /* caller passes X0 (n x total_sp, column major) */
+    SPCA_START_REFERENCE = 1,    /* reference rand_r generator, SP k seeded with k
+                                    (c++/src/gpower/gpower_commons.h:21-47), in float */
+    SPCA_START_REFERENCE_F64 = 2 /* the same generator evaluated in double, as the reference's
+                                    double-precision runs draw it, then stored as float */
+} spca_start;
+
+/* storage forms a matrix handle may carry (bit mask) */
+#define SPCA_STORE_FP32 (1u << 0)
+#define SPCA_STORE_BF16X3 (1u << 1)
+#define SPCA_STORE_FP16 (1u << 2)
+
+/* ---- settings / statistics ---------------------------------------------- */
+/* mirrors OptimizationSettings (c++/src/class/optimization_settings.h:71-123) */
+typedef struct spca_settings {
+    int formulation;          /* spca_formulation */
+    unsigned constraint;      /* s: constraintParameter */
+    double penalty;           /* gamma: penaltyParameter */
+    double tolerance;         /* tolerance */
+    int max_iter;             /* maximumIterations */
+    int total_sp;             /* totalStartingPoints */
+    int batch;                /* batchSize (0 -> total_sp) */
+    int use_otf;              /* useOTF: refill converged slots (not yet supported: must be 0) */
+    int termination;          /* spca_termination */
+    int precision;            /* spca_precision */
+    int start;                /* spca_start */
+    int check_every;          /* host reads the convergence flag every k iterations (>=1) */
+} spca_settings;
+
+/* mirrors OptimizationStatistics (c++/src/class/optimization_statistics.h:27-46) */
+typedef struct spca_stats {
+    unsigned it;              /* total batch iterations done (sum over batches) */
+    double fval;              /* best reference-native objective (lagged value) */
+    double error;             /* final batch max error of the last batch */
+    double loop_ms;           /* iteration loop time, hipEvents (totalTrueComputationTime) */
+    double wall_ms;           /* whole call, host clock (totalElapsedTime) */
+    unsigned n;               /* length of x */
+    int best_sp;              /* global index of the SP that gave fval */
+    int total_sp;             /* after the reference clamp/rounding */
+    unsigned constraint;      /* s after the reference clamp (s <= n) */
+    double ev_fp64;           /* FP64 explained variance of x_out: |A x|_2^2 or |A x|_1 */
+    double gemm_ax_ms;        /* summed kernel time of the A*X products (hipEvents) */
+    double gemm_atz_ms;       /* summed kernel time of the A^T*Z products */
+    double step_ms;           /* summed time of epilogue + formulation step + reduction kernels */
+    unsigned long long sp_iterations; /* sum over SPs of iterations while active */
+} spca_stats;
+
+/* per starting point result (reference values / iters / cardinalities) */
+typedef struct spca_sp_result {
+    float value;              /* final reference-native objective */
+    int iters;                /* iterations this SP performed */
+    int nnz;                  /* nonzeros of its final iterate */
+    int converged;            /* 1 if its own err < tol at the end */
+    double ev_fp64;           /* FP64 explained variance of its normalised final iterate */
+} spca_sp_result;
+
+typedef struct spca_ctx_s* spca_ctx;
+typedef uint64_t spca_matrix;
+typedef uint64_t spca_job;
+
+/* ---- context ------------------------------------------------------------ */
+/* Select device `device`, create the context's stream and set the HBM budget in bytes
+ * (0 = 90% of free device memory).  No reference counterpart: the reference creates its
+ * cuBLAS handle in the front end (c++/src/frontends/gpu_console.cu:64-72). */
+int spca_ctx_create(int device, uint64_t hbm_budget_bytes, spca_ctx* out);
+int spca_ctx_destroy(spca_ctx ctx);
+/* last error message on this context (thread-local to the context), never NULL */
+const char* spca_last_error(spca_ctx ctx);
+/* library version string */
+const char* spca_version(void);
+
+/* ---- resident matrices ---------------------------------------------------- */
+/* Copy A (host, FP32, m x n, column major, leading dimension ld) to the device and build
+ * the operand layouts for the storage forms in `storage` (SPCA_STORE_* mask).  `key` lets
+ * later uploads of the same data share the resident copy (0 = no sharing).
+ * Replaces the front end's H2D copy, c++/src/frontends/gpu_console.cu:43-62. */
+int spca_matrix_upload(spca_ctx ctx, const float* A, int m, int n, int ld, unsigned storage,
+                       uint64_t key, spca_matrix* out);
+/* Same, from a caller-owned device buffer (FP32, column major, leading dimension ld). */
+int spca_matrix_from_device(spca_ctx ctx, const float* d_A, int m, int n, int ld, unsigned storage,
+                            uint64_t key, spca_matrix* out);
+/* Add storage forms to an existing matrix. */
+int spca_matrix_add_storage(spca_ctx ctx, spca_matrix mat, unsigned storage);
+int spca_matrix_release(spca_ctx ctx, spca_matrix mat);
+/* device bytes held by a matrix (all its storage forms) */
+int spca_matrix_bytes(spca_ctx ctx, spca_matrix mat, uint64_t* bytes);
+
+/* ---- solve ---------------------------------------------------------------- */
+/* Solve on a resident matrix.  X0: n x total_sp column-major host starting points when
+ * settings->start == SPCA_START_EXPLICIT (else ignored, may be NULL).  x_out (n floats)
+ * receives the best iterate normalised to unit 2-norm.  per_sp (total_sp entries) and
+ * X_final (n x total_sp host, column major; final iterates as held, unnormalised for
+ * penalised formulations) are optional.  Semantics: c++/src/gpower/gpower_commons.h:67-177
+ * for the iteration, with the reference GPU path's thresholds
+ * (c++/src/gpugpower/my_cublas_wrapper.h:240-255,335-395,571-634). */
+int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0,
+               float* x_out, spca_stats* stats, spca_sp_result* per_sp, float* X_final);
+
+/* Drop-in for GPUSolver::denseDataSolver<float>(handle, m, n, d_B, h_x, settings, stats,
+ * LD_M, LD_N) (c++/src/gpugpower/gpu_sparse_PCA_solver.h:29-35): d_B is a caller-owned
+ * device matrix with leading dimension LD_M; h_x (n floats, host) receives the result.
+ * The matrix is uploaded for the call and released on return, as the reference holds
+ * nothing between calls.  LD_N is accepted for signature parity and ignored. */
+int spca_dense_solver_device(spca_ctx ctx, unsigned m, unsigned n, const float* d_B, float* h_x,
+                             const spca_settings* settings, spca_stats* stats, unsigned LD_M,
+                             unsigned LD_N);
+
+/* Drop-in for MulticoreSolver::denseDataSolver<float>(B, ldB, x, m, n, settings, stats)
+ * (c++/src/gpower/sparse_PCA_solver.h:36-40), computed on the GPU: B is host memory.
+ * Returns the status; *fval_out receives the best value the reference returns. */
+int spca_dense_solver_host(spca_ctx ctx, const float* B, int ldB, float* x, unsigned m, unsigned n,
+                           const spca_settings* settings, spca_stats* stats, double* fval_out);
+
+/* ---- reference settings helpers ------------------------------------------- */
+/* reference OptimizationSettings defaults (c++/src/class/optimization_settings.h:101-123) */
+void spca_settings_default(spca_settings* st);
+/* reference chceckInputAndModifyIt(n) (c++/src/class/optimization_settings.h:156-168):
+ * clamp s <= n, total_sp rounded up to a multiple of batch; returns the batch count */
+int spca_settings_check(spca_settings* st, unsigned n);
+/* isConstrainedProblem / isL1ConstrainedProblem / isL1PenalizedProblem, :125-153 */
+int spca_is_constrained(int formulation);
+int spca_is_l1_constrained(int formulation);
+int spca_is_l1_penalized(int formulation);
+int spca_is_l1_variance(int formulation);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SPCA_MI355X_H_ */

@@ -4,7 +4,7 @@ import sys
 import pytest
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-for p in (REPO, os.path.join(REPO, "oracle"), os.path.join(REPO, "24am_amd")):
+for p in (REPO, os.path.join(REPO, "oracle"), os.path.join(REPO, "24am_amd"), os.path.dirname(__file__)):
     if p not in sys.path:
         sys.path.insert(0, p)
 
@@ -12,3 +12,12 @@ for p in (REPO, os.path.join(REPO, "oracle"), os.path.join(REPO, "24am_amd")):
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
     config.addinivalue_line("markers", "slow: longer CPU oracle runs")
+
+
+@pytest.fixture(scope="session")
+def ctx():
+    """One solver context per test process (device 0)."""
+    import spca_mi355x as S
+    c = S.Context(0)
+    yield c
+    c.close()

@@ -1,0 +1,67 @@
+"""Shared inputs and checks for the GPU parity tests (test infrastructure)."""
+import numpy as np
+
+import oracle as O
+
+L2_VAR = (0, 2, 4, 6)
+
+
+def spiked(m, n, k=20, strength=4.0, seed=0):
+    """Noise N(0,1) plus one planted sparse loading (k nonzeros) of the given strength."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, n))
+    v = np.zeros(n)
+    idx = rng.choice(n, k, replace=False)
+    v[idx] = rng.standard_normal(k)
+    v /= np.linalg.norm(v)
+    u = rng.standard_normal(m)
+    A += strength * np.sqrt(n / k) * 0.2 * np.outer(u, v)
+    return A.astype(np.float32), idx
+
+
+def unit_gaussian_starts(n, B, seed=1):
+    rng = np.random.default_rng(seed)
+    X0 = rng.standard_normal((n, B))
+    X0 /= np.linalg.norm(X0, axis=0)
+    return X0.astype(np.float32)
+
+
+def default_gamma(f, m):
+    """Penalties that leave a few dozen survivors on spiked(m, 1000)."""
+    return {4: 9.0, 5: 9.0 * m, 6: 3.0, 7: 3.0 * np.sqrt(m)}.get(f, 0.0)
+
+
+def explained_variance(A, X, f):
+    """FP64 EV of each normalised column: |A x|_2^2 (L2 variance) or |A x|_1."""
+    A = A.astype(np.float64)
+    Xn = X.astype(np.float64) / np.linalg.norm(X.astype(np.float64), axis=0)
+    Z = A @ Xn
+    return np.sum(Z * Z, axis=0) if f in L2_VAR else np.sum(np.abs(Z), axis=0)
+
+
+def tie_margin(A, X_prev, f, s, gamma):
+    """Relative gap between the threshold and the nearest competing value in the last
+    step, per SP, computed from the oracle's previous iterate (FP64).  A support that
+    differs only where this gap is within rounding is a documented magnitude tie."""
+    A = A.astype(np.float64)
+    Z = A @ X_prev
+    if f in L2_VAR:
+        if f >= 4:
+            Z = Z / np.linalg.norm(Z, axis=0)
+    else:
+        Z = np.sign(Z)
+    V = A.T @ Z
+    out = np.empty(V.shape[1])
+    for j in range(V.shape[1]):
+        a = np.abs(V[:, j])
+        if f in (0, 1):
+            srt = np.sort(a)[::-1]
+            out[j] = (srt[s - 1] - srt[s]) / srt[s - 1] if s < a.size else np.inf
+        elif f in (2, 3):
+            _, _, lam, _ = O.soft_threshold(V[:, j], s, O.LAMBDA_GPU)
+            out[j] = np.min(np.abs(a - lam)) / lam
+        elif f in (4, 5):
+            out[j] = np.min(np.abs(a * a - gamma)) / gamma
+        else:
+            out[j] = np.min(np.abs(a - gamma)) / gamma
+    return out

@@ -1,0 +1,334 @@
+"""Parity of the HIP solver (through the C ABI) with the CPU oracle and with results the
+reference published.  Needs an MI355X.
+
+Tolerances, per starting point (SP), against the FP64 oracle on the same seeded inputs:
+  FP32     explained variance (EV) within 1e-6 relative, objective within 1e-5
+  bf16x3   EV within 1e-5 relative, objective within 1e-4
+  fp16     EV within 1e-3 relative, objective within 5e-3 (fp16 keeps 11 bits; the
+           penalised objectives subtract gamma, which loses more)
+Supports after one step from the same iterate must be identical, except for an SP whose
+step had a magnitude tie: the threshold and its nearest competitor within TIE of each
+other (relative: 1e-5 FP32, 1e-4 bf16x3, 1e-3 fp16), where rounding of A*X decides the
+order; such SPs are counted and bounded (<= 5%; fp16, whose A*X error is ~1e-3, <= 35%).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+import spca_mi355x as S
+from spca_cases import default_gamma, explained_variance, spiked, tie_margin, unit_gaussian_starts
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+TOL = {  # precision: (EV, objective, tie, share of SPs allowed to sit on a tie in one step)
+    S.PREC_FP32: (1e-6, 1e-5, 1e-5, 0.05),
+    S.PREC_BF16X3: (1e-5, 1e-4, 1e-4, 0.05),
+    S.PREC_FP16: (1e-3, 5e-3, 1e-3, 0.35),
+}
+
+
+@pytest.fixture(scope="module")
+def small():
+    m, n, B = 200, 1000, 64
+    A, planted = spiked(m, n)
+    return A, unit_gaussian_starts(n, B), planted
+
+
+@pytest.fixture(scope="module")
+def small_mat(ctx, small):
+    A = small[0]
+    M = ctx.upload(A, storage=S.STORE_FP32 | S.STORE_BF16X3 | S.STORE_FP16)
+    yield M
+    M.release()
+
+
+def _settings(f, it, prec, B, s=10, m=200, tol=0.0):
+    return S.OptimizationSettings(formulation=f, constraintParameter=s, penaltyParameter=default_gamma(f, m),
+                                  tolerance=tol, maximumIterations=it, totalStartingPoints=B, batchSize=B,
+                                  precision=prec)
+
+
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3, S.PREC_FP16])
+@pytest.mark.parametrize("f", range(8))
+def test_step_parity(small, small_mat, f, prec):
+    """One AM iteration from identical iterates (the GPU's own iterate after t steps),
+    GPU vs FP64 oracle: supports identical except at this step's magnitude ties."""
+    A, X0, _ = small
+    m, n = A.shape
+    B, s, g = X0.shape[1], 10, default_gamma(f, m)
+    ev_tol, val_tol, tie_tol, tie_share = TOL[prec]
+    for t in (0, 5, 25):
+        print(f"case f={f} prec={prec} t={t} m={m} n={n} B={B}", flush=True)
+        Xt = X0 if t == 0 else small_mat.solve(_settings(f, t, prec, B), X0=X0, want_X=True)[1]
+        # a penalty can zero an SP out; its next iterate is then NaN (0/0) in both
+        alive = np.all(np.isfinite(Xt), axis=0) & np.any(Xt != 0, axis=0)
+        stats = S.OptimizationStatistics()
+        _, Xg = small_mat.solve(_settings(f, 1, prec, B), X0=Xt, stats=stats, want_X=True)
+        o = O.solve(A.astype(np.float64), f, s=s, gamma=g, tol=0.0, max_iter=1, total_sp=B,
+                    X0=Xt.astype(np.float64), want_X=True, lambda_rule=O.LAMBDA_GPU)
+        tie = tie_margin(A, Xt.astype(np.float64), f, s, g) < tie_tol
+        supp = np.all((Xg != 0) == (o["X"] != 0), axis=0)
+        chk = alive & ~tie
+        scale = np.abs(o["X"]).max(axis=0)
+        xerr = np.abs(Xg - o["X"]).max(axis=0) / np.where(scale > 0, scale, 1)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            vrel = np.where(stats.values == o["sp_val"], 0.0,
+                            np.abs(stats.values - o["sp_val"]) / np.abs(o["sp_val"]))
+        print(f"  t={t}: alive {alive.sum()} ties {tie[alive].sum()} supports equal {supp[alive].mean():.3f} "
+              f"x err {np.nanmax(xerr[chk]) if chk.any() else 0:.2e} value rel {np.nanmax(vrel[chk]) if chk.any() else 0:.2e}",
+              flush=True)
+        assert np.all(supp[chk]), np.nonzero(chk & ~supp)[0]
+        assert tie[alive].mean() <= tie_share
+        assert np.all(xerr[chk] <= 10 * val_tol), np.nanmax(xerr[chk])
+        assert np.all(vrel[chk] <= val_tol), np.nanmax(vrel[chk])
+        np.testing.assert_array_equal(stats.cardinalities[alive], (Xg != 0).sum(axis=0)[alive])
+
+
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3, S.PREC_FP16])
+@pytest.mark.parametrize("f", range(8))
+def test_trajectory_parity(small, small_mat, f, prec):
+    """60 iterations from the same seeded starts: per-SP explained variance of the final
+    iterate against the FP64 oracle.  An SP whose path branched on an earlier magnitude
+    tie may end on another support; those are bounded (<= 10%) and left out of the EV bound."""
+    A, X0, _ = small
+    m, n = A.shape
+    B, it, s, g = X0.shape[1], 60, 10, default_gamma(f, m)
+    ev_tol = TOL[prec][0]
+    print(f"case f={f} prec={prec} m={m} n={n} B={B} it={it}", flush=True)
+    stats = S.OptimizationStatistics()
+    x, Xg = small_mat.solve(_settings(f, it, prec, B), X0=X0, stats=stats, want_X=True)
+    o = O.solve(A.astype(np.float64), f, s=s, gamma=g, tol=0.0, max_iter=it, total_sp=B, X0=X0.astype(np.float64),
+                want_X=True, lambda_rule=O.LAMBDA_GPU)
+    alive_o = np.all(np.isfinite(o["X"]), axis=0) & np.any(o["X"] != 0, axis=0)
+    alive_g = np.all(np.isfinite(Xg), axis=0) & np.any(Xg != 0, axis=0)
+    np.testing.assert_array_equal(alive_g, alive_o)
+    supp = np.all((Xg != 0) == (o["X"] != 0), axis=0)
+    ev_o = explained_variance(A, o["X"][:, alive_o], f)
+    ev_rel = np.abs(stats.sp_ev_fp64[alive_o] - ev_o) / ev_o
+    same = supp[alive_o]
+    print(f"  alive {alive_o.sum()} supports equal {same.mean():.3f} EV rel max {ev_rel[same].max():.2e} "
+          f"median {np.median(ev_rel[same]):.2e}", flush=True)
+    assert stats.it == it
+    assert same.mean() >= 0.90
+    assert np.mean(ev_rel[same] <= ev_tol) >= 0.95, np.sort(ev_rel[same])[-5:]
+    if alive_o.any():
+        assert abs(np.linalg.norm(x.astype(np.float64)) - 1) < 1e-6
+        assert stats.best_sp == int(np.nanargmax(np.where(alive_g, stats.values, -np.inf)))
+
+
+def test_constrained_invariants(small, small_mat):
+    # ||x||_2 = 1, nnz = s for L0 (tie-free data), ||x||_1/||x||_2 = sqrt(s) for L1
+    A, X0, _ = small
+    for f in range(4):
+        st = S.OptimizationSettings(formulation=f, constraintParameter=10, tolerance=0.0, maximumIterations=15,
+                                    totalStartingPoints=64, batchSize=64)
+        _, X = small_mat.solve(st, X0=X0, want_X=True)
+        Xd = X.astype(np.float64)
+        np.testing.assert_allclose(np.linalg.norm(Xd, axis=0), 1.0, atol=2e-6)
+        if f < 2:
+            assert np.all((X != 0).sum(axis=0) == 10)
+        else:
+            np.testing.assert_allclose(np.abs(Xd).sum(0) / np.linalg.norm(Xd, axis=0), np.sqrt(10), rtol=2e-6)
+
+
+def test_planted_support_recovered(small, small_mat):
+    A, X0, planted = small
+    st = S.OptimizationSettings(formulation=0, constraintParameter=20, tolerance=1e-6, maximumIterations=100,
+                                totalStartingPoints=64, batchSize=64)
+    x = small_mat.solve(st, X0=X0)
+    o = O.solve(A.astype(np.float64), 0, s=20, tol=1e-6, max_iter=100, total_sp=64, X0=X0.astype(np.float64),
+                lambda_rule=O.LAMBDA_GPU)
+    assert set(np.nonzero(x)[0]) == set(np.nonzero(o["x"])[0])
+    assert len(set(np.nonzero(x)[0]) & set(planted)) >= 15
+
+
+def test_deterministic_reruns(small, small_mat):
+    A, X0, _ = small
+    for f in (0, 3, 5):
+        st = S.OptimizationSettings(formulation=f, constraintParameter=10, penaltyParameter=default_gamma(f, 200),
+                                    tolerance=0.0, maximumIterations=20, totalStartingPoints=64, batchSize=64,
+                                    precision=S.PREC_BF16X3)
+        a = small_mat.solve(st, X0=X0, want_X=True)[1]
+        b = small_mat.solve(st, X0=X0, want_X=True)[1]
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+@pytest.mark.parametrize("term", [S.TERM_BATCH_MAX, S.TERM_PER_SP])
+def test_batches_and_termination(small, small_mat, term):
+    # 100 SPs in batches of 32 -> 4 batches, 128 SPs (reference clamp); tol > 0
+    A, _, _ = small
+    n = A.shape[1]
+    X0 = unit_gaussian_starts(n, 128, seed=7)
+    st = S.OptimizationSettings(formulation=0, constraintParameter=10, tolerance=1e-4, maximumIterations=50,
+                                totalStartingPoints=100, batchSize=32, termination=term)
+    stats = S.OptimizationStatistics()
+    x = small_mat.solve(st, X0=X0, stats=stats)
+    o = O.solve(A.astype(np.float64), 0, s=10, tol=1e-4, max_iter=50, total_sp=100, batch=32,
+                X0=X0.astype(np.float64), termination=term, lambda_rule=O.LAMBDA_GPU)
+    assert len(stats.values) == 128 and o["total_sp"] == 128
+    same_it = np.mean(stats.iters == o["sp_iters"])
+    rel = np.abs(stats.values - o["sp_val"]) / o["sp_val"]
+    print(f"  iterations equal {same_it:.3f}, total it {stats.it}/{o['it']}, max rel {rel.max():.2e}")
+    assert same_it >= 0.95
+    assert np.median(rel) < 1e-6
+    assert abs(stats.fval - o["fval"]) <= 1e-5 * o["fval"]
+
+
+def _boxplot_rows():
+    rows = []
+    with open(os.path.join(GOLD, "ref_paper_experiment_boxplots.txt")) as fh:
+        for line in fh:
+            parts = line.strip().rstrip(",").split(",")
+            rows.append((int(parts[0]), np.array([float(v) for v in parts[1:]])))
+    return rows
+
+
+@pytest.fixture(scope="module")
+def boxplot_mat(ctx):
+    A = O.generate_problem(512, 2048, uniform_scale=True).astype(np.float32)
+    M = ctx.upload(A)
+    yield M
+    M.release()
+
+
+@pytest.mark.parametrize("row", range(12))
+def test_reference_boxplot_values(boxplot_mat, row):
+    # c++/results/paper_experiment_boxplots.txt: per-SP objective of 1000 reference SPs,
+    # L0_constrained_L2_PCA, 20 iterations, tol 1e-6, printed with 6 significant digits.
+    s, ref = _boxplot_rows()[row]
+    print(f"case boxplot s={s}", flush=True)
+    st = S.OptimizationSettings(formulation=0, constraintParameter=s, tolerance=1e-6, maximumIterations=20,
+                                totalStartingPoints=1000, batchSize=1000, start=S.START_REFERENCE_F64)
+    stats = S.OptimizationStatistics()
+    boxplot_mat.solve(st, stats=stats)
+    rel = np.abs(stats.values.astype(np.float64) - ref) / ref
+    exact = np.mean(["%g" % v == "%g" % r for v, r in zip(stats.values, ref)])
+    print(f"  s={s}: 6-digit matches {exact:.3f}, rel max {rel.max():.2e}, share <= 1e-5: {np.mean(rel <= 1e-5):.4f}")
+    assert np.mean(rel <= 1e-5) >= 0.99
+    assert abs(stats.fval - ref.max()) <= 1e-5 * ref.max()
+
+
+def _integer_rows():
+    rows = json.load(open(os.path.join(GOLD, "ref_cpu_integer_results.json")))["rows"]
+    return [r for r in rows if not r["otf"]]
+
+
+@pytest.mark.parametrize("idx", range(len(_integer_rows())))
+def test_reference_integer_results(ctx, idx):
+    # nnz of the returned x and the total iteration count of the reference's 1-thread runs
+    r = _integer_rows()[idx]
+    m, n = r["m"], r["n"]
+    print(f"case {r['source']}:{r['line']}", flush=True)
+    A = O.generate_problem(m, n, uniform_scale=False)
+    if r["ld_quirk"]:
+        ld = m if m % 16 == 0 else (m // 16 + 1) * 16
+        flat = np.zeros(ld * n)
+        flat[: m * n] = np.asfortranarray(A).reshape(-1, order="F")
+        A = flat.reshape((ld, n), order="F")[:m]
+    M = ctx.upload(np.asarray(A, dtype=np.float32))
+    try:
+        st = S.OptimizationSettings(formulation=r["formulation"], constraintParameter=r["s"],
+                                    penaltyParameter=r["gamma"], tolerance=r["tol"], maximumIterations=r["max_iter"],
+                                    totalStartingPoints=r["total_sp"], batchSize=r["batch"],
+                                    start=S.START_REFERENCE_F64)
+        stats = S.OptimizationStatistics()
+        x = M.solve(st, stats=stats)
+    finally:
+        M.release()
+    assert stats.it == r["it"]
+    nnz = int((x != 0).sum())
+    if nnz != r["nnz"]:
+        # allowed only where the best SP's last step had coordinates on the threshold:
+        # rerun that SP alone in FP64 and check every coordinate the supports disagree on
+        j = stats.best_sp
+        x0 = O.starting_point(n, r["s"], r["formulation"] < 4, j)[:, None]
+        A64 = np.asarray(A, dtype=np.float64)
+        o = O.solve(A64, r["formulation"], s=r["s"], gamma=r["gamma"], tol=0.0, max_iter=r["max_iter"], total_sp=1,
+                    X0=x0, want_X=True, lambda_rule=O.LAMBDA_GPU)
+        op = O.solve(A64, r["formulation"], s=r["s"], gamma=r["gamma"], tol=0.0, max_iter=r["max_iter"] - 1,
+                     total_sp=1, X0=x0, want_X=True, lambda_rule=O.LAMBDA_GPU)
+        assert int((o["x"] != 0).sum()) == r["nnz"]
+        diff = np.nonzero((x != 0) != (o["x"] != 0))[0]
+        z = A64 @ op["X"][:, 0]
+        z = np.sign(z) if r["formulation"] in (1, 3, 5, 7) else z / np.linalg.norm(z)
+        v = A64.T @ z
+        q = v * v if r["formulation"] in (4, 5) else np.abs(v)
+        margin = np.abs(q[diff] - r["gamma"]) / r["gamma"]
+        print(f"  nnz {nnz} vs {r['nnz']}: coordinates {diff} at relative distance {margin} from gamma")
+        assert len(diff) <= 2 and np.all(margin < 1e-4)
+
+
+def test_drop_in_host_entry(ctx, small):
+    # MulticoreSolver::denseDataSolver-shaped entry computes the same as spca_solve
+    import ctypes
+    A = np.asfortranarray(small[0])
+    m, n = A.shape
+    st = S.OptimizationSettings(formulation=1, constraintParameter=10, tolerance=0.0, maximumIterations=10,
+                                totalStartingPoints=64, batchSize=64, start=S.START_REFERENCE)
+    x1 = np.zeros(n, dtype=np.float32)
+    fv = ctypes.c_double()
+    rc = S.lib().spca_dense_solver_host(ctx.ptr, A.ctypes.data_as(ctypes.c_void_p), m,
+                                        x1.ctypes.data_as(ctypes.c_void_p), m, n, ctypes.byref(st._c()), None,
+                                        ctypes.byref(fv))
+    assert rc == S.SPCA_OK
+    M = ctx.upload(A)
+    stats = S.OptimizationStatistics()
+    x2 = M.solve(st, stats=stats)
+    M.release()
+    assert np.array_equal(x1, x2) and fv.value == stats.fval
+
+
+def _hip():
+    import ctypes
+    h = ctypes.CDLL("libamdhip64.so")
+    h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    h.hipFree.argtypes = [ctypes.c_void_p]
+    return h
+
+
+def test_drop_in_device_entry(ctx, small):
+    # GPUSolver::denseDataSolver-shaped entry: caller-owned device matrix with LD_M padding
+    import ctypes
+    A = small[0]
+    m, n = A.shape
+    ldm = 256
+    Ap = np.zeros((ldm, n), dtype=np.float32, order="F")
+    Ap[:m] = A
+    hip = _hip()
+    d = ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(d), Ap.nbytes) == 0
+    try:
+        assert hip.hipMemcpy(d, Ap.ctypes.data_as(ctypes.c_void_p), Ap.nbytes, 1) == 0  # host to device
+        st = S.OptimizationSettings(formulation=3, constraintParameter=10, tolerance=0.0, maximumIterations=10,
+                                    totalStartingPoints=64, batchSize=64, start=S.START_REFERENCE)
+        x1 = np.zeros(n, dtype=np.float32)
+        rc = S.lib().spca_dense_solver_device(ctx.ptr, m, n, d, x1.ctypes.data_as(ctypes.c_void_p),
+                                              ctypes.byref(st._c()), None, ldm, n)
+        assert rc == S.SPCA_OK, S.lib().spca_last_error(ctx.ptr)
+    finally:
+        hip.hipFree(d)
+    M = ctx.upload(A)
+    x2 = M.solve(st)
+    M.release()
+    assert np.array_equal(x1, x2)
+
+
+def test_errors_are_reported(ctx, small_mat, small):
+    X0 = small[1]
+    bad = S.OptimizationSettings(formulation=9, totalStartingPoints=64, batchSize=64)
+    with pytest.raises(S.SpcaError) as e:
+        small_mat.solve(bad, X0=X0)
+    assert e.value.code == S.SPCA_EINVAL
+    otf = S.OptimizationSettings(useOTF=True, totalStartingPoints=64, batchSize=64)
+    with pytest.raises(S.SpcaError):
+        small_mat.solve(otf, X0=X0)
+    # budget: a context with a 1 MiB budget cannot hold a 200 x 1000 matrix in two layouts
+    with S.Context(hbm_budget_bytes=1 << 20) as c2:
+        with pytest.raises(S.SpcaError) as e:
+            c2.upload(small[0])
+        assert e.value.code == S.SPCA_EBUDGET
