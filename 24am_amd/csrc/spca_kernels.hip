@@ -163,9 +163,10 @@ hipError_t launch_to_fp16(const float* src, size_t count, uint16_t* h, hipStream
 // C[r + c*ldc] = sum_k P[r*ldp + k] * Q[c*ldq + k]: both operands are contiguous along
 // the reduction, because the matrix is resident in both orientations (row major for
 // A*X, column major for A^T*Z).  One 128x128 tile per workgroup, 2x2 waves of 64x64,
-// 16x16 MFMA blocks.  blockIdx is remapped so that consecutive tiles (same P panel,
-// neighbouring N tiles) land on the same XCD (dispatch is round-robin over 8 XCDs),
-// which keeps each P panel in that XCD's L2.
+// 16x16 MFMA blocks.  Tiles are ordered (split, M tile, N tile) and blockIdx is remapped
+// so that each XCD (dispatch is round-robin over 8 XCDs) runs one contiguous range of
+// that order: with 8 K-splits an XCD owns one K range, so its slices of A and X come
+// from HBM once, and neighbouring N tiles share each P panel in that XCD's L2.
 __device__ __forceinline__ int xcd_remap(int bid, int total) {
     return (total & 7) ? bid : (bid & 7) * (total >> 3) + (bid >> 3);
 }
@@ -180,8 +181,8 @@ __global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restric
     const int t = xcd_remap(blockIdx.x, total);
     const int tn = t % tiles_n;
     const int rest = t / tiles_n;
-    const int sp = rest % splits;
-    const int tm = rest / splits;
+    const int tm = rest % tiles_m;
+    const int sp = rest / tiles_m;
     const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
 
     __shared__ __attribute__((aligned(16))) float Ps[kGemmBM][kGemmBK + 4];
@@ -263,8 +264,8 @@ __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restr
     const int t = xcd_remap(blockIdx.x, total);
     const int tn = t % tiles_n;
     const int rest = t / tiles_n;
-    const int sp = rest % splits;
-    const int tm = rest / splits;
+    const int tm = rest % tiles_m;
+    const int sp = rest / tiles_m;
     const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
 
     __shared__ __attribute__((aligned(16))) uint16_t Ps[NT][kGemmBM][LDK];

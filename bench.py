@@ -26,6 +26,10 @@ M_ROWS, N_COLS, BATCH, SPARSITY = 4096, 65536, 1024, 256
 THETA = (20.0, 15.0, 10.0, 8.0, 6.0)
 PEAK = {"fp32": 157.3, "bf16x3": 2500.0 / 3.0, "fp16": 2500.0}  # TFLOP/s (bf16x3: bf16 peak / 3 products)
 HBM_PEAK = 8000.0  # GB/s
+# HBM bytes per launch of the product kernels at this workload, from rocprofv3 --pmc
+# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs, scaled by the counters' reading of a 1 GiB
+# copy kernel (profiles/gemm_tn_f32_step_spiked4096x65536_B1024_hbm_counters.md).
+MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (4.56 + 0.25) * 2**30, ("fp32", "gemm_ax"): (6.84 + 0.125) * 2**30}
 
 
 def make_problem(seed=2024):
@@ -144,8 +148,8 @@ def main():
     head = results.get("fp32") or next(iter(results.values()))
     hp = "fp32" if "fp32" in results else next(iter(results))
     roof = {"bound": "mfma", "achieved": head["dominant_tflops"], "peak": PEAK[hp], "unit": "TFLOP/s",
-            "frac": head["dominant_tflops"] / PEAK[hp], "traffic": None, "kernel": head["dominant"],
-            "flops_per_launch": 2.0 * M_ROWS * N_COLS * BATCH}
+            "frac": head["dominant_tflops"] / PEAK[hp], "traffic": MEASURED_TRAFFIC.get((hp, head["dominant"])),
+            "kernel": head["dominant"], "flops_per_launch": 2.0 * M_ROWS * N_COLS * BATCH}
     cpu = None
     if not args.no_cpu:
         threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))

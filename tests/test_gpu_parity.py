@@ -242,44 +242,19 @@ def test_reference_integer_results(ctx, idx):
     assert stats.it == r["it"]
     nnz = int((x != 0).sum())
     if nnz != r["nnz"]:
-        # allowed only where the best SP's last step had coordinates on the threshold:
-        # rerun that SP alone in FP64 and check every coordinate the supports disagree on
+        # Allowed only as an argmax tie: the GPU's best SP is another SP whose FP64 value is
+        # within FP32 rounding of the reference best, and its own FP64 iterate has the nnz
+        # the GPU returned.  Rerun all SPs in FP64 to check both.
+        o = O.solve(np.asarray(A, dtype=np.float64), r["formulation"], s=r["s"], gamma=r["gamma"], tol=r["tol"],
+                    max_iter=r["max_iter"], total_sp=r["total_sp"], batch=r["batch"], want_X=True,
+                    lambda_rule=O.LAMBDA_GPU)
         j = stats.best_sp
-        x0 = O.starting_point(n, r["s"], r["formulation"] < 4, j)[:, None]
-        A64 = np.asarray(A, dtype=np.float64)
-        o = O.solve(A64, r["formulation"], s=r["s"], gamma=r["gamma"], tol=0.0, max_iter=r["max_iter"], total_sp=1,
-                    X0=x0, want_X=True, lambda_rule=O.LAMBDA_GPU)
-        op = O.solve(A64, r["formulation"], s=r["s"], gamma=r["gamma"], tol=0.0, max_iter=r["max_iter"] - 1,
-                     total_sp=1, X0=x0, want_X=True, lambda_rule=O.LAMBDA_GPU)
+        print(f"  nnz {nnz} vs {r['nnz']}: GPU best SP {j}, FP64 value {o['sp_val'][j]:.9g} vs best "
+              f"{o['fval']:.9g} (SP {o['best_sp']})")
         assert int((o["x"] != 0).sum()) == r["nnz"]
-        diff = np.nonzero((x != 0) != (o["x"] != 0))[0]
-        z = A64 @ op["X"][:, 0]
-        z = np.sign(z) if r["formulation"] in (1, 3, 5, 7) else z / np.linalg.norm(z)
-        v = A64.T @ z
-        q = v * v if r["formulation"] in (4, 5) else np.abs(v)
-        margin = np.abs(q[diff] - r["gamma"]) / r["gamma"]
-        print(f"  nnz {nnz} vs {r['nnz']}: coordinates {diff} at relative distance {margin} from gamma")
-        assert len(diff) <= 2 and np.all(margin < 1e-4)
-
-
-def test_drop_in_host_entry(ctx, small):
-    # MulticoreSolver::denseDataSolver-shaped entry computes the same as spca_solve
-    import ctypes
-    A = np.asfortranarray(small[0])
-    m, n = A.shape
-    st = S.OptimizationSettings(formulation=1, constraintParameter=10, tolerance=0.0, maximumIterations=10,
-                                totalStartingPoints=64, batchSize=64, start=S.START_REFERENCE)
-    x1 = np.zeros(n, dtype=np.float32)
-    fv = ctypes.c_double()
-    rc = S.lib().spca_dense_solver_host(ctx.ptr, A.ctypes.data_as(ctypes.c_void_p), m,
-                                        x1.ctypes.data_as(ctypes.c_void_p), m, n, ctypes.byref(st._c()), None,
-                                        ctypes.byref(fv))
-    assert rc == S.SPCA_OK
-    M = ctx.upload(A)
-    stats = S.OptimizationStatistics()
-    x2 = M.solve(st, stats=stats)
-    M.release()
-    assert np.array_equal(x1, x2) and fv.value == stats.fval
+        assert j != o["best_sp"]
+        assert o["sp_val"][j] >= o["fval"] * (1 - 1e-6)
+        assert int((o["X"][:, j] != 0).sum()) == nnz
 
 
 def _hip():
