@@ -311,10 +311,14 @@ void reference_starting_point(float* out, unsigned n, unsigned s, bool constrain
     for (unsigned i = 0; i < n; i++) out[i] = (float)V[i];
 }
 
-int choose_splits(int mp, int np, int bp) {
-    const int tiles = (mp / kGemmBM) * (bp / kGemmBN);
+// K-splits of A*X: enough workgroups to fill 256 CUs for a couple of rounds (128x128
+// tiles run ~3 per CU, 256x256 tiles one per CU), each split at least 512 deep.
+int choose_splits(int precision, int mp, int np, int bp) {
+    const bool big = gemm_big_tiles(precision, mp, bp);
+    const int tiles = big ? (mp / 256) * (bp / 256) : (mp / kGemmBM) * (bp / kGemmBN);
+    const int target = big ? 512 : 2048;
     int splits = 1;
-    while (tiles * splits < 2048 && np % (kGemmBK * splits * 2) == 0 && np / (splits * 2) >= 512) splits *= 2;
+    while (tiles * splits < target && np % (kGemmBK * splits * 2) == 0 && np / (splits * 2) >= 512) splits *= 2;
     return splits;
 }
 
@@ -368,7 +372,7 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
     if (st.constraint < 1) st.constraint = 1;
     const int batch = st.batch;
     const int bp = round_up(batch, kPadB);
-    const int splits = choose_splits(M->mp, M->np, bp);
+    const int splits = choose_splits(st.precision, M->mp, M->np, bp);
     if ((rc = ensure_workspace(ctx, M->np, M->mp, bp, splits))) return rc;
     Workspace& w = ctx->ws;
     hipStream_t s = ctx->stream;

@@ -62,7 +62,9 @@ hipError_t launch_split_bf16(const float* src, size_t count, uint16_t* hi, uint1
 hipError_t launch_to_fp16(const float* src, size_t count, uint16_t* h, hipStream_t st);
 
 // C[r + c*ldc + split*split_stride] = sum_{k in split} P[r*ldp + k] * Q[c*ldq + k]
-// M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.
+// M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.  16-bit products with M and N
+// multiples of 256 run the 256x256-tile kernel (gemm_big_tiles).
+int gemm_big_tiles(int precision, int M, int N);
 hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo,
                           long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq,
                           float* C, long ldc, long split_stride, int M, int N, int K, int splits,

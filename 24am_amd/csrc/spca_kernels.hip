@@ -356,11 +356,276 @@ __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restr
         }
 }
 
+// Large-tile variant of the 16-bit products: 256x256 output tile per workgroup, 8 waves
+// as 2 (M) x 4 (N), each wave 128 x 64 (8 x 4 MFMA blocks, 128 accumulators).  Each wave
+// keeps its B fragments for the whole stage and streams 8 A fragments past them, which
+// cuts LDS reads per MFMA by 1.5x against the 128x128 kernel and halves L2 traffic per
+// flop.  One 80 KiB LDS stage (bf16x3) with the next stage prefetched in registers.
+template <int PREC>
+__global__ __launch_bounds__(512, 1) void gemm_tn_16_big_kernel(const uint16_t* __restrict__ Phi,
+                                                                const uint16_t* __restrict__ Plo, long ldp,
+                                                                const uint16_t* __restrict__ Qhi,
+                                                                const uint16_t* __restrict__ Qlo, long ldq,
+                                                                float* __restrict__ C, long ldc, long split_stride,
+                                                                int tiles_m, int tiles_n, int splits, int ksplit,
+                                                                const int* __restrict__ done) {
+    if (done && *done) return;
+    constexpr int NP = (PREC == 1) ? 2 : 1;
+    constexpr int BM = 256, BN = 256, BK = kGemmBK, LDK = BK + 8;
+    const int total = tiles_m * tiles_n * splits;
+    const int t = xcd_remap(blockIdx.x, total);
+    const int tn = t % tiles_n;
+    const int rest = t / tiles_n;
+    const int tm = rest % tiles_m;
+    const int sp = rest / tiles_m;
+    const long m0 = (long)tm * BM, n0 = (long)tn * BN, k0 = (long)sp * ksplit;
+
+    __shared__ __attribute__((aligned(16))) uint16_t Ps[NP][BM][LDK];
+    __shared__ __attribute__((aligned(16))) uint16_t Qs[NP][BN][LDK];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 2, wn = wave & 3;
+    // a 256 x 32 tile of 16-bit values is 1024 chunks of 16 bytes: 2 per thread per part
+    const int lrow = tid >> 2, lc8 = (tid & 3) * 8;
+    const uint16_t* Pg[NP];
+    const uint16_t* Qg[NP];
+    Pg[0] = Phi + (m0 + lrow) * ldp + k0 + lc8;
+    Qg[0] = Qhi + (n0 + lrow) * ldq + k0 + lc8;
+    if constexpr (NP == 2) {
+        Pg[1] = Plo + (m0 + lrow) * ldp + k0 + lc8;
+        Qg[1] = Qlo + (n0 + lrow) * ldq + k0 + lc8;
+    }
+    u4 pr[NP][2], qr[NP][2];
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            pr[p][i] = *(const u4*)(Pg[p] + (long)(128 * i) * ldp);
+            qr[p][i] = *(const u4*)(Qg[p] + (long)(128 * i) * ldq);
+        }
+    f4 acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    const int kof = 8 * (lane >> 4);
+    for (int kt = 0; kt < ksplit; kt += BK) {
+        __syncthreads();
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                *(u4*)&Ps[p][lrow + 128 * i][lc8] = pr[p][i];
+                *(u4*)&Qs[p][lrow + 128 * i][lc8] = qr[p][i];
+            }
+        __syncthreads();
+        if (kt + BK < ksplit) {
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                Pg[p] += BK;
+                Qg[p] += BK;
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    pr[p][i] = *(const u4*)(Pg[p] + (long)(128 * i) * ldp);
+                    qr[p][i] = *(const u4*)(Qg[p] + (long)(128 * i) * ldq);
+                }
+            }
+        }
+        u4 b[NP][4];
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[p][j] = *(const u4*)&Qs[p][wn * 64 + j * 16 + (lane & 15)][kof];
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            u4 a[NP];
+#pragma unroll
+            for (int p = 0; p < NP; p++) a[p] = *(const u4*)&Ps[p][wm * 128 + i * 16 + (lane & 15)][kof];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if constexpr (PREC == 1) {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[1][j]), acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[1]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+                } else {
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        __builtin_bit_cast(h8, a[0]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
+                }
+            }
+        }
+    }
+    float* Cb = C + (long)sp * split_stride;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long row = m0 + wm * 128 + i * 16 + 4 * (lane >> 4);
+            const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+            *(f4*)(Cb + row + col * ldc) = acc[i][j];
+        }
+}
+
+// Pipelined large-tile variant: the same 256x256 tile and wave layout, but stages are
+// filled by direct global->LDS loads (global_load_lds_dwordx4: each lane's 16 bytes land at
+// the wave's LDS base + lane*16) into a ring of NSTAGE buffers, so NSTAGE-1 stages of loads
+// are in flight while one is computed and no registers stage the data.  LDS rows are 64
+// bytes unpadded; the 16-byte chunks of row r are XOR-swizzled by (r>>1)&3, so the 16 rows x
+// 4 chunks a fragment read touches fall in distinct banks.  One barrier per stage: after it,
+// every wave's loads of stage k have landed and every wave has finished reading stage k-1,
+// whose buffer the loads of stage k+NSTAGE-1 then reuse.
+__device__ __forceinline__ void wait_vmcnt(int n) {
+    // s_waitcnt with vmcnt = n (6 bits) and expcnt/lgkmcnt left at their maxima
+    switch (n) {
+        case 0: __builtin_amdgcn_s_waitcnt(0x0F70); break;
+        case 4: __builtin_amdgcn_s_waitcnt(0x0F74); break;
+        case 8: __builtin_amdgcn_s_waitcnt(0x0F78); break;
+        case 16: __builtin_amdgcn_s_waitcnt(0x4F70); break;
+        default: __builtin_amdgcn_s_waitcnt(0x0F70); break;
+    }
+}
+
+template <int PREC, int NSTAGE>
+__global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t* __restrict__ Phi,
+                                                                 const uint16_t* __restrict__ Plo, long ldp,
+                                                                 const uint16_t* __restrict__ Qhi,
+                                                                 const uint16_t* __restrict__ Qlo, long ldq,
+                                                                 float* __restrict__ C, long ldc, long split_stride,
+                                                                 int tiles_m, int tiles_n, int splits, int ksplit,
+                                                                 const int* __restrict__ done) {
+    if (done && *done) return;
+    constexpr int NP = (PREC == 1) ? 2 : 1;
+    constexpr int BM = 256, BN = 256, BK = kGemmBK;
+    constexpr int ROWB = BK * 2;          // bytes per LDS row
+    constexpr int OPB = BM * ROWB;        // bytes per operand part per stage (16 KiB)
+    constexpr int STB = 2 * NP * OPB;     // bytes per stage
+    constexpr int LPS = 2 * NP * 2;       // global_load_lds instructions per wave per stage
+    const int total = tiles_m * tiles_n * splits;
+    const int t = xcd_remap(blockIdx.x, total);
+    const int tn = t % tiles_n;
+    const int rest = t / tiles_n;
+    const int tm = rest % tiles_m;
+    const int sp = rest / tiles_m;
+    const long m0 = (long)tm * BM, n0 = (long)tn * BN, k0 = (long)sp * ksplit;
+
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[NSTAGE * STB];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 2, wn = wave & 3;
+    const int nk = ksplit / BK;
+
+    // loader: wave w fills rows [32w, 32w+32) of every operand part, 16 rows per instruction
+    const int lrow = lane >> 2;
+    const int lslot = lane & 3;
+    const uint16_t* src[2][NP];
+    src[0][0] = Phi;
+    src[1][0] = Qhi;
+    if constexpr (NP == 2) {
+        src[0][1] = Plo;
+        src[1][1] = Qlo;
+    }
+    long rowoff[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int row = 32 * wave + 16 * i + lrow;
+        const int g = lslot ^ ((row >> 1) & 3);
+        rowoff[0][i] = (m0 + row) * ldp + k0 + g * 8;
+        rowoff[1][i] = (n0 + row) * ldq + k0 + g * 8;
+    }
+    auto issue = [&](int ks, int buf) {
+        const long kk = (long)ks * BK;
+#pragma unroll
+        for (int op = 0; op < 2; op++)
+#pragma unroll
+            for (int p = 0; p < NP; p++)
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    uint8_t* dst = lds + buf * STB + (op * NP + p) * OPB + (32 * wave + 16 * i) * ROWB;
+                    __builtin_amdgcn_global_load_lds((const void*)(src[op][p] + rowoff[op][i] + kk),
+                                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+                }
+    };
+    f4 acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    const int fchunk = lane >> 4;  // logical 16-byte chunk a lane reads
+#pragma unroll
+    for (int s0 = 0; s0 < NSTAGE - 1; s0++)
+        if (s0 < nk) issue(s0, s0);
+    for (int k = 0; k < nk; k++) {
+        if (k + NSTAGE - 2 < nk) wait_vmcnt((NSTAGE - 2) * LPS);
+        else wait_vmcnt(0);
+        __syncthreads();
+        if (k + NSTAGE - 1 < nk) issue(k + NSTAGE - 1, (k + NSTAGE - 1) % NSTAGE);
+        const uint8_t* sb = lds + (k % NSTAGE) * STB;
+        u4 b[NP][4];
+#pragma unroll
+        for (int p = 0; p < NP; p++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int r = wn * 64 + j * 16 + (lane & 15);
+                b[p][j] = *(const u4*)(sb + (NP + p) * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
+            }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            u4 a[NP];
+            const int r = wm * 128 + i * 16 + (lane & 15);
+#pragma unroll
+            for (int p = 0; p < NP; p++) a[p] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
+            if constexpr (PREC == 1) {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[1][j]), acc[i][j], 0, 0, 0);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[1]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                        __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+            } else {
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                        __builtin_bit_cast(h8, a[0]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+    float* Cb = C + (long)sp * split_stride;
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long row = m0 + wm * 128 + i * 16 + 4 * (lane >> 4);
+            const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+            *(f4*)(Cb + row + col * ldc) = acc[i][j];
+        }
+}
+
+int gemm_big_tiles(int precision, int M, int N) { return precision != 0 && M % 256 == 0 && N % 256 == 0; }
+
 hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo, long ldp,
                           const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq, float* C,
                           long ldc, long split_stride, int M, int N, int K, int splits, const int* done,
                           hipStream_t st) {
     if (M % kGemmBM || N % kGemmBN || splits < 1 || K % (kGemmBK * splits)) return hipErrorInvalidValue;
+    if (gemm_big_tiles(precision, M, N)) {
+        const int tiles_m = M / 256, tiles_n = N / 256, ksplit = K / splits;
+        const int grid = tiles_m * tiles_n * splits;
+        if (precision == 1)
+            gemm_tn_16_pipe_kernel<1, 2><<<grid, 512, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride,
+                                                               tiles_m, tiles_n, splits, ksplit, done);
+        else
+            gemm_tn_16_pipe_kernel<2, 4><<<grid, 512, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride,
+                                                               tiles_m, tiles_n, splits, ksplit, done);
+        return hipGetLastError();
+    }
     const int tiles_m = M / kGemmBM, tiles_n = N / kGemmBN, ksplit = K / splits;
     const int grid = tiles_m * tiles_n * splits;
     if (precision == 0) {
@@ -441,13 +706,18 @@ hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int 
 constexpr int kBins = 2048;
 constexpr double kFix = 1099511627776.0;  // 2^40: n * 2^40 < 2^63 for n < 2^23
 
+constexpr int kCand = 2048;  // candidates of the answer bin resolved in LDS (else the radix fallback)
+
 struct SelShared {
     uint32_t cnt[kBins];
     uint64_t s1[kBins];
     uint64_t s2[kBins];
+    uint64_t cand[kCand];  // (|v| bits << 32) | index, sorted descending
     double red[kSelThreads / 64];
     int ires[4];
     uint64_t ures[2];
+    int ncand;
+    double dres[4];
 };
 
 __device__ __forceinline__ int lin_bin(float a, float scale) {
@@ -531,7 +801,13 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
     if (f < 4) {
         // ---- constrained: select, then x <- x/|x|_2
         float mloc = 0.f;
-        for (int q = 0; q < nq; q++) mloc = fmaxf(mloc, fabsf(LDV(q)));
+        for (int q0 = 0; q0 < nq; q0 += 8) {
+            float vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);  // past n the bounded view reads 0
+#pragma unroll
+            for (int u = 0; u < 8; u++) mloc = fmaxf(mloc, fabsf(vv[u]));
+        }
         const float M = block_max<kSelThreads>(mloc, sh.red);
         const bool l1c = (f == 2 || f == 3);
         const float scale = M > 0.f ? (float)kBins / M : 0.f;
@@ -543,14 +819,21 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
             sh.s2[b] = 0;
         }
         __syncthreads();
-        for (int q = 0; q < nq; q++) {
-            const float av = fabsf(LDV(q));
-            const int b = lin_bin(av, scale);
-            atomicAdd(&sh.cnt[b], 1u);
-            if (l1c && av > 0.f) {
-                const double u = (double)av * invM;
-                atomicAdd((unsigned long long*)&sh.s1[b], (unsigned long long)(u * kFix + 0.5));
-                atomicAdd((unsigned long long*)&sh.s2[b], (unsigned long long)(u * u * kFix + 0.5));
+        for (int q0 = 0; q0 < nq; q0 += 8) {
+            float vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (q0 + u >= nq) break;
+                const float av = fabsf(vv[u]);
+                const int b = lin_bin(av, scale);
+                atomicAdd(&sh.cnt[b], 1u);
+                if (l1c && av > 0.f) {
+                    const double w = (double)av * invM;
+                    atomicAdd((unsigned long long*)&sh.s1[b], (unsigned long long)(w * kFix + 0.5));
+                    atomicAdd((unsigned long long*)&sh.s2[b], (unsigned long long)(w * w * kFix + 0.5));
+                }
             }
         }
         __syncthreads();
@@ -562,14 +845,137 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
             });
         __syncthreads();
         const int bstar = sh.ires[0];
+        const uint32_t cA_hist = bstar >= 0 ? (uint32_t)sh.ires[1] : 0u;
+        const uint64_t s1A_fix = bstar >= 0 ? sh.ures[0] : 0ull, s2A_fix = bstar >= 0 ? sh.ures[1] : 0ull;
+        const uint32_t cnt_b = bstar >= 0 ? sh.cnt[bstar] : 0u;
+        double norm2 = 0;
+        bool fast = bstar >= 0 && cnt_b <= (uint32_t)kCand;
+        if (bstar < 0 && l1c) fast = true;  // no trigger: every element survives, sums over all
+        if (fast) {
+            // ---- compaction: the answer bin's elements into LDS; exact FP64 sums of the bins above
+            if (tid == 0) sh.ncand = 0;
+            __syncthreads();
+            double c_loc = 0, s1_loc = 0, s2_loc = 0;
+            for (int q0 = 0; q0 < nq; q0 += 8) {
+                float vv[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const int idx = tid + (q0 + u) * kSelThreads;
+                    if (idx >= a.n) continue;
+                    const float av = fabsf(vv[u]);
+                    const int b = lin_bin(av, scale);
+                    if (b > bstar) {
+                        c_loc += 1.0;
+                        s1_loc += (double)av;
+                        s2_loc += (double)av * (double)av;
+                    } else if (b == bstar) {
+                        const int slot = atomicAdd(&sh.ncand, 1);
+                        if (slot < kCand) sh.cand[slot] = ((uint64_t)__float_as_uint(av) << 32) | (uint32_t)idx;
+                    }
+                }
+            }
+            const double cAd = block_sum<kSelThreads>(c_loc, sh.red);
+            const double S1A = block_sum<kSelThreads>(s1_loc, sh.red);
+            const double S2A = block_sum<kSelThreads>(s2_loc, sh.red);
+            const int nc = min(sh.ncand, kCand);
+            // bitonic sort of the candidates, descending by (|v|, index): a fixed order
+            int npow = 1;
+            while (npow < nc) npow <<= 1;
+            for (int i = nc + tid; i < npow; i += kSelThreads) sh.cand[i] = 0ull;
+            __syncthreads();
+            for (int k = 2; k <= npow; k <<= 1)
+                for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                    for (int i = tid; i < npow; i += kSelThreads) {
+                        const int ixj = i ^ jj;
+                        if (ixj > i) {
+                            const uint64_t x0 = sh.cand[i], x1 = sh.cand[ixj];
+                            const bool desc = (i & k) == 0;
+                            if (desc ? (x0 < x1) : (x0 > x1)) {
+                                sh.cand[i] = x1;
+                                sh.cand[ixj] = x0;
+                            }
+                        }
+                    }
+                    __syncthreads();
+                }
+            if (!l1c) {
+                // ---- L0 constrained: tau = the (s - above)-th largest candidate (counting ties)
+                const uint32_t r = a.s - cA_hist;
+                tau = (r >= 1 && (int)r <= nc) ? __uint_as_float((uint32_t)(sh.cand[r - 1] >> 32)) : 0.f;
+                double cs = 0;
+                for (int i = tid; i < nc; i += kSelThreads) {
+                    const float av = __uint_as_float((uint32_t)(sh.cand[i] >> 32));
+                    if (!(av < tau)) cs += (double)av * (double)av;
+                }
+                norm2 = S2A + block_sum<kSelThreads>(cs, sh.red);
+            } else {
+                // ---- L1 constrained: the first candidate (descending) where the reference scan
+                // triggers, r(a) >= w over the elements above it; one wave walks 64 at a time
+                if (tid < 64) {
+                    const int lane = tid;
+                    double cc = cAd, c1 = S1A, c2 = S2A;  // carried sums of everything above the chunk
+                    int found = -1;
+                    double fc = 0, f1 = 0, f2 = 0;
+                    for (int c0 = 0; c0 < nc && found < 0; c0 += 64) {
+                        const int i = c0 + lane;
+                        const double av = i < nc ? (double)__uint_as_float((uint32_t)(sh.cand[i] >> 32)) : 0.0;
+                        double p0 = i < nc ? 1.0 : 0.0, p1 = av, p2 = av * av;
+                        // inclusive prefix over lanes 0..lane, fixed shuffle order
+                        for (int o = 1; o < 64; o <<= 1) {
+                            const double t0 = __shfl_up(p0, o, 64), t1 = __shfl_up(p1, o, 64), t2 = __shfl_up(p2, o, 64);
+                            if (lane >= o) { p0 += t0; p1 += t1; p2 += t2; }
+                        }
+                        // sums over the elements before candidate i
+                        const double e0 = cc + p0 - (i < nc ? 1.0 : 0.0), e1 = c1 + p1 - av, e2 = c2 + p2 - av * av;
+                        const bool trig = i < nc && pred_r(e0 + 1.0, e1 + av, e2 + av * av, av, wd);
+                        const uint64_t bal = __ballot(trig);
+                        if (bal) {
+                            const int first = __ffsll((long long)bal) - 1;
+                            found = c0 + first;
+                            fc = __shfl(e0, first, 64);
+                            f1 = __shfl(e1, first, 64);
+                            f2 = __shfl(e2, first, 64);
+                        } else {
+                            cc += __shfl(p0, 63, 64);
+                            c1 += __shfl(p1, 63, 64);
+                            c2 += __shfl(p2, 63, 64);
+                        }
+                    }
+                    if (lane == 0) {
+                        if (found < 0) { fc = cc; f1 = c1; f2 = c2; }
+                        sh.dres[0] = fc;
+                        sh.dres[1] = f1;
+                        sh.dres[2] = f2;
+                    }
+                }
+                __syncthreads();
+                const double c = sh.dres[0], S1 = sh.dres[1], S2 = sh.dres[2];
+                const double w2 = wd * wd;
+                const double qa = c * (c - w2);
+                const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
+                const double qc = S1 * S1 - w2 * S2;
+                lam = (float)((-qb - sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa));
+                if (bstar >= 0) {
+                    // every element of the survivor set lies above lam: sum of (a - lam)^2 in closed form
+                    const double lf = (double)lam;
+                    norm2 = S2 - 2.0 * lf * S1 + c * lf * lf;
+                } else {
+                    double acc = 0;
+                    for (int q = 0; q < nq; q++) {
+                        const float t2 = fabsf(LDV(q)) - lam;
+                        if (t2 > 0.f) acc += (double)t2 * (double)t2;
+                    }
+                    norm2 = block_sum<kSelThreads>(acc, sh.red);
+                }
+            }
+        } else {
+        // ---- fallback (answer bin larger than the LDS list): radix passes over the column
         uint32_t prefix = 0;  // key of the answer within bin bstar
-        uint32_t cA = 0, rank = a.s;
-        uint64_t s1A = 0, s2A = 0;
+        uint32_t cA = cA_hist, rank = a.s - cA_hist;
+        uint64_t s1A = s1A_fix, s2A = s2A_fix;
         if (bstar >= 0) {
-            cA = (uint32_t)sh.ires[1];
-            s1A = sh.ures[0];
-            s2A = sh.ures[1];
-            rank = a.s - cA;
             const double lam_b = (double)bstar * Md / (double)kBins;
             for (int pass = 0; pass < 4; pass++) {
                 const int shift = 24 - 8 * pass;
@@ -614,29 +1020,19 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
             }
         }
         __syncthreads();
-        double norm2 = 0;
         if (!l1c) {
-            // ---- L0 constrained: tau = s-th largest |v| (counting ties), keep |v| >= tau
-            // (my_cublas_wrapper.h:240-255,623-626)
             tau = bstar >= 0 ? __uint_as_float(prefix) : 0.f;
             for (int q = 0; q < nq; q++) {
                 const float v = LDV(q);
                 if (!(fabsf(v) < tau)) norm2 += (double)v * (double)v;
             }
         } else {
-            // ---- L1 constrained: survivors are the elements above the answer key, or all
-            // n elements when no element triggers the scan
             double c = 0, S1 = 0, S2 = 0;
             for (int q = 0; q < nq; q++) {
                 const float av = fabsf(LDV(q));
-                bool in;
-                if (bstar < 0) {
-                    in = (tid + q * kSelThreads) < a.n;
-                } else {
-                    const int b = lin_bin(av, scale);
-                    in = b > bstar || (b == bstar && __float_as_uint(av) > prefix);
-                }
-                if (in) {
+                const int b = lin_bin(av, scale);
+                const bool in = b > bstar || (b == bstar && __float_as_uint(av) > prefix);
+                if (in && (tid + q * kSelThreads) < a.n) {
                     c += 1.0;
                     S1 += (double)av;
                     S2 += (double)av * (double)av;
@@ -656,6 +1052,7 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
             }
         }
         norm2 = block_sum<kSelThreads>(norm2, sh.red);
+        }
         const float nf = (float)sqrt(norm2);
         scale_out = 1.0f / nf;
         fval = sp.fz[j];
@@ -663,20 +1060,23 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
         // ---- penalised (my_cublas_wrapper.h:170-238,636-710), gamma in the iterate's precision
         const float g = (float)a.gamma;
         double acc = 0;
-        if (f == 4 || f == 5) {
-            for (int q = 0; q < nq; q++) {
-                const float v = LDV(q);
-                const float t = v * v - g;
-                if (t > 0.f) acc += (double)t;
+        const bool l0p = (f == 4 || f == 5);
+        for (int q0 = 0; q0 < nq; q0 += 8) {
+            float vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);  // zeros past n contribute nothing
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (l0p) {
+                    const float t = vv[u] * vv[u] - g;
+                    if (t > 0.f) acc += (double)t;
+                } else {
+                    const float t = fabsf(vv[u]) - g;
+                    if (t > 0.f) acc += (double)t * (double)t;
+                }
             }
-            fval = (float)block_sum<kSelThreads>(acc, sh.red);
-        } else {
-            for (int q = 0; q < nq; q++) {
-                const float t = fabsf(LDV(q)) - g;
-                if (t > 0.f) acc += (double)t * (double)t;
-            }
-            fval = (float)sqrt(block_sum<kSelThreads>(acc, sh.red));
         }
+        fval = l0p ? (float)block_sum<kSelThreads>(acc, sh.red) : (float)sqrt(block_sum<kSelThreads>(acc, sh.red));
         lam = g;
     }
     // ---- write the iterate (+ operand forms), count nonzeros, update the SP record
@@ -686,25 +1086,33 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
         const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + base, (uint32_t)a.n * 4u);
         const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)(xop.hi + base) : (const void*)xop.f32, a.precision ? (uint32_t)a.n * 2u : 0u);
         const __amdgpu_buffer_rsrc_t lr = col_rsrc(xop.lo ? (const void*)(xop.lo + base) : (const void*)xop.f32, a.precision == 1 ? (uint32_t)a.n * 2u : 0u);
-        for (int q = 0; q < nq; q++) {
-            const float v = LDV(q);
-            float x;
-            if (f == 0 || f == 1) x = fabsf(v) < tau ? 0.f : v;
-            else if (f == 2 || f == 3 || f == 6 || f == 7) {
-                const float t2 = fabsf(v) - lam;
-                x = t2 > 0.f ? t2 * sgnf(v) : 0.f;
-            } else {
-                x = (v * v - lam) <= 0.f ? 0.f : v;  // L0 penalised: lam holds gamma
-            }
-            if (f < 4) x = x * scale_out;
-            __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, tid * 4, q * kSelThreads * 4, 0);
-            nz += (x != 0.f && (tid + q * kSelThreads) < a.n) ? 1.0 : 0.0;
-            if (a.precision == 1) {
-                const uint16_t h = f2bf(x);
-                __builtin_amdgcn_raw_buffer_store_b16(h, hr, tid * 2, q * kSelThreads * 2, 0);
-                __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, tid * 2, q * kSelThreads * 2, 0);
-            } else if (a.precision == 2) {
-                __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, tid * 2, q * kSelThreads * 2, 0);
+        for (int q0 = 0; q0 < nq; q0 += 8) {
+            float vv[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int q = q0 + u;
+                if (q >= nq) break;
+                const float v = vv[u];
+                float x;
+                if (f == 0 || f == 1) x = fabsf(v) < tau ? 0.f : v;
+                else if (f == 2 || f == 3 || f == 6 || f == 7) {
+                    const float t2 = fabsf(v) - lam;
+                    x = t2 > 0.f ? t2 * sgnf(v) : 0.f;
+                } else {
+                    x = (v * v - lam) <= 0.f ? 0.f : v;  // L0 penalised: lam holds gamma
+                }
+                if (f < 4) x = x * scale_out;
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, tid * 4, q * kSelThreads * 4, 0);
+                nz += (x != 0.f && (tid + q * kSelThreads) < a.n) ? 1.0 : 0.0;
+                if (a.precision == 1) {
+                    const uint16_t h = f2bf(x);
+                    __builtin_amdgcn_raw_buffer_store_b16(h, hr, tid * 2, q * kSelThreads * 2, 0);
+                    __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, tid * 2, q * kSelThreads * 2, 0);
+                } else if (a.precision == 2) {
+                    __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, tid * 2, q * kSelThreads * 2, 0);
+                }
             }
         }
     }

@@ -90,8 +90,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; the batched solve shards over starting points "
-                                   "across GPUs only by design (see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this build measures one GPU; sharding the starting points over several GPUs is a "
+                                   "design, not built, so --gpus must be 1"}))
         sys.exit(2)
     import spca_mi355x as S
 
