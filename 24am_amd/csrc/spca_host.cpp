@@ -359,7 +359,6 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
     if (!valid_formulation(st.formulation)) return fail(ctx, SPCA_EINVAL, "formulation must be 0..7");
     if (st.precision < 0 || st.precision > 2) return fail(ctx, SPCA_EINVAL, "precision must be 0..2");
     if (st.max_iter < 1 || st.total_sp < 1 || st.batch < 0) return fail(ctx, SPCA_EINVAL, "bad iteration/SP counts");
-    if (st.use_otf) return fail(ctx, SPCA_EINVAL, "on-the-fly refill is not supported by this build");
     if (st.start < SPCA_START_EXPLICIT || st.start > SPCA_START_REFERENCE_F64)
         return fail(ctx, SPCA_EINVAL, "start must be a spca_start value");
     if (st.start == SPCA_START_EXPLICIT && !X0) return fail(ctx, SPCA_EINVAL, "X0 is required for explicit starts");
@@ -412,6 +411,152 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
     std::vector<float> x_best(n, 0.f);
     hipEvent_t* loop_ev = ctx->loop_ev;
 
+    if (st.use_otf) {
+        // ---- on-the-fly refill (reference OTF driver, sparse_PCA_solver.h:83-169): every SP is
+        // resident in a device pool; finished slots take the next SP on the device.
+        const int total = st.total_sp;
+        float* pool = nullptr;
+        float* xbest = nullptr;
+        int* ibuf = nullptr;
+        float* fbuf = nullptr;
+        uint64_t ob = 0;
+        auto release = [&]() {
+            (void)hipStreamSynchronize(s);
+            dev_free(ctx, pool, (uint64_t)sizeof(float) * w.np * total);
+            dev_free(ctx, xbest, (uint64_t)sizeof(float) * w.np);
+            dev_free(ctx, ibuf, (uint64_t)sizeof(int) * (3 * (size_t)w.bp + 3 + 2 * (size_t)total));
+            dev_free(ctx, fbuf, (uint64_t)sizeof(float) * (1 + (size_t)total));
+        };
+        if ((rc = dev_alloc(ctx, &pool, (size_t)w.np * total, &ob)) || (rc = dev_alloc(ctx, &xbest, (size_t)w.np, &ob)) ||
+            (rc = dev_alloc(ctx, &ibuf, 3 * (size_t)w.bp + 3 + 2 * (size_t)total, &ob)) ||
+            (rc = dev_alloc(ctx, &fbuf, 1 + (size_t)total, &ob))) {
+            release();
+            return rc;
+        }
+        OtfState o;
+        o.order = ibuf;
+        o.cur_it = ibuf + w.bp;
+        o.refill = ibuf + 2 * w.bp;
+        o.generated = ibuf + 3 * w.bp;
+        o.sp_iters = ibuf + 3 * w.bp + 3;
+        o.sp_nnz = o.sp_iters + total;
+        o.best_val = fbuf;
+        o.sp_val = fbuf + 1;
+        o.pool = pool;
+        o.xbest = xbest;
+        // pool of all starting points, padded rows zero
+        {
+            std::vector<float> hp((size_t)n * total);
+            for (int k = 0; k < total; k++) {
+                float* col = hp.data() + (size_t)k * n;
+                if (st.start == SPCA_START_EXPLICIT)
+                    std::memcpy(col, X0 + (size_t)k * n, sizeof(float) * n);
+                else if (st.start == SPCA_START_REFERENCE_F64)
+                    reference_starting_point<double>(col, (unsigned)n, st.constraint, constrained, (unsigned)k);
+                else
+                    reference_starting_point<float>(col, (unsigned)n, st.constraint, constrained, (unsigned)k);
+            }
+            HIP_TRY(ctx, hipMemsetAsync(pool, 0, sizeof(float) * (size_t)w.np * total, s));
+            HIP_TRY(ctx, hipMemcpy2DAsync(pool, sizeof(float) * w.np, hp.data(), sizeof(float) * n, sizeof(float) * n,
+                                          total, hipMemcpyHostToDevice, s));
+            HIP_TRY(ctx, hipMemsetAsync(w.X, 0, sizeof(float) * (size_t)w.np * w.bp, s));
+            HIP_TRY(ctx, hipMemcpyAsync(w.X, pool, sizeof(float) * (size_t)w.np * batch, hipMemcpyDeviceToDevice, s));
+            if (st.precision) {
+                HIP_TRY(ctx, hipMemsetAsync(w.Xhi, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
+                HIP_TRY(ctx, hipMemsetAsync(w.Xlo, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
+                HIP_TRY(ctx, launch_x_operand(xop, st.precision, w.np, batch, s));
+            }
+            std::vector<int> hi(3 * (size_t)w.bp + 3 + 2 * (size_t)total, 0);
+            for (int i = 0; i < batch; i++) hi[i] = i;  // order
+            hi[3 * w.bp] = batch;                       // generated
+            hi[3 * w.bp + 1] = -1;
+            hi[3 * w.bp + 2] = -1;
+            std::vector<float> hf(1 + (size_t)total, 0.f);
+            hf[0] = -1.f;  // the_best_solution_value starts at -1
+            HIP_TRY(ctx, hipMemcpyAsync(ibuf, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice, s));
+            HIP_TRY(ctx, hipMemcpyAsync(fbuf, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, s));
+            HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
+            HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 4), s));
+            std::vector<int> act(w.bp, 0);
+            std::fill(act.begin(), act.begin() + batch, 1);
+            HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
+            HIP_TRY(ctx, hipStreamSynchronize(s));
+        }
+        // every SP needs at most max_iter iterations and every loop iteration advances all slots
+        const long cap = (long)total * st.max_iter / batch + st.max_iter + 1;
+        HIP_TRY(ctx, hipEventRecord(loop_ev[0], s));
+        for (long it = 0; it < cap; it++) {
+            hipEvent_t e0, e1;
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
+            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, M->np, w.X, w.Xhi, w.Xlo, M->np, w.Zpart,
+                                        M->mp, zsplit, M->mp, bp, M->np, splits, sp.done, s));
+            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
+            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, st.precision, batch,
+                                       sp, s));
+            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
+            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, M->mp, w.Z, w.Zhi, w.Zlo, M->mp, w.V,
+                                        M->np, 0, M->np, bp, M->mp, 1, sp.done, s));
+            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
+            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
+            HIP_TRY(ctx, launch_otf_converge(sp, o, batch, st.tolerance, st.max_iter, total, s));
+            HIP_TRY(ctx, launch_otf_refill(sp, o, xop, st.precision, n, w.np, batch, s));
+            HIP_TRY(ctx, hipEventRecord(e1, s));
+            if ((it + 1) % check_every == 0 || it + 1 == cap) {
+                HIP_TRY(ctx, hipMemcpyAsync(ctx->h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+                HIP_TRY(ctx, hipStreamSynchronize(s));
+                pt.flush();
+                if (ctx->h_done[0]) break;
+            }
+        }
+        HIP_TRY(ctx, hipEventRecord(loop_ev[1], s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        pt.flush();
+        {
+            float t = 0;
+            HIP_TRY(ctx, hipEventElapsedTime(&t, loop_ev[0], loop_ev[1]));
+            loop_ms += t;
+        }
+        int done_h[2], gen_h[3];
+        float bestv;
+        std::vector<float> spv(total);
+        std::vector<int> spi(total), spn(total);
+        HIP_TRY(ctx, hipMemcpyAsync(done_h, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(gen_h, o.generated, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(&bestv, o.best_val, sizeof(float), hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(spv.data(), o.sp_val, sizeof(float) * total, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(spi.data(), o.sp_iters, sizeof(int) * total, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(spn.data(), o.sp_nnz, sizeof(int) * total, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(x_best.data(), xbest, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        double evb = 0;
+        HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, m, n, xbest, w.np, 1, spca_is_l1_variance(st.formulation), w.ev, s));
+        HIP_TRY(ctx, hipMemcpyAsync(&evb, w.ev, sizeof(double), hipMemcpyDeviceToHost, s));
+        float emax;
+        HIP_TRY(ctx, hipMemcpyAsync(&emax, sp.errmax, sizeof(float), hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipStreamSynchronize(s));
+        release();
+        total_it = (unsigned)done_h[1];
+        best = bestv;
+        best_sp = gen_h[2];
+        last_err = emax;
+        for (int k = 0; k < total; k++) sp_its += (unsigned long long)spi[k];
+        if (per_sp)
+            for (int k = 0; k < total; k++) {
+                per_sp[k].value = spv[k];
+                per_sp[k].iters = spi[k];
+                per_sp[k].nnz = spn[k];
+                per_sp[k].converged = spi[k] < st.max_iter;
+                per_sp[k].ev_fp64 = (k == best_sp) ? evb : std::nan("");
+            }
+        if (stats) stats->ev_fp64 = evb;
+        (void)X_final;  // slots are reused: final iterates per SP are not kept in OTF mode
+    } else
     for (int bt = 0; bt < total_batches; bt++) {
         const int shift = bt * batch;
         // ---- starting points
@@ -554,6 +699,73 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
     return SPCA_OK;
 }
 
+// FP64 re-solve on the support of x (host vector of n floats)
+int resolve_impl(spca_ctx ctx, Matrix* M, int formulation, const float* x, int max_iter, double tol, double* y_out,
+                 double* ev_out, int* iters_out) {
+    if (!valid_formulation(formulation) || !x || max_iter < 1) return fail(ctx, SPCA_EINVAL, "bad re-solve arguments");
+    const int n = M->n, m = M->m;
+    std::vector<int> idx;
+    double nx = 0;
+    for (int i = 0; i < n; i++)
+        if (x[i] != 0.f) {
+            idx.push_back(i);
+            nx += (double)x[i] * (double)x[i];
+        }
+    const int k = (int)idx.size();
+    if (k == 0 || !(nx > 0) || !std::isfinite(nx)) return fail(ctx, SPCA_EINVAL, "re-solve needs a finite nonzero x");
+    std::vector<double> y0(k);
+    for (int b = 0; b < k; b++) y0[b] = (double)x[idx[b]] / std::sqrt(nx);
+    const int nb = (m + 255) / 256;
+    double* dbuf = nullptr;
+    int* ibuf = nullptr;
+    uint64_t bytes = 0;
+    const size_t nd = (size_t)m * k + k + m + k + nb + 1;
+    int rc;
+    if ((rc = dev_alloc(ctx, &dbuf, nd, &bytes)) || (rc = dev_alloc(ctx, &ibuf, (size_t)k + 1, &bytes))) {
+        dev_free(ctx, dbuf, sizeof(double) * nd);
+        return rc;
+    }
+    double* As = dbuf;
+    double* y = As + (size_t)m * k;
+    double* z = y + k;
+    double* yp = z + m;
+    double* part = yp + k;
+    double* ev = part + nb;
+    int* didx = ibuf;
+    int* flag = ibuf + k;
+    hipStream_t s = ctx->stream;
+    auto done = [&](int code) {
+        (void)hipStreamSynchronize(s);
+        dev_free(ctx, dbuf, sizeof(double) * nd);
+        dev_free(ctx, ibuf, sizeof(int) * ((size_t)k + 1));
+        return code;
+    };
+    const int zero = 0;
+    const double dzero = 0;
+    if (hipMemcpyAsync(didx, idx.data(), sizeof(int) * k, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(y, y0.data(), sizeof(double) * k, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(flag, &zero, sizeof(int), hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(ev, &dzero, sizeof(double), hipMemcpyHostToDevice, s) != hipSuccess ||
+        launch_resolve(M->a32_cm, M->mp, m, didx, k, spca_is_l1_variance(formulation), As, y, z, yp, part, ev, flag,
+                       tol, max_iter, s) != hipSuccess)
+        return done(fail(ctx, SPCA_EDEVICE, "re-solve launch failed"));
+    std::vector<double> yh(k);
+    double evh = 0;
+    int fl = 0;
+    if (hipMemcpyAsync(yh.data(), y, sizeof(double) * k, hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&evh, ev, sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipMemcpyAsync(&fl, flag, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+        return done(fail(ctx, SPCA_EDEVICE, "re-solve copy failed"));
+    if (y_out) {
+        std::fill(y_out, y_out + n, 0.0);
+        for (int b = 0; b < k; b++) y_out[idx[b]] = yh[b];
+    }
+    if (ev_out) *ev_out = evh;
+    if (iters_out) *iters_out = fl ? -1 : max_iter;  // -1: converged before max_iter
+    return done(SPCA_OK);
+}
+
 }  // namespace
 
 extern "C" {
@@ -575,6 +787,8 @@ void spca_settings_default(spca_settings* st) {
     st->precision = SPCA_PREC_FP32;
     st->start = SPCA_START_EXPLICIT;
     st->check_every = 1;
+    st->resolve_iters = 0;
+    st->resolve_tol = 1e-12;
 }
 
 int spca_settings_check(spca_settings* st, unsigned n) {
@@ -692,7 +906,24 @@ int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, con
     if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
     (void)hipSetDevice(ctx->device);
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    return solve_impl(ctx, itm->second, settings, X0, x_out, stats, per_sp, X_final);
+    int rc = solve_impl(ctx, itm->second, settings, X0, x_out, stats, per_sp, X_final);
+    if (rc == SPCA_OK && settings->resolve_iters > 0 && stats) {
+        int its = 0;
+        rc = resolve_impl(ctx, itm->second, settings->formulation, x_out, settings->resolve_iters,
+                          settings->resolve_tol, nullptr, &stats->ev_resolve_fp64, &its);
+        stats->resolve_iterations = its;
+    }
+    return rc;
+}
+
+int spca_resolve_support(spca_ctx ctx, spca_matrix mat, int formulation, const float* x, int max_iter, double tol,
+                         double* y_out, double* ev_out, int* iters_out) {
+    if (!ctx) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto itm = ctx->mats.find(mat);
+    if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+    (void)hipSetDevice(ctx->device);
+    return resolve_impl(ctx, itm->second, formulation, x, max_iter, tol, y_out, ev_out, iters_out);
 }
 
 int spca_dense_solver_device(spca_ctx ctx, unsigned m, unsigned n, const float* d_B, float* h_x,

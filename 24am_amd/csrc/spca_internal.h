@@ -43,6 +43,21 @@ struct Operand {
     uint16_t* lo;        // bf16 lo part (bf16x3 only)
 };
 
+// On-the-fly (OTF) slot refill state (reference OTF driver, c++/src/gpower/sparse_PCA_solver.h:83-169).
+struct OtfState {
+    int* order;        // global SP index held by each slot
+    int* cur_it;       // iterations of the slot's current SP
+    int* refill;       // 1: the slot takes a new SP before the next iteration
+    int* generated;    // [0]: SPs handed out so far; [1]: slot whose column becomes the best this
+                       // iteration (-1 none); [2]: global index of the best SP so far
+    float* best_val;   // [0]: best value so far
+    float* sp_val;     // per global SP: value at its last termination
+    int* sp_iters;     // per global SP: iterations at its last termination
+    int* sp_nnz;       // per global SP: nonzeros at its last termination
+    const float* pool; // all starting points, ld = padded n
+    float* xbest;      // iterate of the best SP (padded n)
+};
+
 struct StepArgs {
     int formulation;
     int precision;       // spca_precision
@@ -83,11 +98,26 @@ hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipS
 hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_iter, int per_sp,
                            hipStream_t st);
 
+// OTF: per-slot termination, best-so-far, assignment of new SPs to finished slots in slot
+// order; then the refill kernel saves the best column and loads the new SPs.
+hipError_t launch_otf_converge(SpState sp, OtfState o, int batch, double tol, int max_iter, int total_sp,
+                               hipStream_t st);
+hipError_t launch_otf_refill(SpState sp, OtfState o, Operand xop, int precision, int n, int ldx, int batch,
+                             hipStream_t st);
+
 // Write the operand forms of X from its FP32 values (for starting points).
 hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st);
 
 // FP64 explained variance of each normalised column of X: |A x|_2^2 or |A x|_1.
 hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* X, int ldx, int batch,
                           int l1, double* ev, hipStream_t st);
+
+// FP64 re-solve on a support (indices idx[0..k)): max_iter iterations of z = map(A_S y),
+// y = A_S^T z / |A_S^T z|, stopping (flag) when the objective changes by <= tol relative.
+// ev[0] receives the objective of the final y.  Buffers: As m*k, y k, z m, yp k,
+// part ceil(m/256), ev 1, flag 1 (all device; y holds the unit start, flag zeroed).
+hipError_t launch_resolve(const float* Acm, int lda, int m, const int* idx, int k, int l1, double* As, double* y,
+                          double* z, double* yp, double* part, double* ev, int* flag, double tol, int max_iter,
+                          hipStream_t st);
 
 }  // namespace spca

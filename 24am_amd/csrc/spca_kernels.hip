@@ -570,12 +570,19 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                 const int r = wn * 64 + j * 16 + (lane & 15);
                 b[p][j] = *(const u4*)(sb + (NP + p) * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
             }
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            u4 a[NP];
+        // A fragments double-buffered: the reads for block i+1 are in flight while the MFMAs of
+        // block i issue
+        u4 abuf[2][NP];
+        auto load_a = [&](int i, u4* dst) {
             const int r = wm * 128 + i * 16 + (lane & 15);
 #pragma unroll
-            for (int p = 0; p < NP; p++) a[p] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
+            for (int p = 0; p < NP; p++) dst[p] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
+        };
+        load_a(0, abuf[0]);
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            if (i + 1 < 8) load_a(i + 1, abuf[(i + 1) & 1]);
+            const u4* a = abuf[i & 1];
             if constexpr (PREC == 1) {
 #pragma unroll
                 for (int j = 0; j < 4; j++)
@@ -595,6 +602,9 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
                         __builtin_bit_cast(h8, a[0]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
             }
+            // schedule: this block's reads of the next A fragments first, then its MFMAs
+            if (i + 1 < 8) __builtin_amdgcn_sched_group_barrier(0x100, NP, 0);
+            __builtin_amdgcn_sched_group_barrier(0x008, 4 * (PREC == 1 ? 3 : 1), 0);
         }
     }
     float* Cb = C + (long)sp * split_stride;
@@ -1181,6 +1191,132 @@ hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_it
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- OTF slot refill
+// Reference OTF loop body (c++/src/gpower/sparse_PCA_solver.h:120-163): every slot counts
+// an iteration; a slot terminates when err < tol (its count is >= 1, so the reference's
+// "it > 0" holds) or at max_iter; terminated slots update the best with a strict '>' in
+// slot order and take the next SP while any remain.  Slots with nothing left keep
+// iterating, as in the reference.  One 1024-thread workgroup; slots in chunks of 1024
+// with a carried prefix count, so the assignment order is the slot order.
+__global__ __launch_bounds__(1024) void otf_converge_kernel(SpState sp, OtfState o, int batch, double tol,
+                                                            int max_iter, int total_sp) {
+    if (*sp.done) return;
+    __shared__ int wsum[16];
+    __shared__ float wbest[16];
+    __shared__ int wbesti[16];
+    __shared__ int wbestg[16];
+    __shared__ int s_bglob;
+    __shared__ int s_carry, s_any;
+    __shared__ float s_best;
+    __shared__ int s_bslot;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) {
+        s_carry = 0;
+        s_any = 0;
+        s_best = o.best_val[0];
+        s_bslot = -1;
+        s_bglob = -1;
+    }
+    __syncthreads();
+    const int gen0 = o.generated[0];
+    for (int c0 = 0; c0 < batch; c0 += 1024) {
+        const int i = c0 + tid;
+        int term = 0;
+        float v = 0.f;
+        int g = -1;
+        if (i < batch) {
+            g = o.order[i];
+            const int cur = o.cur_it[i] + 1;
+            o.cur_it[i] = cur;
+            const float e = sp.err[i];
+            term = ((double)e < tol) || cur >= max_iter;
+            v = sp.val[i];
+            if (term) {
+                o.sp_val[g] = v;
+                o.sp_iters[g] = cur;
+                o.sp_nnz[g] = sp.nnz[i];
+            } else {
+                s_any = 1;  // benign: every writer stores 1
+            }
+        }
+        // best among the terminated slots of this chunk: largest value, first slot on ties
+        float bv = (term && v == v) ? v : -INFINITY;
+        int bi = (term && v == v) ? i : 0x7fffffff;
+        int bg = g;
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ov = __shfl_down(bv, off, 64);
+            const int oi = __shfl_down(bi, off, 64);
+            const int og = __shfl_down(bg, off, 64);
+            if (ov > bv || (ov == bv && oi < bi)) { bv = ov; bi = oi; bg = og; }
+        }
+        // exclusive prefix of terminated slots (slot order)
+        const uint64_t bal = __ballot(term);
+        const int before = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) { wsum[w] = __popcll(bal); wbest[w] = bv; wbesti[w] = bi; wbestg[w] = bg; }
+        __syncthreads();
+        int off = 0, tot = 0;
+        for (int k = 0; k < 16; k++) {
+            if (k < w) off += wsum[k];
+            tot += wsum[k];
+        }
+        if (term) {
+            const int rank = s_carry + off + before;
+            if (gen0 + rank < total_sp) {
+                o.order[i] = gen0 + rank;
+                o.cur_it[i] = 0;
+                o.refill[i] = 1;
+                s_any = 1;
+            }
+        }
+        if (tid == 0) {
+            // fold this chunk's best into the running best (strict '>', earlier chunks first)
+            float cb = -INFINITY;
+            int ci = 0x7fffffff, cg = -1;
+            for (int k = 0; k < 16; k++)
+                if (wbest[k] > cb || (wbest[k] == cb && wbesti[k] < ci)) { cb = wbest[k]; ci = wbesti[k]; cg = wbestg[k]; }
+            if (ci != 0x7fffffff && s_best < cb) { s_best = cb; s_bslot = ci; s_bglob = cg; }
+            s_carry += tot;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        o.generated[0] = min(total_sp, gen0 + s_carry);
+        o.generated[1] = s_bslot;
+        if (s_bglob >= 0) o.generated[2] = s_bglob;
+        o.best_val[0] = s_best;
+        sp.done[1] += 1;
+        if (!s_any) sp.done[0] = 1;
+    }
+}
+hipError_t launch_otf_converge(SpState sp, OtfState o, int batch, double tol, int max_iter, int total_sp,
+                               hipStream_t st) {
+    otf_converge_kernel<<<1, 1024, 0, st>>>(sp, o, batch, tol, max_iter, total_sp);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void otf_refill_kernel(SpState sp, OtfState o, Operand xop, int precision, int n,
+                                                          int ldx, int batch) {
+    const int i = blockIdx.x;
+    const long base = (long)i * ldx;
+    if (o.generated[1] == i)
+        for (int k = threadIdx.x; k < n; k += 256) o.xbest[k] = xop.f32[base + k];
+    if (!o.refill[i]) return;
+    __syncthreads();
+    const float* src = o.pool + (long)o.order[i] * ldx;
+    for (int k = threadIdx.x; k < n; k += 256) write_operand(xop, precision, base + k, src[k]);
+    if (threadIdx.x == 0) {
+        sp.val[i] = 0.f;
+        sp.err[i] = 0.f;
+        sp.nnz[i] = 0;
+        o.refill[i] = 0;
+    }
+}
+hipError_t launch_otf_refill(SpState sp, OtfState o, Operand xop, int precision, int n, int ldx, int batch,
+                             hipStream_t st) {
+    otf_refill_kernel<<<batch, 256, 0, st>>>(sp, o, xop, precision, n, ldx, batch);
+    return hipGetLastError();
+}
+
 __global__ void x_operand_kernel(Operand xop, int precision, int ldx, int batch) {
     const long total = (long)ldx * batch;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
@@ -1260,6 +1396,90 @@ hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* 
                           double* ev, hipStream_t st) {
     if (m > 64 * kEvThreads) return hipErrorInvalidValue;
     ev_fp64_kernel<<<batch, kEvThreads, 0, st>>>(Acm, lda, m, n, X, ldx, l1, ev);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- FP64 support re-solve
+// y on the support S of the returned iterate: L2 variance maximises |A_S y|_2 (power
+// iteration on A_S^T A_S), L1 variance maximises |A_S y|_1 (the AM sign iteration restricted
+// to S).  A_S is gathered once into an m x k FP64 matrix; each iteration is three launches
+// with fixed-order reductions, so the result is reproducible.
+__global__ void gather_support_kernel(const float* __restrict__ A, int lda, int m, const int* __restrict__ idx, int k,
+                                      double* __restrict__ As) {
+    const long total = (long)m * k;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e % m), b = (int)(e / m);
+        As[e] = (double)A[(long)idx[b] * lda + i];
+    }
+}
+// z = A_S y, mapped (sign for L1 variance), per-block partial of the objective
+__global__ __launch_bounds__(256) void resolve_az_kernel(const double* __restrict__ As, int m, int k,
+                                                         const double* __restrict__ y, int l1, double* __restrict__ z,
+                                                         double* __restrict__ part) {
+    __shared__ double red[4];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    double zi = 0;
+    if (i < m)
+        for (int b = 0; b < k; b++) zi += As[(long)b * m + i] * y[b];
+    const double contrib = i < m ? (l1 ? fabs(zi) : zi * zi) : 0.0;
+    if (i < m) z[i] = l1 ? (double)((zi > 0) - (zi < 0)) : zi;
+    const double t = block_sum<256>(contrib, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+// y' = A_S^T z, one block per support column
+__global__ __launch_bounds__(256) void resolve_atz_kernel(const double* __restrict__ As, int m,
+                                                          const double* __restrict__ z, double* __restrict__ yp) {
+    __shared__ double red[4];
+    const int b = blockIdx.x;
+    double acc = 0;
+    for (int i = threadIdx.x; i < m; i += 256) acc += As[(long)b * m + i] * z[i];
+    const double t = block_sum<256>(acc, red);
+    if (threadIdx.x == 0) yp[b] = t;
+}
+// objective of the current y (sum of the partials), normalise y' into y, convergence flag
+__global__ __launch_bounds__(256) void resolve_norm_kernel(int k, int nparts, const double* __restrict__ part,
+                                                           double* __restrict__ yp, double* __restrict__ y,
+                                                           double* __restrict__ ev, int* __restrict__ flag, double tol,
+                                                           int it) {
+    __shared__ double red[4];
+    if (flag[0]) return;
+    double e = 0;
+    for (int p = threadIdx.x; p < nparts; p += 256) e += part[p];
+    e = block_sum<256>(e, red);
+    double nn = 0;
+    for (int b = threadIdx.x; b < k; b += 256) nn += yp[b] * yp[b];
+    nn = block_sum<256>(nn, red);
+    const double inv = 1.0 / sqrt(nn);
+    if (threadIdx.x == 0) {
+        const double prev = ev[0];
+        ev[0] = e;  // objective of the y this iteration started from
+        if (it > 0 && fabs(e - prev) <= tol * fabs(e)) flag[0] = 1;
+    }
+    __syncthreads();
+    if (flag[0]) return;
+    for (int b = threadIdx.x; b < k; b += 256) y[b] = yp[b] * inv;
+}
+__global__ __launch_bounds__(256) void resolve_sum_kernel(int nparts, const double* __restrict__ part,
+                                                          double* __restrict__ ev) {
+    __shared__ double red[4];
+    double e = 0;
+    for (int p = threadIdx.x; p < nparts; p += 256) e += part[p];
+    e = block_sum<256>(e, red);
+    if (threadIdx.x == 0) ev[0] = e;
+}
+hipError_t launch_resolve(const float* Acm, int lda, int m, const int* idx, int k, int l1, double* As, double* y,
+                          double* z, double* yp, double* part, double* ev, int* flag, double tol, int max_iter,
+                          hipStream_t st) {
+    gather_support_kernel<<<1024, 256, 0, st>>>(Acm, lda, m, idx, k, As);
+    const int nb = (m + 255) / 256;
+    for (int it = 0; it < max_iter; it++) {
+        resolve_az_kernel<<<nb, 256, 0, st>>>(As, m, k, y, l1, z, part);
+        resolve_atz_kernel<<<k, 256, 0, st>>>(As, m, z, yp);
+        resolve_norm_kernel<<<1, 256, 0, st>>>(k, nb, part, yp, y, ev, flag, tol, it);
+    }
+    // objective of the final y
+    resolve_az_kernel<<<nb, 256, 0, st>>>(As, m, k, y, l1, z, part);
+    resolve_sum_kernel<<<1, 256, 0, st>>>(nb, part, ev);
     return hipGetLastError();
 }
 

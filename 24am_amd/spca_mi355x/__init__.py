@@ -34,6 +34,7 @@ EXPORTS = [
     "spca_matrix_from_device", "spca_matrix_add_storage", "spca_matrix_release", "spca_matrix_bytes", "spca_solve",
     "spca_dense_solver_device", "spca_dense_solver_host", "spca_settings_default", "spca_settings_check",
     "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
+    "spca_resolve_support",
 ]
 
 
@@ -60,6 +61,7 @@ class _Settings(ctypes.Structure):
         ("tolerance", ctypes.c_double), ("max_iter", ctypes.c_int), ("total_sp", ctypes.c_int),
         ("batch", ctypes.c_int), ("use_otf", ctypes.c_int), ("termination", ctypes.c_int),
         ("precision", ctypes.c_int), ("start", ctypes.c_int), ("check_every", ctypes.c_int),
+        ("resolve_iters", ctypes.c_int), ("resolve_tol", ctypes.c_double),
     ]
 
 
@@ -70,6 +72,7 @@ class _Stats(ctypes.Structure):
         ("best_sp", ctypes.c_int), ("total_sp", ctypes.c_int), ("constraint", ctypes.c_uint),
         ("ev_fp64", ctypes.c_double), ("gemm_ax_ms", ctypes.c_double), ("gemm_atz_ms", ctypes.c_double),
         ("step_ms", ctypes.c_double), ("sp_iterations", ctypes.c_ulonglong),
+        ("ev_resolve_fp64", ctypes.c_double), ("resolve_iterations", ctypes.c_int),
     ]
 
 
@@ -109,6 +112,7 @@ def lib():
             "spca_is_l1_constrained": (I, [I]),
             "spca_is_l1_penalized": (I, [I]),
             "spca_is_l1_variance": (I, [I]),
+            "spca_resolve_support": (I, [P, U64, I, P, I, ctypes.c_double, P, P, ctypes.POINTER(I)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -139,6 +143,8 @@ class OptimizationSettings:
         self.termination = TERM_BATCH_MAX
         self.start = START_EXPLICIT
         self.check_every = 1
+        self.resolve_iters = 0  # > 0: FP64 re-solve on the support of the result
+        self.resolve_tol = 1e-12
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -164,7 +170,7 @@ class OptimizationSettings:
         return _Settings(int(self.formulation), int(self.constraintParameter), float(self.penaltyParameter),
                          float(self.tolerance), int(self.maximumIterations), int(self.totalStartingPoints),
                          int(self.batchSize), int(bool(self.useOTF)), int(self.termination), int(self.precision),
-                         int(self.start), int(self.check_every))
+                         int(self.start), int(self.check_every), int(self.resolve_iters), float(self.resolve_tol))
 
 
 class OptimizationStatistics:
@@ -188,6 +194,8 @@ class OptimizationStatistics:
         self.gemm_atz_ms = 0.0
         self.step_ms = 0.0
         self.sp_iterations = 0
+        self.ev_resolve_fp64 = None
+        self.resolve_iterations = 0
 
     def _fill(self, s, per_sp=None):
         self.it, self.fval, self.error = s.it, s.fval, s.error
@@ -196,6 +204,7 @@ class OptimizationStatistics:
         self.n, self.best_sp, self.ev_fp64 = s.n, s.best_sp, s.ev_fp64
         self.gemm_ax_ms, self.gemm_atz_ms, self.step_ms = s.gemm_ax_ms, s.gemm_atz_ms, s.step_ms
         self.sp_iterations = s.sp_iterations
+        self.ev_resolve_fp64, self.resolve_iterations = s.ev_resolve_fp64, s.resolve_iterations
         if per_sp is not None:
             self.values = np.array([r.value for r in per_sp], dtype=np.float32)
             self.iters = np.array([r.iters for r in per_sp], dtype=np.int32)
@@ -261,6 +270,18 @@ class Matrix:
 
     def add_storage(self, storage):
         _check(self.ctx.ptr, lib().spca_matrix_add_storage(self.ctx.ptr, self.handle, storage))
+
+    def resolve_support(self, x, formulation, max_iter=1000, tol=1e-12):
+        """FP64 re-solve on the support of x; returns (y (n doubles), ev, iterations or -1 if converged)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        y = np.zeros(self.n, dtype=np.float64)
+        ev = ctypes.c_double()
+        its = ctypes.c_int()
+        _check(self.ctx.ptr, lib().spca_resolve_support(self.ctx.ptr, self.handle, int(formulation),
+                                                        x.ctypes.data_as(ctypes.c_void_p), max_iter, tol,
+                                                        y.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ev),
+                                                        ctypes.byref(its)))
+        return y, ev.value, its.value
 
     def release(self):
         if self.handle:

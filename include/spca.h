@@ -96,6 +96,8 @@ typedef struct spca_settings {
     int precision;            /* spca_precision */
     int start;                /* spca_start */
     int check_every;          /* host reads the convergence flag every k iterations (>=1) */
+    int resolve_iters;        /* > 0: FP64 re-solve on the support of x_out, at most this many iterations */
+    double resolve_tol;       /* relative objective change that stops the re-solve */
 } spca_settings;
 
 /* mirrors OptimizationStatistics (c++/src/class/optimization_statistics.h:27-46) */
@@ -114,6 +116,8 @@ typedef struct spca_stats {
     double gemm_atz_ms;       /* summed kernel time of the A^T*Z products */
     double step_ms;           /* summed time of epilogue + formulation step + reduction kernels */
     unsigned long long sp_iterations; /* sum over SPs of iterations while active */
+    double ev_resolve_fp64;   /* explained variance after the FP64 support re-solve (0 if not run) */
+    int resolve_iterations;   /* iterations the re-solve ran */
 } spca_stats;
 
 /* per starting point result (reference values / iters / cardinalities) */
@@ -181,6 +185,15 @@ int spca_dense_solver_device(spca_ctx ctx, unsigned m, unsigned n, const float* 
  * Returns the status; *fval_out receives the best value the reference returns. */
 int spca_dense_solver_host(spca_ctx ctx, const float* B, int ldB, float* x, unsigned m, unsigned n,
                            const spca_settings* settings, spca_stats* stats, double* fval_out);
+
+/* FP64 re-solve restricted to the support of x (its nonzeros), on the GPU: the unit vector y
+ * on that support maximising |A y|_2 (L2-variance formulations: power iteration on
+ * A_S^T A_S) or |A y|_1 (L1-variance formulations: the AM sign iteration restricted to S),
+ * started from x normalised.  Stops when the objective changes by <= tol (relative) or after
+ * max_iter iterations.  y_out (n doubles, zero off the support) and *ev_out receive the
+ * result.  No reference counterpart: it refines the solver's output on a fixed support. */
+int spca_resolve_support(spca_ctx ctx, spca_matrix mat, int formulation, const float* x, int max_iter, double tol,
+                         double* y_out, double* ev_out, int* iters_out);
 
 /* ---- reference settings helpers ------------------------------------------- */
 /* reference OptimizationSettings defaults (c++/src/class/optimization_settings.h:101-123) */

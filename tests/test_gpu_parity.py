@@ -178,6 +178,82 @@ def test_batches_and_termination(small, small_mat, term):
     assert abs(stats.fval - o["fval"]) <= 1e-5 * o["fval"]
 
 
+@pytest.mark.parametrize("f", [0, 3, 4, 7])
+def test_otf_refill_matches_oracle(small, small_mat, f):
+    # on-the-fly refill (reference OTF driver): 200 SPs through 32 slots, refilled on the device
+    A, _, _ = small
+    n = A.shape[1]
+    X0 = unit_gaussian_starts(n, 224, seed=11)  # 200 rounded up to a multiple of 32
+    g = default_gamma(f, A.shape[0])
+    print(f"case otf f={f}", flush=True)
+    st = S.OptimizationSettings(formulation=f, constraintParameter=10, penaltyParameter=g, tolerance=1e-4,
+                                maximumIterations=30, totalStartingPoints=200, batchSize=32, useOTF=True)
+    stats = S.OptimizationStatistics()
+    x = small_mat.solve(st, X0=X0, stats=stats)
+    o = O.solve(A.astype(np.float64), f, s=10, gamma=g, tol=1e-4, max_iter=30, total_sp=200, batch=32, otf=True,
+                X0=X0.astype(np.float64), lambda_rule=O.LAMBDA_GPU)
+    assert len(stats.values) == 224
+    same_it = np.mean(stats.iters == o["sp_iters"])
+    with np.errstate(invalid="ignore"):
+        rel = np.abs(stats.values - o["sp_val"]) / np.abs(o["sp_val"])
+    print(f"  total it {stats.it}/{o['it']} iterations equal {same_it:.3f} median rel {np.nanmedian(rel):.2e} "
+          f"best {stats.fval:.7g}/{o['fval']:.7g} (SP {stats.best_sp}/{o['best_sp']})")
+    assert same_it >= 0.95
+    assert abs(int(stats.it) - int(o["it"])) <= 2
+    assert np.nanmedian(rel) < 1e-6
+    assert abs(stats.fval - o["fval"]) <= 1e-5 * abs(o["fval"])
+    # the best SP may differ only as an argmax tie: its FP64 value equals the best to rounding
+    assert stats.best_sp == o["best_sp"] or o["sp_val"][stats.best_sp] >= o["fval"] * (1 - 1e-6)
+    np.testing.assert_allclose(x, o["x"], atol=2e-5)
+    assert abs(stats.ev_fp64 - explained_variance(A, o["x"][:, None], f)[0]) <= 1e-5 * stats.ev_fp64
+
+
+def test_resolve_l2_is_top_eigenvector(small, small_mat):
+    # FP64 re-solve on the support of the returned x: for L2 variance it is the top
+    # eigenvector of A_S^T A_S (LAPACK eigh in FP64 pins it)
+    A, X0, _ = small
+    st = S.OptimizationSettings(formulation=2, constraintParameter=10, tolerance=1e-6, maximumIterations=100,
+                                totalStartingPoints=64, batchSize=64, resolve_iters=2000, resolve_tol=1e-15)
+    stats = S.OptimizationStatistics()
+    x = small_mat.solve(st, X0=X0, stats=stats)
+    Sx = np.nonzero(x)[0]
+    A64 = A.astype(np.float64)
+    w, V = np.linalg.eigh(A64[:, Sx].T @ A64[:, Sx])
+    print(f"  support {len(Sx)}: ev(x) {stats.ev_fp64:.12g} re-solve {stats.ev_resolve_fp64:.12g} eigh {w[-1]:.12g}")
+    assert abs(stats.ev_resolve_fp64 - w[-1]) <= 1e-10 * w[-1]
+    assert stats.ev_resolve_fp64 >= stats.ev_fp64 * (1 - 1e-12)
+    y, ev, _ = small_mat.resolve_support(x, 2, max_iter=2000, tol=1e-15)
+    assert abs(ev - w[-1]) <= 1e-10 * w[-1]
+    assert np.all(y[np.setdiff1d(np.arange(A.shape[1]), Sx)] == 0)
+    assert abs(abs(y[Sx] @ V[:, -1]) - 1) < 1e-8
+
+
+def test_resolve_l1_brute_force(ctx):
+    # L1 variance on a 10 x 4 support: max |A_S y|_1 over unit y = max over sign vectors u of
+    # |A_S^T u|_2; the best of a few starts reaches it and no start exceeds it
+    import itertools
+    rng = np.random.default_rng(3)
+    A = rng.standard_normal((10, 50)).astype(np.float32)
+    Sx = np.array([3, 17, 29, 41])
+    A64 = A.astype(np.float64)[:, Sx]
+    best = max(np.linalg.norm(A64.T @ np.array(u)) for u in itertools.product((-1.0, 1.0), repeat=10))
+    M = ctx.upload(A)
+    try:
+        evs = []
+        for t in range(8):
+            x = np.zeros(50, dtype=np.float32)
+            x[Sx] = rng.standard_normal(4)
+            y, ev, _ = M.resolve_support(x, 1, max_iter=500, tol=1e-15)
+            evs.append(ev)
+            xs = x[Sx].astype(np.float64) / np.linalg.norm(x[Sx])
+            assert ev >= np.abs(A64 @ xs).sum() * (1 - 1e-12)
+            assert ev <= best * (1 + 1e-12)
+            assert abs(np.abs(A64 @ y[Sx]).sum() - ev) <= 1e-12 * ev
+        assert abs(max(evs) - best) <= 1e-12 * best
+    finally:
+        M.release()
+
+
 def _boxplot_rows():
     rows = []
     with open(os.path.join(GOLD, "ref_paper_experiment_boxplots.txt")) as fh:
@@ -213,8 +289,7 @@ def test_reference_boxplot_values(boxplot_mat, row):
 
 
 def _integer_rows():
-    rows = json.load(open(os.path.join(GOLD, "ref_cpu_integer_results.json")))["rows"]
-    return [r for r in rows if not r["otf"]]
+    return json.load(open(os.path.join(GOLD, "ref_cpu_integer_results.json")))["rows"]
 
 
 @pytest.mark.parametrize("idx", range(len(_integer_rows())))
@@ -234,7 +309,7 @@ def test_reference_integer_results(ctx, idx):
         st = S.OptimizationSettings(formulation=r["formulation"], constraintParameter=r["s"],
                                     penaltyParameter=r["gamma"], tolerance=r["tol"], maximumIterations=r["max_iter"],
                                     totalStartingPoints=r["total_sp"], batchSize=r["batch"],
-                                    start=S.START_REFERENCE_F64)
+                                    useOTF=r["otf"], start=S.START_REFERENCE_F64)
         stats = S.OptimizationStatistics()
         x = M.solve(st, stats=stats)
     finally:
@@ -246,7 +321,7 @@ def test_reference_integer_results(ctx, idx):
         # within FP32 rounding of the reference best, and its own FP64 iterate has the nnz
         # the GPU returned.  Rerun all SPs in FP64 to check both.
         o = O.solve(np.asarray(A, dtype=np.float64), r["formulation"], s=r["s"], gamma=r["gamma"], tol=r["tol"],
-                    max_iter=r["max_iter"], total_sp=r["total_sp"], batch=r["batch"], want_X=True,
+                    max_iter=r["max_iter"], total_sp=r["total_sp"], batch=r["batch"], otf=r["otf"], want_X=True,
                     lambda_rule=O.LAMBDA_GPU)
         j = stats.best_sp
         print(f"  nnz {nnz} vs {r['nnz']}: GPU best SP {j}, FP64 value {o['sp_val'][j]:.9g} vs best "
@@ -299,9 +374,9 @@ def test_errors_are_reported(ctx, small_mat, small):
     with pytest.raises(S.SpcaError) as e:
         small_mat.solve(bad, X0=X0)
     assert e.value.code == S.SPCA_EINVAL
-    otf = S.OptimizationSettings(useOTF=True, totalStartingPoints=64, batchSize=64)
+    bad_start = S.OptimizationSettings(start=7, totalStartingPoints=64, batchSize=64)
     with pytest.raises(S.SpcaError):
-        small_mat.solve(otf, X0=X0)
+        small_mat.solve(bad_start, X0=X0)
     # budget: a context with a 1 MiB budget cannot hold a 200 x 1000 matrix in two layouts
     with S.Context(hbm_budget_bytes=1 << 20) as c2:
         with pytest.raises(S.SpcaError) as e:
