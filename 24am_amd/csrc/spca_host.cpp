@@ -12,8 +12,12 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <thread>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -59,21 +63,35 @@ struct Workspace {
     uint64_t bytes = 0;
 };
 
+// Everything one solve needs besides the matrix: its stream, workspace, a pinned flag for
+// the host's convergence checks and hipEvents for timing.  The context owns one engine for
+// synchronous calls; the request server owns one per concurrent job.
+struct Engine {
+    hipStream_t stream = nullptr;
+    Workspace ws;
+    int* h_done = nullptr;
+    std::vector<hipEvent_t> events;  // pool of (start, stop) pairs for phase timing
+    hipEvent_t loop_ev[2] = {nullptr, nullptr};
+    const std::atomic<int>* cancel = nullptr;  // set by the request server; checked at host syncs
+};
+
 }  // namespace spca
+
+struct spca_server;
 
 struct spca_ctx_s {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;  // the default engine's stream
     uint64_t budget = 0;
     uint64_t used = 0;
-    std::mutex mu;
+    std::mutex alloc_mu;  // guards `used`
+    std::mutex mu;        // guards the matrix table and the default engine
+    std::mutex err_mu;    // guards err
     std::string err;
     std::map<uint64_t, spca::Matrix*> mats;
     uint64_t next_handle = 1;
-    spca::Workspace ws;
-    int* h_done = nullptr;  // pinned
-    std::vector<hipEvent_t> events;  // pool of (start, stop) pairs for phase timing
-    hipEvent_t loop_ev[2] = {nullptr, nullptr};
+    spca::Engine eng;  // default engine (synchronous calls)
+    spca_server* server = nullptr;
 };
 
 using namespace spca;
@@ -82,8 +100,14 @@ namespace {
 
 const char* kVersion = "spca_mi355x 0.1 (gfx950)";
 
+thread_local std::string tl_err;  // this thread's last error (worker threads read their own)
+
 int fail(spca_ctx ctx, int code, const std::string& msg) {
-    if (ctx) ctx->err = msg;
+    tl_err = msg;
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->err_mu);
+        ctx->err = msg;
+    }
     return code;
 }
 
@@ -96,13 +120,36 @@ int fail(spca_ctx ctx, int code, const std::string& msg) {
     } while (0)
 
 template <class T>
-int dev_alloc(spca_ctx ctx, T** p, size_t count, uint64_t* acct) {
+int dev_alloc(spca_ctx ctx, T** p, size_t count, uint64_t* acct, std::string* err = nullptr) {
     const uint64_t b = (uint64_t)count * sizeof(T);
-    if (ctx->used + b > ctx->budget)
-        return fail(ctx, SPCA_EBUDGET, "allocation of " + std::to_string(b) + " bytes exceeds the HBM budget");
+    {
+        std::lock_guard<std::mutex> g(ctx->alloc_mu);
+        if (ctx->used + b > ctx->budget) {
+            const std::string m = "allocation of " + std::to_string(b) + " bytes exceeds the HBM budget";
+            if (err) *err = m;
+            else {
+                tl_err = m;
+                std::lock_guard<std::mutex> g2(ctx->err_mu);
+                ctx->err = m;
+            }
+            return SPCA_EBUDGET;
+        }
+        ctx->used += b;
+    }
     hipError_t e = hipMalloc((void**)p, b ? b : 16);
-    if (e != hipSuccess) return fail(ctx, SPCA_ENOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    ctx->used += b;
+    if (e != hipSuccess) {
+        std::lock_guard<std::mutex> g(ctx->alloc_mu);
+        ctx->used -= b;
+        *p = nullptr;
+        const std::string m = std::string("hipMalloc: ") + hipGetErrorString(e);
+        if (err) *err = m;
+        else {
+            tl_err = m;
+            std::lock_guard<std::mutex> g2(ctx->err_mu);
+            ctx->err = m;
+        }
+        return SPCA_ENOMEM;
+    }
     *acct += b;
     return SPCA_OK;
 }
@@ -110,6 +157,7 @@ template <class T>
 void dev_free(spca_ctx ctx, T*& p, uint64_t bytes_of_p) {
     if (p) {
         (void)hipFree(p);
+        std::lock_guard<std::mutex> g(ctx->alloc_mu);
         ctx->used -= bytes_of_p;
         p = nullptr;
     }
@@ -122,6 +170,7 @@ void free_matrix(spca_ctx ctx, Matrix* M) {
     void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    std::lock_guard<std::mutex> g(ctx->alloc_mu);
     ctx->used -= M->bytes;
     delete M;
 }
@@ -234,40 +283,77 @@ int make_matrix(spca_ctx ctx, const float* src, bool src_on_device, int m, int n
     return SPCA_OK;
 }
 
-void free_workspace(spca_ctx ctx) {
-    Workspace& w = ctx->ws;
-    (void)hipStreamSynchronize(ctx->stream);
+void free_workspace(spca_ctx ctx, Engine& E) {
+    Workspace& w = E.ws;
+    (void)hipStreamSynchronize(E.stream);
     void* ptrs[] = {w.X, w.Xhi, w.Xlo, w.Zpart, w.Z, w.Zhi, w.Zlo, w.V, w.fstate, w.istate, w.ev};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
-    ctx->used -= w.bytes;
+    {
+        std::lock_guard<std::mutex> g(ctx->alloc_mu);
+        ctx->used -= w.bytes;
+    }
     w = Workspace();
 }
 
-int ensure_workspace(spca_ctx ctx, int np, int mp, int bp, int splits) {
-    Workspace& w = ctx->ws;
+// device bytes a solve of this shape holds in its workspace
+uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
+    const uint64_t xs = (uint64_t)np * bp, zs = (uint64_t)mp * bp;
+    return xs * (4 + 2 + 2 + 4) + zs * splits * 4 + zs * (4 + 2 + 2) + (uint64_t)4 * bp * 4 +
+           ((uint64_t)3 * bp + 4) * 4 + (uint64_t)bp * 8;
+}
+
+int ensure_workspace(spca_ctx ctx, Engine& E, int np, int mp, int bp, int splits, std::string* err = nullptr) {
+    Workspace& w = E.ws;
     if (w.X && w.np == np && w.mp == mp && w.bp == bp && w.splits >= splits) return SPCA_OK;
-    free_workspace(ctx);
+    free_workspace(ctx, E);
     w.np = np;
     w.mp = mp;
     w.bp = bp;
     w.splits = splits;
     const size_t xs = (size_t)np * bp, zs = (size_t)mp * bp;
     int rc;
-    if ((rc = dev_alloc(ctx, &w.X, xs, &w.bytes)) || (rc = dev_alloc(ctx, &w.Xhi, xs, &w.bytes)) ||
-        (rc = dev_alloc(ctx, &w.Xlo, xs, &w.bytes)) || (rc = dev_alloc(ctx, &w.Zpart, zs * splits, &w.bytes)) ||
-        (rc = dev_alloc(ctx, &w.Z, zs, &w.bytes)) || (rc = dev_alloc(ctx, &w.Zhi, zs, &w.bytes)) ||
-        (rc = dev_alloc(ctx, &w.Zlo, zs, &w.bytes)) || (rc = dev_alloc(ctx, &w.V, xs, &w.bytes)) ||
-        (rc = dev_alloc(ctx, &w.fstate, (size_t)4 * bp, &w.bytes)) ||
-        (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + 4, &w.bytes)) ||
-        (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes))) {
-        free_workspace(ctx);
+    if ((rc = dev_alloc(ctx, &w.X, xs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.Xhi, xs, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.Xlo, xs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.Zpart, zs * splits, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.Z, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.Zhi, zs, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.Zlo, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.V, xs, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.fstate, (size_t)4 * bp, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + 4, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err))) {
+        free_workspace(ctx, E);
         return rc;
     }
-    HIP_TRY(ctx, hipMemsetAsync(w.Z, 0, zs * sizeof(float), ctx->stream));
-    HIP_TRY(ctx, hipMemsetAsync(w.Zhi, 0, zs * sizeof(uint16_t), ctx->stream));
-    HIP_TRY(ctx, hipMemsetAsync(w.Zlo, 0, zs * sizeof(uint16_t), ctx->stream));
+    if (hipMemsetAsync(w.Z, 0, zs * sizeof(float), E.stream) != hipSuccess ||
+        hipMemsetAsync(w.Zhi, 0, zs * sizeof(uint16_t), E.stream) != hipSuccess ||
+        hipMemsetAsync(w.Zlo, 0, zs * sizeof(uint16_t), E.stream) != hipSuccess) {
+        free_workspace(ctx, E);
+        return SPCA_EDEVICE;
+    }
     return SPCA_OK;
+}
+
+int engine_init(Engine& E) {
+    if (hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking) != hipSuccess) return SPCA_EDEVICE;
+    if (hipHostMalloc((void**)&E.h_done, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess) return SPCA_ENOMEM;
+    if (hipEventCreate(&E.loop_ev[0]) != hipSuccess || hipEventCreate(&E.loop_ev[1]) != hipSuccess)
+        return SPCA_EDEVICE;
+    return SPCA_OK;
+}
+
+void engine_destroy(spca_ctx ctx, Engine& E) {
+    if (E.stream) (void)hipStreamSynchronize(E.stream);
+    free_workspace(ctx, E);
+    for (hipEvent_t e : E.events) (void)hipEventDestroy(e);
+    E.events.clear();
+    for (hipEvent_t& e : E.loop_ev)
+        if (e) {
+            (void)hipEventDestroy(e);
+            e = nullptr;
+        }
+    if (E.h_done) (void)hipHostFree(E.h_done);
+    E.h_done = nullptr;
+    if (E.stream) (void)hipStreamDestroy(E.stream);
+    E.stream = nullptr;
 }
 
 SpState state_of(Workspace& w) {
@@ -325,19 +411,19 @@ int choose_splits(int precision, int mp, int np, int bp) {
 // Kernel time per phase (0: A*X, 1: A^T*Z, 2: epilogue/step/convergence), from
 // hipEvent pairs recorded on the solver stream and folded in after each host sync.
 struct PhaseTimer {
-    spca_ctx ctx;
+    Engine* eng;
     size_t used = 0;  // events handed out since the last flush (always pairs)
     double ms[3] = {0, 0, 0};
     std::vector<int> phase;
     hipError_t pair(hipEvent_t* a, hipEvent_t* b, int ph) {
-        while (ctx->events.size() < used + 2) {
+        while (eng->events.size() < used + 2) {
             hipEvent_t e;
             hipError_t r = hipEventCreate(&e);
             if (r != hipSuccess) return r;
-            ctx->events.push_back(e);
+            eng->events.push_back(e);
         }
-        *a = ctx->events[used];
-        *b = ctx->events[used + 1];
+        *a = eng->events[used];
+        *b = eng->events[used + 1];
         used += 2;
         phase.push_back(ph);
         return hipSuccess;
@@ -345,14 +431,14 @@ struct PhaseTimer {
     void flush() {
         for (size_t i = 0; i < phase.size(); i++) {
             float t = 0;
-            if (hipEventElapsedTime(&t, ctx->events[2 * i], ctx->events[2 * i + 1]) == hipSuccess) ms[phase[i]] += t;
+            if (hipEventElapsedTime(&t, eng->events[2 * i], eng->events[2 * i + 1]) == hipSuccess) ms[phase[i]] += t;
         }
         phase.clear();
         used = 0;
     }
 };
 
-int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float* X0, float* x_out,
+int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, const float* X0, float* x_out,
                spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
     const auto t_wall0 = std::chrono::steady_clock::now();
     spca_settings st = *st_in;
@@ -372,9 +458,9 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
     const int batch = st.batch;
     const int bp = round_up(batch, kPadB);
     const int splits = choose_splits(st.precision, M->mp, M->np, bp);
-    if ((rc = ensure_workspace(ctx, M->np, M->mp, bp, splits))) return rc;
-    Workspace& w = ctx->ws;
-    hipStream_t s = ctx->stream;
+    if ((rc = ensure_workspace(ctx, E, M->np, M->mp, bp, splits))) return rc;
+    Workspace& w = E.ws;
+    hipStream_t s = E.stream;
     SpState sp = state_of(w);
     const bool constrained = st.formulation < 4;
     const int check_every = st.check_every > 0 ? st.check_every : 1;
@@ -407,9 +493,9 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
     unsigned total_it = 0;
     unsigned long long sp_its = 0;
     double loop_ms = 0, last_err = 0;
-    PhaseTimer pt{ctx};
+    PhaseTimer pt{&E};
     std::vector<float> x_best(n, 0.f);
-    hipEvent_t* loop_ev = ctx->loop_ev;
+    hipEvent_t* loop_ev = E.loop_ev;
 
     if (st.use_otf) {
         // ---- on-the-fly refill (reference OTF driver, sparse_PCA_solver.h:83-169): every SP is
@@ -509,10 +595,14 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
             HIP_TRY(ctx, launch_otf_refill(sp, o, xop, st.precision, n, w.np, batch, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             if ((it + 1) % check_every == 0 || it + 1 == cap) {
-                HIP_TRY(ctx, hipMemcpyAsync(ctx->h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+                HIP_TRY(ctx, hipMemcpyAsync(E.h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
                 HIP_TRY(ctx, hipStreamSynchronize(s));
                 pt.flush();
-                if (ctx->h_done[0]) break;
+                if (E.h_done[0]) break;
+                if (E.cancel && E.cancel->load()) {
+                    release();
+                    return fail(ctx, SPCA_ECANCELLED, "cancelled");
+                }
             }
         }
         HIP_TRY(ctx, hipEventRecord(loop_ev[1], s));
@@ -611,10 +701,11 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
                                          st.termination == SPCA_TERM_PER_SP, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             if ((it + 1) % check_every == 0 || it + 1 == st.max_iter) {
-                HIP_TRY(ctx, hipMemcpyAsync(ctx->h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+                HIP_TRY(ctx, hipMemcpyAsync(E.h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
                 HIP_TRY(ctx, hipStreamSynchronize(s));
                 pt.flush();
-                if (ctx->h_done[0]) break;
+                if (E.h_done[0]) break;
+                if (E.cancel && E.cancel->load()) return fail(ctx, SPCA_ECANCELLED, "cancelled");
             }
         }
         HIP_TRY(ctx, hipEventRecord(loop_ev[1], s));
@@ -700,8 +791,8 @@ int solve_impl(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float*
 }
 
 // FP64 re-solve on the support of x (host vector of n floats)
-int resolve_impl(spca_ctx ctx, Matrix* M, int formulation, const float* x, int max_iter, double tol, double* y_out,
-                 double* ev_out, int* iters_out) {
+int resolve_impl(spca_ctx ctx, Engine& E, Matrix* M, int formulation, const float* x, int max_iter, double tol,
+                 double* y_out, double* ev_out, int* iters_out) {
     if (!valid_formulation(formulation) || !x || max_iter < 1) return fail(ctx, SPCA_EINVAL, "bad re-solve arguments");
     const int n = M->n, m = M->m;
     std::vector<int> idx;
@@ -733,7 +824,7 @@ int resolve_impl(spca_ctx ctx, Matrix* M, int formulation, const float* x, int m
     double* ev = part + nb;
     int* didx = ibuf;
     int* flag = ibuf + k;
-    hipStream_t s = ctx->stream;
+    hipStream_t s = E.stream;
     auto done = [&](int code) {
         (void)hipStreamSynchronize(s);
         dev_free(ctx, dbuf, sizeof(double) * nd);
@@ -766,9 +857,156 @@ int resolve_impl(spca_ctx ctx, Matrix* M, int formulation, const float* x, int m
     return done(SPCA_OK);
 }
 
+int solve_and_resolve(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* settings, const float* X0,
+                      float* x_out, spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
+    int rc = solve_impl(ctx, E, M, settings, X0, x_out, stats, per_sp, X_final);
+    if (rc == SPCA_OK && settings->resolve_iters > 0 && stats) {
+        int its = 0;
+        rc = resolve_impl(ctx, E, M, settings->formulation, x_out, settings->resolve_iters, settings->resolve_tol,
+                          nullptr, &stats->ev_resolve_fp64, &its);
+        stats->resolve_iterations = its;
+    }
+    return rc;
+}
+
+
+// ---------------------------------------------------------------- request server
+// Continuous batching: jobs wait in a FIFO queue; each worker thread owns an engine (its
+// own stream and workspace) and takes the next job as soon as it is free and the job's
+// workspace fits the HBM budget.  A job that does not fit waits for a running job to
+// return its memory; it fails with SPCA_EBUDGET only when nothing else is running.  Work
+// from different workers overlaps on the device.  Kernels are deterministic and every job
+// has its own buffers, so a job's result does not depend on what runs beside it.
+struct Job {
+    uint64_t id = 0;
+    uint64_t mat = 0;
+    Matrix* M = nullptr;
+    spca_settings st{};
+    std::vector<float> X0;
+    std::vector<float> x;
+    spca_stats stats{};
+    std::vector<spca_sp_result> per_sp;
+    bool want_per_sp = false;
+    int status = SPCA_EAGAIN;
+    int state = SPCA_JOB_QUEUED;
+    std::atomic<int> cancel{0};
+    std::string err;
+    std::chrono::steady_clock::time_point t_submit, t_start, t_end;
+    int np = 0, mp = 0, bp = 0, splits = 0;
+};
+
+}  // namespace
+
+struct spca_server {
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Job*> queue;
+    std::map<uint64_t, Job*> jobs;
+    std::vector<std::thread> workers;
+    std::vector<Engine> engines;
+    bool stop = false;
+    uint64_t next_id = 1;
+    int running = 0;
+};
+
+namespace {
+
+void matrix_unref(spca_ctx ctx, uint64_t h) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto itm = ctx->mats.find(h);
+    if (itm == ctx->mats.end()) return;
+    if (--itm->second->refs <= 0) {
+        free_matrix(ctx, itm->second);
+        ctx->mats.erase(itm);
+    }
+}
+
+void worker_main(spca_ctx ctx, spca_server* S, int w) {
+    (void)hipSetDevice(ctx->device);
+    Engine& E = S->engines[w];
+    for (;;) {
+        Job* J = nullptr;
+        {
+            std::unique_lock<std::mutex> lk(S->mu);
+            S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
+            if (S->queue.empty()) break;  // stop requested and nothing queued
+            J = S->queue.front();
+            S->queue.pop_front();
+            J->state = SPCA_JOB_RUNNING;
+            S->running++;
+        }
+        // admission by HBM footprint
+        int rc;
+        for (;;) {
+            std::string err;
+            rc = ensure_workspace(ctx, E, J->np, J->mp, J->bp, J->splits, &err);
+            if (rc != SPCA_EBUDGET) break;
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->running <= 1 || J->cancel.load() || S->stop) {
+                J->err = err;
+                break;
+            }
+            S->cv.wait_for(lk, std::chrono::milliseconds(50));
+        }
+        J->t_start = std::chrono::steady_clock::now();
+        if (rc == SPCA_OK && J->cancel.load()) rc = SPCA_ECANCELLED;
+        if (rc == SPCA_OK) {
+            E.cancel = &J->cancel;
+            const float* X0 = J->X0.empty() ? nullptr : J->X0.data();
+            rc = solve_and_resolve(ctx, E, J->M, &J->st, X0, J->x.data(), &J->stats,
+                                   J->want_per_sp ? J->per_sp.data() : nullptr, nullptr);
+            E.cancel = nullptr;
+            if (rc != SPCA_OK) J->err = tl_err;
+        }
+        J->t_end = std::chrono::steady_clock::now();
+        // keep the workspace only if the next queued job has the same shape
+        bool keep = false;
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (!S->queue.empty()) {
+                const Job* Nx = S->queue.front();
+                keep = Nx->np == E.ws.np && Nx->mp == E.ws.mp && Nx->bp == E.ws.bp && Nx->splits <= E.ws.splits;
+            }
+        }
+        if (!keep) free_workspace(ctx, E);
+        matrix_unref(ctx, J->mat);
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            J->status = rc;
+            J->state = rc == SPCA_ECANCELLED ? SPCA_JOB_CANCELLED : SPCA_JOB_DONE;
+            S->running--;
+        }
+        S->cv.notify_all();
+    }
+    free_workspace(ctx, E);
+}
+
+void server_stop(spca_ctx ctx) {
+    spca_server* S = ctx->server;
+    if (!S) return;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->stop = true;
+        for (Job* J : S->queue) {
+            J->status = SPCA_ECANCELLED;
+            J->state = SPCA_JOB_CANCELLED;
+            J->cancel.store(1);
+        }
+        S->queue.clear();
+        for (auto& kv : S->jobs) kv.second->cancel.store(1);
+    }
+    S->cv.notify_all();
+    for (auto& t : S->workers) t.join();
+    for (auto& E : S->engines) engine_destroy(ctx, E);
+    for (auto& kv : S->jobs) delete kv.second;
+    delete S;
+    ctx->server = nullptr;
+}
+
 }  // namespace
 
 extern "C" {
+
 
 const char* spca_version(void) { return kVersion; }
 
@@ -815,41 +1053,41 @@ int spca_ctx_create(int device, uint64_t hbm_budget_bytes, spca_ctx* out) {
     if (hipSetDevice(device) != hipSuccess) return SPCA_EDEVICE;
     spca_ctx c = new spca_ctx_s();
     c->device = device;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    int rc = engine_init(c->eng);
+    if (rc) {
+        engine_destroy(c, c->eng);
         delete c;
-        return SPCA_EDEVICE;
+        return rc;
     }
+    c->stream = c->eng.stream;
     size_t fr = 0, tot = 0;
     (void)hipMemGetInfo(&fr, &tot);
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(0.9 * (double)fr);
-    if (hipHostMalloc((void**)&c->h_done, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess ||
-        hipEventCreate(&c->loop_ev[0]) != hipSuccess || hipEventCreate(&c->loop_ev[1]) != hipSuccess) {
-        (void)hipStreamDestroy(c->stream);
-        delete c;
-        return SPCA_ENOMEM;
-    }
     *out = c;
     return SPCA_OK;
 }
 
 int spca_ctx_destroy(spca_ctx ctx) {
     if (!ctx) return SPCA_EINVAL;
+    server_stop(ctx);
     {
         std::lock_guard<std::mutex> g(ctx->mu);
         (void)hipSetDevice(ctx->device);
         for (auto& kv : ctx->mats) free_matrix(ctx, kv.second);
         ctx->mats.clear();
-        free_workspace(ctx);
-        for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
-        for (hipEvent_t e : ctx->loop_ev) (void)hipEventDestroy(e);
-        (void)hipHostFree(ctx->h_done);
-        (void)hipStreamDestroy(ctx->stream);
+        engine_destroy(ctx, ctx->eng);
     }
     delete ctx;
     return SPCA_OK;
 }
 
-const char* spca_last_error(spca_ctx ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* spca_last_error(spca_ctx ctx) {
+    static thread_local std::string copy;
+    if (!ctx) return "null context";
+    std::lock_guard<std::mutex> g(ctx->err_mu);
+    copy = ctx->err;
+    return copy.c_str();
+}
 
 int spca_matrix_upload(spca_ctx ctx, const float* A, int m, int n, int ld, unsigned storage, uint64_t key,
                        spca_matrix* out) {
@@ -906,13 +1144,162 @@ int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, con
     if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
     (void)hipSetDevice(ctx->device);
     if (stats) std::memset(stats, 0, sizeof(*stats));
-    int rc = solve_impl(ctx, itm->second, settings, X0, x_out, stats, per_sp, X_final);
-    if (rc == SPCA_OK && settings->resolve_iters > 0 && stats) {
-        int its = 0;
-        rc = resolve_impl(ctx, itm->second, settings->formulation, x_out, settings->resolve_iters,
-                          settings->resolve_tol, nullptr, &stats->ev_resolve_fp64, &its);
-        stats->resolve_iterations = its;
+    return solve_and_resolve(ctx, ctx->eng, itm->second, settings, X0, x_out, stats, per_sp, X_final);
+}
+
+
+int spca_server_start(spca_ctx ctx, int workers) {
+    if (!ctx || workers < 1 || workers > 16) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    if (ctx->server) return SPCA_OK;
+    (void)hipSetDevice(ctx->device);
+    spca_server* S = new spca_server();
+    S->engines.resize(workers);
+    for (auto& E : S->engines) {
+        int rc = engine_init(E);
+        if (rc) {
+            for (auto& E2 : S->engines) engine_destroy(ctx, E2);
+            delete S;
+            return fail(ctx, rc, "engine creation failed");
+        }
     }
+    ctx->server = S;
+    for (int w = 0; w < workers; w++) S->workers.emplace_back(worker_main, ctx, S, w);
+    return SPCA_OK;
+}
+
+int spca_server_stop(spca_ctx ctx) {
+    if (!ctx) return SPCA_EINVAL;
+    server_stop(ctx);
+    return SPCA_OK;
+}
+
+int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, int want_per_sp,
+                spca_job* out) {
+    if (!ctx || !settings || !out) return SPCA_EINVAL;
+    spca_server* S = ctx->server;
+    if (!S) return fail(ctx, SPCA_EINVAL, "no request server: call spca_server_start first");
+    Job* J = new Job();
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        auto itm = ctx->mats.find(mat);
+        if (itm == ctx->mats.end()) {
+            delete J;
+            return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+        }
+        Matrix* M = itm->second;
+        J->st = *settings;
+        if (!valid_formulation(J->st.formulation) || J->st.precision < 0 || J->st.precision > 2 || J->st.max_iter < 1 ||
+            J->st.total_sp < 1 || J->st.batch < 0 || (J->st.start == SPCA_START_EXPLICIT && !X0)) {
+            delete J;
+            return fail(ctx, SPCA_EINVAL, "bad job settings");
+        }
+        const unsigned need = J->st.precision == 0 ? SPCA_STORE_FP32
+                              : J->st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
+        (void)hipSetDevice(ctx->device);
+        int rc = add_storage(ctx, M, need);
+        if (rc) {
+            delete J;
+            return rc;
+        }
+        spca_settings chk = J->st;
+        spca_settings_check(&chk, (unsigned)M->n);
+        J->M = M;
+        J->mat = mat;
+        J->np = M->np;
+        J->mp = M->mp;
+        J->bp = round_up(chk.batch, kPadB);
+        J->splits = choose_splits(J->st.precision, M->mp, M->np, J->bp);
+        const uint64_t need_bytes = workspace_bytes(J->np, J->mp, J->bp, J->splits);
+        if (need_bytes + M->bytes > ctx->budget) {
+            delete J;
+            return fail(ctx, SPCA_EBUDGET, "job workspace exceeds the HBM budget");
+        }
+        if (J->st.start == SPCA_START_EXPLICIT) J->X0.assign(X0, X0 + (size_t)M->n * chk.total_sp);
+        J->x.assign(M->n, 0.f);
+        J->want_per_sp = want_per_sp != 0;
+        if (J->want_per_sp) J->per_sp.resize(chk.total_sp);
+        M->refs++;
+    }
+    J->t_submit = std::chrono::steady_clock::now();
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        J->id = S->next_id++;
+        S->jobs[J->id] = J;
+        S->queue.push_back(J);
+        *out = J->id;
+    }
+    S->cv.notify_all();
+    return SPCA_OK;
+}
+
+int spca_poll(spca_ctx ctx, spca_job job, int* state) {
+    if (!ctx || !ctx->server || !state) return SPCA_EINVAL;
+    spca_server* S = ctx->server;
+    std::lock_guard<std::mutex> lk(S->mu);
+    auto it = S->jobs.find(job);
+    if (it == S->jobs.end()) return SPCA_ENOTFOUND;
+    *state = it->second->state;
+    return SPCA_OK;
+}
+
+int spca_cancel(spca_ctx ctx, spca_job job) {
+    if (!ctx || !ctx->server) return SPCA_EINVAL;
+    spca_server* S = ctx->server;
+    Job* J = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        auto it = S->jobs.find(job);
+        if (it == S->jobs.end()) return SPCA_ENOTFOUND;
+        J = it->second;
+        J->cancel.store(1);
+        for (auto q = S->queue.begin(); q != S->queue.end(); ++q)
+            if (*q == J) {
+                S->queue.erase(q);
+                J->status = SPCA_ECANCELLED;
+                J->state = SPCA_JOB_CANCELLED;
+                break;
+            }
+    }
+    if (J->state == SPCA_JOB_CANCELLED && J->M) {
+        // a queued job never reached a worker: drop its matrix reference here
+        if (J->t_start.time_since_epoch().count() == 0) matrix_unref(ctx, J->mat);
+        J->M = nullptr;
+    }
+    S->cv.notify_all();
+    return SPCA_OK;
+}
+
+int spca_wait(spca_ctx ctx, spca_job job, float* x_out, spca_stats* stats, spca_sp_result* per_sp,
+              spca_job_times* times) {
+    if (!ctx || !ctx->server) return SPCA_EINVAL;
+    spca_server* S = ctx->server;
+    Job* J = nullptr;
+    {
+        std::unique_lock<std::mutex> lk(S->mu);
+        auto it = S->jobs.find(job);
+        if (it == S->jobs.end()) return SPCA_ENOTFOUND;
+        J = it->second;
+        S->cv.wait(lk, [&] { return J->state == SPCA_JOB_DONE || J->state == SPCA_JOB_CANCELLED; });
+        S->jobs.erase(it);
+    }
+    const int rc = J->status;
+    if (rc == SPCA_OK) {
+        if (x_out) std::memcpy(x_out, J->x.data(), sizeof(float) * J->x.size());
+        if (stats) *stats = J->stats;
+        if (per_sp && J->want_per_sp) std::memcpy(per_sp, J->per_sp.data(), sizeof(spca_sp_result) * J->per_sp.size());
+    } else if (!J->err.empty()) {
+        std::lock_guard<std::mutex> g(ctx->err_mu);
+        ctx->err = J->err;
+    }
+    if (times) {
+        using ms = std::chrono::duration<double, std::milli>;
+        const bool started = J->t_start.time_since_epoch().count() != 0;
+        times->queue_ms = started ? ms(J->t_start - J->t_submit).count() : 0.0;
+        times->run_ms = started ? ms(J->t_end - J->t_start).count() : 0.0;
+        times->latency_ms = started ? ms(J->t_end - J->t_submit).count() : 0.0;
+    }
+    delete J;
     return rc;
 }
 
@@ -923,7 +1310,7 @@ int spca_resolve_support(spca_ctx ctx, spca_matrix mat, int formulation, const f
     auto itm = ctx->mats.find(mat);
     if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
     (void)hipSetDevice(ctx->device);
-    return resolve_impl(ctx, itm->second, formulation, x, max_iter, tol, y_out, ev_out, iters_out);
+    return resolve_impl(ctx, ctx->eng, itm->second, formulation, x, max_iter, tol, y_out, ev_out, iters_out);
 }
 
 int spca_dense_solver_device(spca_ctx ctx, unsigned m, unsigned n, const float* d_B, float* h_x,

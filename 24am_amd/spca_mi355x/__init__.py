@@ -34,8 +34,10 @@ EXPORTS = [
     "spca_matrix_from_device", "spca_matrix_add_storage", "spca_matrix_release", "spca_matrix_bytes", "spca_solve",
     "spca_dense_solver_device", "spca_dense_solver_host", "spca_settings_default", "spca_settings_check",
     "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
-    "spca_resolve_support",
+    "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
+    "spca_wait",
 ]
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
 
 
 class SPCA_Formulation(enum.IntEnum):
@@ -76,6 +78,10 @@ class _Stats(ctypes.Structure):
     ]
 
 
+class _JobTimes(ctypes.Structure):
+    _fields_ = [("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
+
+
 class _SpResult(ctypes.Structure):
     _fields_ = [("value", ctypes.c_float), ("iters", ctypes.c_int), ("nnz", ctypes.c_int),
                 ("converged", ctypes.c_int), ("ev_fp64", ctypes.c_double)]
@@ -113,6 +119,12 @@ def lib():
             "spca_is_l1_penalized": (I, [I]),
             "spca_is_l1_variance": (I, [I]),
             "spca_resolve_support": (I, [P, U64, I, P, I, ctypes.c_double, P, P, ctypes.POINTER(I)]),
+            "spca_server_start": (I, [P, I]),
+            "spca_server_stop": (I, [P]),
+            "spca_submit": (I, [P, U64, ctypes.POINTER(_Settings), P, I, ctypes.POINTER(U64)]),
+            "spca_poll": (I, [P, U64, ctypes.POINTER(I)]),
+            "spca_cancel": (I, [P, U64]),
+            "spca_wait": (I, [P, U64, P, ctypes.POINTER(_Stats), P, ctypes.POINTER(_JobTimes)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -249,6 +261,13 @@ class Context:
         except Exception:
             pass
 
+    def start_server(self, workers=4):
+        """Start the request server: up to `workers` jobs run at once, each on its own stream."""
+        _check(self._p, lib().spca_server_start(self._p, workers))
+
+    def stop_server(self):
+        _check(self._p, lib().spca_server_stop(self._p))
+
     def upload(self, A, storage=STORE_FP32, key=0):
         """A: (m x n) float32 array (any order); copied column major to the device."""
         A = np.asfortranarray(A, dtype=np.float32)
@@ -311,6 +330,52 @@ class Matrix:
         if stats is not None:
             stats._fill(s, res)
         return (x, Xf) if want_X else x
+
+
+class Job:
+    """A submitted solve (continuous batching through the context's request server)."""
+
+    def __init__(self, matrix, handle, total):
+        self.matrix, self.handle, self.total = matrix, handle, total
+
+    def poll(self):
+        st = ctypes.c_int()
+        _check(self.matrix.ctx.ptr, lib().spca_poll(self.matrix.ctx.ptr, self.handle, ctypes.byref(st)))
+        return st.value
+
+    def cancel(self):
+        _check(self.matrix.ctx.ptr, lib().spca_cancel(self.matrix.ctx.ptr, self.handle))
+
+    def wait(self, stats=None, per_sp=True):
+        """Blocks until done; returns (x, times dict).  Raises SpcaError on failure/cancel."""
+        x = np.zeros(self.matrix.n, dtype=np.float32)
+        s = _Stats()
+        res = (_SpResult * self.total)() if per_sp else None
+        t = _JobTimes()
+        rc = lib().spca_wait(self.matrix.ctx.ptr, self.handle, x.ctypes.data_as(ctypes.c_void_p), ctypes.byref(s),
+                             ctypes.cast(res, ctypes.c_void_p) if res is not None else None, ctypes.byref(t))
+        _check(self.matrix.ctx.ptr, rc)
+        if stats is not None:
+            stats._fill(s, res)
+        return x, {"queue_ms": t.queue_ms, "run_ms": t.run_ms, "latency_ms": t.latency_ms}
+
+
+def _submit(matrix, settings, X0=None, per_sp=True):
+    c = settings._c()
+    total = _total_after_check(c, matrix.n)
+    X0p = None
+    if c.start == START_EXPLICIT:
+        X0 = np.asfortranarray(X0, dtype=np.float32)
+        if X0.shape != (matrix.n, total):
+            raise ValueError(f"X0 must be {matrix.n} x {total}")
+        X0p = X0.ctypes.data_as(ctypes.c_void_p)
+    h = ctypes.c_uint64()
+    _check(matrix.ctx.ptr, lib().spca_submit(matrix.ctx.ptr, matrix.handle, ctypes.byref(c), X0p, int(per_sp),
+                                             ctypes.byref(h)))
+    return Job(matrix, h.value, total)
+
+
+Matrix.submit = _submit
 
 
 def _total_after_check(c, n):

@@ -186,6 +186,35 @@ int spca_dense_solver_device(spca_ctx ctx, unsigned m, unsigned n, const float* 
 int spca_dense_solver_host(spca_ctx ctx, const float* B, int ldB, float* x, unsigned m, unsigned n,
                            const spca_settings* settings, spca_stats* stats, double* fval_out);
 
+/* ---- request server (continuous batching) ---------------------------------- */
+/* No reference counterpart: the reference solves one request per call.  A server runs up to
+ * `workers` jobs at once, each on its own HIP stream with its own workspace; queued jobs are
+ * admitted first-in first-out as soon as a worker is free and their workspace fits the HBM
+ * budget (matrices stay resident and are shared by handle).  Results are independent of
+ * what runs beside a job. */
+#define SPCA_JOB_QUEUED 0
+#define SPCA_JOB_RUNNING 1
+#define SPCA_JOB_DONE 2
+#define SPCA_JOB_CANCELLED 3
+
+typedef struct spca_job_times {
+    double queue_ms;          /* submit -> start */
+    double run_ms;            /* start -> finish */
+    double latency_ms;        /* submit -> finish */
+} spca_job_times;
+
+int spca_server_start(spca_ctx ctx, int workers);
+int spca_server_stop(spca_ctx ctx);  /* cancels queued jobs, waits for running ones */
+/* X0 is copied at submission; want_per_sp asks for per-SP results in spca_wait. */
+int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, int want_per_sp,
+                spca_job* out);
+int spca_poll(spca_ctx ctx, spca_job job, int* state);  /* SPCA_JOB_* */
+int spca_cancel(spca_ctx ctx, spca_job job);
+/* Blocks until the job ends, copies its results (x_out: n floats; per_sp: total_sp entries)
+ * and retires the handle.  Returns the job's status (SPCA_ECANCELLED if cancelled). */
+int spca_wait(spca_ctx ctx, spca_job job, float* x_out, spca_stats* stats, spca_sp_result* per_sp,
+              spca_job_times* times);
+
 /* FP64 re-solve restricted to the support of x (its nonzeros), on the GPU: the unit vector y
  * on that support maximising |A y|_2 (L2-variance formulations: power iteration on
  * A_S^T A_S) or |A y|_1 (L1-variance formulations: the AM sign iteration restricted to S),

@@ -382,3 +382,48 @@ def test_errors_are_reported(ctx, small_mat, small):
         with pytest.raises(S.SpcaError) as e:
             c2.upload(small[0])
         assert e.value.code == S.SPCA_EBUDGET
+
+
+def test_serving_alone_equals_mix(ctx):
+    """Continuous batching: jobs of different shapes, formulations and precisions submitted
+    while others run return bitwise the results they give when solved alone (own buffers,
+    deterministic kernels), and a queued job can be cancelled."""
+    rng = np.random.default_rng(5)
+    mats = []
+    for (m, n, seed) in ((200, 1000, 1), (300, 2500, 2), (128, 4096, 3)):
+        A, _ = spiked(m, n, seed=seed)
+        mats.append(ctx.upload(A, storage=S.STORE_FP32 | S.STORE_BF16X3, key=100 + seed))
+    specs = []
+    for k in range(12):
+        M = mats[k % 3]
+        f = int(rng.integers(0, 8))
+        B = [32, 64, 128][k % 3]
+        prec = [S.PREC_FP32, S.PREC_BF16X3][k % 2]
+        st = S.OptimizationSettings(formulation=f, constraintParameter=10, penaltyParameter=default_gamma(f, M.m),
+                                    tolerance=1e-5, maximumIterations=40, totalStartingPoints=B, batchSize=B,
+                                    precision=prec, useOTF=(k % 4 == 3))
+        specs.append((M, st, unit_gaussian_starts(M.n, B, seed=k)))
+    alone = []
+    for M, st, X0 in specs:
+        stats = S.OptimizationStatistics()
+        x = M.solve(st, X0=X0, stats=stats)
+        alone.append((x, stats.values.copy(), stats.fval))
+    ctx.start_server(4)
+    try:
+        jobs = [M.submit(st, X0=X0) for M, st, X0 in specs]
+        # one extra job, cancelled while it may still be queued
+        extra = specs[0][0].submit(specs[0][1], X0=specs[0][2])
+        extra.cancel()
+        for j, (xa, va, fa) in zip(jobs, alone):
+            stats = S.OptimizationStatistics()
+            x, times = j.wait(stats=stats)
+            assert times["latency_ms"] >= times["run_ms"] >= 0
+            assert np.array_equal(x.view(np.uint32), xa.view(np.uint32))
+            assert np.array_equal(stats.values.view(np.uint32), va.view(np.uint32)) and stats.fval == fa
+        with pytest.raises(S.SpcaError) as e:
+            extra.wait()
+        assert e.value.code == S.SPCA_ECANCELLED
+    finally:
+        ctx.stop_server()
+        for M in mats:
+            M.release()
