@@ -283,9 +283,8 @@ int make_matrix(spca_ctx ctx, const float* src, bool src_on_device, int m, int n
     return SPCA_OK;
 }
 
-void free_workspace(spca_ctx ctx, Engine& E) {
-    Workspace& w = E.ws;
-    (void)hipStreamSynchronize(E.stream);
+void free_ws(spca_ctx ctx, Workspace& w, hipStream_t s) {
+    (void)hipStreamSynchronize(s);
     void* ptrs[] = {w.X, w.Xhi, w.Xlo, w.Zpart, w.Z, w.Zhi, w.Zlo, w.V, w.fstate, w.istate, w.ev};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -295,6 +294,7 @@ void free_workspace(spca_ctx ctx, Engine& E) {
     }
     w = Workspace();
 }
+void free_workspace(spca_ctx ctx, Engine& E) { free_ws(ctx, E.ws, E.stream); }
 
 // device bytes a solve of this shape holds in its workspace
 uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
@@ -303,10 +303,10 @@ uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
            ((uint64_t)3 * bp + 4) * 4 + (uint64_t)bp * 8;
 }
 
-int ensure_workspace(spca_ctx ctx, Engine& E, int np, int mp, int bp, int splits, std::string* err = nullptr) {
-    Workspace& w = E.ws;
+int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp, int splits,
+              std::string* err = nullptr) {
     if (w.X && w.np == np && w.mp == mp && w.bp == bp && w.splits >= splits) return SPCA_OK;
-    free_workspace(ctx, E);
+    free_ws(ctx, w, s);
     w.np = np;
     w.mp = mp;
     w.bp = bp;
@@ -320,16 +320,19 @@ int ensure_workspace(spca_ctx ctx, Engine& E, int np, int mp, int bp, int splits
         (rc = dev_alloc(ctx, &w.fstate, (size_t)4 * bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + 4, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err))) {
-        free_workspace(ctx, E);
+        free_ws(ctx, w, s);
         return rc;
     }
-    if (hipMemsetAsync(w.Z, 0, zs * sizeof(float), E.stream) != hipSuccess ||
-        hipMemsetAsync(w.Zhi, 0, zs * sizeof(uint16_t), E.stream) != hipSuccess ||
-        hipMemsetAsync(w.Zlo, 0, zs * sizeof(uint16_t), E.stream) != hipSuccess) {
-        free_workspace(ctx, E);
+    if (hipMemsetAsync(w.Z, 0, zs * sizeof(float), s) != hipSuccess ||
+        hipMemsetAsync(w.Zhi, 0, zs * sizeof(uint16_t), s) != hipSuccess ||
+        hipMemsetAsync(w.Zlo, 0, zs * sizeof(uint16_t), s) != hipSuccess) {
+        free_ws(ctx, w, s);
         return SPCA_EDEVICE;
     }
     return SPCA_OK;
+}
+int ensure_workspace(spca_ctx ctx, Engine& E, int np, int mp, int bp, int splits, std::string* err = nullptr) {
+    return ensure_ws(ctx, E.ws, E.stream, np, mp, bp, splits, err);
 }
 
 int engine_init(Engine& E) {
@@ -399,13 +402,154 @@ void reference_starting_point(float* out, unsigned n, unsigned s, bool constrain
 
 // K-splits of A*X: enough workgroups to fill 256 CUs for a couple of rounds (128x128
 // tiles run ~3 per CU, 256x256 tiles one per CU), each split at least 512 deep.
-int choose_splits(int precision, int mp, int np, int bp) {
-    const bool big = gemm_big_tiles(precision, mp, bp);
+int choose_splits(int precision, int mp, int np, int bp, bool allow_big = true) {
+    const bool big = allow_big && gemm_big_tiles(precision, mp, bp);
     const int tiles = big ? (mp / 256) * (bp / 256) : (mp / kGemmBM) * (bp / kGemmBN);
     const int target = big ? 512 : 2048;
     int splits = 1;
     while (tiles * splits < target && np % (kGemmBK * splits * 2) == 0 && np / (splits * 2) >= 512) splits *= 2;
     return splits;
+}
+
+Operand x_operand(Workspace& w, int precision) {
+    return Operand{w.X, precision ? w.Xhi : nullptr, precision == 1 ? w.Xlo : nullptr};
+}
+Operand z_operand(Workspace& w, int precision) {
+    return Operand{w.Z, precision ? w.Zhi : nullptr, precision == 1 ? w.Zlo : nullptr};
+}
+
+// Starting points with global indices shift .. shift+cnt-1 (n floats each) into out.
+void make_starts(const spca_settings& st, int n, const float* X0, int shift, int cnt, float* out) {
+    const bool constrained = st.formulation < 4;
+    for (int jj = 0; jj < cnt; jj++) {
+        float* col = out + (size_t)jj * n;
+        if (st.start == SPCA_START_EXPLICIT)
+            std::memcpy(col, X0 + (size_t)(shift + jj) * n, sizeof(float) * n);
+        else if (st.start == SPCA_START_REFERENCE_F64)
+            reference_starting_point<double>(col, (unsigned)n, st.constraint, constrained, (unsigned)(shift + jj));
+        else
+            reference_starting_point<float>(col, (unsigned)n, st.constraint, constrained, (unsigned)(shift + jj));
+    }
+}
+
+// Start of a batch: X <- the starting points in hX (padded rows and columns zero), its
+// operand forms, and the per-SP state reset (val = err = 0, iters = nnz = 0,
+// active = (j < batch), done = 0).  Returns after the copies have completed.
+int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& st, int n, int batch,
+               const float* hX) {
+    const Operand xop = x_operand(w, st.precision);
+    const SpState sp = state_of(w);
+    HIP_TRY(ctx, hipMemsetAsync(w.X, 0, sizeof(float) * (size_t)w.np * w.bp, s));
+    HIP_TRY(ctx, hipMemcpy2DAsync(w.X, sizeof(float) * w.np, hX, sizeof(float) * n, sizeof(float) * n, batch,
+                                  hipMemcpyHostToDevice, s));
+    if (st.precision) {
+        HIP_TRY(ctx, hipMemsetAsync(w.Xhi, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
+        HIP_TRY(ctx, hipMemsetAsync(w.Xlo, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
+        HIP_TRY(ctx, launch_x_operand(xop, st.precision, w.np, batch, s));
+    }
+    HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
+    HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 4), s));
+    std::vector<int> act(w.bp, 0);
+    std::fill(act.begin(), act.begin() + batch, 1);
+    HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return SPCA_OK;
+}
+
+// What a solve accumulates over its batches.
+struct Results {
+    double best = -1;
+    int best_sp = -1;
+    unsigned total_it = 0;
+    unsigned long long sp_its = 0;
+    double last_err = 0;
+    double ev_best = 0;
+    std::vector<float> x_best;
+};
+
+// End of a batch: per-SP values, iterations, nonzeros and FP64 explained variance, the
+// final iterates if asked, and the best of the batch (first maximum, as
+// sparse_PCA_solver.h:212-231) against the best so far.
+int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const spca_settings& st, int shift, int batch,
+                  bool want_ev, spca_sp_result* per_sp, float* X_final, Results& R) {
+    const SpState sp = state_of(w);
+    const int n = M->n;
+    std::vector<float> hval(batch), herr(batch);
+    std::vector<int> hit(batch), hnnz(batch);
+    std::vector<double> hev(batch, 0.0);
+    int done_h[2];
+    float emax;
+    HIP_TRY(ctx, hipMemcpyAsync(done_h, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(hval.data(), sp.val, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(herr.data(), sp.err, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(hit.data(), sp.iters, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(hnnz.data(), sp.nnz, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(&emax, sp.errmax, sizeof(float), hipMemcpyDeviceToHost, s));
+    if (want_ev) {
+        HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, M->m, n, w.X, w.np, batch, spca_is_l1_variance(st.formulation),
+                                    w.ev, s));
+        HIP_TRY(ctx, hipMemcpyAsync(hev.data(), w.ev, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    R.total_it += (unsigned)done_h[1];
+    R.last_err = emax;
+    for (int jj = 0; jj < batch; jj++) R.sp_its += (unsigned long long)hit[jj];
+    int sel = 0;
+    float bv = hval[0];
+    for (int jj = 1; jj < batch; jj++)
+        if (hval[jj] > bv) {
+            bv = hval[jj];
+            sel = jj;
+        }
+    if (per_sp) {
+        for (int jj = 0; jj < batch; jj++) {
+            spca_sp_result& r = per_sp[shift + jj];
+            r.value = hval[jj];
+            r.iters = hit[jj];
+            r.nnz = hnnz[jj];
+            r.converged = (double)herr[jj] < st.tolerance;
+            r.ev_fp64 = hev[jj];
+        }
+    }
+    if (X_final)
+        HIP_TRY(ctx, hipMemcpy2DAsync(X_final + (size_t)shift * n, sizeof(float) * n, w.X, sizeof(float) * w.np,
+                                      sizeof(float) * n, batch, hipMemcpyDeviceToHost, s));
+    if (R.best < (double)bv || R.best_sp < 0) {
+        R.best = bv;
+        R.best_sp = shift + sel;
+        R.x_best.assign(n, 0.f);
+        HIP_TRY(ctx, hipMemcpyAsync(R.x_best.data(), w.X + (size_t)sel * w.np, sizeof(float) * n,
+                                    hipMemcpyDeviceToHost, s));
+        R.ev_best = hev[sel];
+    }
+    HIP_TRY(ctx, hipStreamSynchronize(s));
+    return SPCA_OK;
+}
+
+// final x <- x/|x|_2 (sparse_PCA_solver.h:236-237) and the statistics
+void finish_solve(const Results& R, int n, const spca_settings& st, float* x_out, spca_stats* stats, double loop_ms,
+                  const double* phase_ms, std::chrono::steady_clock::time_point t_wall0) {
+    double nx = 0;
+    for (int i = 0; i < n; i++) nx += (double)R.x_best[i] * (double)R.x_best[i];
+    const float inv = 1.0f / (float)std::sqrt(nx);
+    for (int i = 0; i < n; i++) x_out[i] = R.x_best[i] * inv;
+    if (stats) {
+        stats->it = R.total_it;
+        stats->fval = R.best;
+        stats->error = R.last_err;
+        stats->loop_ms = loop_ms;
+        stats->wall_ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall0).count();
+        stats->n = (unsigned)n;
+        stats->best_sp = R.best_sp;
+        stats->total_sp = st.total_sp;
+        stats->constraint = st.constraint;
+        stats->gemm_ax_ms = phase_ms ? phase_ms[0] : 0.0;
+        stats->gemm_atz_ms = phase_ms ? phase_ms[1] : 0.0;
+        stats->step_ms = phase_ms ? phase_ms[2] : 0.0;
+        stats->sp_iterations = R.sp_its;
+        stats->ev_fp64 = R.ev_best;
+    }
 }
 
 // Kernel time per phase (0: A*X, 1: A^T*Z, 2: epilogue/step/convergence), from
@@ -438,9 +582,10 @@ struct PhaseTimer {
     }
 };
 
-int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, const float* X0, float* x_out,
-               spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
-    const auto t_wall0 = std::chrono::steady_clock::now();
+// Checks a request and normalises its settings (clamp of s, batch rounding); makes sure the
+// matrix holds the storage the precision needs.  Returns the number of batches in *nb.
+int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float* X0, const float* x_out,
+                     spca_settings* st_out, int* nb) {
     spca_settings st = *st_in;
     if (!valid_formulation(st.formulation)) return fail(ctx, SPCA_EINVAL, "formulation must be 0..7");
     if (st.precision < 0 || st.precision > 2) return fail(ctx, SPCA_EINVAL, "precision must be 0..2");
@@ -452,9 +597,20 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     const unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
     int rc = add_storage(ctx, M, need);
     if (rc) return rc;
-    const int n = M->n, m = M->m;
-    const int total_batches = spca_settings_check(&st, (unsigned)n);
+    *nb = spca_settings_check(&st, (unsigned)M->n);
     if (st.constraint < 1) st.constraint = 1;
+    *st_out = st;
+    return SPCA_OK;
+}
+
+int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, const float* X0, float* x_out,
+               spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
+    const auto t_wall0 = std::chrono::steady_clock::now();
+    spca_settings st;
+    int total_batches = 0;
+    int rc = prepare_settings(ctx, M, st_in, X0, x_out, &st, &total_batches);
+    if (rc) return rc;
+    const int n = M->n, m = M->m;
     const int batch = st.batch;
     const int bp = round_up(batch, kPadB);
     const int splits = choose_splits(st.precision, M->mp, M->np, bp);
@@ -462,11 +618,10 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     Workspace& w = E.ws;
     hipStream_t s = E.stream;
     SpState sp = state_of(w);
-    const bool constrained = st.formulation < 4;
     const int check_every = st.check_every > 0 ? st.check_every : 1;
 
-    Operand xop{w.X, st.precision ? w.Xhi : nullptr, st.precision == 1 ? w.Xlo : nullptr};
-    Operand zop{w.Z, st.precision ? w.Zhi : nullptr, st.precision == 1 ? w.Zlo : nullptr};
+    const Operand xop = x_operand(w, st.precision);
+    const Operand zop = z_operand(w, st.precision);
     const float* P1_32 = M->a32_rm;
     const uint16_t* P1_hi = st.precision == 1 ? M->hi_rm : M->h_rm;
     const uint16_t* P1_lo = M->lo_rm;
@@ -485,16 +640,9 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     sa.batch = batch;
 
     std::vector<float> hX((size_t)n * batch);
-    std::vector<float> hval(batch), herr(batch);
-    std::vector<int> hit(batch), hnnz(batch);
-    std::vector<double> hev(batch);
-    double best = -1;
-    int best_sp = -1;
-    unsigned total_it = 0;
-    unsigned long long sp_its = 0;
-    double loop_ms = 0, last_err = 0;
+    Results R;
+    double loop_ms = 0;
     PhaseTimer pt{&E};
-    std::vector<float> x_best(n, 0.f);
     hipEvent_t* loop_ev = E.loop_ev;
 
     if (st.use_otf) {
@@ -533,15 +681,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         // pool of all starting points, padded rows zero
         {
             std::vector<float> hp((size_t)n * total);
-            for (int k = 0; k < total; k++) {
-                float* col = hp.data() + (size_t)k * n;
-                if (st.start == SPCA_START_EXPLICIT)
-                    std::memcpy(col, X0 + (size_t)k * n, sizeof(float) * n);
-                else if (st.start == SPCA_START_REFERENCE_F64)
-                    reference_starting_point<double>(col, (unsigned)n, st.constraint, constrained, (unsigned)k);
-                else
-                    reference_starting_point<float>(col, (unsigned)n, st.constraint, constrained, (unsigned)k);
-            }
+            make_starts(st, n, X0, 0, total, hp.data());
             HIP_TRY(ctx, hipMemsetAsync(pool, 0, sizeof(float) * (size_t)w.np * total, s));
             HIP_TRY(ctx, hipMemcpy2DAsync(pool, sizeof(float) * w.np, hp.data(), sizeof(float) * n, sizeof(float) * n,
                                           total, hipMemcpyHostToDevice, s));
@@ -623,7 +763,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         HIP_TRY(ctx, hipMemcpyAsync(spv.data(), o.sp_val, sizeof(float) * total, hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipMemcpyAsync(spi.data(), o.sp_iters, sizeof(int) * total, hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipMemcpyAsync(spn.data(), o.sp_nnz, sizeof(int) * total, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(x_best.data(), xbest, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        R.x_best.assign(n, 0.f);
+        HIP_TRY(ctx, hipMemcpyAsync(R.x_best.data(), xbest, sizeof(float) * n, hipMemcpyDeviceToHost, s));
         double evb = 0;
         HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, m, n, xbest, w.np, 1, spca_is_l1_variance(st.formulation), w.ev, s));
         HIP_TRY(ctx, hipMemcpyAsync(&evb, w.ev, sizeof(double), hipMemcpyDeviceToHost, s));
@@ -631,11 +772,13 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         HIP_TRY(ctx, hipMemcpyAsync(&emax, sp.errmax, sizeof(float), hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipStreamSynchronize(s));
         release();
-        total_it = (unsigned)done_h[1];
-        best = bestv;
-        best_sp = gen_h[2];
-        last_err = emax;
-        for (int k = 0; k < total; k++) sp_its += (unsigned long long)spi[k];
+        R.total_it = (unsigned)done_h[1];
+        R.best = bestv;
+        R.best_sp = gen_h[2];
+        R.last_err = emax;
+        R.ev_best = evb;
+        const int best_sp = R.best_sp;
+        for (int k = 0; k < total; k++) R.sp_its += (unsigned long long)spi[k];
         if (per_sp)
             for (int k = 0; k < total; k++) {
                 per_sp[k].value = spv[k];
@@ -644,38 +787,12 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
                 per_sp[k].converged = spi[k] < st.max_iter;
                 per_sp[k].ev_fp64 = (k == best_sp) ? evb : std::nan("");
             }
-        if (stats) stats->ev_fp64 = evb;
         (void)X_final;  // slots are reused: final iterates per SP are not kept in OTF mode
     } else
     for (int bt = 0; bt < total_batches; bt++) {
         const int shift = bt * batch;
-        // ---- starting points
-        for (int jj = 0; jj < batch; jj++) {
-            float* col = hX.data() + (size_t)jj * n;
-            if (st.start == SPCA_START_EXPLICIT)
-                std::memcpy(col, X0 + (size_t)(shift + jj) * n, sizeof(float) * n);
-            else if (st.start == SPCA_START_REFERENCE_F64)
-                reference_starting_point<double>(col, (unsigned)n, st.constraint, constrained, (unsigned)(shift + jj));
-            else
-                reference_starting_point<float>(col, (unsigned)n, st.constraint, constrained, (unsigned)(shift + jj));
-        }
-        HIP_TRY(ctx, hipMemsetAsync(w.X, 0, sizeof(float) * (size_t)w.np * w.bp, s));
-        HIP_TRY(ctx, hipMemcpy2DAsync(w.X, sizeof(float) * w.np, hX.data(), sizeof(float) * n, sizeof(float) * n,
-                                      batch, hipMemcpyHostToDevice, s));
-        if (st.precision) {
-            HIP_TRY(ctx, hipMemsetAsync(w.Xhi, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
-            HIP_TRY(ctx, hipMemsetAsync(w.Xlo, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
-            HIP_TRY(ctx, launch_x_operand(xop, st.precision, w.np, batch, s));
-        }
-        // ---- per-SP state: val = err = 0, iters = nnz = 0, active = (j < batch), done = 0
-        HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
-        HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 4), s));
-        {
-            std::vector<int> act(w.bp, 0);
-            std::fill(act.begin(), act.begin() + batch, 1);
-            HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
-            HIP_TRY(ctx, hipStreamSynchronize(s));
-        }
+        make_starts(st, n, X0, shift, batch, hX.data());
+        if ((rc = load_batch(ctx, s, w, st, n, batch, hX.data()))) return rc;
         HIP_TRY(ctx, hipEventRecord(loop_ev[0], s));
         for (int it = 0; it < st.max_iter; it++) {
             hipEvent_t e0, e1;
@@ -716,77 +833,11 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventElapsedTime(&t, loop_ev[0], loop_ev[1]));
             loop_ms += t;
         }
-        // ---- batch results
-        int done_h[2];
-        float emax;
-        HIP_TRY(ctx, hipMemcpyAsync(done_h, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(hval.data(), sp.val, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(herr.data(), sp.err, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(hit.data(), sp.iters, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(hnnz.data(), sp.nnz, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipMemcpyAsync(&emax, sp.errmax, sizeof(float), hipMemcpyDeviceToHost, s));
-        HIP_TRY(ctx, hipStreamSynchronize(s));
-        total_it += (unsigned)done_h[1];
-        last_err = emax;
-        for (int jj = 0; jj < batch; jj++) sp_its += (unsigned long long)hit[jj];
-        // best of batch: first maximum, as sparse_PCA_solver.h:212-231
-        int sel = 0;
-        float bv = hval[0];
-        for (int jj = 1; jj < batch; jj++)
-            if (hval[jj] > bv) {
-                bv = hval[jj];
-                sel = jj;
-            }
-        const bool want_ev = per_sp != nullptr || stats != nullptr;
-        if (want_ev) {
-            HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, m, n, w.X, w.np, batch, spca_is_l1_variance(st.formulation),
-                                        w.ev, s));
-            HIP_TRY(ctx, hipMemcpyAsync(hev.data(), w.ev, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
-            HIP_TRY(ctx, hipStreamSynchronize(s));
-        }
-        if (per_sp) {
-            for (int jj = 0; jj < batch; jj++) {
-                spca_sp_result& r = per_sp[shift + jj];
-                r.value = hval[jj];
-                r.iters = hit[jj];
-                r.nnz = hnnz[jj];
-                r.converged = (double)herr[jj] < st.tolerance;
-                r.ev_fp64 = hev[jj];
-            }
-        }
-        if (X_final)
-            HIP_TRY(ctx, hipMemcpy2DAsync(X_final + (size_t)shift * n, sizeof(float) * n, w.X, sizeof(float) * w.np,
-                                          sizeof(float) * n, batch, hipMemcpyDeviceToHost, s));
-        if (best < (double)bv || best_sp < 0) {
-            best = bv;
-            best_sp = shift + sel;
-            HIP_TRY(ctx, hipMemcpyAsync(x_best.data(), w.X + (size_t)sel * w.np, sizeof(float) * n,
-                                        hipMemcpyDeviceToHost, s));
-            if (stats) stats->ev_fp64 = hev[sel];
-        }
-        HIP_TRY(ctx, hipStreamSynchronize(s));
+        if ((rc = collect_batch(ctx, s, M, w, st, shift, batch, per_sp != nullptr || stats != nullptr, per_sp, X_final,
+                                R)))
+            return rc;
     }
-    // final x <- x/|x|_2 (sparse_PCA_solver.h:236-237)
-    double nx = 0;
-    for (int i = 0; i < n; i++) nx += (double)x_best[i] * (double)x_best[i];
-    const float inv = 1.0f / (float)std::sqrt(nx);
-    for (int i = 0; i < n; i++) x_out[i] = x_best[i] * inv;
-    if (stats) {
-        stats->it = total_it;
-        stats->fval = best;
-        stats->error = last_err;
-        stats->loop_ms = loop_ms;
-        stats->wall_ms =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_wall0).count();
-        stats->n = (unsigned)n;
-        stats->best_sp = best_sp;
-        stats->total_sp = st.total_sp;
-        stats->constraint = st.constraint;
-        stats->gemm_ax_ms = pt.ms[0];
-        stats->gemm_atz_ms = pt.ms[1];
-        stats->step_ms = pt.ms[2];
-        stats->sp_iterations = sp_its;
-    }
+    finish_solve(R, n, st, x_out, stats, loop_ms, pt.ms, t_wall0);
     return SPCA_OK;
 }
 
@@ -869,6 +920,196 @@ int solve_and_resolve(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* s
     return rc;
 }
 
+
+// ---------------------------------------------------------------- fused multi-job solve
+// Jobs of different shapes and formulations iterate in lockstep: each AM iteration of all
+// of them is five launches (launch_fused_iteration), and a chunk of check_every
+// iterations is captured once into a hipGraph and replayed.  Each job keeps its own
+// workspace, state and termination; a finished job's work items return at once, and the
+// host stops when every job has finished.  Per job the tile order, K split and kernel
+// bodies are those of a solo solve with 128x128 tiles, so FP32 results equal a solo
+// solve's bit for bit.
+struct FusedRun {
+    Matrix* M = nullptr;
+    spca_settings st{};
+    Workspace ws;
+    int splits = 0;
+};
+
+int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigned flags) {
+    const auto t_wall0 = std::chrono::steady_clock::now();
+    if (!jobs || njobs < 1) return fail(ctx, SPCA_EINVAL, "fused solve needs at least one job");
+    hipStream_t s = E.stream;
+    std::vector<FusedRun> run(njobs);
+    for (int k = 0; k < njobs; k++) jobs[k].status = SPCA_EAGAIN;
+    FusedJob* d_jobs = nullptr;
+    int* d_done = nullptr;
+    uint64_t tb = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<hipEvent_t> chunk_ev;
+    auto cleanup = [&](int code) {
+        (void)hipStreamSynchronize(s);
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        for (hipEvent_t e : chunk_ev) (void)hipEventDestroy(e);
+        dev_free(ctx, d_jobs, sizeof(FusedJob) * (uint64_t)njobs);
+        dev_free(ctx, d_done, sizeof(int) * 4);
+        for (auto& r : run) free_ws(ctx, r.ws, s);
+        for (int k = 0; k < njobs; k++)
+            if (jobs[k].status == SPCA_EAGAIN) jobs[k].status = code;
+        return code;
+    };
+    // ---- jobs: settings, matrices (held for the call), plans, workspaces, starting points
+    int rc;
+    int precision = -1, check_every = 1 << 30, max_iter = 0;
+    for (int k = 0; k < njobs; k++) {
+        spca_fused_job& J = jobs[k];
+        FusedRun& r = run[k];
+        auto it = ctx->mats.find(J.matrix);  // the caller holds ctx->mu
+        if (it != ctx->mats.end()) r.M = it->second;
+        if (J.stats) std::memset(J.stats, 0, sizeof(*J.stats));
+        if (!r.M || !J.settings) {
+            J.status = SPCA_EINVAL;
+            return cleanup(fail(ctx, SPCA_EINVAL, "fused job " + std::to_string(k) + ": bad matrix or settings"));
+        }
+        int nb = 0;
+        if ((rc = prepare_settings(ctx, r.M, J.settings, J.X0, J.x_out, &r.st, &nb))) {
+            J.status = rc;
+            return cleanup(rc);
+        }
+        if (nb != 1 || r.st.use_otf) {
+            J.status = SPCA_EINVAL;
+            return cleanup(fail(ctx, SPCA_EINVAL, "fused job " + std::to_string(k) +
+                                                      ": fused jobs run one batch (total_sp <= batch) without OTF"));
+        }
+        if (precision >= 0 && r.st.precision != precision) {
+            J.status = SPCA_EINVAL;
+            return cleanup(fail(ctx, SPCA_EINVAL, "fused jobs must share one precision"));
+        }
+        precision = r.st.precision;
+        check_every = std::min(check_every, std::max(1, r.st.check_every));
+        max_iter = std::max(max_iter, r.st.max_iter);
+        const int bp = round_up(r.st.batch, kPadB);
+        r.splits = choose_splits(precision, r.M->mp, r.M->np, bp, false);
+        if ((rc = ensure_ws(ctx, r.ws, s, r.M->np, r.M->mp, bp, r.splits))) {
+            J.status = rc;
+            return cleanup(rc);
+        }
+        std::vector<float> hX((size_t)r.M->n * r.st.batch);
+        make_starts(r.st, r.M->n, J.X0, 0, r.st.batch, hX.data());
+        if ((rc = load_batch(ctx, s, r.ws, r.st, r.M->n, r.st.batch, hX.data()))) return cleanup(rc);
+    }
+    // ---- job table
+    std::vector<FusedJob> tab(njobs);
+    int t1 = 0, t2 = 0, c0 = 0;
+    for (int k = 0; k < njobs; k++) {
+        FusedRun& r = run[k];
+        Matrix* M = r.M;
+        FusedJob& f = tab[k];
+        f.p1_32 = M->a32_rm;
+        f.p1_hi = precision == 1 ? M->hi_rm : M->h_rm;
+        f.p1_lo = M->lo_rm;
+        f.p2_32 = M->a32_cm;
+        f.p2_hi = precision == 1 ? M->hi_cm : M->h_cm;
+        f.p2_lo = M->lo_cm;
+        f.Zpart = r.ws.Zpart;
+        f.V = r.ws.V;
+        f.xop = x_operand(r.ws, precision);
+        f.zop = z_operand(r.ws, precision);
+        f.sp = state_of(r.ws);
+        f.sa.formulation = r.st.formulation;
+        f.sa.precision = precision;
+        f.sa.n = M->n;
+        f.sa.ldv = M->np;
+        f.sa.s = r.st.constraint;
+        f.sa.gamma = r.st.penalty;
+        f.sa.batch = r.st.batch;
+        f.mp = M->mp;
+        f.np = M->np;
+        f.bp = r.ws.bp;
+        f.splits = r.splits;
+        f.zsplit = (long)M->mp * f.bp;
+        f.tol = r.st.tolerance;
+        f.max_iter = r.st.max_iter;
+        f.per_sp = r.st.termination == SPCA_TERM_PER_SP;
+        f.t1 = t1;
+        f.t2 = t2;
+        f.c0 = c0;
+        t1 += (f.mp / kGemmBM) * (f.bp / kGemmBN) * f.splits;
+        t2 += (f.np / kGemmBM) * (f.bp / kGemmBN);
+        c0 += r.st.batch;
+    }
+    if ((rc = dev_alloc(ctx, &d_jobs, (size_t)njobs, &tb)) || (rc = dev_alloc(ctx, &d_done, 4, &tb)))
+        return cleanup(rc);
+    if (hipMemcpyAsync(d_jobs, tab.data(), sizeof(FusedJob) * njobs, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemsetAsync(d_done, 0, sizeof(int) * 4, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+        return cleanup(fail(ctx, SPCA_EDEVICE, "fused job table upload failed"));
+    const int chunk = std::min(check_every, max_iter);
+    if (flags & SPCA_FUSED_GRAPH) {
+        hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
+        for (int i = 0; i < chunk && e == hipSuccess; i++)
+            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, d_done, s);
+        hipError_t e2 = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess) e = e2;
+        if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess)
+            return cleanup(fail(ctx, SPCA_EDEVICE, std::string("fused iteration graph: ") + hipGetErrorString(e)));
+    }
+    // ---- iterate in chunks until every job has finished
+    const int max_chunks = (max_iter + chunk - 1) / chunk + 1;
+    auto new_event = [&](hipEvent_t* ev) {
+        if (hipEventCreate(ev) != hipSuccess) return false;
+        chunk_ev.push_back(*ev);
+        return true;
+    };
+    hipEvent_t ev0;
+    if (!new_event(&ev0) || hipEventRecord(ev0, s) != hipSuccess)
+        return cleanup(fail(ctx, SPCA_EDEVICE, "event record failed"));
+    int chunks = 0;
+    for (; chunks < max_chunks; chunks++) {
+        hipError_t e = hipSuccess;
+        if (gexec)
+            e = hipGraphLaunch(gexec, s);
+        else
+            for (int i = 0; i < chunk && e == hipSuccess; i++)
+                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, d_done, s);
+        hipEvent_t ev;
+        if (e == hipSuccess && !new_event(&ev)) e = hipErrorOutOfMemory;
+        if (e == hipSuccess) e = hipEventRecord(ev, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(E.h_done, d_done, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return cleanup(fail(ctx, SPCA_EDEVICE, std::string("fused iteration: ") + hipGetErrorString(e)));
+        if (E.h_done[0] >= njobs) {
+            chunks++;
+            break;
+        }
+    }
+    if (E.h_done[0] < njobs) return cleanup(fail(ctx, SPCA_EDEVICE, "fused jobs did not finish"));
+    // ---- per-job results
+    for (int k = 0; k < njobs; k++) {
+        spca_fused_job& J = jobs[k];
+        FusedRun& r = run[k];
+        Results R;
+        if ((rc = collect_batch(ctx, s, r.M, r.ws, r.st, 0, r.st.batch, J.per_sp != nullptr || J.stats != nullptr,
+                                J.per_sp, nullptr, R)))
+            return cleanup(rc);
+        // the chunk in which the job finished bounds its loop time
+        const int c = std::min(chunks, std::max(1, (int)((R.total_it + chunk - 1) / chunk)));
+        float t = 0;
+        (void)hipEventElapsedTime(&t, chunk_ev[0], chunk_ev[c]);
+        finish_solve(R, r.M->n, r.st, J.x_out, J.stats, t, nullptr, t_wall0);
+        if (J.stats && r.st.resolve_iters > 0) {
+            int its = 0;
+            if ((rc = resolve_impl(ctx, E, r.M, r.st.formulation, J.x_out, r.st.resolve_iters, r.st.resolve_tol,
+                                   nullptr, &J.stats->ev_resolve_fp64, &its)))
+                return cleanup(rc);
+            J.stats->resolve_iterations = its;
+        }
+        J.status = SPCA_OK;
+    }
+    return cleanup(SPCA_OK);
+}
 
 // ---------------------------------------------------------------- request server
 // Continuous batching: jobs wait in a FIFO queue; each worker thread owns an engine (its
@@ -1147,6 +1388,13 @@ int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, con
     return solve_and_resolve(ctx, ctx->eng, itm->second, settings, X0, x_out, stats, per_sp, X_final);
 }
 
+
+int spca_solve_fused(spca_ctx ctx, spca_fused_job* jobs, int njobs, unsigned flags) {
+    if (!ctx) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    return fused_impl(ctx, ctx->eng, jobs, njobs, flags);
+}
 
 int spca_server_start(spca_ctx ctx, int workers) {
     if (!ctx || workers < 1 || workers > 16) return SPCA_EINVAL;

@@ -68,6 +68,28 @@ struct StepArgs {
     int batch;           // true number of SPs in this batch
 };
 
+// One job of a fused multi-job iteration: every field the five per-iteration kernels of a
+// solo solve take, plus the job's first tile in each product's tile list and its first
+// column in the list of starting points.  Jobs of one fused launch share the precision.
+struct FusedJob {
+    const float* p1_32;     // A row major (mp x np, ld np): A*X operand
+    const uint16_t* p1_hi;
+    const uint16_t* p1_lo;
+    const float* p2_32;     // A column major (ld mp): A^T*Z operand
+    const uint16_t* p2_hi;
+    const uint16_t* p2_lo;
+    float* Zpart;           // splits x (mp x bp) partial products of A*X
+    float* V;               // np x bp
+    Operand xop, zop;
+    SpState sp;
+    StepArgs sa;
+    long zsplit;            // mp * bp
+    double tol;
+    int mp, np, bp, splits;
+    int t1, t2, c0;         // first A*X tile, first A^T*Z tile, first column of this job
+    int max_iter, per_sp;
+};
+
 // ---- launchers (spca_kernels.hip); every one enqueues on `st` and returns a hip status
 hipError_t launch_pad_copy_f32(const float* src, int m, int n, int lds, float* dst, int ldd, int mp,
                                int np, hipStream_t st);
@@ -104,6 +126,14 @@ hipError_t launch_otf_converge(SpState sp, OtfState o, int batch, double tol, in
                                hipStream_t st);
 hipError_t launch_otf_refill(SpState sp, OtfState o, Operand xop, int precision, int n, int ldx, int batch,
                              hipStream_t st);
+
+// One AM iteration of every job in `jobs` (device array, sorted by t1/t2/c0) as five
+// launches: A*X tiles of all jobs, the z epilogue of all columns, A^T*Z tiles, the
+// formulation step of all columns and the per-job convergence (a finished job adds 1 to
+// *all_done).  128x128 tiles; per job the same tile order, K split and kernel bodies as a
+// solo solve, so a job's FP32 results do not depend on the other jobs.
+hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
+                                  int* all_done, hipStream_t st);
 
 // Write the operand forms of X from its FP32 values (for starting points).
 hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st);

@@ -171,20 +171,10 @@ __device__ __forceinline__ int xcd_remap(int bid, int total) {
     return (total & 7) ? bid : (bid & 7) * (total >> 3) + (bid >> 3);
 }
 
-__global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restrict__ P, long ldp,
-                                                          const float* __restrict__ Q, long ldq,
-                                                          float* __restrict__ C, long ldc, long split_stride,
-                                                          int tiles_m, int tiles_n, int splits, int ksplit,
-                                                          const int* __restrict__ done) {
-    if (done && *done) return;
-    const int total = tiles_m * tiles_n * splits;
-    const int t = xcd_remap(blockIdx.x, total);
-    const int tn = t % tiles_n;
-    const int rest = t / tiles_n;
-    const int tm = rest % tiles_m;
-    const int sp = rest / tiles_m;
-    const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
-
+// One 128x128 FP32 output tile: C[m0.., n0..] (+ split offset) over K range [k0, k0+ksplit).
+__device__ __forceinline__ void gemm_f32_tile(const float* __restrict__ P, long ldp, const float* __restrict__ Q,
+                                              long ldq, float* __restrict__ C, long ldc, long split_stride, int sp,
+                                              long m0, long n0, long k0, int ksplit) {
     __shared__ __attribute__((aligned(16))) float Ps[kGemmBM][kGemmBK + 4];
     __shared__ __attribute__((aligned(16))) float Qs[kGemmBN][kGemmBK + 4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -246,27 +236,33 @@ __global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restric
         }
 }
 
-// 16-bit operand products.  PREC 1: bf16x3 (Phi*Qhi + Phi*Qlo + Plo*Qhi), PREC 2: fp16.
-// One 16x16x32 MFMA covers a whole 32-deep K stage; lane l supplies row l%16 and
-// k = 8*(l/16) .. +7 of each operand.
-template <int PREC>
-__global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restrict__ Phi,
-                                                         const uint16_t* __restrict__ Plo, long ldp,
-                                                         const uint16_t* __restrict__ Qhi,
-                                                         const uint16_t* __restrict__ Qlo, long ldq,
-                                                         float* __restrict__ C, long ldc, long split_stride,
-                                                         int tiles_m, int tiles_n, int splits, int ksplit,
-                                                         const int* __restrict__ done) {
+__global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restrict__ P, long ldp,
+                                                          const float* __restrict__ Q, long ldq,
+                                                          float* __restrict__ C, long ldc, long split_stride,
+                                                          int tiles_m, int tiles_n, int splits, int ksplit,
+                                                          const int* __restrict__ done) {
     if (done && *done) return;
-    constexpr int NT = (PREC == 1) ? 2 : 1;  // number of parts per operand
-    constexpr int LDK = kGemmBK + 8;         // 80-byte rows: 16-byte aligned
     const int total = tiles_m * tiles_n * splits;
     const int t = xcd_remap(blockIdx.x, total);
     const int tn = t % tiles_n;
     const int rest = t / tiles_n;
     const int tm = rest % tiles_m;
     const int sp = rest / tiles_m;
-    const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
+    gemm_f32_tile(P, ldp, Q, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM, (long)tn * kGemmBN,
+                  (long)sp * ksplit, ksplit);
+}
+
+// 16-bit operand products.  PREC 1: bf16x3 (Phi*Qhi + Phi*Qlo + Plo*Qhi), PREC 2: fp16.
+// One 16x16x32 MFMA covers a whole 32-deep K stage; lane l supplies row l%16 and
+// k = 8*(l/16) .. +7 of each operand.
+template <int PREC>
+__device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, const uint16_t* __restrict__ Plo,
+                                            long ldp, const uint16_t* __restrict__ Qhi,
+                                            const uint16_t* __restrict__ Qlo, long ldq, float* __restrict__ C,
+                                            long ldc, long split_stride, int sp, long m0, long n0, long k0,
+                                            int ksplit) {
+    constexpr int NT = (PREC == 1) ? 2 : 1;  // number of parts per operand
+    constexpr int LDK = kGemmBK + 8;         // 80-byte rows: 16-byte aligned
 
     __shared__ __attribute__((aligned(16))) uint16_t Ps[NT][kGemmBM][LDK];
     __shared__ __attribute__((aligned(16))) uint16_t Qs[NT][kGemmBN][LDK];
@@ -354,6 +350,25 @@ __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restr
             const long col = n0 + wn * 64 + j * 16 + (lane & 15);
             *(f4*)(Cb + row + col * ldc) = acc[i][j];
         }
+}
+
+template <int PREC>
+__global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restrict__ Phi,
+                                                         const uint16_t* __restrict__ Plo, long ldp,
+                                                         const uint16_t* __restrict__ Qhi,
+                                                         const uint16_t* __restrict__ Qlo, long ldq,
+                                                         float* __restrict__ C, long ldc, long split_stride,
+                                                         int tiles_m, int tiles_n, int splits, int ksplit,
+                                                         const int* __restrict__ done) {
+    if (done && *done) return;
+    const int total = tiles_m * tiles_n * splits;
+    const int t = xcd_remap(blockIdx.x, total);
+    const int tn = t % tiles_n;
+    const int rest = t / tiles_n;
+    const int tm = rest % tiles_m;
+    const int sp = rest / tiles_m;
+    gemm16_tile<PREC>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM,
+                      (long)tn * kGemmBN, (long)sp * ksplit, ksplit);
 }
 
 // Large-tile variant of the 16-bit products: 256x256 output tile per workgroup, 8 waves
@@ -670,10 +685,9 @@ __device__ __forceinline__ void write_operand(Operand op, int precision, long id
 // One workgroup per SP column.  z = sum of split partials (fixed order).  Constrained:
 // L1 variance f = |z|_1 then z = sgn z; L2 variance f = |z|_2 (z unchanged)
 // (c++/src/gpower/gpower_commons.h:75-101).  Penalised: z = sgn z or z/|z|_2 (:150-166).
-__global__ __launch_bounds__(256) void prep_z_kernel(const float* __restrict__ Zpart, int splits,
-                                                     long split_stride, int m, int ldz, Operand zop,
-                                                     int formulation, int precision, SpState sp) {
-    const int j = blockIdx.x;
+__device__ __forceinline__ void prep_z_body(int j, const float* __restrict__ Zpart, int splits, long split_stride,
+                                            int m, int ldz, Operand zop, int formulation, int precision,
+                                            SpState sp) {
     if (*sp.done || !sp.active[j]) return;
     __shared__ double red[4];
     const float* zc = Zpart + (long)j * ldz;
@@ -697,6 +711,11 @@ __global__ __launch_bounds__(256) void prep_z_kernel(const float* __restrict__ Z
         else if (!constrained) z = z * inv;
         write_operand(zop, precision, idx, z);
     }
+}
+__global__ __launch_bounds__(256) void prep_z_kernel(const float* __restrict__ Zpart, int splits,
+                                                     long split_stride, int m, int ldz, Operand zop,
+                                                     int formulation, int precision, SpState sp) {
+    prep_z_body(blockIdx.x, Zpart, splits, split_stride, m, ldz, zop, formulation, precision, sp);
 }
 hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int m, int ldz, Operand zop,
                          int formulation, int precision, int batch, SpState sp, hipStream_t st) {
@@ -793,9 +812,7 @@ __device__ void scan_hist(SelShared& sh, bool l1c, uint32_t rank, uint32_t cA, u
     if (lane == 0 && fmax < 0) sh.ires[0] = -1;
 }
 
-__global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restrict__ V, Operand xop, StepArgs a,
-                                                           SpState sp) {
-    const int j = blockIdx.x;
+__device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Operand xop, StepArgs a, SpState sp) {
     if (*sp.done || !sp.active[j]) return;
     __shared__ SelShared sh;
     const int tid = threadIdx.x;
@@ -1136,6 +1153,10 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
     }
 }
 
+__global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restrict__ V, Operand xop, StepArgs a,
+                                                           SpState sp) {
+    step_body(blockIdx.x, V, xop, a, sp);
+}
 hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st) {
     if (a.n <= 0 || (long)a.n * 4 > 0x7fffffffL) return hipErrorInvalidValue;
     step_kernel<<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp);
@@ -1146,9 +1167,10 @@ hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipS
 // Batch rule (gpu_sparse_PCA_solver.h:176-186, sparse_PCA_solver.h:203-207): error =
 // max_j err_j with NaN skipped by the '<' update; stop iff it > 0 and error < tol.
 // Per-SP rule: each active SP stops on its own error or at max_iter.
-__global__ __launch_bounds__(256) void converge_kernel(SpState sp, int batch, int it, double tol, int max_iter,
-                                                       int per_sp) {
+__device__ __forceinline__ void converge_body(SpState sp, int batch, int it, double tol, int max_iter, int per_sp,
+                                              int* all_done) {
     if (*sp.done) return;
+    if (it < 0) it = sp.done[1];  // fused launches read the iteration index from the device
     __shared__ float wmax[4];
     __shared__ int wany[4];
     float e = 0.f;
@@ -1183,12 +1205,104 @@ __global__ __launch_bounds__(256) void converge_kernel(SpState sp, int batch, in
         int stop;
         if (per_sp) stop = !a;
         else stop = it > 0 && (double)m < tol;
-        if (stop || it + 1 >= max_iter) sp.done[0] = 1;
+        if (stop || it + 1 >= max_iter) {
+            sp.done[0] = 1;
+            if (all_done) atomicAdd(all_done, 1);
+        }
     }
+}
+__global__ __launch_bounds__(256) void converge_kernel(SpState sp, int batch, int it, double tol, int max_iter,
+                                                       int per_sp) {
+    converge_body(sp, batch, it, tol, max_iter, per_sp, nullptr);
 }
 hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_iter, int per_sp, hipStream_t st) {
     converge_kernel<<<1, 256, 0, st>>>(sp, batch, it, tol, max_iter, per_sp);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- fused multi-job iteration
+// Each launch covers the work items of all jobs; a workgroup finds its job by binary search
+// over the jobs' first item (the job table is a few KB and stays in cache) and runs the same
+// body as the solo kernel on the job's own buffers.  Finished jobs return at once.
+template <int FIELD>
+__device__ __forceinline__ int job_first(const FusedJob* __restrict__ jobs, int k) {
+    return FIELD == 0 ? jobs[k].t1 : FIELD == 1 ? jobs[k].t2 : jobs[k].c0;
+}
+template <int FIELD>
+__device__ __forceinline__ int find_job(const FusedJob* __restrict__ jobs, int njobs, int item) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (job_first<FIELD>(jobs, mid) <= item) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// WHICH 0: Zpart = A*X (K split), WHICH 1: V = A^T*Z.
+template <int PREC, int WHICH>
+__global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restrict__ jobs, int njobs, int total) {
+    const int t = xcd_remap(blockIdx.x, total);
+    const int k = find_job<WHICH>(jobs, njobs, t);
+    const FusedJob& J = jobs[k];
+    if (*J.sp.done) return;
+    const int lt = t - (WHICH == 0 ? J.t1 : J.t2);
+    const int tiles_m = (WHICH == 0 ? J.mp : J.np) / kGemmBM, tiles_n = J.bp / kGemmBN;
+    const int tn = lt % tiles_n;
+    const int rest = lt / tiles_n;
+    const int tm = rest % tiles_m;
+    const int sp = rest / tiles_m;
+    const int ksplit = WHICH == 0 ? J.np / J.splits : J.mp;
+    const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
+    if constexpr (PREC == 0) {
+        if constexpr (WHICH == 0)
+            gemm_f32_tile(J.p1_32, J.np, J.xop.f32, J.np, J.Zpart, J.mp, J.zsplit, sp, m0, n0, k0, ksplit);
+        else
+            gemm_f32_tile(J.p2_32, J.mp, J.zop.f32, J.mp, J.V, J.np, 0, 0, m0, n0, k0, ksplit);
+    } else {
+        if constexpr (WHICH == 0)
+            gemm16_tile<PREC>(J.p1_hi, J.p1_lo, J.np, J.xop.hi, J.xop.lo, J.np, J.Zpart, J.mp, J.zsplit, sp, m0, n0,
+                              k0, ksplit);
+        else
+            gemm16_tile<PREC>(J.p2_hi, J.p2_lo, J.mp, J.zop.hi, J.zop.lo, J.mp, J.V, J.np, 0, 0, m0, n0, k0, ksplit);
+    }
+}
+
+__global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __restrict__ jobs, int njobs) {
+    const int k = find_job<2>(jobs, njobs, blockIdx.x);
+    const FusedJob& J = jobs[k];
+    prep_z_body(blockIdx.x - J.c0, J.Zpart, J.splits, J.zsplit, J.mp, J.mp, J.zop, J.sa.formulation, J.sa.precision,
+                J.sp);
+}
+
+__global__ __launch_bounds__(kSelThreads) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {
+    const int k = find_job<2>(jobs, njobs, blockIdx.x);
+    const FusedJob& J = jobs[k];
+    step_body(blockIdx.x - J.c0, J.V, J.xop, J.sa, J.sp);
+}
+
+__global__ __launch_bounds__(256) void fused_converge_kernel(const FusedJob* __restrict__ jobs, int* all_done) {
+    const FusedJob& J = jobs[blockIdx.x];
+    converge_body(J.sp, J.sa.batch, -1, J.tol, J.max_iter, J.per_sp, all_done);
+}
+
+template <int PREC>
+hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int* all_done,
+                        hipStream_t st) {
+    fused_gemm_kernel<PREC, 0><<<tiles1, 256, 0, st>>>(jobs, njobs, tiles1);
+    fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
+    fused_gemm_kernel<PREC, 1><<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
+    fused_step_kernel<<<cols, kSelThreads, 0, st>>>(jobs, njobs);
+    fused_converge_kernel<<<njobs, 256, 0, st>>>(jobs, all_done);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
+                                  int* all_done, hipStream_t st) {
+    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1) return hipErrorInvalidValue;
+    if (precision == 0) return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, all_done, st);
+    if (precision == 1) return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, all_done, st);
+    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, all_done, st);
 }
 
 // ---------------------------------------------------------------- OTF slot refill

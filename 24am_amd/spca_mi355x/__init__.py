@@ -35,8 +35,9 @@ EXPORTS = [
     "spca_dense_solver_device", "spca_dense_solver_host", "spca_settings_default", "spca_settings_check",
     "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
-    "spca_wait",
+    "spca_wait", "spca_solve_fused",
 ]
+FUSED_GRAPH = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
 
 
@@ -87,6 +88,12 @@ class _SpResult(ctypes.Structure):
                 ("converged", ctypes.c_int), ("ev_fp64", ctypes.c_double)]
 
 
+class _FusedJob(ctypes.Structure):
+    _fields_ = [("matrix", ctypes.c_uint64), ("settings", ctypes.POINTER(_Settings)), ("X0", ctypes.c_void_p),
+                ("x_out", ctypes.c_void_p), ("stats", ctypes.POINTER(_Stats)), ("per_sp", ctypes.c_void_p),
+                ("status", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -125,6 +132,7 @@ def lib():
             "spca_poll": (I, [P, U64, ctypes.POINTER(I)]),
             "spca_cancel": (I, [P, U64]),
             "spca_wait": (I, [P, U64, P, ctypes.POINTER(_Stats), P, ctypes.POINTER(_JobTimes)]),
+            "spca_solve_fused": (I, [P, ctypes.POINTER(_FusedJob), I, U]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -267,6 +275,40 @@ class Context:
 
     def stop_server(self):
         _check(self._p, lib().spca_server_stop(self._p))
+
+    def solve_fused(self, jobs, graph=True, per_sp=True):
+        """Solve uneven jobs together, one fused launch per phase and iteration.
+        jobs: list of (matrix, settings, X0 or None).  Returns a list of
+        (x, OptimizationStatistics) in the order given."""
+        keep, cj = [], (_FusedJob * len(jobs))()
+        outs = []
+        for k, (M, settings, X0) in enumerate(jobs):
+            c = settings._c()
+            total = _total_after_check(c, M.n)
+            X0p = None
+            if c.start == START_EXPLICIT:
+                X0 = np.asfortranarray(X0, dtype=np.float32)
+                if X0.shape != (M.n, total):
+                    raise ValueError(f"job {k}: X0 must be {M.n} x {total}")
+                X0p = X0.ctypes.data_as(ctypes.c_void_p)
+            x = np.zeros(M.n, dtype=np.float32)
+            st = _Stats()
+            res = (_SpResult * total)() if per_sp else None
+            keep += [c, X0, x, st, res]
+            cj[k].matrix = M.handle
+            cj[k].settings = ctypes.pointer(c)
+            cj[k].X0 = X0p
+            cj[k].x_out = x.ctypes.data_as(ctypes.c_void_p)
+            cj[k].stats = ctypes.pointer(st)
+            cj[k].per_sp = ctypes.cast(res, ctypes.c_void_p) if res is not None else None
+            outs.append((x, st, res))
+        _check(self._p, lib().spca_solve_fused(self._p, cj, len(jobs), FUSED_GRAPH if graph else 0))
+        result = []
+        for x, st, res in outs:
+            S = OptimizationStatistics()
+            S._fill(st, res)
+            result.append((x, S))
+        return result
 
     def upload(self, A, storage=STORE_FP32, key=0):
         """A: (m x n) float32 array (any order); copied column major to the device."""

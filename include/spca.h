@@ -171,6 +171,29 @@ int spca_matrix_bytes(spca_ctx ctx, spca_matrix mat, uint64_t* bytes);
 int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0,
                float* x_out, spca_stats* stats, spca_sp_result* per_sp, float* X_final);
 
+/* One job of a fused multi-job solve.  The fields mean what spca_solve's arguments mean;
+ * status receives the job's own result code. */
+typedef struct spca_fused_job {
+    spca_matrix matrix;
+    const spca_settings* settings; /* one batch (total_sp <= batch), use_otf = 0 */
+    const float* X0;
+    float* x_out;                  /* n floats */
+    spca_stats* stats;             /* optional; loop_ms counts to the check at which the job had finished */
+    spca_sp_result* per_sp;        /* optional, batch entries */
+    int status;
+} spca_fused_job;
+
+#define SPCA_FUSED_GRAPH 1u /* replay the fused iteration from a captured hipGraph */
+
+/* Solve uneven jobs (different matrices, shapes, formulations, s, batch sizes) together:
+ * every AM iteration of all jobs is one fused launch per phase, and a chunk of
+ * min(check_every) iterations is captured into a hipGraph (SPCA_FUSED_GRAPH).  All jobs
+ * share one precision.  Each job terminates on its own rule; a job's FP32 results equal
+ * those of a solo spca_solve of it with the same plan (128x128 product tiles).  No
+ * counterpart in the reference, whose GPU solver runs one job per call
+ * (c++/src/gpugpower/gpu_sparse_PCA_solver.h:29-211). */
+int spca_solve_fused(spca_ctx ctx, spca_fused_job* jobs, int njobs, unsigned flags);
+
 /* Drop-in for GPUSolver::denseDataSolver<float>(handle, m, n, d_B, h_x, settings, stats,
  * LD_M, LD_N) (c++/src/gpugpower/gpu_sparse_PCA_solver.h:29-35): d_B is a caller-owned
  * device matrix with leading dimension LD_M; h_x (n floats, host) receives the result.
