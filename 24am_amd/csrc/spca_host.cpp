@@ -202,8 +202,8 @@ int add_storage(spca_ctx ctx, Matrix* M, unsigned storage) {
         if ((rc = dev_alloc(ctx, &M->hi_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_cm, cnt, &M->bytes)) ||
             (rc = dev_alloc(ctx, &M->hi_rm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_rm, cnt, &M->bytes)))
             return rc;
-        HIP_TRY(ctx, launch_split_bf16(M->a32_cm, cnt, M->hi_cm, M->lo_cm, st));
-        HIP_TRY(ctx, launch_split_bf16(rm, cnt, M->hi_rm, M->lo_rm, st));
+        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 1, M->hi_cm, M->lo_cm, st));
+        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 1, M->hi_rm, M->lo_rm, st));
         M->storage |= SPCA_STORE_BF16X3;
     }
     if (want & SPCA_STORE_FP16) {
@@ -211,8 +211,8 @@ int add_storage(spca_ctx ctx, Matrix* M, unsigned storage) {
         const float* rm = M->a32_rm ? M->a32_rm : rm_tmp;
         if ((rc = dev_alloc(ctx, &M->h_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->h_rm, cnt, &M->bytes)))
             return rc;
-        HIP_TRY(ctx, launch_to_fp16(M->a32_cm, cnt, M->h_cm, st));
-        HIP_TRY(ctx, launch_to_fp16(rm, cnt, M->h_rm, st));
+        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 2, M->h_cm, nullptr, st));
+        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 2, M->h_rm, nullptr, st));
         M->storage |= SPCA_STORE_FP16;
     }
     HIP_TRY(ctx, hipStreamSynchronize(st));
@@ -412,10 +412,10 @@ int choose_splits(int precision, int mp, int np, int bp, bool allow_big = true) 
 }
 
 Operand x_operand(Workspace& w, int precision) {
-    return Operand{w.X, precision ? w.Xhi : nullptr, precision == 1 ? w.Xlo : nullptr};
+    return Operand{w.X, precision ? w.Xhi : nullptr, precision == 1 ? w.Xlo : nullptr, w.bp};
 }
 Operand z_operand(Workspace& w, int precision) {
-    return Operand{w.Z, precision ? w.Zhi : nullptr, precision == 1 ? w.Zlo : nullptr};
+    return Operand{w.Z, precision ? w.Zhi : nullptr, precision == 1 ? w.Zlo : nullptr, w.bp};
 }
 
 // Starting points with global indices shift .. shift+cnt-1 (n floats each) into out.
@@ -629,6 +629,9 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     const uint16_t* P2_hi = st.precision == 1 ? M->hi_cm : M->h_cm;
     const uint16_t* P2_lo = M->lo_cm;
     const long zsplit = (long)M->mp * bp;
+    // leading dimensions of the product operands: FP32 row strides, 16-bit (K-blocked) row counts
+    const long ld1p = st.precision ? M->mp : M->np, ld1q = st.precision ? bp : M->np;
+    const long ld2p = st.precision ? M->np : M->mp, ld2q = st.precision ? bp : M->mp;
 
     StepArgs sa;
     sa.formulation = st.formulation;
@@ -715,7 +718,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, M->np, w.X, w.Xhi, w.Xlo, M->np, w.Zpart,
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                         M->mp, zsplit, M->mp, bp, M->np, splits, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
@@ -725,7 +728,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, M->mp, w.Z, w.Zhi, w.Zlo, M->mp, w.V,
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V,
                                         M->np, 0, M->np, bp, M->mp, 1, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
@@ -798,7 +801,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, M->np, w.X, w.Xhi, w.Xlo, M->np, w.Zpart,
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                         M->mp, zsplit, M->mp, bp, M->np, splits, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
@@ -808,7 +811,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, M->mp, w.Z, w.Zhi, w.Zlo, M->mp, w.V,
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V,
                                         M->np, 0, M->np, bp, M->mp, 1, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));

@@ -36,11 +36,14 @@ struct SpState {
     float* errmax;   // [0]: batch max error of the last iteration
 };
 
-// Operand buffers of one iterate matrix (n or m rows, padded batch columns).
+// Operand buffers of one iterate matrix (n or m rows, padded batch columns).  The FP32
+// values are column major; the 16-bit forms the products read are K-blocked: element i of
+// column j at ((i / 32) * bp + j) * 32 + i % 32.
 struct Operand {
     float* f32;          // FP32 values (also the master copy of iterates)
     uint16_t* hi;        // bf16 hi part, or fp16 values
     uint16_t* lo;        // bf16 lo part (bf16x3 only)
+    int bp;              // columns (padded batch)
 };
 
 // On-the-fly (OTF) slot refill state (reference OTF driver, c++/src/gpower/sparse_PCA_solver.h:83-169).
@@ -95,10 +98,14 @@ hipError_t launch_pad_copy_f32(const float* src, int m, int n, int lds, float* d
                                int np, hipStream_t st);
 hipError_t launch_transpose_f32(const float* src, int rows, int cols, int lds, float* dst, int ldd,
                                 hipStream_t st);
-hipError_t launch_split_bf16(const float* src, size_t count, uint16_t* hi, uint16_t* lo, hipStream_t st);
-hipError_t launch_to_fp16(const float* src, size_t count, uint16_t* h, hipStream_t st);
+// src: R rows x K (row r at src + r*lds) -> K-blocked 16-bit form (precision 1: bf16 hi + lo,
+// 2: fp16 in hi): element (r, k) at ((k / 32) * R + r) * 32 + k % 32.  K % 32 == 0.
+hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int precision, uint16_t* hi, uint16_t* lo,
+                             hipStream_t st);
 
-// C[r + c*ldc + split*split_stride] = sum_{k in split} P[r*ldp + k] * Q[c*ldq + k]
+// C[r + c*ldc + split*split_stride] = sum_{k in split} P(r, k) * Q(c, k)
+// FP32: P(r, k) = P32[r*ldp + k], Q(c, k) = Q32[c*ldq + k].  16-bit: K-blocked operands with
+// ldp / ldq rows, P(r, k) = P[((k / 32) * ldp + r) * 32 + k % 32] (likewise Q).
 // M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.  16-bit products with M and N
 // multiples of 256 run the 256x256-tile kernel (gemm_big_tiles).
 int gemm_big_tiles(int precision, int M, int N);

@@ -137,25 +137,52 @@ hipError_t launch_transpose_f32(const float* src, int rows, int cols, int lds, f
     return hipGetLastError();
 }
 
-__global__ void split_bf16_kernel(const float* __restrict__ src, size_t count, uint16_t* __restrict__ hi,
-                                  uint16_t* __restrict__ lo) {
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x) {
-        const float x = src[e];
-        const uint16_t h = f2bf(x);
-        hi[e] = h;
-        lo[e] = f2bf(x - bf2f(h));
+// 16-bit product operands are stored K-blocked: element (r, k) of an operand with R rows
+// (the non-reduction dimension, padded) sits at ((k / 32) * R + r) * 32 + k % 32, so the
+// 32-deep K stage of any row range is one contiguous run of 64-byte rows.  The products'
+// per-stage loads are then dense blocks instead of 64-byte pieces one operand row apart
+// (128 KiB for a 65536-wide operand), which keeps L2 lines and TLB entries whole.
+__device__ __forceinline__ long blk_index(int r, int k, int R) { return ((long)(k >> 5) * R + r) * 32 + (k & 31); }
+
+// src: R rows x K (row r at src + r*lds, K contiguous) -> blocked bf16 hi+lo (precision 1)
+// or fp16 (precision 2).  One thread per 32-element chunk; consecutive threads take
+// consecutive rows of one K block, so the 64-byte output rows are written contiguously.
+__global__ void to_blocked_kernel(const float* __restrict__ src, long lds, int R, int K, int precision,
+                                  uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+    const long chunks = (long)R * (K / 32);
+    for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < chunks; c += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(c % R), kb = (int)(c / R);
+        const float* sp = src + (long)r * lds + (long)kb * 32;
+        const long o = c * 32;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const f4 v0 = *(const f4*)(sp + 8 * q), v1 = *(const f4*)(sp + 8 * q + 4);
+            const float x[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
+            u4 hv, lv;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                uint32_t h0, h1, l0 = 0, l1 = 0;
+                if (precision == 1) {
+                    h0 = f2bf(x[2 * e]);
+                    h1 = f2bf(x[2 * e + 1]);
+                    l0 = f2bf(x[2 * e] - bf2f((uint16_t)h0));
+                    l1 = f2bf(x[2 * e + 1] - bf2f((uint16_t)h1));
+                } else {
+                    h0 = f2h(x[2 * e]);
+                    h1 = f2h(x[2 * e + 1]);
+                }
+                hv[e] = h0 | (h1 << 16);
+                lv[e] = l0 | (l1 << 16);
+            }
+            *(u4*)(hi + o + 8 * q) = hv;
+            if (precision == 1) *(u4*)(lo + o + 8 * q) = lv;
+        }
     }
 }
-hipError_t launch_split_bf16(const float* src, size_t count, uint16_t* hi, uint16_t* lo, hipStream_t st) {
-    split_bf16_kernel<<<8192, 256, 0, st>>>(src, count, hi, lo);
-    return hipGetLastError();
-}
-__global__ void to_fp16_kernel(const float* __restrict__ src, size_t count, uint16_t* __restrict__ h) {
-    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < count; e += (size_t)gridDim.x * blockDim.x)
-        h[e] = f2h(src[e]);
-}
-hipError_t launch_to_fp16(const float* src, size_t count, uint16_t* h, hipStream_t st) {
-    to_fp16_kernel<<<8192, 256, 0, st>>>(src, count, h);
+hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int precision, uint16_t* hi, uint16_t* lo,
+                             hipStream_t st) {
+    if (R <= 0 || K <= 0 || K % 32 || (precision != 1 && precision != 2)) return hipErrorInvalidValue;
+    to_blocked_kernel<<<8192, 256, 0, st>>>(src, lds, R, K, precision, hi, lo);
     return hipGetLastError();
 }
 
@@ -272,19 +299,21 @@ __device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, co
     const int lrow = tid >> 2, lc8 = (tid & 3) * 8;
     const uint16_t* Pg[NT];
     const uint16_t* Qg[NT];
-    Pg[0] = Phi + (m0 + lrow) * ldp + k0 + lc8;
-    Qg[0] = Qhi + (n0 + lrow) * ldq + k0 + lc8;
+    // K-blocked operands: ldp / ldq are their row counts; one K stage advances 32 rows' worth
+    const long pstep = ldp * kGemmBK, qstep = ldq * kGemmBK;
+    Pg[0] = Phi + ((k0 >> 5) * ldp + m0 + lrow) * 32 + lc8;
+    Qg[0] = Qhi + ((k0 >> 5) * ldq + n0 + lrow) * 32 + lc8;
     if constexpr (NT == 2) {
-        Pg[1] = Plo + (m0 + lrow) * ldp + k0 + lc8;
-        Qg[1] = Qlo + (n0 + lrow) * ldq + k0 + lc8;
+        Pg[1] = Plo + ((k0 >> 5) * ldp + m0 + lrow) * 32 + lc8;
+        Qg[1] = Qlo + ((k0 >> 5) * ldq + n0 + lrow) * 32 + lc8;
     }
     u4 pr[NT][2], qr[NT][2];
 #pragma unroll
     for (int p = 0; p < NT; p++)
 #pragma unroll
         for (int i = 0; i < 2; i++) {
-            pr[p][i] = *(const u4*)(Pg[p] + (long)(64 * i) * ldp);
-            qr[p][i] = *(const u4*)(Qg[p] + (long)(64 * i) * ldq);
+            pr[p][i] = *(const u4*)(Pg[p] + 64 * 32 * i);
+            qr[p][i] = *(const u4*)(Qg[p] + 64 * 32 * i);
         }
     f4 acc[4][4];
 #pragma unroll
@@ -305,12 +334,12 @@ __device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, co
         if (kt + kGemmBK < ksplit) {
 #pragma unroll
             for (int p = 0; p < NT; p++) {
-                Pg[p] += kGemmBK;
-                Qg[p] += kGemmBK;
+                Pg[p] += pstep;
+                Qg[p] += qstep;
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
-                    pr[p][i] = *(const u4*)(Pg[p] + (long)(64 * i) * ldp);
-                    qr[p][i] = *(const u4*)(Qg[p] + (long)(64 * i) * ldq);
+                    pr[p][i] = *(const u4*)(Pg[p] + 64 * 32 * i);
+                    qr[p][i] = *(const u4*)(Qg[p] + 64 * 32 * i);
                 }
             }
         }
@@ -371,126 +400,22 @@ __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restr
                       (long)tn * kGemmBN, (long)sp * ksplit, ksplit);
 }
 
-// Large-tile variant of the 16-bit products: 256x256 output tile per workgroup, 8 waves
-// as 2 (M) x 4 (N), each wave 128 x 64 (8 x 4 MFMA blocks, 128 accumulators).  Each wave
-// keeps its B fragments for the whole stage and streams 8 A fragments past them, which
-// cuts LDS reads per MFMA by 1.5x against the 128x128 kernel and halves L2 traffic per
-// flop.  One 80 KiB LDS stage (bf16x3) with the next stage prefetched in registers.
-template <int PREC>
-__global__ __launch_bounds__(512, 1) void gemm_tn_16_big_kernel(const uint16_t* __restrict__ Phi,
-                                                                const uint16_t* __restrict__ Plo, long ldp,
-                                                                const uint16_t* __restrict__ Qhi,
-                                                                const uint16_t* __restrict__ Qlo, long ldq,
-                                                                float* __restrict__ C, long ldc, long split_stride,
-                                                                int tiles_m, int tiles_n, int splits, int ksplit,
-                                                                const int* __restrict__ done) {
-    if (done && *done) return;
-    constexpr int NP = (PREC == 1) ? 2 : 1;
-    constexpr int BM = 256, BN = 256, BK = kGemmBK, LDK = BK + 8;
-    const int total = tiles_m * tiles_n * splits;
-    const int t = xcd_remap(blockIdx.x, total);
-    const int tn = t % tiles_n;
-    const int rest = t / tiles_n;
-    const int tm = rest % tiles_m;
-    const int sp = rest / tiles_m;
-    const long m0 = (long)tm * BM, n0 = (long)tn * BN, k0 = (long)sp * ksplit;
-
-    __shared__ __attribute__((aligned(16))) uint16_t Ps[NP][BM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t Qs[NP][BN][LDK];
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 2, wn = wave & 3;
-    // a 256 x 32 tile of 16-bit values is 1024 chunks of 16 bytes: 2 per thread per part
-    const int lrow = tid >> 2, lc8 = (tid & 3) * 8;
-    const uint16_t* Pg[NP];
-    const uint16_t* Qg[NP];
-    Pg[0] = Phi + (m0 + lrow) * ldp + k0 + lc8;
-    Qg[0] = Qhi + (n0 + lrow) * ldq + k0 + lc8;
-    if constexpr (NP == 2) {
-        Pg[1] = Plo + (m0 + lrow) * ldp + k0 + lc8;
-        Qg[1] = Qlo + (n0 + lrow) * ldq + k0 + lc8;
-    }
-    u4 pr[NP][2], qr[NP][2];
-#pragma unroll
-    for (int p = 0; p < NP; p++)
-#pragma unroll
-        for (int i = 0; i < 2; i++) {
-            pr[p][i] = *(const u4*)(Pg[p] + (long)(128 * i) * ldp);
-            qr[p][i] = *(const u4*)(Qg[p] + (long)(128 * i) * ldq);
-        }
-    f4 acc[8][4];
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
-
-    const int kof = 8 * (lane >> 4);
-    for (int kt = 0; kt < ksplit; kt += BK) {
-        __syncthreads();
-#pragma unroll
-        for (int p = 0; p < NP; p++)
-#pragma unroll
-            for (int i = 0; i < 2; i++) {
-                *(u4*)&Ps[p][lrow + 128 * i][lc8] = pr[p][i];
-                *(u4*)&Qs[p][lrow + 128 * i][lc8] = qr[p][i];
-            }
-        __syncthreads();
-        if (kt + BK < ksplit) {
-#pragma unroll
-            for (int p = 0; p < NP; p++) {
-                Pg[p] += BK;
-                Qg[p] += BK;
-#pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    pr[p][i] = *(const u4*)(Pg[p] + (long)(128 * i) * ldp);
-                    qr[p][i] = *(const u4*)(Qg[p] + (long)(128 * i) * ldq);
-                }
-            }
-        }
-        u4 b[NP][4];
-#pragma unroll
-        for (int p = 0; p < NP; p++)
-#pragma unroll
-            for (int j = 0; j < 4; j++) b[p][j] = *(const u4*)&Qs[p][wn * 64 + j * 16 + (lane & 15)][kof];
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
-            u4 a[NP];
-#pragma unroll
-            for (int p = 0; p < NP; p++) a[p] = *(const u4*)&Ps[p][wm * 128 + i * 16 + (lane & 15)][kof];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if constexpr (PREC == 1) {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[1][j]), acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf8, a[1]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
-                } else {
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                        __builtin_bit_cast(h8, a[0]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
-                }
-            }
-        }
-    }
-    float* Cb = C + (long)sp * split_stride;
-#pragma unroll
-    for (int i = 0; i < 8; i++)
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const long row = m0 + wm * 128 + i * 16 + 4 * (lane >> 4);
-            const long col = n0 + wn * 64 + j * 16 + (lane & 15);
-            *(f4*)(Cb + row + col * ldc) = acc[i][j];
-        }
-}
-
-// Pipelined large-tile variant: the same 256x256 tile and wave layout, but stages are
-// filled by direct global->LDS loads (global_load_lds_dwordx4: each lane's 16 bytes land at
-// the wave's LDS base + lane*16) into a ring of NSTAGE buffers, so NSTAGE-1 stages of loads
-// are in flight while one is computed and no registers stage the data.  LDS rows are 64
-// bytes unpadded; the 16-byte chunks of row r are XOR-swizzled by (r>>1)&3, so the 16 rows x
-// 4 chunks a fragment read touches fall in distinct banks.  One barrier per stage: after it,
-// every wave's loads of stage k have landed and every wave has finished reading stage k-1,
-// whose buffer the loads of stage k+NSTAGE-1 then reuse.
+// Pipelined large-tile variant: 256x256 output tile per workgroup, 8 waves as 2 (M) x 4 (N),
+// each wave 128 x 64 (8 x 4 MFMA blocks, 128 accumulators).  Stages are filled by direct
+// global->LDS loads (global_load_lds_dwordx4: each lane's 16 bytes land at the wave's LDS
+// base + lane*16) into a ring of NSTAGE buffers, so NSTAGE-1 stages of loads are in flight
+// while one is computed and no registers stage the data; the K-blocked operands make each
+// wave's 16-row load one contiguous KiB.  LDS rows are 64 bytes unpadded; the 16-byte chunks
+// of row r are XOR-swizzled by (r>>1)&3, so the 16 rows x 4 chunks a fragment read touches
+// fall in distinct banks.  One barrier per stage: each wave first waits (vmcnt) for its own
+// loads of stage k, so after the barrier every wave's loads of stage k have landed and every
+// wave has finished reading stage k-1, whose buffer the loads of stage k+NSTAGE-1 then reuse.
+// The barrier is the bare s_barrier: __syncthreads() adds a workgroup release fence, which
+// the compiler implements by waiting for ALL outstanding loads (vmcnt(0)), including the
+// later stages still in flight, and that would cut the lookahead to one stage.  The explicit
+// per-wave wait above provides the ordering the fence would.  Each wave reads all its A and B
+// fragments of the stage first, then issues the stage's MFMAs, so LDS latency is paid once
+// per stage and the MFMAs of the two waves sharing a SIMD overlap the other's reads.
 __device__ __forceinline__ void wait_vmcnt(int n) {
     // s_waitcnt with vmcnt = n (6 bits) and expcnt/lgkmcnt left at their maxima
     switch (n) {
@@ -540,16 +465,17 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
         src[0][1] = Plo;
         src[1][1] = Qlo;
     }
+    // K-blocked operands with ldp / ldq rows: one K stage of an operand is rows*32 elements
+    const long stage_elems[2] = {ldp * BK, ldq * BK};
     long rowoff[2][2];
 #pragma unroll
     for (int i = 0; i < 2; i++) {
         const int row = 32 * wave + 16 * i + lrow;
         const int g = lslot ^ ((row >> 1) & 3);
-        rowoff[0][i] = (m0 + row) * ldp + k0 + g * 8;
-        rowoff[1][i] = (n0 + row) * ldq + k0 + g * 8;
+        rowoff[0][i] = ((k0 >> 5) * ldp + m0 + row) * 32 + g * 8;
+        rowoff[1][i] = ((k0 >> 5) * ldq + n0 + row) * 32 + g * 8;
     }
     auto issue = [&](int ks, int buf) {
-        const long kk = (long)ks * BK;
 #pragma unroll
         for (int op = 0; op < 2; op++)
 #pragma unroll
@@ -557,8 +483,9 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
 #pragma unroll
                 for (int i = 0; i < 2; i++) {
                     uint8_t* dst = lds + buf * STB + (op * NP + p) * OPB + (32 * wave + 16 * i) * ROWB;
-                    __builtin_amdgcn_global_load_lds((const void*)(src[op][p] + rowoff[op][i] + kk),
-                                                     (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+                    __builtin_amdgcn_global_load_lds(
+                        (const void*)(src[op][p] + rowoff[op][i] + (long)ks * stage_elems[op]),
+                        (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
                 }
     };
     f4 acc[8][4];
@@ -574,7 +501,7 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
     for (int k = 0; k < nk; k++) {
         if (k + NSTAGE - 2 < nk) wait_vmcnt((NSTAGE - 2) * LPS);
         else wait_vmcnt(0);
-        __syncthreads();
+        __builtin_amdgcn_s_barrier();
         if (k + NSTAGE - 1 < nk) issue(k + NSTAGE - 1, (k + NSTAGE - 1) % NSTAGE);
         const uint8_t* sb = lds + (k % NSTAGE) * STB;
         u4 b[NP][4];
@@ -585,19 +512,17 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                 const int r = wn * 64 + j * 16 + (lane & 15);
                 b[p][j] = *(const u4*)(sb + (NP + p) * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
             }
-        // A fragments double-buffered: the reads for block i+1 are in flight while the MFMAs of
-        // block i issue
-        u4 abuf[2][NP];
-        auto load_a = [&](int i, u4* dst) {
-            const int r = wm * 128 + i * 16 + (lane & 15);
-#pragma unroll
-            for (int p = 0; p < NP; p++) dst[p] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
-        };
-        load_a(0, abuf[0]);
+        // all A fragments of the stage first, then the MFMAs
+        u4 afr[8][NP];
 #pragma unroll
         for (int i = 0; i < 8; i++) {
-            if (i + 1 < 8) load_a(i + 1, abuf[(i + 1) & 1]);
-            const u4* a = abuf[i & 1];
+            const int r = wm * 128 + i * 16 + (lane & 15);
+#pragma unroll
+            for (int p = 0; p < NP; p++) afr[i][p] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const u4* a = afr[i];
             if constexpr (PREC == 1) {
 #pragma unroll
                 for (int j = 0; j < 4; j++)
@@ -617,9 +542,6 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
                         __builtin_bit_cast(h8, a[0]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
             }
-            // schedule: this block's reads of the next A fragments first, then its MFMAs
-            if (i + 1 < 8) __builtin_amdgcn_sched_group_barrier(0x100, NP, 0);
-            __builtin_amdgcn_sched_group_barrier(0x008, 4 * (PREC == 1 ? 3 : 1), 0);
         }
     }
     float* Cb = C + (long)sp * split_stride;
@@ -671,14 +593,16 @@ __device__ __forceinline__ bool is_constrained(int f) { return f < 4; }
 __device__ __forceinline__ bool is_l1_variance(int f) { return f == 1 || f == 3 || f == 5 || f == 7; }
 __device__ __forceinline__ float sgnf(float v) { return (float)((v > 0.f) - (v < 0.f)); }
 
-__device__ __forceinline__ void write_operand(Operand op, int precision, long idx, float v) {
-    op.f32[idx] = v;
+// Element i of column j: FP32 column major (ld), 16-bit forms K-blocked over op.bp columns.
+__device__ __forceinline__ void write_operand(Operand op, int precision, long ld, int i, int j, float v) {
+    op.f32[(long)j * ld + i] = v;
     if (precision == 1) {
+        const long b = blk_index(j, i, op.bp);
         const uint16_t h = f2bf(v);
-        op.hi[idx] = h;
-        op.lo[idx] = f2bf(v - bf2f(h));
+        op.hi[b] = h;
+        op.lo[b] = f2bf(v - bf2f(h));
     } else if (precision == 2) {
-        op.hi[idx] = f2h(v);
+        op.hi[blk_index(j, i, op.bp)] = f2h(v);
     }
 }
 
@@ -709,7 +633,7 @@ __device__ __forceinline__ void prep_z_body(int j, const float* __restrict__ Zpa
         float z = zop.f32[idx];
         if (l1v) z = sgnf(z);
         else if (!constrained) z = z * inv;
-        write_operand(zop, precision, idx, z);
+        write_operand(zop, precision, ldz, i, j, z);
     }
 }
 __global__ __launch_bounds__(256) void prep_z_kernel(const float* __restrict__ Zpart, int splits,
@@ -1111,8 +1035,12 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     {
         const long base = (long)j * a.ldv;
         const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + base, (uint32_t)a.n * 4u);
-        const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)(xop.hi + base) : (const void*)xop.f32, a.precision ? (uint32_t)a.n * 2u : 0u);
-        const __amdgpu_buffer_rsrc_t lr = col_rsrc(xop.lo ? (const void*)(xop.lo + base) : (const void*)xop.f32, a.precision == 1 ? (uint32_t)a.n * 2u : 0u);
+        // 16-bit forms are K-blocked: element i of column j at ((i/32)*bp + j)*32 + i%32; the
+        // views cover the whole operand (ldv * bp elements) and elements i >= n are skipped
+        const uint32_t opb = a.precision ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
+        const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
+        const __amdgpu_buffer_rsrc_t lr =
+            col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, a.precision == 1 ? opb : 0u);
         for (int q0 = 0; q0 < nq; q0 += 8) {
             float vv[8];
 #pragma unroll
@@ -1133,12 +1061,16 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 if (f < 4) x = x * scale_out;
                 __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, tid * 4, q * kSelThreads * 4, 0);
                 nz += (x != 0.f && (tid + q * kSelThreads) < a.n) ? 1.0 : 0.0;
-                if (a.precision == 1) {
-                    const uint16_t h = f2bf(x);
-                    __builtin_amdgcn_raw_buffer_store_b16(h, hr, tid * 2, q * kSelThreads * 2, 0);
-                    __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, tid * 2, q * kSelThreads * 2, 0);
-                } else if (a.precision == 2) {
-                    __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, tid * 2, q * kSelThreads * 2, 0);
+                const int i = tid + q * kSelThreads;
+                if (a.precision && i < a.n) {
+                    const uint32_t bo = (uint32_t)(blk_index(j, i, xop.bp) * 2);
+                    if (a.precision == 1) {
+                        const uint16_t h = f2bf(x);
+                        __builtin_amdgcn_raw_buffer_store_b16(h, hr, bo, 0, 0);
+                        __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, bo, 0, 0);
+                    } else {
+                        __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, bo, 0, 0);
+                    }
                 }
             }
         }
@@ -1159,6 +1091,7 @@ __global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restri
 }
 hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st) {
     if (a.n <= 0 || (long)a.n * 4 > 0x7fffffffL) return hipErrorInvalidValue;
+    if (a.precision && (long)a.ldv * xop.bp * 2 > 0xffffffffL) return hipErrorInvalidValue;  // 32-bit store offsets
     step_kernel<<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp);
     return hipGetLastError();
 }
@@ -1261,10 +1194,10 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
             gemm_f32_tile(J.p2_32, J.mp, J.zop.f32, J.mp, J.V, J.np, 0, 0, m0, n0, k0, ksplit);
     } else {
         if constexpr (WHICH == 0)
-            gemm16_tile<PREC>(J.p1_hi, J.p1_lo, J.np, J.xop.hi, J.xop.lo, J.np, J.Zpart, J.mp, J.zsplit, sp, m0, n0,
+            gemm16_tile<PREC>(J.p1_hi, J.p1_lo, J.mp, J.xop.hi, J.xop.lo, J.bp, J.Zpart, J.mp, J.zsplit, sp, m0, n0,
                               k0, ksplit);
         else
-            gemm16_tile<PREC>(J.p2_hi, J.p2_lo, J.mp, J.zop.hi, J.zop.lo, J.mp, J.V, J.np, 0, 0, m0, n0, k0, ksplit);
+            gemm16_tile<PREC>(J.p2_hi, J.p2_lo, J.np, J.zop.hi, J.zop.lo, J.bp, J.V, J.np, 0, 0, m0, n0, k0, ksplit);
     }
 }
 
@@ -1417,7 +1350,7 @@ __global__ __launch_bounds__(256) void otf_refill_kernel(SpState sp, OtfState o,
     if (!o.refill[i]) return;
     __syncthreads();
     const float* src = o.pool + (long)o.order[i] * ldx;
-    for (int k = threadIdx.x; k < n; k += 256) write_operand(xop, precision, base + k, src[k]);
+    for (int k = threadIdx.x; k < n; k += 256) write_operand(xop, precision, ldx, k, i, src[k]);
     if (threadIdx.x == 0) {
         sp.val[i] = 0.f;
         sp.err[i] = 0.f;
@@ -1434,7 +1367,7 @@ hipError_t launch_otf_refill(SpState sp, OtfState o, Operand xop, int precision,
 __global__ void x_operand_kernel(Operand xop, int precision, int ldx, int batch) {
     const long total = (long)ldx * batch;
     for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x)
-        write_operand(xop, precision, e, xop.f32[e]);
+        write_operand(xop, precision, ldx, (int)(e % ldx), (int)(e / ldx), xop.f32[e]);
 }
 hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st) {
     if (precision == 0) return hipSuccess;

@@ -39,12 +39,17 @@ def explained_variance(A, X, f):
     return np.sum(Z * Z, axis=0) if f in L2_VAR else np.sum(np.abs(Z), axis=0)
 
 
-def tie_margin(A, X_prev, f, s, gamma):
+def tie_margin(A, X_prev, f, s, gamma, ztol=0.0):
     """Relative gap between the threshold and the nearest competing value in the last
     step, per SP, computed from the oracle's previous iterate (FP64).  A support that
     differs only where this gap is within rounding is a documented magnitude tie."""
     A = A.astype(np.float64)
     Z = A @ X_prev
+    # L1 variance maps z to sgn z: an entry of z within the product's rounding (ztol, relative
+    # to the rms of z) of zero may flip sign, which moves every entry of A^T sgn(z); such an
+    # SP counts as a tie (margin 0)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        zrel = np.min(np.abs(Z), axis=0) / np.sqrt(np.mean(Z * Z, axis=0)) if f not in L2_VAR else None
     if f in L2_VAR:
         if f >= 4:
             Z = Z / np.linalg.norm(Z, axis=0)
@@ -64,4 +69,4 @@ def tie_margin(A, X_prev, f, s, gamma):
             out[j] = np.min(np.abs(a * a - gamma)) / gamma
         else:
             out[j] = np.min(np.abs(a - gamma)) / gamma
-    return out
+    return out if zrel is None else np.where(zrel < ztol, 0.0, out)

@@ -9,7 +9,10 @@ Tolerances, per starting point (SP), against the FP64 oracle on the same seeded 
 Supports after one step from the same iterate must be identical, except for an SP whose
 step had a magnitude tie: the threshold and its nearest competitor within TIE of each
 other (relative: 1e-5 FP32, 1e-4 bf16x3, 1e-3 fp16), where rounding of A*X decides the
-order; such SPs are counted and bounded (<= 5%; fp16, whose A*X error is ~1e-3, <= 35%).
+order; such SPs are counted and bounded (<= 5%; fp16, whose A*X error is ~1e-3, <= 50%).
+For L1-variance formulations (z -> sgn z) an entry of z = A x within the product's
+rounding of zero (ZTOL relative to the rms of z) can flip sign and move every entry of
+A^T sgn(z): such an SP also counts as a tie.
 """
 import json
 import os
@@ -27,8 +30,14 @@ GOLD = os.path.join(os.path.dirname(__file__), "golden")
 TOL = {  # precision: (EV, objective, tie, share of SPs allowed to sit on a tie in one step)
     S.PREC_FP32: (1e-6, 1e-5, 1e-5, 0.05),
     S.PREC_BF16X3: (1e-5, 1e-4, 1e-4, 0.05),
-    S.PREC_FP16: (1e-3, 5e-3, 1e-3, 0.35),
+    S.PREC_FP16: (1e-3, 5e-3, 1e-3, 0.5),
 }
+
+
+# relative size (to the rms of z = A x) below which an entry of z may change sign through
+# the product's rounding: about 3x the rms product error, which is ~sqrt(2) times the
+# operand rounding (fp16 2^-12 rms ~1.4e-4, bf16x3 ~2^-17, FP32 ~2^-24 plus accumulation)
+ZTOL = {S.PREC_FP32: 1e-6, S.PREC_BF16X3: 3e-5, S.PREC_FP16: 6e-4}
 
 
 @pytest.fixture(scope="module")
@@ -52,25 +61,53 @@ def _settings(f, it, prec, B, s=10, m=200, tol=0.0):
                                   precision=prec)
 
 
+@pytest.fixture(scope="module")
+def wide():
+    """Padded batch and rows multiples of 256: the 16-bit products run the 256x256-tile kernel."""
+    m, n, B = 300, 1000, 256
+    A, planted = spiked(m, n, seed=7)
+    return A, unit_gaussian_starts(n, B, seed=8), planted
+
+
+@pytest.fixture(scope="module")
+def wide_mat(ctx, wide):
+    M = ctx.upload(wide[0], storage=S.STORE_FP32 | S.STORE_BF16X3 | S.STORE_FP16)
+    yield M
+    M.release()
+
+
 @pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3, S.PREC_FP16])
 @pytest.mark.parametrize("f", range(8))
 def test_step_parity(small, small_mat, f, prec):
     """One AM iteration from identical iterates (the GPU's own iterate after t steps),
     GPU vs FP64 oracle: supports identical except at this step's magnitude ties."""
-    A, X0, _ = small
+    _step_parity(small, small_mat, f, prec)
+
+
+@pytest.mark.parametrize("prec", [S.PREC_BF16X3, S.PREC_FP16])
+@pytest.mark.parametrize("f", range(8))
+def test_step_parity_large_tiles(wide, wide_mat, f, prec):
+    """As test_step_parity with B = 256 and 300 x 1000 (padded 512 x 1024), where both
+    16-bit products run the pipelined 256x256-tile kernel."""
+    _step_parity(wide, wide_mat, f, prec)
+
+
+def _step_parity(data, mat, f, prec):
+    small_mat = mat
+    A, X0, _ = data
     m, n = A.shape
     B, s, g = X0.shape[1], 10, default_gamma(f, m)
     ev_tol, val_tol, tie_tol, tie_share = TOL[prec]
     for t in (0, 5, 25):
         print(f"case f={f} prec={prec} t={t} m={m} n={n} B={B}", flush=True)
-        Xt = X0 if t == 0 else small_mat.solve(_settings(f, t, prec, B), X0=X0, want_X=True)[1]
+        Xt = X0 if t == 0 else small_mat.solve(_settings(f, t, prec, B, m=m), X0=X0, want_X=True)[1]
         # a penalty can zero an SP out; its next iterate is then NaN (0/0) in both
         alive = np.all(np.isfinite(Xt), axis=0) & np.any(Xt != 0, axis=0)
         stats = S.OptimizationStatistics()
-        _, Xg = small_mat.solve(_settings(f, 1, prec, B), X0=Xt, stats=stats, want_X=True)
+        _, Xg = small_mat.solve(_settings(f, 1, prec, B, m=m), X0=Xt, stats=stats, want_X=True)
         o = O.solve(A.astype(np.float64), f, s=s, gamma=g, tol=0.0, max_iter=1, total_sp=B,
                     X0=Xt.astype(np.float64), want_X=True, lambda_rule=O.LAMBDA_GPU)
-        tie = tie_margin(A, Xt.astype(np.float64), f, s, g) < tie_tol
+        tie = tie_margin(A, Xt.astype(np.float64), f, s, g, ztol=ZTOL[prec]) < tie_tol
         supp = np.all((Xg != 0) == (o["X"] != 0), axis=0)
         chk = alive & ~tie
         scale = np.abs(o["X"]).max(axis=0)
