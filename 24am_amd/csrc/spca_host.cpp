@@ -75,6 +75,23 @@ struct Engine {
     const std::atomic<int>* cancel = nullptr;  // set by the request server; checked at host syncs
 };
 
+// Launch plan of the two products of one solve shape: K split of A*X and the tile kernel.
+struct Plan {
+    int splits = 1;
+    int big = 0;       // 1: 256x256-tile kernel for both products
+    double ms = 0;     // measured A*X + epilogue + A^T*Z time when tuned (0: heuristic plan)
+};
+struct PlanKey {
+    int precision, mp, np, bp, allow_big;
+    bool operator<(const PlanKey& o) const {
+        if (precision != o.precision) return precision < o.precision;
+        if (mp != o.mp) return mp < o.mp;
+        if (np != o.np) return np < o.np;
+        if (bp != o.bp) return bp < o.bp;
+        return allow_big < o.allow_big;
+    }
+};
+
 }  // namespace spca
 
 struct spca_server;
@@ -92,6 +109,11 @@ struct spca_ctx_s {
     uint64_t next_handle = 1;
     spca::Engine eng;  // default engine (synchronous calls)
     spca_server* server = nullptr;
+    // product plans: measured per shape when tuning is on, else the fixed heuristic
+    std::mutex plan_mu;
+    int tuning = 0;
+    std::string tune_path;
+    std::map<spca::PlanKey, spca::Plan> plans;
 };
 
 using namespace spca;
@@ -552,6 +574,181 @@ void finish_solve(const Results& R, int n, const spca_settings& st, float* x_out
     }
 }
 
+// ---- product plans and the autotuner
+Plan heuristic_plan(int precision, int mp, int np, int bp, bool allow_big) {
+    Plan p;
+    p.big = allow_big && gemm_big_tiles(precision, mp, bp);
+    p.splits = choose_splits(precision, mp, np, bp, p.big != 0);
+    return p;
+}
+
+// Plan cache file: {"plans": [{"precision": P, "mp": .., "np": .., "bp": .., "allow_big": ..,
+// "splits": .., "big": .., "ms": ..}, ...]}
+void load_plans(spca_ctx ctx) {
+    FILE* f = std::fopen(ctx->tune_path.c_str(), "r");
+    if (!f) return;
+    std::string txt;
+    char buf[4096];
+    size_t r;
+    while ((r = std::fread(buf, 1, sizeof buf, f)) > 0) txt.append(buf, r);
+    std::fclose(f);
+    size_t pos = 0;
+    while ((pos = txt.find('{', pos + 1)) != std::string::npos) {
+        const size_t end = txt.find('}', pos);
+        if (end == std::string::npos) break;
+        const std::string obj = txt.substr(pos, end - pos + 1);
+        auto num = [&](const char* key, double* v) {
+            const size_t k = obj.find(std::string("\"") + key + "\"");
+            if (k == std::string::npos) return false;
+            const size_t c = obj.find(':', k);
+            return c != std::string::npos && std::sscanf(obj.c_str() + c + 1, "%lf", v) == 1;
+        };
+        double P, mp, np, bp, ab, sp, bg, ms = 0;
+        if (num("precision", &P) && num("mp", &mp) && num("np", &np) && num("bp", &bp) && num("allow_big", &ab) &&
+            num("splits", &sp) && num("big", &bg)) {
+            num("ms", &ms);
+            Plan p;
+            p.splits = (int)sp;
+            p.big = (int)bg;
+            p.ms = ms;
+            ctx->plans[PlanKey{(int)P, (int)mp, (int)np, (int)bp, (int)ab}] = p;
+        }
+        pos = end;
+    }
+}
+
+void save_plans(spca_ctx ctx) {
+    if (ctx->tune_path.empty()) return;
+    const std::string tmp = ctx->tune_path + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    if (!f) return;
+    std::fprintf(f, "{\"plans\": [");
+    bool first = true;
+    for (auto& kv : ctx->plans) {
+        if (kv.second.ms <= 0) continue;  // only measured plans are worth keeping
+        std::fprintf(f, "%s\n  {\"precision\": %d, \"mp\": %d, \"np\": %d, \"bp\": %d, \"allow_big\": %d, "
+                        "\"splits\": %d, \"big\": %d, \"ms\": %.5f}",
+                     first ? "" : ",", kv.first.precision, kv.first.mp, kv.first.np, kv.first.bp,
+                     kv.first.allow_big, kv.second.splits, kv.second.big, kv.second.ms);
+        first = false;
+    }
+    std::fprintf(f, "\n]}\n");
+    std::fclose(f);
+    std::rename(tmp.c_str(), ctx->tune_path.c_str());
+}
+
+// Times every valid plan of this shape on the job's own matrix and starting points (already
+// in w.X): A*X, the z epilogue and A^T*Z, one warm-up and three timed runs each, and keeps the
+// fastest.  Candidates: K splits 1..16 of A*X (each split at least 256 deep, partial sums
+// within 512 MiB) and, where the shape allows, both tile kernels.
+int tune_plan(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, Workspace& w, bool allow_big,
+              Plan* out) {
+    hipStream_t s = E.stream;
+    const int mp = M->mp, np = M->np, bp = w.bp, prec = st.precision;
+    const SpState sp = state_of(w);
+    const Operand zop = z_operand(w, prec);
+    const long zsplit = (long)mp * bp;
+    const long ld1p = prec ? mp : np, ld1q = prec ? bp : np, ld2p = prec ? np : mp, ld2q = prec ? bp : mp;
+    const float* P1_32 = M->a32_rm;
+    const uint16_t* P1_hi = prec == 1 ? M->hi_rm : M->h_rm;
+    const float* P2_32 = M->a32_cm;
+    const uint16_t* P2_hi = prec == 1 ? M->hi_cm : M->h_cm;
+    hipEvent_t e0, e1;
+    HIP_TRY(ctx, hipEventCreate(&e0));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(ctx, SPCA_EDEVICE, "event create failed");
+    }
+    Plan best;
+    best.ms = 1e30;
+    int rc = SPCA_OK;
+    const int bigs = allow_big && gemm_big_tiles(prec, mp, bp) ? 2 : 1;
+    for (int bi = 0; bi < bigs && rc == SPCA_OK; bi++)
+        for (int splits = 1; splits <= std::min(16, w.splits) && rc == SPCA_OK; splits *= 2) {
+            if (np % (kGemmBK * splits) || np / splits < 256) break;
+            float tot = 0;
+            for (int r = 0; r < 4; r++) {
+                hipError_t e = hipEventRecord(e0, s);
+                if (e == hipSuccess)
+                    e = launch_gemm_tn(prec, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart, mp,
+                                       zsplit, mp, bp, np, splits, bi, nullptr, s);
+                if (e == hipSuccess)
+                    e = launch_prep_z(w.Zpart, splits, zsplit, mp, mp, zop, st.formulation, prec, st.batch, sp, s);
+                if (e == hipSuccess)
+                    e = launch_gemm_tn(prec, P2_32, P2_hi, M->lo_cm, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V, np, 0, np, bp,
+                                       mp, 1, bi, nullptr, s);
+                if (e == hipSuccess) e = hipEventRecord(e1, s);
+                if (e == hipSuccess) e = hipEventSynchronize(e1);
+                float t = 0;
+                if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+                if (e != hipSuccess) {
+                    rc = fail(ctx, SPCA_EDEVICE, std::string("tuning: ") + hipGetErrorString(e));
+                    break;
+                }
+                if (r > 0) tot += t;
+            }
+            const double ms = tot / 3.0;
+            if (rc == SPCA_OK && ms < best.ms) {
+                best.ms = ms;
+                best.splits = splits;
+                best.big = bi;
+            }
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc == SPCA_OK) *out = best;
+    return rc;
+}
+
+// The plan of a solve: cached (tuned or loaded), else tuned now when tuning is on (with the
+// first batch's starting points), else the heuristic.  Tuning leaves the workspace sized for
+// the largest candidate split.
+PlanKey plan_key(int precision, int mp, int np, int bp, bool allow_big) {
+    // the tile choice only exists for 16-bit shapes the large tiles fit, so FP32 solves and
+    // such shapes share one plan between solo and fused solves
+    return PlanKey{precision, mp, np, bp, allow_big && gemm_big_tiles(precision, mp, bp) ? 1 : 0};
+}
+
+// The cached plan of a shape, else the heuristic (never tunes).
+Plan peek_plan(spca_ctx ctx, int precision, int mp, int np, int bp, bool allow_big) {
+    std::lock_guard<std::mutex> g(ctx->plan_mu);
+    auto it = ctx->plans.find(plan_key(precision, mp, np, bp, allow_big));
+    return it != ctx->plans.end() ? it->second : heuristic_plan(precision, mp, np, bp, allow_big);
+}
+
+int plan_for(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, int bp, bool allow_big, const float* X0,
+             Plan* out, Workspace* wsp = nullptr) {
+    const PlanKey key = plan_key(st.precision, M->mp, M->np, bp, allow_big);
+    allow_big = key.allow_big != 0;
+    {
+        std::lock_guard<std::mutex> g(ctx->plan_mu);
+        auto it = ctx->plans.find(key);
+        if (it != ctx->plans.end()) {
+            *out = it->second;
+            return SPCA_OK;
+        }
+        if (!ctx->tuning) {
+            *out = heuristic_plan(st.precision, M->mp, M->np, bp, allow_big);
+            return SPCA_OK;
+        }
+    }
+    Workspace& w = wsp ? *wsp : E.ws;
+    int max_splits = 16;
+    while (max_splits > 1 && (uint64_t)M->mp * bp * 4 * max_splits > (512ull << 20)) max_splits /= 2;
+    int rc = ensure_ws(ctx, w, E.stream, M->np, M->mp, bp, max_splits);
+    if (rc) return rc;
+    std::vector<float> hX((size_t)M->n * st.batch);
+    make_starts(st, M->n, X0, 0, st.batch, hX.data());
+    if ((rc = load_batch(ctx, E.stream, w, st, M->n, st.batch, hX.data()))) return rc;
+    Plan p;
+    if ((rc = tune_plan(ctx, E, M, st, w, allow_big, &p))) return rc;
+    std::lock_guard<std::mutex> g(ctx->plan_mu);
+    ctx->plans[key] = p;
+    save_plans(ctx);
+    *out = p;
+    return SPCA_OK;
+}
+
 // Kernel time per phase (0: A*X, 1: A^T*Z, 2: epilogue/step/convergence), from
 // hipEvent pairs recorded on the solver stream and folded in after each host sync.
 struct PhaseTimer {
@@ -613,7 +810,9 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     const int n = M->n, m = M->m;
     const int batch = st.batch;
     const int bp = round_up(batch, kPadB);
-    const int splits = choose_splits(st.precision, M->mp, M->np, bp);
+    Plan plan;
+    if ((rc = plan_for(ctx, E, M, st, bp, true, X0, &plan))) return rc;
+    const int splits = plan.splits;
     if ((rc = ensure_workspace(ctx, E, M->np, M->mp, bp, splits))) return rc;
     Workspace& w = E.ws;
     hipStream_t s = E.stream;
@@ -719,7 +918,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
-                                        M->mp, zsplit, M->mp, bp, M->np, splits, sp.done, s));
+                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -729,7 +928,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V,
-                                        M->np, 0, M->np, bp, M->mp, 1, sp.done, s));
+                                        M->np, 0, M->np, bp, M->mp, 1, plan.big, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -802,7 +1001,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
-                                        M->mp, zsplit, M->mp, bp, M->np, splits, sp.done, s));
+                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -812,7 +1011,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V,
-                                        M->np, 0, M->np, bp, M->mp, 1, sp.done, s));
+                                        M->np, 0, M->np, bp, M->mp, 1, plan.big, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -994,7 +1193,12 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         check_every = std::min(check_every, std::max(1, r.st.check_every));
         max_iter = std::max(max_iter, r.st.max_iter);
         const int bp = round_up(r.st.batch, kPadB);
-        r.splits = choose_splits(precision, r.M->mp, r.M->np, bp, false);
+        Plan plan;
+        if ((rc = plan_for(ctx, E, r.M, r.st, bp, false, J.X0, &plan, &r.ws))) {
+            J.status = rc;
+            return cleanup(rc);
+        }
+        r.splits = plan.splits;
         if ((rc = ensure_ws(ctx, r.ws, s, r.M->np, r.M->mp, bp, r.splits))) {
             J.status = rc;
             return cleanup(rc);
@@ -1392,6 +1596,25 @@ int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, con
 }
 
 
+int spca_set_tuning(spca_ctx ctx, int mode, const char* cache_path) {
+    if (!ctx || mode < 0 || mode > 1) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->plan_mu);
+    ctx->tuning = mode;
+    ctx->tune_path = cache_path ? cache_path : "";
+    if (!ctx->tune_path.empty()) load_plans(ctx);
+    return SPCA_OK;
+}
+
+int spca_get_plan(spca_ctx ctx, int precision, int m, int n, int batch, int fused, int* splits, int* big,
+                  double* tuned_ms) {
+    if (!ctx || m <= 0 || n <= 0 || batch <= 0 || precision < 0 || precision > 2) return SPCA_EINVAL;
+    const Plan p = peek_plan(ctx, precision, round_up(m, kPadMN), round_up(n, kPadMN), round_up(batch, kPadB), !fused);
+    if (splits) *splits = p.splits;
+    if (big) *big = p.big;
+    if (tuned_ms) *tuned_ms = p.ms;
+    return SPCA_OK;
+}
+
 int spca_solve_fused(spca_ctx ctx, spca_fused_job* jobs, int njobs, unsigned flags) {
     if (!ctx) return SPCA_EINVAL;
     std::lock_guard<std::mutex> g(ctx->mu);
@@ -1460,7 +1683,7 @@ int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, co
         J->np = M->np;
         J->mp = M->mp;
         J->bp = round_up(chk.batch, kPadB);
-        J->splits = choose_splits(J->st.precision, M->mp, M->np, J->bp);
+        J->splits = peek_plan(ctx, J->st.precision, M->mp, M->np, J->bp, true).splits;
         const uint64_t need_bytes = workspace_bytes(J->np, J->mp, J->bp, J->splits);
         if (need_bytes + M->bytes > ctx->budget) {
             delete J;

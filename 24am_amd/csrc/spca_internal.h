@@ -106,12 +106,13 @@ hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int preci
 // C[r + c*ldc + split*split_stride] = sum_{k in split} P(r, k) * Q(c, k)
 // FP32: P(r, k) = P32[r*ldp + k], Q(c, k) = Q32[c*ldq + k].  16-bit: K-blocked operands with
 // ldp / ldq rows, P(r, k) = P[((k / 32) * ldp + r) * 32 + k % 32] (likewise Q).
-// M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.  16-bit products with M and N
-// multiples of 256 run the 256x256-tile kernel (gemm_big_tiles).
+// M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.  big = 1 runs the pipelined 256x256-tile
+// kernel, allowed for 16-bit products with M and N multiples of 256 (gemm_big_tiles); else
+// 128x128 tiles.
 int gemm_big_tiles(int precision, int M, int N);
 hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo,
                           long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq,
-                          float* C, long ldc, long split_stride, int M, int N, int K, int splits,
+                          float* C, long ldc, long split_stride, int M, int N, int K, int splits, int big,
                           const int* done, hipStream_t st);
 
 // Sum split partials of Z, per-column statistic and map (sign / normalise), write the

@@ -559,10 +559,11 @@ int gemm_big_tiles(int precision, int M, int N) { return precision != 0 && M % 2
 
 hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo, long ldp,
                           const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq, float* C,
-                          long ldc, long split_stride, int M, int N, int K, int splits, const int* done,
+                          long ldc, long split_stride, int M, int N, int K, int splits, int big, const int* done,
                           hipStream_t st) {
     if (M % kGemmBM || N % kGemmBN || splits < 1 || K % (kGemmBK * splits)) return hipErrorInvalidValue;
-    if (gemm_big_tiles(precision, M, N)) {
+    if (big && !gemm_big_tiles(precision, M, N)) return hipErrorInvalidValue;
+    if (big) {
         const int tiles_m = M / 256, tiles_n = N / 256, ksplit = K / splits;
         const int grid = tiles_m * tiles_n * splits;
         if (precision == 1)

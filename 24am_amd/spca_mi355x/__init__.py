@@ -35,7 +35,7 @@ EXPORTS = [
     "spca_dense_solver_device", "spca_dense_solver_host", "spca_settings_default", "spca_settings_check",
     "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
-    "spca_wait", "spca_solve_fused",
+    "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan",
 ]
 FUSED_GRAPH = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
@@ -133,6 +133,9 @@ def lib():
             "spca_cancel": (I, [P, U64]),
             "spca_wait": (I, [P, U64, P, ctypes.POINTER(_Stats), P, ctypes.POINTER(_JobTimes)]),
             "spca_solve_fused": (I, [P, ctypes.POINTER(_FusedJob), I, U]),
+            "spca_set_tuning": (I, [P, I, ctypes.c_char_p]),
+            "spca_get_plan": (I, [P, I, I, I, I, I, ctypes.POINTER(I), ctypes.POINTER(I),
+                                  ctypes.POINTER(ctypes.c_double)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -275,6 +278,18 @@ class Context:
 
     def stop_server(self):
         _check(self._p, lib().spca_server_stop(self._p))
+
+    def set_tuning(self, mode, cache_path=None):
+        """mode 1: time the product plans of each new solve shape and keep the fastest
+        (cached in memory and, with cache_path, in a JSON file); mode 0: fixed heuristic."""
+        _check(self._p, lib().spca_set_tuning(self._p, int(mode), cache_path.encode() if cache_path else None))
+
+    def plan(self, precision, m, n, batch, fused=False):
+        """(splits, big, tuned_ms) of the plan a solve of this shape uses now."""
+        sp, bg, ms = ctypes.c_int(), ctypes.c_int(), ctypes.c_double()
+        _check(self._p, lib().spca_get_plan(self._p, int(precision), int(m), int(n), int(batch), int(fused),
+                                            ctypes.byref(sp), ctypes.byref(bg), ctypes.byref(ms)))
+        return sp.value, bg.value, ms.value
 
     def solve_fused(self, jobs, graph=True, per_sp=True):
         """Solve uneven jobs together, one fused launch per phase and iteration.

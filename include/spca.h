@@ -194,6 +194,21 @@ typedef struct spca_fused_job {
  * (c++/src/gpugpower/gpu_sparse_PCA_solver.h:29-211). */
 int spca_solve_fused(spca_ctx ctx, spca_fused_job* jobs, int njobs, unsigned flags);
 
+/* ---- product plans (autotuner) --------------------------------------------- */
+/* A solve shape (precision, padded m, n, batch) runs its two products with a plan: the K
+ * split of A*X and the tile kernel (128x128, or 256x256 for 16-bit shapes it fits).
+ * mode 0 (default): a fixed heuristic.  mode 1: the first solve of a shape times every valid
+ * plan on its own matrix and starting points and keeps the fastest.  cache_path (may be
+ * NULL): measured plans are loaded from and saved to this JSON file.  A plan fixes the
+ * summation order of A*X, so results are reproducible for a given plan; solo and fused
+ * solves of a shape without a tile choice share one plan.  No counterpart in the reference
+ * (cuBLAS chooses its own kernels, c++/src/gpugpower/my_cublas_wrapper.h:100-133). */
+int spca_set_tuning(spca_ctx ctx, int mode, const char* cache_path);
+/* The plan a solve of this shape would use now (fused != 0: the fused path's 128x128 tiles);
+ * tuned_ms is the measured time of A*X + epilogue + A^T*Z, 0 for an untuned plan. */
+int spca_get_plan(spca_ctx ctx, int precision, int m, int n, int batch, int fused, int* splits, int* big,
+                  double* tuned_ms);
+
 /* Drop-in for GPUSolver::denseDataSolver<float>(handle, m, n, d_B, h_x, settings, stats,
  * LD_M, LD_N) (c++/src/gpugpower/gpu_sparse_PCA_solver.h:29-35): d_B is a caller-owned
  * device matrix with leading dimension LD_M; h_x (n floats, host) receives the result.
