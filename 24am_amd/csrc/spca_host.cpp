@@ -305,6 +305,108 @@ int make_matrix(spca_ctx ctx, const float* src, bool src_on_device, int m, int n
     return SPCA_OK;
 }
 
+// ---- synthetic spiked matrices (device noise, planted sparse loadings)
+// Normal variate of counter (a, b, stream) in double: Box-Muller on Philox outputs 0 and 1.
+double host_normal(uint32_t a, uint32_t b, uint32_t stream, uint64_t seed) {
+    uint32_t c[4] = {a, b, stream, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const double u0 = ((double)c[0] + 0.5) * 2.3283064365386963e-10, u1 = ((double)c[1] + 0.5) * 2.3283064365386963e-10;
+    return std::sqrt(-2.0 * std::log(u0)) * std::cos(6.283185307179586 * u1);
+}
+
+// A = N + sum_k sqrt(theta_k) g_k v_k^T: N(0,1) noise N from Philox counters (r, c, 0);
+// loading k: spike_nnz distinct columns, the t-th candidate column being output 0 of
+// counter (t, k, 3) mod n (repeats skipped), values v from counters (t, k, 2) normalised to
+// unit 2-norm, g_k[r] from counters (r, k, 1).  loadings (n x nspikes, optional) receives v.
+int make_generated(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const double* theta, int spike_nnz,
+                   unsigned storage, uint64_t key, float* loadings, spca_matrix* out) {
+    if (!ctx || !out || m <= 0 || n <= 0 || nspikes < 0 || (nspikes > 0 && (!theta || spike_nnz < 1)))
+        return fail(ctx, SPCA_EINVAL, "bad generator arguments");
+    if (key && !loadings) {
+        for (auto& kv : ctx->mats) {
+            Matrix* M = kv.second;
+            if (M->key == key && M->m == m && M->n == n) {
+                int rc = add_storage(ctx, M, storage);
+                if (rc) return rc;
+                M->refs++;
+                *out = kv.first;
+                return SPCA_OK;
+            }
+        }
+    }
+    Matrix* M = new Matrix();
+    M->m = m;
+    M->n = n;
+    M->mp = round_up(m, kPadMN);
+    M->np = round_up(n, kPadMN);
+    M->key = key;
+    M->refs = 1;
+    int rc = dev_alloc(ctx, &M->a32_cm, (size_t)M->mp * M->np, &M->bytes);
+    if (rc) {
+        delete M;
+        return rc;
+    }
+    hipStream_t st = ctx->stream;
+    const int nnz = std::min(spike_nnz, n);
+    int* didx = nullptr;
+    float* dbuf = nullptr;
+    uint64_t tb = 0;
+    auto done = [&](int code) {
+        (void)hipStreamSynchronize(st);
+        dev_free(ctx, didx, sizeof(int) * (uint64_t)std::max(nnz, 1));
+        dev_free(ctx, dbuf, sizeof(float) * (uint64_t)(std::max(nnz, 1) + m));
+        if (code != SPCA_OK) free_matrix(ctx, M);
+        return code;
+    };
+    if (hipSuccess != launch_gen_noise(M->a32_cm, m, n, M->mp, M->np, seed, st))
+        return done(fail(ctx, SPCA_EDEVICE, "noise generation failed"));
+    if (loadings) std::memset(loadings, 0, sizeof(float) * (size_t)n * nspikes);
+    if (nspikes > 0) {
+        if ((rc = dev_alloc(ctx, &didx, (size_t)nnz, &tb)) || (rc = dev_alloc(ctx, &dbuf, (size_t)nnz + m, &tb)))
+            return done(rc);
+        for (int k = 0; k < nspikes; k++) {
+            std::vector<int> idx;
+            std::vector<char> used(n, 0);
+            for (uint32_t t = 0; (int)idx.size() < nnz; t++) {
+                uint32_t c[4] = {t, (uint32_t)k, 3u, 0u};
+                philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+                const int col = (int)(c[0] % (uint32_t)n);
+                if (!used[col]) {
+                    used[col] = 1;
+                    idx.push_back(col);
+                }
+            }
+            std::vector<double> v(nnz);
+            double nv = 0;
+            for (int t = 0; t < nnz; t++) {
+                v[t] = host_normal((uint32_t)t, (uint32_t)k, 2u, seed);
+                nv += v[t] * v[t];
+            }
+            nv = std::sqrt(nv);
+            std::vector<float> h((size_t)nnz + m);
+            for (int t = 0; t < nnz; t++) {
+                v[t] /= nv;
+                h[t] = (float)(std::sqrt(theta[k]) * v[t]);
+                if (loadings) loadings[(size_t)k * n + idx[t]] = (float)v[t];
+            }
+            for (int r = 0; r < m; r++) h[nnz + r] = (float)host_normal((uint32_t)r, (uint32_t)k, 1u, seed);
+            // the copies are synchronous with respect to the host buffers, which change per loading
+            if (hipMemcpyAsync(didx, idx.data(), sizeof(int) * nnz, hipMemcpyHostToDevice, st) != hipSuccess ||
+                hipMemcpyAsync(dbuf, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+                launch_plant(M->a32_cm, m, M->mp, didx, dbuf, nnz, dbuf + nnz, st) != hipSuccess ||
+                hipStreamSynchronize(st) != hipSuccess)
+                return done(fail(ctx, SPCA_EDEVICE, "planting a loading failed"));
+        }
+    }
+    if (hipStreamSynchronize(st) != hipSuccess) return done(fail(ctx, SPCA_EDEVICE, "generation failed"));
+    if ((rc = add_storage(ctx, M, storage))) return done(rc);
+    done(SPCA_OK);
+    const uint64_t h = ctx->next_handle++;
+    ctx->mats[h] = M;
+    *out = h;
+    return SPCA_OK;
+}
+
 void free_ws(spca_ctx ctx, Workspace& w, hipStream_t s) {
     (void)hipStreamSynchronize(s);
     void* ptrs[] = {w.X, w.Xhi, w.Xlo, w.Zpart, w.Z, w.Zhi, w.Zlo, w.V, w.fstate, w.istate, w.ev};
@@ -442,6 +544,7 @@ Operand z_operand(Workspace& w, int precision) {
 
 // Starting points with global indices shift .. shift+cnt-1 (n floats each) into out.
 void make_starts(const spca_settings& st, int n, const float* X0, int shift, int cnt, float* out) {
+    if (st.start == SPCA_START_GAUSSIAN) return;  // generated on the device by load_batch
     const bool constrained = st.formulation < 4;
     for (int jj = 0; jj < cnt; jj++) {
         float* col = out + (size_t)jj * n;
@@ -458,12 +561,15 @@ void make_starts(const spca_settings& st, int n, const float* X0, int shift, int
 // operand forms, and the per-SP state reset (val = err = 0, iters = nnz = 0,
 // active = (j < batch), done = 0).  Returns after the copies have completed.
 int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& st, int n, int batch,
-               const float* hX) {
+               const float* hX, int shift = 0) {
     const Operand xop = x_operand(w, st.precision);
     const SpState sp = state_of(w);
     HIP_TRY(ctx, hipMemsetAsync(w.X, 0, sizeof(float) * (size_t)w.np * w.bp, s));
-    HIP_TRY(ctx, hipMemcpy2DAsync(w.X, sizeof(float) * w.np, hX, sizeof(float) * n, sizeof(float) * n, batch,
-                                  hipMemcpyHostToDevice, s));
+    if (st.start == SPCA_START_GAUSSIAN)
+        HIP_TRY(ctx, launch_gen_starts(w.X, w.np, n, batch, shift, st.seed, s));
+    else
+        HIP_TRY(ctx, hipMemcpy2DAsync(w.X, sizeof(float) * w.np, hX, sizeof(float) * n, sizeof(float) * n, batch,
+                                      hipMemcpyHostToDevice, s));
     if (st.precision) {
         HIP_TRY(ctx, hipMemsetAsync(w.Xhi, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
         HIP_TRY(ctx, hipMemsetAsync(w.Xlo, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
@@ -507,7 +613,9 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
     HIP_TRY(ctx, hipMemcpyAsync(hit.data(), sp.iters, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(hnnz.data(), sp.nnz, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(&emax, sp.errmax, sizeof(float), hipMemcpyDeviceToHost, s));
-    if (want_ev) {
+    // per-SP explained variance when per-SP results are asked for; else only the batch's best
+    // SP needs it (for stats), which is computed below once it is known
+    if (want_ev && per_sp) {
         HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, M->m, n, w.X, w.np, batch, spca_is_l1_variance(st.formulation),
                                     w.ev, s));
         HIP_TRY(ctx, hipMemcpyAsync(hev.data(), w.ev, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
@@ -542,6 +650,12 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
         R.x_best.assign(n, 0.f);
         HIP_TRY(ctx, hipMemcpyAsync(R.x_best.data(), w.X + (size_t)sel * w.np, sizeof(float) * n,
                                     hipMemcpyDeviceToHost, s));
+        if (want_ev && !per_sp) {
+            HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, M->m, n, w.X + (size_t)sel * w.np, w.np, 1,
+                                        spca_is_l1_variance(st.formulation), w.ev, s));
+            HIP_TRY(ctx, hipMemcpyAsync(&hev[sel], w.ev, sizeof(double), hipMemcpyDeviceToHost, s));
+            HIP_TRY(ctx, hipStreamSynchronize(s));
+        }
         R.ev_best = hev[sel];
     }
     HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -737,7 +851,7 @@ int plan_for(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, int bp
     while (max_splits > 1 && (uint64_t)M->mp * bp * 4 * max_splits > (512ull << 20)) max_splits /= 2;
     int rc = ensure_ws(ctx, w, E.stream, M->np, M->mp, bp, max_splits);
     if (rc) return rc;
-    std::vector<float> hX((size_t)M->n * st.batch);
+    std::vector<float> hX(st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)M->n * st.batch);
     make_starts(st, M->n, X0, 0, st.batch, hX.data());
     if ((rc = load_batch(ctx, E.stream, w, st, M->n, st.batch, hX.data()))) return rc;
     Plan p;
@@ -787,7 +901,7 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
     if (!valid_formulation(st.formulation)) return fail(ctx, SPCA_EINVAL, "formulation must be 0..7");
     if (st.precision < 0 || st.precision > 2) return fail(ctx, SPCA_EINVAL, "precision must be 0..2");
     if (st.max_iter < 1 || st.total_sp < 1 || st.batch < 0) return fail(ctx, SPCA_EINVAL, "bad iteration/SP counts");
-    if (st.start < SPCA_START_EXPLICIT || st.start > SPCA_START_REFERENCE_F64)
+    if (st.start < SPCA_START_EXPLICIT || st.start > SPCA_START_GAUSSIAN)
         return fail(ctx, SPCA_EINVAL, "start must be a spca_start value");
     if (st.start == SPCA_START_EXPLICIT && !X0) return fail(ctx, SPCA_EINVAL, "X0 is required for explicit starts");
     if (!x_out) return fail(ctx, SPCA_EINVAL, "x_out is required");
@@ -841,7 +955,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     sa.gamma = st.penalty;
     sa.batch = batch;
 
-    std::vector<float> hX((size_t)n * batch);
+    std::vector<float> hX(st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)n * batch);
     Results R;
     double loop_ms = 0;
     PhaseTimer pt{&E};
@@ -882,11 +996,14 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         o.xbest = xbest;
         // pool of all starting points, padded rows zero
         {
-            std::vector<float> hp((size_t)n * total);
+            std::vector<float> hp(st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)n * total);
             make_starts(st, n, X0, 0, total, hp.data());
             HIP_TRY(ctx, hipMemsetAsync(pool, 0, sizeof(float) * (size_t)w.np * total, s));
-            HIP_TRY(ctx, hipMemcpy2DAsync(pool, sizeof(float) * w.np, hp.data(), sizeof(float) * n, sizeof(float) * n,
-                                          total, hipMemcpyHostToDevice, s));
+            if (st.start == SPCA_START_GAUSSIAN)
+                HIP_TRY(ctx, launch_gen_starts(pool, w.np, n, total, 0, st.seed, s));
+            else
+                HIP_TRY(ctx, hipMemcpy2DAsync(pool, sizeof(float) * w.np, hp.data(), sizeof(float) * n,
+                                              sizeof(float) * n, total, hipMemcpyHostToDevice, s));
             HIP_TRY(ctx, hipMemsetAsync(w.X, 0, sizeof(float) * (size_t)w.np * w.bp, s));
             HIP_TRY(ctx, hipMemcpyAsync(w.X, pool, sizeof(float) * (size_t)w.np * batch, hipMemcpyDeviceToDevice, s));
             if (st.precision) {
@@ -994,7 +1111,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     for (int bt = 0; bt < total_batches; bt++) {
         const int shift = bt * batch;
         make_starts(st, n, X0, shift, batch, hX.data());
-        if ((rc = load_batch(ctx, s, w, st, n, batch, hX.data()))) return rc;
+        if ((rc = load_batch(ctx, s, w, st, n, batch, hX.data(), shift))) return rc;
         HIP_TRY(ctx, hipEventRecord(loop_ev[0], s));
         for (int it = 0; it < st.max_iter; it++) {
             hipEvent_t e0, e1;
@@ -1203,7 +1320,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
             J.status = rc;
             return cleanup(rc);
         }
-        std::vector<float> hX((size_t)r.M->n * r.st.batch);
+        std::vector<float> hX(r.st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)r.M->n * r.st.batch);
         make_starts(r.st, r.M->n, J.X0, 0, r.st.batch, hX.data());
         if ((rc = load_batch(ctx, s, r.ws, r.st, r.M->n, r.st.batch, hX.data()))) return cleanup(rc);
     }
@@ -1551,6 +1668,28 @@ int spca_matrix_from_device(spca_ctx ctx, const float* d_A, int m, int n, int ld
     std::lock_guard<std::mutex> g(ctx->mu);
     (void)hipSetDevice(ctx->device);
     return make_matrix(ctx, d_A, true, m, n, ld, storage, key, out);
+}
+
+int spca_matrix_generate(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const double* theta,
+                         int spike_nnz, unsigned storage, uint64_t key, float* loadings, spca_matrix* out) {
+    if (!ctx) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    return make_generated(ctx, m, n, seed, nspikes, theta, spike_nnz, storage, key, loadings, out);
+}
+
+int spca_matrix_download(spca_ctx ctx, spca_matrix mat, float* A, int ld) {
+    if (!ctx || !A) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto it = ctx->mats.find(mat);
+    if (it == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+    Matrix* M = it->second;
+    if (ld < M->m) return fail(ctx, SPCA_EINVAL, "ld < m");
+    (void)hipSetDevice(ctx->device);
+    HIP_TRY(ctx, hipMemcpy2DAsync(A, sizeof(float) * ld, M->a32_cm, sizeof(float) * M->mp, sizeof(float) * M->m,
+                                  M->n, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SPCA_OK;
 }
 
 int spca_matrix_add_storage(spca_ctx ctx, spca_matrix mat, unsigned storage) {

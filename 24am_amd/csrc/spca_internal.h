@@ -24,6 +24,24 @@ constexpr int kSelMaxEPT = 64;  // elements per thread held in registers (n <= 6
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// Philox4x32-10 block (Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as easy as
+// 1, 2, 3", SC 2011): ten rounds on the 128-bit counter c with the 64-bit key (k0, k1).
+__host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+    for (int r = 0; r < 10; r++) {
+        if (r) {
+            k0 += 0x9E3779B9u;
+            k1 += 0xBB67AE85u;
+        }
+        const uint64_t p0 = (uint64_t)0xD2511F53u * c[0], p1 = (uint64_t)0xCD9E8D57u * c[2];
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0, hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        const uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+        c[0] = n0;
+        c[1] = lo1;
+        c[2] = n2;
+        c[3] = lo0;
+    }
+}
+
 // Per-starting-point device state (structure of arrays, length = padded batch).
 struct SpState {
     float* val;      // value of the last completed iteration (reference vals[i].val)
@@ -134,6 +152,15 @@ hipError_t launch_otf_converge(SpState sp, OtfState o, int batch, double tol, in
                                hipStream_t st);
 hipError_t launch_otf_refill(SpState sp, OtfState o, Operand xop, int precision, int n, int ldx, int batch,
                              hipStream_t st);
+
+// Synthetic spiked matrices: A (mp x np column major) <- N(0,1) noise from Philox4x32-10
+// counters (r, c, 0) keyed by seed (padding zero); then A[:, idx[t]] += coef[t] * g, one
+// planted loading per call.
+hipError_t launch_gen_noise(float* A, int m, int n, int mp, int np, uint64_t seed, hipStream_t st);
+// Starting points shift .. shift+batch-1 generated in place (columns of X, ld ldx).
+hipError_t launch_gen_starts(float* X, int ldx, int n, int batch, int shift, uint64_t seed, hipStream_t st);
+hipError_t launch_plant(float* A, int m, int mp, const int* idx, const float* coef, int cnt, const float* g,
+                        hipStream_t st);
 
 // One AM iteration of every job in `jobs` (device array, sorted by t1/t2/c0) as five
 // launches: A*X tiles of all jobs, the z epilogue of all columns, A^T*Z tiles, the

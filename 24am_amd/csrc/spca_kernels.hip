@@ -186,6 +186,68 @@ hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int preci
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- synthetic spiked matrices
+// Counter-based normal variates: philox4x32_10 on counter (a, b, stream, 0) with the 64-bit
+// seed as key; outputs 0 and 1, as u = (x + 0.5) * 2^-32 in (0, 1), go through Box-Muller:
+// sqrt(-2 ln u0) * cos(2 pi u1).
+__device__ inline float philox_normal(uint32_t a, uint32_t b, uint32_t stream, uint64_t seed) {
+    uint32_t c[4] = {a, b, stream, 0u};
+    philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float u0 = ((float)c[0] + 0.5f) * 2.3283064365386963e-10f;
+    const float u1 = ((float)c[1] + 0.5f) * 2.3283064365386963e-10f;
+    return sqrtf(-2.0f * logf(u0)) * cosf(6.283185307179586f * u1);
+}
+
+// A (mp x np column major, zero padding) <- N(0,1) noise, element (r, c) from counter (r, c, 0).
+__global__ void gen_noise_kernel(float* __restrict__ A, int m, int n, int mp, int np, uint64_t seed) {
+    const long total = (long)mp * np;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e % mp), c = (int)(e / mp);
+        A[e] = (r < m && c < n) ? philox_normal((uint32_t)r, (uint32_t)c, 0u, seed) : 0.f;
+    }
+}
+// A[:, idx[t]] += coef[t] * g for t < cnt (one planted loading; g has m entries)
+__global__ void plant_kernel(float* __restrict__ A, int m, int mp, const int* __restrict__ idx,
+                             const float* __restrict__ coef, int cnt, const float* __restrict__ g) {
+    const long total = (long)m * cnt;
+    for (long e = blockIdx.x * (long)blockDim.x + threadIdx.x; e < total; e += (long)gridDim.x * blockDim.x) {
+        const int r = (int)(e % m), t = (int)(e / m);
+        float* a = A + (long)idx[t] * mp + r;
+        *a = *a + coef[t] * g[r];
+    }
+}
+// X[:, j] (j < batch, ld ldx) <- unit-normalised normals of counters (i, shift + j, 4), i < n;
+// rows n..ldx stay as they are (zero).  One workgroup per column; the norm in FP64.
+__global__ __launch_bounds__(256) void gen_starts_kernel(float* __restrict__ X, int ldx, int n, int shift,
+                                                         uint64_t seed) {
+    __shared__ double red[4];
+    const int j = blockIdx.x;
+    float* x = X + (long)j * ldx;
+    double acc = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const float v = philox_normal((uint32_t)i, (uint32_t)(shift + j), 4u, seed);
+        x[i] = v;
+        acc += (double)v * v;
+    }
+    const float inv = (float)(1.0 / sqrt(block_sum<256>(acc, red)));
+    for (int i = threadIdx.x; i < n; i += 256) x[i] = x[i] * inv;
+}
+hipError_t launch_gen_starts(float* X, int ldx, int n, int batch, int shift, uint64_t seed, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    gen_starts_kernel<<<batch, 256, 0, st>>>(X, ldx, n, shift, seed);
+    return hipGetLastError();
+}
+hipError_t launch_gen_noise(float* A, int m, int n, int mp, int np, uint64_t seed, hipStream_t st) {
+    gen_noise_kernel<<<8192, 256, 0, st>>>(A, m, n, mp, np, seed);
+    return hipGetLastError();
+}
+hipError_t launch_plant(float* A, int m, int mp, const int* idx, const float* coef, int cnt, const float* g,
+                        hipStream_t st) {
+    if (cnt <= 0) return hipSuccess;
+    plant_kernel<<<1024, 256, 0, st>>>(A, m, mp, idx, coef, cnt, g);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- products
 // C[r + c*ldc] = sum_k P[r*ldp + k] * Q[c*ldq + k]: both operands are contiguous along
 // the reduction, because the matrix is resident in both orientations (row major for
@@ -1174,9 +1236,24 @@ __device__ __forceinline__ int find_job(const FusedJob* __restrict__ jobs, int n
 }
 
 // WHICH 0: Zpart = A*X (K split), WHICH 1: V = A^T*Z.
+// Windowed XCD mapping for the fused tile list: the list is cut into windows of 8*kWin tiles
+// and XCD x takes the x-th run of kWin consecutive tiles of each window (the last, partial
+// window maps straight through).  All XCDs work inside one window at a time, so the tiles in
+// flight span one or two jobs' matrices rather than eight regions of a list that covers
+// every job (fewer distinct pages and L2 sets at once), and each XCD still gets runs of
+// consecutive tiles that share operand panels.
+constexpr int kWin = 96;
+__device__ __forceinline__ int xcd_window_remap(int bid, int total) {
+    const int win = 8 * kWin;
+    const int base = bid / win * win;
+    if (base + win > total) return bid;
+    const int r = bid - base;
+    return base + (r & 7) * kWin + (r >> 3);
+}
+
 template <int PREC, int WHICH>
 __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restrict__ jobs, int njobs, int total) {
-    const int t = xcd_remap(blockIdx.x, total);
+    const int t = xcd_window_remap(blockIdx.x, total);
     const int k = find_job<WHICH>(jobs, njobs, t);
     const FusedJob& J = jobs[k];
     if (*J.sp.done) return;

@@ -26,7 +26,7 @@ STATUS_NAMES = {0: "SPCA_OK", 1: "SPCA_EINVAL", 2: "SPCA_ENOMEM", 3: "SPCA_EBUDG
 STORE_FP32, STORE_BF16X3, STORE_FP16 = 1, 2, 4
 PREC_FP32, PREC_BF16X3, PREC_FP16 = 0, 1, 2
 TERM_BATCH_MAX, TERM_PER_SP = 0, 1
-START_EXPLICIT, START_REFERENCE, START_REFERENCE_F64 = 0, 1, 2
+START_EXPLICIT, START_REFERENCE, START_REFERENCE_F64, START_GAUSSIAN = 0, 1, 2, 3
 
 # every symbol include/spca.h declares
 EXPORTS = [
@@ -35,7 +35,8 @@ EXPORTS = [
     "spca_dense_solver_device", "spca_dense_solver_host", "spca_settings_default", "spca_settings_check",
     "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
-    "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan",
+    "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan", "spca_matrix_generate",
+    "spca_matrix_download",
 ]
 FUSED_GRAPH = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
@@ -64,7 +65,7 @@ class _Settings(ctypes.Structure):
         ("tolerance", ctypes.c_double), ("max_iter", ctypes.c_int), ("total_sp", ctypes.c_int),
         ("batch", ctypes.c_int), ("use_otf", ctypes.c_int), ("termination", ctypes.c_int),
         ("precision", ctypes.c_int), ("start", ctypes.c_int), ("check_every", ctypes.c_int),
-        ("resolve_iters", ctypes.c_int), ("resolve_tol", ctypes.c_double),
+        ("resolve_iters", ctypes.c_int), ("resolve_tol", ctypes.c_double), ("seed", ctypes.c_uint64),
     ]
 
 
@@ -134,6 +135,8 @@ def lib():
             "spca_wait": (I, [P, U64, P, ctypes.POINTER(_Stats), P, ctypes.POINTER(_JobTimes)]),
             "spca_solve_fused": (I, [P, ctypes.POINTER(_FusedJob), I, U]),
             "spca_set_tuning": (I, [P, I, ctypes.c_char_p]),
+            "spca_matrix_generate": (I, [P, I, I, U64, I, P, I, U, U64, P, ctypes.POINTER(U64)]),
+            "spca_matrix_download": (I, [P, U64, P, I]),
             "spca_get_plan": (I, [P, I, I, I, I, I, ctypes.POINTER(I), ctypes.POINTER(I),
                                   ctypes.POINTER(ctypes.c_double)]),
         }
@@ -168,6 +171,7 @@ class OptimizationSettings:
         self.check_every = 1
         self.resolve_iters = 0  # > 0: FP64 re-solve on the support of the result
         self.resolve_tol = 1e-12
+        self.seed = 0  # START_GAUSSIAN: key of the device-generated starting points
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -193,7 +197,8 @@ class OptimizationSettings:
         return _Settings(int(self.formulation), int(self.constraintParameter), float(self.penaltyParameter),
                          float(self.tolerance), int(self.maximumIterations), int(self.totalStartingPoints),
                          int(self.batchSize), int(bool(self.useOTF)), int(self.termination), int(self.precision),
-                         int(self.start), int(self.check_every), int(self.resolve_iters), float(self.resolve_tol))
+                         int(self.start), int(self.check_every), int(self.resolve_iters), float(self.resolve_tol),
+                         int(self.seed))
 
 
 class OptimizationStatistics:
@@ -325,6 +330,20 @@ class Context:
             result.append((x, S))
         return result
 
+    def generate(self, m, n, seed, theta=(), spike_nnz=0, storage=STORE_FP32, key=0, want_loadings=False):
+        """Spiked m x n matrix generated on the device (Philox noise + planted loadings).
+        Returns the Matrix, and with want_loadings also the n x len(theta) loadings."""
+        th = np.ascontiguousarray(theta, dtype=np.float64)
+        L = np.zeros((n, len(th)), dtype=np.float32, order="F") if want_loadings else None
+        h = ctypes.c_uint64()
+        _check(self._p, lib().spca_matrix_generate(self._p, m, n, int(seed), len(th),
+                                                   th.ctypes.data_as(ctypes.c_void_p) if len(th) else None,
+                                                   int(spike_nnz), storage, key,
+                                                   L.ctypes.data_as(ctypes.c_void_p) if want_loadings else None,
+                                                   ctypes.byref(h)))
+        M = Matrix(self, h.value, m, n)
+        return (M, L) if want_loadings else M
+
     def upload(self, A, storage=STORE_FP32, key=0):
         """A: (m x n) float32 array (any order); copied column major to the device."""
         A = np.asfortranarray(A, dtype=np.float32)
@@ -338,6 +357,13 @@ class Context:
 class Matrix:
     def __init__(self, ctx, handle, m, n):
         self.ctx, self.handle, self.m, self.n = ctx, handle, m, n
+
+    def download(self):
+        """The resident FP32 matrix as an (m x n) array."""
+        A = np.zeros((self.m, self.n), dtype=np.float32, order="F")
+        _check(self.ctx.ptr, lib().spca_matrix_download(self.ctx.ptr, self.handle, A.ctypes.data_as(ctypes.c_void_p),
+                                                        self.m))
+        return A
 
     def nbytes(self):
         b = ctypes.c_uint64()

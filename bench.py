@@ -66,6 +66,77 @@ def calibrate_gamma(A, v1):
     return out
 
 
+def uneven_specs(count, seed=4):
+    """Uneven jobs: m (samples) uniform in [256, 8192], n (variables) log-uniform in
+    [4096, 262144], formulation uniform over the 8, s from {16, 64, 256, 1024} (below n), B from
+    {32, 128, 512}; each A is a spiked model like the headline one (5 loadings of 256 nonzeros,
+    strengths 20, 15, 10, 8, 6, N(0,1) noise), generated on the device from its own seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(count):
+        m = int(rng.integers(256, 8193))
+        n = int(round(np.exp(rng.uniform(np.log(4096), np.log(262144)))))
+        f = int(rng.integers(0, 8))
+        s = min(int(rng.choice([16, 64, 256, 1024])), n - 1)
+        B = int(rng.choice([32, 128, 512]))
+        out.append(dict(m=m, n=n, f=f, s=s, B=B, seed=1000 + k))
+    return out
+
+
+def noise_gamma(f, s, m, n):
+    """Penalty that keeps about s noise coordinates at the step: the s/(2n) upper quantile t of
+    |N(0,1)| for L2 variance (z normalised, so (A^T z)_i ~ N(0,1)), scaled by sqrt(m) for L1
+    variance (z = sgn, (A^T z)_i ~ N(0,m)); squared for the L0 penalty."""
+    from scipy.stats import norm
+    t = float(norm.isf(s / (2.0 * n)))
+    if f in (5, 7):
+        t *= np.sqrt(m)
+    return t * t if f in (4, 5) else t
+
+
+def run_uneven(S, ctx, count, iters, graph=True):
+    """The uneven jobs solved two ways on one context: one fused solve (one launch per phase
+    per iteration for all jobs, replayed from a hipGraph) and one solo solve per job.  tol = 0
+    and a fixed iteration count, so both do the same SP-iterations."""
+    specs = uneven_specs(count)
+    t0 = time.time()
+    mats = {}
+    jobs = []
+    for sp in specs:
+        M = ctx.generate(sp["m"], sp["n"], sp["seed"], theta=THETA, spike_nnz=SPARSITY, key=sp["seed"])
+        mats[sp["seed"]] = M
+        st = S.OptimizationSettings(formulation=sp["f"], constraintParameter=sp["s"],
+                                    penaltyParameter=noise_gamma(sp["f"], sp["s"], sp["m"], sp["n"]),
+                                    tolerance=0.0, maximumIterations=iters, totalStartingPoints=sp["B"],
+                                    batchSize=sp["B"], precision=S.PREC_FP32, start=S.START_GAUSSIAN,
+                                    seed=sp["seed"], check_every=iters)
+        jobs.append((M, st, None))
+    gen_s = time.time() - t0
+    a_bytes = sum(M.nbytes() for M in mats.values())
+    print(f"# uneven: {count} jobs, A resident {a_bytes / 2**30:.1f} GiB, generated in {gen_s:.1f}s",
+          file=sys.stderr, flush=True)
+    sp_it = sum(sp["B"] * iters for sp in specs)
+    # warm-up (first-touch of every matrix and plan), then the timed fused solve
+    ctx.solve_fused(jobs, graph=graph, per_sp=False)
+    t0 = time.time()
+    fused = ctx.solve_fused(jobs, graph=graph, per_sp=False)
+    fused_s = time.time() - t0
+    t0 = time.time()
+    solo = [M.solve(st, per_sp=False) for (M, st, _) in jobs]
+    solo_s = time.time() - t0
+    same = all(np.array_equal(xf.view(np.uint32), xs.view(np.uint32)) for (xf, _), xs in zip(fused, solo))
+    for M in mats.values():
+        M.release()
+    flops = sum(4.0 * sp["m"] * sp["n"] * sp["B"] * iters for sp in specs)
+    res = {"jobs": count, "iterations": iters, "sp_iterations": sp_it, "A_GiB": a_bytes / 2**30,
+           "fused_s": fused_s, "solo_s": solo_s, "fused_sp_it_per_s": sp_it / fused_s, "solo_sp_it_per_s": sp_it / solo_s,
+           "fused_over_solo": solo_s / fused_s, "fused_tflops": flops / fused_s / 1e12,
+           "bitwise_equal_to_solo": same, "graph": graph, "dtype": "f32"}
+    print(f"# uneven: fused {fused_s:.2f}s ({res['fused_sp_it_per_s']:.0f} SP-it/s, {res['fused_tflops']:.1f} TFLOP/s), "
+          f"solo {solo_s:.2f}s, bitwise equal {same}", file=sys.stderr, flush=True)
+    return res
+
+
 def cpu_baseline(A, X0, threads):
     """Oracle restatement on the host (host BLAS gemm pair + per-column steps), FP32,
     L0_constrained_L2_PCA, on a bounded sample: the full matrix, 128 SPs, 3 iterations."""
@@ -88,6 +159,9 @@ def main():
     ap.add_argument("--formulations", default="0,1,2,3,4,5,6,7")
     ap.add_argument("--precisions", default="fp32,bf16x3,fp16")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--uneven-jobs", type=int, default=64, help="uneven fused-job workload size (0: skip)")
+    ap.add_argument("--uneven-iters", type=int, default=20)
+    ap.add_argument("--uneven-graph", type=int, default=1, help="replay the fused iteration from a hipGraph")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; sharding the starting points over several GPUs is a "
@@ -145,6 +219,8 @@ def main():
                           "kernel_ms_per_step": {"gemm_ax": ax_avg, "gemm_atz": atz_avg, "step": step_ms / launches},
                           "dominant": dom, "dominant_tflops": flops / (dom_ms / 1e3) / 1e12}
         M.release()
+        uneven = (run_uneven(S, ctx, args.uneven_jobs, args.uneven_iters, graph=bool(args.uneven_graph))
+                  if args.uneven_jobs > 0 else None)
     head = results.get("fp32") or next(iter(results.values()))
     hp = "fp32" if "fp32" in results else next(iter(results))
     roof = {"bound": "mfma", "achieved": head["dominant_tflops"], "peak": PEAK[hp], "unit": "TFLOP/s",
@@ -166,6 +242,7 @@ def main():
         "forms": {p: {k: v for k, v in r.items() if k != "per_formulation"} for p, r in results.items()},
         "per_formulation": {p: r["per_formulation"] for p, r in results.items()},
     }
+    line["uneven_jobs"] = uneven
     if "fp32" in results and "bf16x3" in results:
         line["bf16x3_over_fp32"] = results["bf16x3"]["sp_it_per_s"] / results["fp32"]["sp_it_per_s"]
     print(json.dumps(line))

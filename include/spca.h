@@ -72,8 +72,11 @@ typedef enum spca_start {
     SPCA_START_EXPLICIT = 0,     /* caller passes X0 (n x total_sp, column major) */
     SPCA_START_REFERENCE = 1,    /* reference rand_r generator, SP k seeded with k
                                     (c++/src/gpower/gpower_commons.h:21-47), in float */
-    SPCA_START_REFERENCE_F64 = 2 /* the same generator evaluated in double, as the reference's
+    SPCA_START_REFERENCE_F64 = 2, /* the same generator evaluated in double, as the reference's
                                     double-precision runs draw it, then stored as float */
+    SPCA_START_GAUSSIAN = 3      /* generated on the device: SP k is the unit-normalised vector of
+                                    Box-Muller normals of Philox4x32-10 counters (i, k, 4) keyed
+                                    by settings.seed (see spca_matrix_generate) */
 } spca_start;
 
 /* storage forms a matrix handle may carry (bit mask) */
@@ -91,13 +94,15 @@ typedef struct spca_settings {
     int max_iter;             /* maximumIterations */
     int total_sp;             /* totalStartingPoints */
     int batch;                /* batchSize (0 -> total_sp) */
-    int use_otf;              /* useOTF: refill converged slots (not yet supported: must be 0) */
+    int use_otf;              /* useOTF: a finished SP's slot takes the next SP on the device
+                                 (c++/src/gpower/sparse_PCA_solver.h:83-169) */
     int termination;          /* spca_termination */
     int precision;            /* spca_precision */
     int start;                /* spca_start */
     int check_every;          /* host reads the convergence flag every k iterations (>=1) */
     int resolve_iters;        /* > 0: FP64 re-solve on the support of x_out, at most this many iterations */
     double resolve_tol;       /* relative objective change that stops the re-solve */
+    uint64_t seed;            /* key of SPCA_START_GAUSSIAN starting points */
 } spca_settings;
 
 /* mirrors OptimizationStatistics (c++/src/class/optimization_statistics.h:27-46) */
@@ -157,6 +162,19 @@ int spca_matrix_from_device(spca_ctx ctx, const float* d_A, int m, int n, int ld
 /* Add storage forms to an existing matrix. */
 int spca_matrix_add_storage(spca_ctx ctx, spca_matrix mat, unsigned storage);
 int spca_matrix_release(spca_ctx ctx, spca_matrix mat);
+/* Synthetic spiked matrix generated on the device (no host copy of A):
+ * A = N + sum_k sqrt(theta[k]) g_k v_k^T, m x n.  N(0,1) noise N and g_k ~ N(0, I_m) are
+ * Box-Muller normals of Philox4x32-10 outputs 0 and 1 (counter (r, c, 0) for N[r, c],
+ * (r, k, 1) for g_k[r], key = seed); v_k has spike_nnz distinct columns (the t-th candidate
+ * is output 0 of counter (t, k, 3) mod n, repeats skipped) with values from counters
+ * (t, k, 2), normalised to unit 2-norm.  loadings (optional, n x nspikes column major)
+ * receives the v_k.  key != 0 shares a resident matrix as spca_matrix_upload does.  For
+ * benchmarks and tests at sizes a host copy would not serve; the reference has no
+ * counterpart (its experiments build data on the host). */
+int spca_matrix_generate(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const double* theta,
+                         int spike_nnz, unsigned storage, uint64_t key, float* loadings, spca_matrix* out);
+/* copy the resident FP32 matrix back (column major, leading dimension ld >= m) */
+int spca_matrix_download(spca_ctx ctx, spca_matrix mat, float* A, int ld);
 /* device bytes held by a matrix (all its storage forms) */
 int spca_matrix_bytes(spca_ctx ctx, spca_matrix mat, uint64_t* bytes);
 
