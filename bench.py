@@ -137,6 +137,69 @@ def run_uneven(S, ctx, count, iters, graph=True):
     return res
 
 
+def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5):
+    """Request stream: Poisson arrivals at `rate` jobs/s for `seconds`, each job on one of
+    `pool_size` resident seeded matrices (shapes drawn as the uneven jobs), with its own
+    formulation, s and B; tol = 1e-5, at most 200 iterations, device Gaussian starts.  The
+    server admits jobs by HBM footprint under `budget` bytes and runs up to `workers` at once.
+    Latency = submit -> result, measured by the server.  Returns the summary and, per job,
+    the FP64 explained variance of its result (for the cross-precision comparison)."""
+    specs = uneven_specs(pool_size, seed=11)
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    arrivals = []
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > seconds:
+            break
+        arrivals.append(t)
+    store = {S.PREC_FP32: S.STORE_FP32, S.PREC_BF16X3: S.STORE_BF16X3, S.PREC_FP16: S.STORE_FP16}[precision]
+    draws = [(int(rng.integers(0, pool_size)), int(rng.integers(0, 8)), int(rng.choice([16, 64, 256, 1024])),
+              int(rng.choice([32, 128, 512]))) for _ in arrivals]
+    with S.Context(0, hbm_budget_bytes=int(budget)) as ctx:
+        t0 = time.time()
+        pool = [ctx.generate(sp["m"], sp["n"], sp["seed"], theta=THETA, spike_nnz=SPARSITY, storage=store,
+                             key=sp["seed"]) for sp in specs]
+        resident = sum(M.nbytes() for M in pool)
+        print(f"# serving {precision}: pool of {pool_size} matrices, {resident / 2**30:.1f} GiB resident "
+              f"(built in {time.time() - t0:.1f}s), {len(arrivals)} arrivals over {seconds}s", file=sys.stderr, flush=True)
+        ctx.start_server(workers)
+        jobs = []
+        t0 = time.time()
+        for k, (a, (pi, f, sv, B)) in enumerate(zip(arrivals, draws)):
+            now = time.time() - t0
+            if a > now:
+                time.sleep(a - now)
+            M, sp = pool[pi], specs[pi]
+            sv = min(sv, M.n - 1)
+            st = S.OptimizationSettings(formulation=f, constraintParameter=sv,
+                                        penaltyParameter=noise_gamma(f, sv, M.m, M.n), tolerance=1e-5,
+                                        maximumIterations=200, totalStartingPoints=B, batchSize=B,
+                                        precision=precision, start=S.START_GAUSSIAN, seed=100000 + k,
+                                        check_every=4)
+            jobs.append((M.submit(st, per_sp=False), B))
+        lat, sp_its, ev = [], 0, []
+        for j, B in jobs:
+            stats = S.OptimizationStatistics()
+            _, times = j.wait(stats=stats, per_sp=False)
+            lat.append(times["latency_ms"])
+            sp_its += stats.sp_iterations
+            ev.append(stats.ev_fp64)
+        span = time.time() - t0
+        ctx.stop_server()
+        for M in pool:
+            M.release()
+    lat = np.array(lat)
+    res = {"precision": {0: "fp32", 1: "bf16x3", 2: "fp16"}[precision], "arrival_rate": rate, "seconds": seconds,
+           "jobs": len(jobs), "workers": workers, "pool_matrices": pool_size, "resident_GiB": resident / 2**30,
+           "hbm_budget_GB": budget / 1e9, "p50_latency_ms": float(np.percentile(lat, 50)),
+           "p99_latency_ms": float(np.percentile(lat, 99)), "max_latency_ms": float(lat.max()),
+           "jobs_per_s": len(jobs) / span, "sp_it_per_s": sp_its / span, "makespan_s": span, "tol": 1e-5}
+    print(f"# serving {res['precision']}: {len(jobs)} jobs in {span:.1f}s ({res['jobs_per_s']:.1f} jobs/s), latency p50 "
+          f"{res['p50_latency_ms']:.0f} ms p99 {res['p99_latency_ms']:.0f} ms", file=sys.stderr, flush=True)
+    return res, np.array(ev)
+
+
 def cpu_baseline(A, X0, threads):
     """Oracle restatement on the host (host BLAS gemm pair + per-column steps), FP32,
     L0_constrained_L2_PCA, on a bounded sample: the full matrix, 128 SPs, 3 iterations."""
@@ -162,6 +225,10 @@ def main():
     ap.add_argument("--uneven-jobs", type=int, default=64, help="uneven fused-job workload size (0: skip)")
     ap.add_argument("--uneven-iters", type=int, default=20)
     ap.add_argument("--uneven-graph", type=int, default=1, help="replay the fused iteration from a hipGraph")
+    ap.add_argument("--serve-seconds", type=float, default=1.0, help="request-stream length (0: skip serving)")
+    ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals per second")
+    ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-pool", type=int, default=24, help="resident seeded matrices the jobs draw from")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; sharding the starting points over several GPUs is a "
@@ -221,6 +288,16 @@ def main():
         M.release()
         uneven = (run_uneven(S, ctx, args.uneven_jobs, args.uneven_iters, graph=bool(args.uneven_graph))
                   if args.uneven_jobs > 0 else None)
+    serving = None
+    if args.serve_seconds > 0:
+        serving = {}
+        evs = {}
+        for name, prec in (("fp32", S.PREC_FP32), ("bf16x3", S.PREC_BF16X3)):
+            serving[name], evs[name] = run_serving(S, prec, args.serve_rate, args.serve_seconds, args.serve_workers,
+                                                   args.serve_pool, 200e9)
+        rel = np.abs(evs["bf16x3"] - evs["fp32"]) / np.abs(evs["fp32"])
+        serving["bf16x3_vs_fp32_ev"] = {"median_rel": float(np.median(rel)), "max_rel": float(rel.max()),
+                                        "share_within_1e-5": float(np.mean(rel <= 1e-5))}
     head = results.get("fp32") or next(iter(results.values()))
     hp = "fp32" if "fp32" in results else next(iter(results))
     roof = {"bound": "mfma", "achieved": head["dominant_tflops"], "peak": PEAK[hp], "unit": "TFLOP/s",
@@ -243,6 +320,10 @@ def main():
         "per_formulation": {p: r["per_formulation"] for p, r in results.items()},
     }
     line["uneven_jobs"] = uneven
+    line["serving"] = serving
+    if serving:
+        line["p99_latency_ms"] = serving["fp32"]["p99_latency_ms"]
+        line["p50_latency_ms"] = serving["fp32"]["p50_latency_ms"]
     if "fp32" in results and "bf16x3" in results:
         line["bf16x3_over_fp32"] = results["bf16x3"]["sp_it_per_s"] / results["fp32"]["sp_it_per_s"]
     print(json.dumps(line))
