@@ -114,6 +114,11 @@ struct spca_ctx_s {
     int tuning = 0;
     std::string tune_path;
     std::map<spca::PlanKey, spca::Plan> plans;
+    // device blocks: live ones by pointer (size class), freed ones kept for reuse by class.
+    // `used` counts both, so live + cached device memory stays within the budget.
+    std::map<void*, uint64_t> live;
+    std::multimap<uint64_t, void*> cache;
+    uint64_t cached = 0;
 };
 
 using namespace spca;
@@ -141,48 +146,105 @@ int fail(spca_ctx ctx, int code, const std::string& msg) {
                         std::string(#expr) + ": " + hipGetErrorString(e_));                        \
     } while (0)
 
+// Device memory goes through a size-class cache: a freed block is kept and handed to the
+// next request of its class, so solves of changing shapes (the request server) do not pay
+// hipMalloc and, above all, hipFree, which waits for the whole device and would serialise
+// concurrent solves.  Classes are 4 per octave (at most 19% slack); cached blocks count
+// against the HBM budget and are released (largest first) when a request would exceed it.
+uint64_t size_class(uint64_t b) {
+    if (b <= 4096) return 4096;
+    int e = 63 - __builtin_clzll(b - 1);  // 2^e < b <= 2^(e+1)
+    const uint64_t base = 1ull << e, q = base / 4;
+    return base + (b - base + q - 1) / q * q;
+}
+
+// Caller holds ctx->alloc_mu.  Releases cached blocks until `need` more bytes fit the budget.
+void evict_for(spca_ctx ctx, uint64_t need) {
+    while (ctx->used + need > ctx->budget && !ctx->cache.empty()) {
+        auto it = std::prev(ctx->cache.end());
+        (void)hipFree(it->second);
+        ctx->used -= it->first;
+        ctx->cached -= it->first;
+        ctx->cache.erase(it);
+    }
+}
+
+void set_err(spca_ctx ctx, std::string* err, const std::string& m) {
+    if (err) {
+        *err = m;
+        return;
+    }
+    tl_err = m;
+    std::lock_guard<std::mutex> g2(ctx->err_mu);
+    ctx->err = m;
+}
+
+int raw_alloc(spca_ctx ctx, void** p, uint64_t b, std::string* err) {
+    const uint64_t cls = size_class(b);
+    std::lock_guard<std::mutex> g(ctx->alloc_mu);
+    auto it = ctx->cache.find(cls);
+    if (it != ctx->cache.end()) {
+        *p = it->second;
+        ctx->cache.erase(it);
+        ctx->cached -= cls;
+        ctx->live[*p] = cls;
+        return SPCA_OK;
+    }
+    evict_for(ctx, cls);
+    if (ctx->used + cls > ctx->budget) {
+        set_err(ctx, err, "allocation of " + std::to_string(b) + " bytes exceeds the HBM budget");
+        return SPCA_EBUDGET;
+    }
+    hipError_t e = hipMalloc(p, cls);
+    if (e == hipErrorOutOfMemory && ctx->cached) {
+        (void)hipGetLastError();
+        evict_for(ctx, ctx->budget);  // release every cached block and retry once
+        e = hipMalloc(p, cls);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        set_err(ctx, err, std::string("hipMalloc: ") + hipGetErrorString(e));
+        return SPCA_ENOMEM;
+    }
+    ctx->used += cls;
+    ctx->live[*p] = cls;
+    return SPCA_OK;
+}
+
+// The block must be idle: every stream that used it has been synchronised.
+void raw_free(spca_ctx ctx, void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> g(ctx->alloc_mu);
+    auto it = ctx->live.find(p);
+    if (it == ctx->live.end()) return;
+    ctx->cache.emplace(it->second, p);
+    ctx->cached += it->second;
+    ctx->live.erase(it);
+}
+
+void release_cache(spca_ctx ctx) {
+    std::lock_guard<std::mutex> g(ctx->alloc_mu);
+    for (auto& kv : ctx->cache) {
+        (void)hipFree(kv.second);
+        ctx->used -= kv.first;
+    }
+    ctx->cache.clear();
+    ctx->cached = 0;
+}
+
 template <class T>
 int dev_alloc(spca_ctx ctx, T** p, size_t count, uint64_t* acct, std::string* err = nullptr) {
     const uint64_t b = (uint64_t)count * sizeof(T);
-    {
-        std::lock_guard<std::mutex> g(ctx->alloc_mu);
-        if (ctx->used + b > ctx->budget) {
-            const std::string m = "allocation of " + std::to_string(b) + " bytes exceeds the HBM budget";
-            if (err) *err = m;
-            else {
-                tl_err = m;
-                std::lock_guard<std::mutex> g2(ctx->err_mu);
-                ctx->err = m;
-            }
-            return SPCA_EBUDGET;
-        }
-        ctx->used += b;
-    }
-    hipError_t e = hipMalloc((void**)p, b ? b : 16);
-    if (e != hipSuccess) {
-        std::lock_guard<std::mutex> g(ctx->alloc_mu);
-        ctx->used -= b;
-        *p = nullptr;
-        const std::string m = std::string("hipMalloc: ") + hipGetErrorString(e);
-        if (err) *err = m;
-        else {
-            tl_err = m;
-            std::lock_guard<std::mutex> g2(ctx->err_mu);
-            ctx->err = m;
-        }
-        return SPCA_ENOMEM;
-    }
-    *acct += b;
-    return SPCA_OK;
+    void* q = nullptr;
+    const int rc = raw_alloc(ctx, &q, b, err);
+    *p = (T*)q;
+    if (rc == SPCA_OK) *acct += b;
+    return rc;
 }
 template <class T>
-void dev_free(spca_ctx ctx, T*& p, uint64_t bytes_of_p) {
-    if (p) {
-        (void)hipFree(p);
-        std::lock_guard<std::mutex> g(ctx->alloc_mu);
-        ctx->used -= bytes_of_p;
-        p = nullptr;
-    }
+void dev_free(spca_ctx ctx, T*& p, uint64_t /*bytes_of_p*/) {
+    raw_free(ctx, (void*)p);
+    p = nullptr;
 }
 
 bool valid_formulation(int f) { return f >= 0 && f <= 7; }
@@ -190,10 +252,7 @@ bool valid_formulation(int f) { return f >= 0 && f <= 7; }
 void free_matrix(spca_ctx ctx, Matrix* M) {
     (void)hipStreamSynchronize(ctx->stream);
     void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    std::lock_guard<std::mutex> g(ctx->alloc_mu);
-    ctx->used -= M->bytes;
+    for (void* p : ptrs) raw_free(ctx, p);
     delete M;
 }
 
@@ -410,12 +469,7 @@ int make_generated(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const
 void free_ws(spca_ctx ctx, Workspace& w, hipStream_t s) {
     (void)hipStreamSynchronize(s);
     void* ptrs[] = {w.X, w.Xhi, w.Xlo, w.Zpart, w.Z, w.Zhi, w.Zlo, w.V, w.fstate, w.istate, w.ev};
-    for (void* p : ptrs)
-        if (p) (void)hipFree(p);
-    {
-        std::lock_guard<std::mutex> g(ctx->alloc_mu);
-        ctx->used -= w.bytes;
-    }
+    for (void* p : ptrs) raw_free(ctx, p);
     w = Workspace();
 }
 void free_workspace(spca_ctx ctx, Engine& E) { free_ws(ctx, E.ws, E.stream); }
@@ -1641,6 +1695,9 @@ int spca_ctx_destroy(spca_ctx ctx) {
         for (auto& kv : ctx->mats) free_matrix(ctx, kv.second);
         ctx->mats.clear();
         engine_destroy(ctx, ctx->eng);
+        release_cache(ctx);
+        for (auto& kv : ctx->live) (void)hipFree(kv.first);  // none expected
+        ctx->live.clear();
     }
     delete ctx;
     return SPCA_OK;
@@ -1734,6 +1791,15 @@ int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, con
     return solve_and_resolve(ctx, ctx->eng, itm->second, settings, X0, x_out, stats, per_sp, X_final);
 }
 
+
+int spca_ctx_memory(spca_ctx ctx, uint64_t* used, uint64_t* cached, uint64_t* budget) {
+    if (!ctx) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->alloc_mu);
+    if (used) *used = ctx->used;
+    if (cached) *cached = ctx->cached;
+    if (budget) *budget = ctx->budget;
+    return SPCA_OK;
+}
 
 int spca_set_tuning(spca_ctx ctx, int mode, const char* cache_path) {
     if (!ctx || mode < 0 || mode > 1) return SPCA_EINVAL;

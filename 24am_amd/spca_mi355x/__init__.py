@@ -36,7 +36,7 @@ EXPORTS = [
     "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
     "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan", "spca_matrix_generate",
-    "spca_matrix_download",
+    "spca_matrix_download", "spca_ctx_memory",
 ]
 FUSED_GRAPH = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
@@ -137,6 +137,7 @@ def lib():
             "spca_set_tuning": (I, [P, I, ctypes.c_char_p]),
             "spca_matrix_generate": (I, [P, I, I, U64, I, P, I, U, U64, P, ctypes.POINTER(U64)]),
             "spca_matrix_download": (I, [P, U64, P, I]),
+            "spca_ctx_memory": (I, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64)]),
             "spca_get_plan": (I, [P, I, I, I, I, I, ctypes.POINTER(I), ctypes.POINTER(I),
                                   ctypes.POINTER(ctypes.c_double)]),
         }
@@ -283,6 +284,12 @@ class Context:
 
     def stop_server(self):
         _check(self._p, lib().spca_server_stop(self._p))
+
+    def memory(self):
+        """(used, cached, budget) bytes of device memory held by the context."""
+        u, c, b = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(self._p, lib().spca_ctx_memory(self._p, ctypes.byref(u), ctypes.byref(c), ctypes.byref(b)))
+        return u.value, c.value, b.value
 
     def set_tuning(self, mode, cache_path=None):
         """mode 1: time the product plans of each new solve shape and keep the fastest

@@ -178,13 +178,14 @@ def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5)
                                         precision=precision, start=S.START_GAUSSIAN, seed=100000 + k,
                                         check_every=4)
             jobs.append((M.submit(st, per_sp=False), B))
-        lat, sp_its, ev = [], 0, []
-        for j, B in jobs:
+        lat, sp_its, ev, flops = [], 0, [], 0.0
+        for (j, B), (pi, _, _, _) in zip(jobs, draws):
             stats = S.OptimizationStatistics()
             _, times = j.wait(stats=stats, per_sp=False)
             lat.append(times["latency_ms"])
             sp_its += stats.sp_iterations
             ev.append(stats.ev_fp64)
+            flops += 4.0 * specs[pi]["m"] * specs[pi]["n"] * B * stats.it
         span = time.time() - t0
         ctx.stop_server()
         for M in pool:
@@ -194,7 +195,8 @@ def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5)
            "jobs": len(jobs), "workers": workers, "pool_matrices": pool_size, "resident_GiB": resident / 2**30,
            "hbm_budget_GB": budget / 1e9, "p50_latency_ms": float(np.percentile(lat, 50)),
            "p99_latency_ms": float(np.percentile(lat, 99)), "max_latency_ms": float(lat.max()),
-           "jobs_per_s": len(jobs) / span, "sp_it_per_s": sp_its / span, "makespan_s": span, "tol": 1e-5}
+           "jobs_per_s": len(jobs) / span, "sp_it_per_s": sp_its / span, "makespan_s": span, "tol": 1e-5,
+           "algorithmic_tflops": flops / span / 1e12}
     print(f"# serving {res['precision']}: {len(jobs)} jobs in {span:.1f}s ({res['jobs_per_s']:.1f} jobs/s), latency p50 "
           f"{res['p50_latency_ms']:.0f} ms p99 {res['p99_latency_ms']:.0f} ms", file=sys.stderr, flush=True)
     return res, np.array(ev)

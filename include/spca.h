@@ -144,6 +144,9 @@ typedef uint64_t spca_job;
  * cuBLAS handle in the front end (c++/src/frontends/gpu_console.cu:64-72). */
 int spca_ctx_create(int device, uint64_t hbm_budget_bytes, spca_ctx* out);
 int spca_ctx_destroy(spca_ctx ctx);
+/* Device memory of the context: used = held (live + cached), cached = freed blocks kept for
+ * reuse by size class (released when a request would exceed the budget), budget = the limit. */
+int spca_ctx_memory(spca_ctx ctx, uint64_t* used, uint64_t* cached, uint64_t* budget);
 /* last error message on this context (thread-local to the context), never NULL */
 const char* spca_last_error(spca_ctx ctx);
 /* library version string */

@@ -464,3 +464,29 @@ def test_serving_alone_equals_mix(ctx):
         ctx.stop_server()
         for M in mats:
             M.release()
+
+
+def test_memory_cache_reuses_blocks():
+    """Freed device blocks are kept by size class and handed to the next request of the same
+    class; held memory (live + cached) stays within the budget, which evicts cached blocks."""
+    A, _ = spiked(200, 1000, seed=21)
+    with S.Context(0, hbm_budget_bytes=64 << 20) as c:
+        M = c.upload(A)
+        used1, cached1, budget = c.memory()  # the host-to-device staging block is cached already
+        assert budget == 64 << 20 and 0 < cached1 < used1 <= budget
+        M.release()
+        used2, cached2, _ = c.memory()
+        assert used2 == used1 and cached2 == used1  # every block kept, still counted
+        M = c.upload(A)
+        assert c.memory() == (used1, cached1, budget)  # the same blocks came back
+        M.release()
+        # a larger matrix needs fresh blocks: cached ones are released to stay in budget
+        B, _ = spiked(1000, 4000, seed=22)
+        M = c.upload(B)
+        used3, cached3, _ = c.memory()
+        assert used3 <= budget
+        x = M.solve(S.OptimizationSettings(formulation=0, constraintParameter=10, tolerance=0.0, maximumIterations=3,
+                                           totalStartingPoints=32, batchSize=32),
+                    X0=unit_gaussian_starts(4000, 32, seed=23))
+        assert np.isfinite(x).all()
+        M.release()
