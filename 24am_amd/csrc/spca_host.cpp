@@ -638,6 +638,21 @@ int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& s
     return SPCA_OK;
 }
 
+// FP64 explained variance of `batch` columns of X (device) into ev (device); the partial
+// sums live in a scratch block that returns to the cache once the kernels are done.
+int run_ev(spca_ctx ctx, hipStream_t s, Matrix* M, const float* X, int ldx, int batch, int formulation, double* ev) {
+    double* scratch = nullptr;
+    uint64_t sb = 0;
+    int rc = dev_alloc(ctx, &scratch, ev_scratch_doubles(M->m, M->n, batch), &sb);
+    if (rc) return rc;
+    hipError_t e = launch_ev_fp64(M->a32_cm, M->mp, M->m, M->n, X, ldx, batch, spca_is_l1_variance(formulation), ev,
+                                  scratch, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    dev_free(ctx, scratch, sb);
+    if (e != hipSuccess) return fail(ctx, SPCA_EDEVICE, std::string("explained variance: ") + hipGetErrorString(e));
+    return SPCA_OK;
+}
+
 // What a solve accumulates over its batches.
 struct Results {
     double best = -1;
@@ -670,8 +685,8 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
     // per-SP explained variance when per-SP results are asked for; else only the batch's best
     // SP needs it (for stats), which is computed below once it is known
     if (want_ev && per_sp) {
-        HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, M->m, n, w.X, w.np, batch, spca_is_l1_variance(st.formulation),
-                                    w.ev, s));
+        int rc = run_ev(ctx, s, M, w.X, w.np, batch, st.formulation, w.ev);
+        if (rc) return rc;
         HIP_TRY(ctx, hipMemcpyAsync(hev.data(), w.ev, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
     }
     HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -705,8 +720,8 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
         HIP_TRY(ctx, hipMemcpyAsync(R.x_best.data(), w.X + (size_t)sel * w.np, sizeof(float) * n,
                                     hipMemcpyDeviceToHost, s));
         if (want_ev && !per_sp) {
-            HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, M->m, n, w.X + (size_t)sel * w.np, w.np, 1,
-                                        spca_is_l1_variance(st.formulation), w.ev, s));
+            int rc = run_ev(ctx, s, M, w.X + (size_t)sel * w.np, w.np, 1, st.formulation, w.ev);
+            if (rc) return rc;
             HIP_TRY(ctx, hipMemcpyAsync(&hev[sel], w.ev, sizeof(double), hipMemcpyDeviceToHost, s));
             HIP_TRY(ctx, hipStreamSynchronize(s));
         }
@@ -1139,7 +1154,10 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         R.x_best.assign(n, 0.f);
         HIP_TRY(ctx, hipMemcpyAsync(R.x_best.data(), xbest, sizeof(float) * n, hipMemcpyDeviceToHost, s));
         double evb = 0;
-        HIP_TRY(ctx, launch_ev_fp64(M->a32_cm, M->mp, m, n, xbest, w.np, 1, spca_is_l1_variance(st.formulation), w.ev, s));
+        if ((rc = run_ev(ctx, s, M, xbest, w.np, 1, st.formulation, w.ev))) {
+            release();
+            return rc;
+        }
         HIP_TRY(ctx, hipMemcpyAsync(&evb, w.ev, sizeof(double), hipMemcpyDeviceToHost, s));
         float emax;
         HIP_TRY(ctx, hipMemcpyAsync(&emax, sp.errmax, sizeof(float), hipMemcpyDeviceToHost, s));

@@ -173,9 +173,11 @@ hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs
 // Write the operand forms of X from its FP32 values (for starting points).
 hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st);
 
-// FP64 explained variance of each normalised column of X: |A x|_2^2 or |A x|_1.
+// FP64 explained variance of each normalised column of X: |A x|_2^2 or |A x|_1.  scratch:
+// ev_scratch_doubles(m, n, batch) doubles of device memory.
+size_t ev_scratch_doubles(int m, int n, int batch, int* chunks = nullptr, int* chunk_len = nullptr);
 hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* X, int ldx, int batch,
-                          int l1, double* ev, hipStream_t st);
+                          int l1, double* ev, double* scratch, hipStream_t st);
 
 // FP64 re-solve on a support (indices idx[0..k)): max_iter iterations of z = map(A_S y),
 // y = A_S^T z / |A_S^T z|, stopping (flag) when the objective changes by <= tol relative.

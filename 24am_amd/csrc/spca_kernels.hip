@@ -1454,73 +1454,123 @@ hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipS
 }
 
 // ---------------------------------------------------------------- FP64 explained variance
-// ev_j = |A xhat_j|_2^2 (L2 variance) or |A xhat_j|_1 (L1 variance), xhat = x/|x|_2, all
-// in FP64 from the FP32 master copy of A.  Nonzeros of x are compacted in index order
-// (ballot prefix counts), so the summation order is fixed.
+// ev_j = |A xhat_j|_2^2 (L2 variance) or |A xhat_j|_1 (L1 variance), xhat = x/|x|_2, all in
+// FP64 from the FP32 master copy of A.  Work is split over (row block, index chunk of x, SP):
+// each workgroup walks its chunk's nonzeros in index order (ballot compaction per 256-index
+// slab) and accumulates its rows of z = A xhat; the chunk partials are summed in chunk order,
+// then rows and row blocks in a fixed order, so the result does not depend on scheduling.
 constexpr int kEvThreads = 256;
-constexpr int kEvList = 2048;
-__global__ __launch_bounds__(kEvThreads) void ev_fp64_kernel(const float* __restrict__ A, int lda, int m, int n,
-                                                              const float* __restrict__ X, int ldx, int l1,
-                                                              double* __restrict__ ev) {
-    const int j = blockIdx.x;
-    const float* x = X + (long)j * ldx;
+constexpr int kEvRows = 4 * kEvThreads;  // rows per workgroup, 4 per thread
+
+__global__ __launch_bounds__(kEvThreads) void ev_norm_kernel(const float* __restrict__ X, int ldx, int n,
+                                                              double* __restrict__ invn) {
     __shared__ double red[kEvThreads / 64];
-    __shared__ int idxs[kEvList];
-    __shared__ double vals[kEvList];
-    __shared__ int wcount[kEvThreads / 64];
+    const float* x = X + (long)blockIdx.x * ldx;
     double nx = 0;
     for (int k = threadIdx.x; k < n; k += kEvThreads) nx += (double)x[k] * (double)x[k];
     nx = block_sum<kEvThreads>(nx, red);
-    const double inv = 1.0 / sqrt(nx);
-    constexpr int RPT = 64;  // rows per thread held in registers (m <= 16384)
-    double acc[RPT];
-#pragma unroll
-    for (int r = 0; r < RPT; r++) acc[r] = 0;
+    if (threadIdx.x == 0) invn[blockIdx.x] = 1.0 / sqrt(nx);
+}
+
+__global__ __launch_bounds__(kEvThreads) void ev_partial_kernel(const float* __restrict__ A, int lda, int m, int n,
+                                                                 const float* __restrict__ X, int ldx,
+                                                                 const double* __restrict__ invn, int chunk_len,
+                                                                 int chunks, double* __restrict__ part, int ldp) {
+    __shared__ int idxs[kEvThreads];
+    __shared__ double vals[kEvThreads];
+    __shared__ int wcount[kEvThreads / 64];
+    const int rb = blockIdx.x, c = blockIdx.y, j = blockIdx.z;
+    const float* x = X + (long)j * ldx;
+    const double inv = invn[j];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    int kbase = 0;
-    while (kbase < n) {
-        // compact up to kEvList nonzeros from [kbase, ...) in index order
-        int listed = 0;
-        int k0 = kbase;
-        while (k0 < n && listed + kEvThreads <= kEvList) {
-            const int k = k0 + threadIdx.x;
-            const float xv = k < n ? x[k] : 0.f;
-            const bool nzf = xv != 0.f;
-            const uint64_t bal = __ballot(nzf);
-            const int before = __popcll(bal & ((1ull << lane) - 1ull));
-            if (lane == 0) wcount[w] = __popcll(bal);
-            __syncthreads();
-            int off = listed;
-            for (int i = 0; i < w; i++) off += wcount[i];
-            if (nzf) { idxs[off + before] = k; vals[off + before] = (double)xv * inv; }
-            int tot = 0;
-            for (int i = 0; i < kEvThreads / 64; i++) tot += wcount[i];
-            __syncthreads();
-            listed += tot;
-            k0 += kEvThreads;
+    double acc[4] = {0, 0, 0, 0};
+    const int k_end = min(n, (c + 1) * chunk_len);
+    for (int k0 = c * chunk_len; k0 < k_end; k0 += kEvThreads) {
+        const int k = k0 + threadIdx.x;
+        const float xv = k < k_end ? x[k] : 0.f;
+        const bool nzf = xv != 0.f;
+        const uint64_t bal = __ballot(nzf);
+        if (lane == 0) wcount[w] = __popcll(bal);
+        __syncthreads();
+        int off = 0, cnt = 0;
+        for (int i = 0; i < kEvThreads / 64; i++) {
+            if (i < w) off += wcount[i];
+            cnt += wcount[i];
         }
-        kbase = k0;
-        for (int t = 0; t < listed; t++) {
+        if (nzf) {
+            const int at = off + __popcll(bal & ((1ull << lane) - 1ull));
+            idxs[at] = k;
+            vals[at] = (double)xv * inv;
+        }
+        __syncthreads();
+        for (int t = 0; t < cnt; t++) {
             const float* col = A + (long)idxs[t] * lda;
-            const double xv = vals[t];
+            const double v = vals[t];
 #pragma unroll
-            for (int r = 0; r < RPT; r++) {
-                const int i = threadIdx.x + r * kEvThreads;
-                if (i < m) acc[r] += (double)col[i] * xv;
+            for (int r = 0; r < 4; r++) {
+                const int i = rb * kEvRows + threadIdx.x + r * kEvThreads;
+                if (i < m) acc[r] += (double)col[i] * v;
             }
         }
         __syncthreads();
     }
+    double* pp = part + ((long)j * chunks + c) * ldp;
+#pragma unroll
+    for (int r = 0; r < 4; r++) pp[rb * kEvRows + threadIdx.x + r * kEvThreads] = acc[r];
+}
+
+__global__ __launch_bounds__(kEvThreads) void ev_reduce_kernel(const double* __restrict__ part, int chunks, int ldp,
+                                                                int m, int l1, double* __restrict__ rpart) {
+    __shared__ double red[kEvThreads / 64];
+    const int rb = blockIdx.x, j = blockIdx.y;
     double s = 0;
 #pragma unroll
-    for (int r = 0; r < RPT; r++) s += l1 ? fabs(acc[r]) : acc[r] * acc[r];
+    for (int r = 0; r < 4; r++) {
+        const int i = rb * kEvRows + threadIdx.x + r * kEvThreads;
+        double z = 0;
+        for (int c = 0; c < chunks; c++) z += part[((long)j * chunks + c) * ldp + i];
+        if (i < m) s += l1 ? fabs(z) : z * z;
+    }
     s = block_sum<kEvThreads>(s, red);
-    if (threadIdx.x == 0) ev[j] = s;
+    if (threadIdx.x == 0) rpart[(long)j * gridDim.x + rb] = s;
 }
+
+__global__ void ev_final_kernel(const double* __restrict__ rpart, int rbs, int batch, double* __restrict__ ev) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= batch) return;
+    double s = 0;
+    for (int b = 0; b < rbs; b++) s += rpart[(long)j * rbs + b];
+    ev[j] = s;
+}
+
+size_t ev_scratch_doubles(int m, int n, int batch, int* chunks_out, int* chunk_len_out) {
+    const int rbs = (m + kEvRows - 1) / kEvRows, ldp = rbs * kEvRows;
+    // up to 256 chunks of at least 256 indices, within 64 MiB of partials
+    const long cap = (64l << 20) / 8 / ((long)batch * ldp);
+    int chunks = (int)std::max(1l, std::min(cap, 256l));
+    chunks = std::min(chunks, (n + kEvThreads - 1) / kEvThreads);
+    int chunk_len = (n + chunks - 1) / chunks;
+    chunk_len = (chunk_len + kEvThreads - 1) / kEvThreads * kEvThreads;
+    chunks = (n + chunk_len - 1) / chunk_len;
+    if (chunks_out) *chunks_out = chunks;
+    if (chunk_len_out) *chunk_len_out = chunk_len;
+    return (size_t)batch * chunks * ldp + (size_t)batch * rbs + (size_t)batch;
+}
+
 hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* X, int ldx, int batch, int l1,
-                          double* ev, hipStream_t st) {
-    if (m > 64 * kEvThreads) return hipErrorInvalidValue;
-    ev_fp64_kernel<<<batch, kEvThreads, 0, st>>>(Acm, lda, m, n, X, ldx, l1, ev);
+                          double* ev, double* scratch, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    int chunks = 0, chunk_len = 0;
+    ev_scratch_doubles(m, n, batch, &chunks, &chunk_len);
+    const int rbs = (m + kEvRows - 1) / kEvRows, ldp = rbs * kEvRows;
+    double* part = scratch;
+    double* rpart = part + (size_t)batch * chunks * ldp;
+    double* invn = rpart + (size_t)batch * rbs;
+    ev_norm_kernel<<<batch, kEvThreads, 0, st>>>(X, ldx, n, invn);
+    ev_partial_kernel<<<dim3(rbs, chunks, batch), kEvThreads, 0, st>>>(Acm, lda, m, n, X, ldx, invn, chunk_len, chunks,
+                                                                       part, ldp);
+    ev_reduce_kernel<<<dim3(rbs, batch), kEvThreads, 0, st>>>(part, chunks, ldp, m, l1, rpart);
+    ev_final_kernel<<<(batch + 255) / 256, 256, 0, st>>>(rpart, rbs, batch, ev);
     return hipGetLastError();
 }
 
