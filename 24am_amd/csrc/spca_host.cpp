@@ -478,7 +478,7 @@ void free_workspace(spca_ctx ctx, Engine& E) { free_ws(ctx, E.ws, E.stream); }
 uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
     const uint64_t xs = (uint64_t)np * bp, zs = (uint64_t)mp * bp;
     return xs * (4 + 2 + 2 + 4) + zs * splits * 4 + zs * (4 + 2 + 2) + (uint64_t)4 * bp * 4 +
-           ((uint64_t)3 * bp + 4) * 4 + (uint64_t)bp * 8;
+           ((uint64_t)3 * bp + 8) * 4 + (uint64_t)bp * 8;
 }
 
 int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp, int splits,
@@ -496,7 +496,7 @@ int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp,
         (rc = dev_alloc(ctx, &w.Z, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.Zhi, zs, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.Zlo, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.V, xs, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.fstate, (size_t)4 * bp, &w.bytes, err)) ||
-        (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + 4, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + 8, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err))) {
         free_ws(ctx, w, s);
         return rc;
@@ -578,6 +578,16 @@ void reference_starting_point(float* out, unsigned n, unsigned s, bool constrain
     for (unsigned i = 0; i < n; i++) out[i] = (float)V[i];
 }
 
+// Largest nonzero count per iterate at which A*X is gathered over the nonzeros.  The gather
+// reads ~4 B per nonzero and row; the dense product costs 2n flops per row and SP at the
+// form's MFMA rate, so the break-even density falls as the form gets faster: 1/64 for FP32,
+// 1/192 for bf16x3 (about 3x FP32), 1/384 for fp16.
+int sparse_cap(const spca_settings& st, int n) {
+    if (!st.sparse_products) return 0;
+    const int div = st.precision == SPCA_PREC_FP32 ? 64 : st.precision == SPCA_PREC_BF16X3 ? 192 : 384;
+    return std::max(1, n / div);
+}
+
 // K-splits of A*X: enough workgroups to fill 256 CUs for a couple of rounds (128x128
 // tiles run ~3 per CU, 256x256 tiles one per CU), each split at least 512 deep.
 int choose_splits(int precision, int mp, int np, int bp, bool allow_big = true) {
@@ -630,7 +640,7 @@ int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& s
         HIP_TRY(ctx, launch_x_operand(xop, st.precision, w.np, batch, s));
     }
     HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
-    HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 4), s));
+    HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 8), s));
     std::vector<int> act(w.bp, 0);
     std::fill(act.begin(), act.begin() + batch, 1);
     HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
@@ -658,6 +668,7 @@ struct Results {
     double best = -1;
     int best_sp = -1;
     unsigned total_it = 0;
+    unsigned sparse_it = 0;
     unsigned long long sp_its = 0;
     double last_err = 0;
     double ev_best = 0;
@@ -674,9 +685,9 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
     std::vector<float> hval(batch), herr(batch);
     std::vector<int> hit(batch), hnnz(batch);
     std::vector<double> hev(batch, 0.0);
-    int done_h[2];
+    int done_h[5];
     float emax;
-    HIP_TRY(ctx, hipMemcpyAsync(done_h, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+    HIP_TRY(ctx, hipMemcpyAsync(done_h, sp.done, sizeof(int) * 5, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(hval.data(), sp.val, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(herr.data(), sp.err, sizeof(float) * batch, hipMemcpyDeviceToHost, s));
     HIP_TRY(ctx, hipMemcpyAsync(hit.data(), sp.iters, sizeof(int) * batch, hipMemcpyDeviceToHost, s));
@@ -691,6 +702,7 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
     }
     HIP_TRY(ctx, hipStreamSynchronize(s));
     R.total_it += (unsigned)done_h[1];
+    R.sparse_it += (unsigned)done_h[4];
     R.last_err = emax;
     for (int jj = 0; jj < batch; jj++) R.sp_its += (unsigned long long)hit[jj];
     int sel = 0;
@@ -754,6 +766,7 @@ void finish_solve(const Results& R, int n, const spca_settings& st, float* x_out
         stats->step_ms = phase_ms ? phase_ms[2] : 0.0;
         stats->sp_iterations = R.sp_its;
         stats->ev_fp64 = R.ev_best;
+        stats->sparse_iterations = R.sparse_it;
     }
 }
 
@@ -1090,7 +1103,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipMemcpyAsync(ibuf, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice, s));
             HIP_TRY(ctx, hipMemcpyAsync(fbuf, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, s));
             HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
-            HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 4), s));
+            HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 8), s));
             std::vector<int> act(w.bp, 0);
             std::fill(act.begin(), act.begin() + batch, 1);
             HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
@@ -1182,6 +1195,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     } else
     for (int bt = 0; bt < total_batches; bt++) {
         const int shift = bt * batch;
+        const int cap = sparse_cap(st, n);
         make_starts(st, n, X0, shift, batch, hX.data());
         if ((rc = load_batch(ctx, s, w, st, n, batch, hX.data(), shift))) return rc;
         HIP_TRY(ctx, hipEventRecord(loop_ev[0], s));
@@ -1189,8 +1203,10 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
+            // one of the two A*X forms runs; the other sees the device flag and returns
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
-                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, s));
+                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done + 3, s));
+            HIP_TRY(ctx, launch_sparse_ax(M->a32_cm, M->mp, M->mp, n, w.X, w.np, w.Zpart, M->mp, batch, sp, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1206,7 +1222,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
             HIP_TRY(ctx, launch_converge(sp, batch, it, st.tolerance, st.max_iter,
-                                         st.termination == SPCA_TERM_PER_SP, s));
+                                         st.termination == SPCA_TERM_PER_SP, cap, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             if ((it + 1) % check_every == 0 || it + 1 == st.max_iter) {
                 HIP_TRY(ctx, hipMemcpyAsync(E.h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
@@ -1398,7 +1414,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     }
     // ---- job table
     std::vector<FusedJob> tab(njobs);
-    int t1 = 0, t2 = 0, c0 = 0;
+    int t1 = 0, t2 = 0, c0 = 0, t3 = 0;
     for (int k = 0; k < njobs; k++) {
         FusedRun& r = run[k];
         Matrix* M = r.M;
@@ -1432,9 +1448,12 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         f.t1 = t1;
         f.t2 = t2;
         f.c0 = c0;
+        f.t3 = t3;
+        f.sparse_cap = sparse_cap(r.st, M->n);
         t1 += (f.mp / kGemmBM) * (f.bp / kGemmBN) * f.splits;
         t2 += (f.np / kGemmBM) * (f.bp / kGemmBN);
         c0 += r.st.batch;
+        t3 += (f.mp + kSparseRows - 1) / kSparseRows * r.st.batch;
     }
     if ((rc = dev_alloc(ctx, &d_jobs, (size_t)njobs, &tb)) || (rc = dev_alloc(ctx, &d_done, 4, &tb)))
         return cleanup(rc);
@@ -1445,7 +1464,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     if (flags & SPCA_FUSED_GRAPH) {
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         for (int i = 0; i < chunk && e == hipSuccess; i++)
-            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, d_done, s);
+            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, d_done, s);
         hipError_t e2 = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = e2;
         if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
@@ -1469,7 +1488,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
             e = hipGraphLaunch(gexec, s);
         else
             for (int i = 0; i < chunk && e == hipSuccess; i++)
-                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, d_done, s);
+                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, d_done, s);
         hipEvent_t ev;
         if (e == hipSuccess && !new_event(&ev)) e = hipErrorOutOfMemory;
         if (e == hipSuccess) e = hipEventRecord(ev, s);
@@ -1664,6 +1683,8 @@ void spca_settings_default(spca_settings* st) {
     st->check_every = 1;
     st->resolve_iters = 0;
     st->resolve_tol = 1e-12;
+    st->seed = 0;
+    st->sparse_products = 1;
 }
 
 int spca_settings_check(spca_settings* st, unsigned n) {

@@ -50,7 +50,9 @@ struct SpState {
     int* iters;      // iterations performed
     int* nnz;        // nonzeros of the iterate
     int* active;     // 1 while the SP iterates
-    int* done;       // [0]: batch finished flag, [1]: iterations counted, [2]: spare
+    int* done;       // [0]: batch finished, [1]: iterations counted, [2]: next A*X is the sparse
+                     // gather, [3]: [0] | [2] (the dense A*X product stands down), [4]: sparse
+                     // A*X iterations counted, [5..7]: spare
     float* errmax;   // [0]: batch max error of the last iteration
 };
 
@@ -108,7 +110,9 @@ struct FusedJob {
     double tol;
     int mp, np, bp, splits;
     int t1, t2, c0;         // first A*X tile, first A^T*Z tile, first column of this job
+    int t3;                 // first (row block, SP) item of the sparse A*X product
     int max_iter, per_sp;
+    int sparse_cap;         // largest nonzero count for the sparse A*X product (0: never)
 };
 
 // ---- launchers (spca_kernels.hip); every one enqueues on `st` and returns a hip status
@@ -142,8 +146,9 @@ hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int 
 // Formulation step on V (A^T Z) for each active SP, writes X (+operand), updates state.
 hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st);
 
-// Batch convergence: batch max error / per-SP freezing, done flag.
-hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_iter, int per_sp,
+// Batch convergence: batch max error / per-SP freezing, done flag; chooses the next A*X
+// product (sparse gather when every iterating SP has at most sparse_cap nonzeros).
+hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_iter, int per_sp, int sparse_cap,
                            hipStream_t st);
 
 // OTF: per-slot termination, best-so-far, assignment of new SPs to finished slots in slot
@@ -168,7 +173,13 @@ hipError_t launch_plant(float* A, int m, int mp, const int* idx, const float* co
 // *all_done).  128x128 tiles; per job the same tile order, K split and kernel bodies as a
 // solo solve, so a job's FP32 results do not depend on the other jobs.
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
-                                  int* all_done, hipStream_t st);
+                                  int items3, int* all_done, hipStream_t st);
+
+// z_j = A x_j over the nonzeros of each active x_j (FP64 accumulation, fixed order) into Z
+// (ld ldz), when sp.done[2] is set; else it returns at once.  A: FP32 column major, ld lda.
+constexpr int kSparseRows = 1024;  // rows per workgroup of the sparse product
+hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z, int ldz,
+                            int batch, SpState sp, hipStream_t st);
 
 // Write the operand forms of X from its FP32 values (for starting points).
 hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st);

@@ -574,16 +574,16 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                 const int r = wn * 64 + j * 16 + (lane & 15);
                 b[p][j] = *(const u4*)(sb + (NP + p) * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
             }
-        // all A fragments of the stage first, then the MFMAs
+        // every A fragment of the stage is read first; the MFMAs then issue in two groups kept
+        // apart by scheduling barriers, so the compiler neither sinks the reads between the
+        // MFMAs nor interleaves the groups
         u4 afr[8][NP];
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
+        auto load_a = [&](int i) {
             const int r = wm * 128 + i * 16 + (lane & 15);
 #pragma unroll
             for (int p = 0; p < NP; p++) afr[i][p] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
-        }
-#pragma unroll
-        for (int i = 0; i < 8; i++) {
+        };
+        auto mma = [&](int i) {
             const u4* a = afr[i];
             if constexpr (PREC == 1) {
 #pragma unroll
@@ -604,7 +604,15 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
                         __builtin_bit_cast(h8, a[0]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
             }
-        }
+        };
+#pragma unroll
+        for (int i = 0; i < 8; i++) load_a(i);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 4; i++) mma(i);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 4; i < 8; i++) mma(i);
     }
     float* Cb = C + (long)sp * split_stride;
 #pragma unroll
@@ -651,6 +659,88 @@ hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, 
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- sparse A*X
+// z_j = A x_j over the nonzeros of x_j, for iterates that are sparse (after a constrained or
+// penalised step): workgroup (row block, SP) walks the SP's nonzeros in index order (ballot
+// compaction of 256-index slabs) and accumulates its 1024 rows in FP64 from the FP32 master
+// A (column major, so each nonzero's column segment is one coalesced read; 4 rows per thread
+// as one 16-byte load, eight nonzeros' loads in flight).  Runs when the
+// previous iteration's convergence kernel set done[2]; the dense product stands down then.
+// The FP64 sum in a fixed order is deterministic and more accurate than any product form.
+constexpr int kSpRows = kSparseRows;  // 4 rows per thread
+__device__ __forceinline__ void sparse_ax_body(int rb, int j, const float* __restrict__ A, int lda, int n,
+                                               const float* __restrict__ X, int ldx, float* __restrict__ Z, int ldz,
+                                               int mp, SpState sp) {
+    if (sp.done[0] || !sp.done[2] || !sp.active[j]) return;
+    __shared__ int idxs[256];
+    __shared__ float vals[256];
+    __shared__ int wcount[4];
+    const float* x = X + (long)j * ldx;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // each thread owns 4 consecutive rows: one 16-byte load per nonzero column
+    const int row0 = rb * kSpRows + 4 * threadIdx.x;
+    const bool in = row0 < mp;  // mp and the row block are multiples of 4 rows
+    double acc[4] = {0, 0, 0, 0};
+    for (int k0 = 0; k0 < n; k0 += 256) {
+        const int k = k0 + threadIdx.x;
+        const float xv = k < n ? x[k] : 0.f;
+        const bool nzf = xv != 0.f;
+        const uint64_t bal = __ballot(nzf);
+        if (lane == 0) wcount[w] = __popcll(bal);
+        __syncthreads();
+        int off = 0, cnt = 0;
+        for (int i = 0; i < 4; i++) {
+            if (i < w) off += wcount[i];
+            cnt += wcount[i];
+        }
+        if (nzf) {
+            const int at = off + __popcll(bal & ((1ull << lane) - 1ull));
+            idxs[at] = k;
+            vals[at] = xv;
+        }
+        __syncthreads();
+        if (in) {
+            // eight columns' loads in flight before their FMAs; the sum stays in index order
+            int t = 0;
+            for (; t + 8 <= cnt; t += 8) {
+                f4 c[8];
+#pragma unroll
+                for (int u = 0; u < 8; u++) c[u] = *(const f4*)(A + (long)idxs[t + u] * lda + row0);
+#pragma unroll
+                for (int u = 0; u < 8; u++) {
+                    const double v = vals[t + u];
+#pragma unroll
+                    for (int r = 0; r < 4; r++) acc[r] += (double)c[u][r] * v;
+                }
+            }
+            for (; t < cnt; t++) {
+                const f4 c = *(const f4*)(A + (long)idxs[t] * lda + row0);
+                const double v = vals[t];
+#pragma unroll
+                for (int r = 0; r < 4; r++) acc[r] += (double)c[r] * v;
+            }
+        }
+        __syncthreads();
+    }
+    if (in) {
+        f4 o;
+#pragma unroll
+        for (int r = 0; r < 4; r++) o[r] = (float)acc[r];
+        *(f4*)(Z + (long)j * ldz + row0) = o;
+    }
+}
+__global__ __launch_bounds__(256) void sparse_ax_kernel(const float* __restrict__ A, int lda, int n,
+                                                        const float* __restrict__ X, int ldx, float* __restrict__ Z,
+                                                        int ldz, int mp, SpState sp) {
+    sparse_ax_body(blockIdx.x, blockIdx.y, A, lda, n, X, ldx, Z, ldz, mp, sp);
+}
+hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z, int ldz,
+                            int batch, SpState sp, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    sparse_ax_kernel<<<dim3((mp + kSpRows - 1) / kSpRows, batch), 256, 0, st>>>(Acm, lda, n, X, ldx, Z, ldz, mp, sp);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- z epilogue
 __device__ __forceinline__ bool is_constrained(int f) { return f < 4; }
 __device__ __forceinline__ bool is_l1_variance(int f) { return f == 1 || f == 3 || f == 5 || f == 7; }
@@ -676,6 +766,7 @@ __device__ __forceinline__ void prep_z_body(int j, const float* __restrict__ Zpa
                                             int m, int ldz, Operand zop, int formulation, int precision,
                                             SpState sp) {
     if (*sp.done || !sp.active[j]) return;
+    if (sp.done[2]) splits = 1;  // the sparse A*X product writes the whole of z into partial 0
     __shared__ double red[4];
     const float* zc = Zpart + (long)j * ldz;
     const bool l1v = is_l1_variance(formulation);
@@ -1163,14 +1254,18 @@ hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipS
 // Batch rule (gpu_sparse_PCA_solver.h:176-186, sparse_PCA_solver.h:203-207): error =
 // max_j err_j with NaN skipped by the '<' update; stop iff it > 0 and error < tol.
 // Per-SP rule: each active SP stops on its own error or at max_iter.
+// Also chooses the next iteration's A*X product: the sparse gather (done[2] = 1) when every
+// SP that keeps iterating has at most sparse_cap nonzeros (0: never); done[3] = done[0] |
+// done[2] is the flag the dense A*X product checks to stand down.
 __device__ __forceinline__ void converge_body(SpState sp, int batch, int it, double tol, int max_iter, int per_sp,
-                                              int* all_done) {
+                                              int sparse_cap, int* all_done) {
     if (*sp.done) return;
     if (it < 0) it = sp.done[1];  // fused launches read the iteration index from the device
     __shared__ float wmax[4];
     __shared__ int wany[4];
+    __shared__ int wnnz[4];
     float e = 0.f;
-    int any = 0;
+    int any = 0, nzmax = 0;
     for (int j = threadIdx.x; j < batch; j += 256) {
         if (!sp.active[j]) continue;
         const float ej = sp.err[j];
@@ -1180,39 +1275,51 @@ __device__ __forceinline__ void converge_body(SpState sp, int batch, int it, dou
             if ((it > 0 && (double)ej < tol) || sp.iters[j] >= max_iter) sp.active[j] = 0;
             else any = 1;
         }
+        if (sp.active[j]) nzmax = max(nzmax, sp.nnz[j]);
     }
     // max with the same NaN-skipping semantics: a NaN never wins a '<' comparison
     for (int o = 32; o > 0; o >>= 1) {
         const float t = __shfl_down(e, o, 64);
         if (e < t) e = t;
         any |= __shfl_down(any, o, 64);
+        nzmax = max(nzmax, __shfl_down(nzmax, o, 64));
     }
-    if ((threadIdx.x & 63) == 0) { wmax[threadIdx.x >> 6] = e; wany[threadIdx.x >> 6] = any; }
+    if ((threadIdx.x & 63) == 0) {
+        wmax[threadIdx.x >> 6] = e;
+        wany[threadIdx.x >> 6] = any;
+        wnnz[threadIdx.x >> 6] = nzmax;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         float m = 0.f;
-        int a = 0;
+        int a = 0, nz = 0;
         for (int w = 0; w < 4; w++) {
             if (m < wmax[w]) m = wmax[w];
             a |= wany[w];
+            nz = max(nz, wnnz[w]);
         }
         sp.errmax[0] = m;
         sp.done[1] += 1;
         int stop;
         if (per_sp) stop = !a;
         else stop = it > 0 && (double)m < tol;
+        const int sparse = sparse_cap > 0 && nz <= sparse_cap;
+        sp.done[2] = sparse;
         if (stop || it + 1 >= max_iter) {
             sp.done[0] = 1;
             if (all_done) atomicAdd(all_done, 1);
         }
+        sp.done[3] = sp.done[0] | sparse;
+        if (sparse && !sp.done[0]) sp.done[4] += 1;  // iterations whose A*X is the sparse gather
     }
 }
 __global__ __launch_bounds__(256) void converge_kernel(SpState sp, int batch, int it, double tol, int max_iter,
-                                                       int per_sp) {
-    converge_body(sp, batch, it, tol, max_iter, per_sp, nullptr);
+                                                       int per_sp, int sparse_cap) {
+    converge_body(sp, batch, it, tol, max_iter, per_sp, sparse_cap, nullptr);
 }
-hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_iter, int per_sp, hipStream_t st) {
-    converge_kernel<<<1, 256, 0, st>>>(sp, batch, it, tol, max_iter, per_sp);
+hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_iter, int per_sp, int sparse_cap,
+                           hipStream_t st) {
+    converge_kernel<<<1, 256, 0, st>>>(sp, batch, it, tol, max_iter, per_sp, sparse_cap);
     return hipGetLastError();
 }
 
@@ -1222,7 +1329,7 @@ hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_it
 // body as the solo kernel on the job's own buffers.  Finished jobs return at once.
 template <int FIELD>
 __device__ __forceinline__ int job_first(const FusedJob* __restrict__ jobs, int k) {
-    return FIELD == 0 ? jobs[k].t1 : FIELD == 1 ? jobs[k].t2 : jobs[k].c0;
+    return FIELD == 0 ? jobs[k].t1 : FIELD == 1 ? jobs[k].t2 : FIELD == 2 ? jobs[k].c0 : jobs[k].t3;
 }
 template <int FIELD>
 __device__ __forceinline__ int find_job(const FusedJob* __restrict__ jobs, int njobs, int item) {
@@ -1256,7 +1363,7 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
     const int t = xcd_window_remap(blockIdx.x, total);
     const int k = find_job<WHICH>(jobs, njobs, t);
     const FusedJob& J = jobs[k];
-    if (*J.sp.done) return;
+    if (J.sp.done[WHICH == 0 ? 3 : 0]) return;  // A*X stands down for the sparse product too
     const int lt = t - (WHICH == 0 ? J.t1 : J.t2);
     const int tiles_m = (WHICH == 0 ? J.mp : J.np) / kGemmBM, tiles_n = J.bp / kGemmBN;
     const int tn = lt % tiles_n;
@@ -1279,6 +1386,14 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
     }
 }
 
+// sparse A*X items: (row block, SP) of each job, row block fastest
+__global__ __launch_bounds__(256) void fused_sparse_ax_kernel(const FusedJob* __restrict__ jobs, int njobs) {
+    const int k = find_job<3>(jobs, njobs, blockIdx.x);
+    const FusedJob& J = jobs[k];
+    const int li = blockIdx.x - J.t3, rbs = (J.mp + kSpRows - 1) / kSpRows;
+    sparse_ax_body(li % rbs, li / rbs, J.p2_32, J.mp, J.sa.n, J.xop.f32, J.np, J.Zpart, J.mp, J.mp, J.sp);
+}
+
 __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __restrict__ jobs, int njobs) {
     const int k = find_job<2>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
@@ -1294,13 +1409,14 @@ __global__ __launch_bounds__(kSelThreads) void fused_step_kernel(const FusedJob*
 
 __global__ __launch_bounds__(256) void fused_converge_kernel(const FusedJob* __restrict__ jobs, int* all_done) {
     const FusedJob& J = jobs[blockIdx.x];
-    converge_body(J.sp, J.sa.batch, -1, J.tol, J.max_iter, J.per_sp, all_done);
+    converge_body(J.sp, J.sa.batch, -1, J.tol, J.max_iter, J.per_sp, J.sparse_cap, all_done);
 }
 
 template <int PREC>
-hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int* all_done,
+hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int items3, int* all_done,
                         hipStream_t st) {
     fused_gemm_kernel<PREC, 0><<<tiles1, 256, 0, st>>>(jobs, njobs, tiles1);
+    fused_sparse_ax_kernel<<<items3, 256, 0, st>>>(jobs, njobs);
     fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
     fused_gemm_kernel<PREC, 1><<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
     fused_step_kernel<<<cols, kSelThreads, 0, st>>>(jobs, njobs);
@@ -1309,11 +1425,11 @@ hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2,
 }
 
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
-                                  int* all_done, hipStream_t st) {
-    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1) return hipErrorInvalidValue;
-    if (precision == 0) return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, all_done, st);
-    if (precision == 1) return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, all_done, st);
-    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, all_done, st);
+                                  int items3, int* all_done, hipStream_t st) {
+    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1 || items3 < 1) return hipErrorInvalidValue;
+    if (precision == 0) return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, items3, all_done, st);
+    if (precision == 1) return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, items3, all_done, st);
+    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, items3, all_done, st);
 }
 
 // ---------------------------------------------------------------- OTF slot refill

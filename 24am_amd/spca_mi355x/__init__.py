@@ -66,6 +66,7 @@ class _Settings(ctypes.Structure):
         ("batch", ctypes.c_int), ("use_otf", ctypes.c_int), ("termination", ctypes.c_int),
         ("precision", ctypes.c_int), ("start", ctypes.c_int), ("check_every", ctypes.c_int),
         ("resolve_iters", ctypes.c_int), ("resolve_tol", ctypes.c_double), ("seed", ctypes.c_uint64),
+        ("sparse_products", ctypes.c_int),
     ]
 
 
@@ -77,6 +78,7 @@ class _Stats(ctypes.Structure):
         ("ev_fp64", ctypes.c_double), ("gemm_ax_ms", ctypes.c_double), ("gemm_atz_ms", ctypes.c_double),
         ("step_ms", ctypes.c_double), ("sp_iterations", ctypes.c_ulonglong),
         ("ev_resolve_fp64", ctypes.c_double), ("resolve_iterations", ctypes.c_int),
+        ("sparse_iterations", ctypes.c_uint),
     ]
 
 
@@ -173,6 +175,7 @@ class OptimizationSettings:
         self.resolve_iters = 0  # > 0: FP64 re-solve on the support of the result
         self.resolve_tol = 1e-12
         self.seed = 0  # START_GAUSSIAN: key of the device-generated starting points
+        self.sparse_products = 1  # gather A*X over the nonzeros of sparse iterates (0: always dense)
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -199,7 +202,7 @@ class OptimizationSettings:
                          float(self.tolerance), int(self.maximumIterations), int(self.totalStartingPoints),
                          int(self.batchSize), int(bool(self.useOTF)), int(self.termination), int(self.precision),
                          int(self.start), int(self.check_every), int(self.resolve_iters), float(self.resolve_tol),
-                         int(self.seed))
+                         int(self.seed), int(self.sparse_products))
 
 
 class OptimizationStatistics:
@@ -225,6 +228,7 @@ class OptimizationStatistics:
         self.sp_iterations = 0
         self.ev_resolve_fp64 = None
         self.resolve_iterations = 0
+        self.sparse_iterations = 0
 
     def _fill(self, s, per_sp=None):
         self.it, self.fval, self.error = s.it, s.fval, s.error
@@ -234,6 +238,7 @@ class OptimizationStatistics:
         self.gemm_ax_ms, self.gemm_atz_ms, self.step_ms = s.gemm_ax_ms, s.gemm_atz_ms, s.step_ms
         self.sp_iterations = s.sp_iterations
         self.ev_resolve_fp64, self.resolve_iterations = s.ev_resolve_fp64, s.resolve_iterations
+        self.sparse_iterations = s.sparse_iterations
         if per_sp is not None:
             self.values = np.array([r.value for r in per_sp], dtype=np.float32)
             self.iters = np.array([r.iters for r in per_sp], dtype=np.int32)

@@ -138,6 +138,47 @@ def run_uneven(S, ctx, count, iters, graph=True):
     return res
 
 
+def run_precision_variant(S, ctx, iters, warmup):
+    """Precision variant at 16384 x 131072: a spiked model like the headline one (5 loadings of
+    256 nonzeros, strengths 20..6, N(0,1) noise), generated on the device and stored as FP32,
+    bf16 hi+lo and fp16; s = 512, B = 2048 device Gaussian starting points; L2 variance / L0
+    constraint and L1 variance / L1 constraint; tol = 0 and a fixed iteration count.  Reports
+    SP-iterations/s per form."""
+    m, n, B, s = 16384, 131072, 2048, 512
+    t0 = time.time()
+    M = ctx.generate(m, n, 77, theta=THETA, spike_nnz=SPARSITY,
+                     storage=S.STORE_FP32 | S.STORE_BF16X3 | S.STORE_FP16)
+    print(f"# precision variant: {m}x{n} resident {M.nbytes() / 2**30:.1f} GiB, built in {time.time() - t0:.1f}s",
+          file=sys.stderr, flush=True)
+    out = {}
+    for name, prec in (("fp32", S.PREC_FP32), ("bf16x3", S.PREC_BF16X3), ("fp16", S.PREC_FP16)):
+        sp_it, ms = 0, 0.0
+        per = {}
+        for f in (0, 3):
+            st = S.OptimizationSettings(formulation=f, constraintParameter=s, tolerance=0.0, maximumIterations=warmup,
+                                        totalStartingPoints=B, batchSize=B, precision=prec, start=S.START_GAUSSIAN,
+                                        seed=9, check_every=1000)
+            if warmup > 0:
+                M.solve(st, per_sp=False)
+            st.maximumIterations = iters
+            stats = S.OptimizationStatistics()
+            M.solve(st, stats=stats, per_sp=False)
+            per[str(f)] = {"ms_per_step": stats.totalTrueComputationTime * 1e3 / stats.it,
+                           "gemm_ax_ms": stats.gemm_ax_ms / stats.it, "gemm_atz_ms": stats.gemm_atz_ms / stats.it,
+                           "ev_fp64": stats.ev_fp64}
+            sp_it += B * stats.it
+            ms += stats.totalTrueComputationTime * 1e3
+        out[name] = {"sp_it_per_s": sp_it / (ms / 1e3), "per_formulation": per,
+                     "algorithmic_tflops": 4.0 * m * n * sp_it / (ms / 1e3) / 1e12}
+        print(f"# precision variant {name}: {out[name]['sp_it_per_s']:.0f} SP-it/s "
+              f"({out[name]['algorithmic_tflops']:.0f} algorithmic TFLOP/s)", file=sys.stderr, flush=True)
+    M.release()
+    out["bf16x3_over_fp32"] = out["bf16x3"]["sp_it_per_s"] / out["fp32"]["sp_it_per_s"]
+    out["fp16_over_fp32"] = out["fp16"]["sp_it_per_s"] / out["fp32"]["sp_it_per_s"]
+    out["workload"] = f"spiked {m}x{n}, B={B}, s={s}, formulations 0 and 3, tol=0 ({iters} iterations)"
+    return out
+
+
 def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5):
     """Request stream: Poisson arrivals at `rate` jobs/s for `seconds`, each job on one of
     `pool_size` resident seeded matrices (shapes drawn as the uneven jobs), with its own
@@ -224,11 +265,12 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--formulations", default="0,1,2,3,4,5,6,7")
-    ap.add_argument("--precisions", default="fp32,bf16x3,fp16")
+    ap.add_argument("--precisions", default="fp32,fp32_dense,bf16x3,fp16")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-jobs", type=int, default=64, help="uneven fused-job workload size (0: skip)")
     ap.add_argument("--uneven-iters", type=int, default=20)
     ap.add_argument("--uneven-graph", type=int, default=1, help="replay the fused iteration from a hipGraph")
+    ap.add_argument("--variant-iters", type=int, default=5, help="precision-variant (16384 x 131072) iterations, 0: skip")
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="request-stream length (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-workers", type=int, default=4)
@@ -246,8 +288,9 @@ def main():
     print(f"# data ready in {time.time() - t0:.1f}s, gamma {gam}", file=sys.stderr, flush=True)
     forms = [int(f) for f in args.formulations.split(",")]
     precs = [p for p in args.precisions.split(",") if p]
-    pcode = {"fp32": S.PREC_FP32, "bf16x3": S.PREC_BF16X3, "fp16": S.PREC_FP16}
-    store = {"fp32": S.STORE_FP32, "bf16x3": S.STORE_BF16X3, "fp16": S.STORE_FP16}
+    # "fp32_dense" is the plain FP32 form: both products dense (no gather over sparse iterates)
+    pcode = {"fp32": S.PREC_FP32, "fp32_dense": S.PREC_FP32, "bf16x3": S.PREC_BF16X3, "fp16": S.PREC_FP16}
+    store = {"fp32": S.STORE_FP32, "fp32_dense": S.STORE_FP32, "bf16x3": S.STORE_BF16X3, "fp16": S.STORE_FP16}
     results = {}
     with S.Context(0) as ctx:
         mask = 0
@@ -261,7 +304,8 @@ def main():
             for f in forms:
                 st = S.OptimizationSettings(formulation=f, constraintParameter=SPARSITY, penaltyParameter=gam.get(f, 0.0),
                                             tolerance=0.0, maximumIterations=args.warmup, totalStartingPoints=BATCH,
-                                            batchSize=BATCH, precision=pcode[p], check_every=1000)
+                                            batchSize=BATCH, precision=pcode[p], check_every=1000,
+                                            sparse_products=0 if p == "fp32_dense" else 1)
                 if args.warmup > 0:
                     M.solve(st, X0=X0, per_sp=False)
                 st.maximumIterations = args.steps
@@ -269,6 +313,7 @@ def main():
                 M.solve(st, X0=X0, stats=stats, per_sp=False)
                 sp_it = BATCH * stats.it
                 per[str(f)] = {"sp_it_per_s": sp_it / stats.totalTrueComputationTime,
+                               "sparse_iterations": stats.sparse_iterations,
                                "ms_per_step": stats.totalTrueComputationTime * 1e3 / stats.it,
                                "gemm_ax_ms": stats.gemm_ax_ms / stats.it, "gemm_atz_ms": stats.gemm_atz_ms / stats.it,
                                "step_ms": stats.step_ms / stats.it, "fval": stats.fval, "ev_fp64": stats.ev_fp64}
@@ -292,6 +337,7 @@ def main():
         M.release()
         uneven = (run_uneven(S, ctx, args.uneven_jobs, args.uneven_iters, graph=bool(args.uneven_graph))
                   if args.uneven_jobs > 0 else None)
+        variant = run_precision_variant(S, ctx, args.variant_iters, 1) if args.variant_iters > 0 else None
     serving = None
     if args.serve_seconds > 0:
         serving = {}
@@ -324,12 +370,15 @@ def main():
         "per_formulation": {p: r["per_formulation"] for p, r in results.items()},
     }
     line["uneven_jobs"] = uneven
+    line["precision_variant"] = variant
     line["serving"] = serving
     if serving:
         line["p99_latency_ms"] = serving["fp32"]["p99_latency_ms"]
         line["p50_latency_ms"] = serving["fp32"]["p50_latency_ms"]
     if "fp32" in results and "bf16x3" in results:
         line["bf16x3_over_fp32"] = results["bf16x3"]["sp_it_per_s"] / results["fp32"]["sp_it_per_s"]
+    if "fp32_dense" in results and "bf16x3" in results:
+        line["bf16x3_over_fp32_dense"] = results["bf16x3"]["sp_it_per_s"] / results["fp32_dense"]["sp_it_per_s"]
     print(json.dumps(line))
 
 

@@ -103,6 +103,10 @@ typedef struct spca_settings {
     int resolve_iters;        /* > 0: FP64 re-solve on the support of x_out, at most this many iterations */
     double resolve_tol;       /* relative objective change that stops the re-solve */
     uint64_t seed;            /* key of SPCA_START_GAUSSIAN starting points */
+    int sparse_products;      /* 1 (default): A*X is gathered over the nonzeros of the iterates
+                                 (FP64 accumulation) once every iterating SP has at most n/64
+                                 (FP32), n/192 (bf16x3) or n/384 (fp16) of them; 0: always the
+                                 dense product of `precision` */
 } spca_settings;
 
 /* mirrors OptimizationStatistics (c++/src/class/optimization_statistics.h:27-46) */
@@ -123,6 +127,7 @@ typedef struct spca_stats {
     unsigned long long sp_iterations; /* sum over SPs of iterations while active */
     double ev_resolve_fp64;   /* explained variance after the FP64 support re-solve (0 if not run) */
     int resolve_iterations;   /* iterations the re-solve ran */
+    unsigned sparse_iterations; /* iterations whose A*X was gathered over the iterates' nonzeros */
 } spca_stats;
 
 /* per starting point result (reference values / iters / cardinalities) */

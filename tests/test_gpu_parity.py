@@ -490,3 +490,24 @@ def test_memory_cache_reuses_blocks():
                     X0=unit_gaussian_starts(4000, 32, seed=23))
         assert np.isfinite(x).all()
         M.release()
+
+
+@pytest.mark.parametrize("f", [0, 1, 3, 4])
+def test_sparse_products_match_dense(small, small_mat, f):
+    """Once the iterates are sparse, A*X is gathered over their nonzeros in FP64; the solve
+    agrees with the dense-product solve (and the sparse form actually ran)."""
+    A, X0, _ = small
+    out = {}
+    for sparse in (0, 1):
+        st = _settings(f, 30, S.PREC_FP32, X0.shape[1])
+        st.sparse_products = sparse
+        stats = S.OptimizationStatistics()
+        x = small_mat.solve(st, X0=X0, stats=stats)
+        out[sparse] = (x, stats)
+    (xd, sd), (xs, ss) = out[0], out[1]
+    assert sd.sparse_iterations == 0
+    if f in (0, 1):  # L0 constraint: s = 10 nonzeros <= n/64, so every iteration after the first
+        assert ss.sparse_iterations == ss.it - 1
+    rel = np.abs(ss.values - sd.values) / np.abs(sd.values)
+    assert np.mean(rel <= 1e-5) >= 0.95, np.sort(rel)[-4:]
+    assert abs(ss.fval - sd.fval) <= 1e-5 * abs(sd.fval)
