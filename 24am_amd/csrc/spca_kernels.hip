@@ -903,6 +903,9 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     // soft map with lam (L1), penalised maps; then multiply by `scale_out`
     float tau = 0.f, lam = 0.f, scale_out = 1.f;
 
+    // L1 constraint with s <= 1 (|x|_1 <= |x|_2) admits only 1-sparse x: the top-1 hard
+    // threshold, which the lambda scan cannot produce (its quadratic degenerates)
+    const bool l1c = (f == 2 || f == 3) && a.s > 1;
     if (f < 4) {
         // ---- constrained: select, then x <- x/|x|_2
         float mloc = 0.f;
@@ -914,7 +917,6 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
             for (int u = 0; u < 8; u++) mloc = fmaxf(mloc, fabsf(vv[u]));
         }
         const float M = block_max<kSelThreads>(mloc, sh.red);
-        const bool l1c = (f == 2 || f == 3);
         const float scale = M > 0.f ? (float)kBins / M : 0.f;
         const double invM = M > 0.f ? 1.0 / (double)M : 0.0;
         const double Md = (double)M;
@@ -1062,7 +1064,12 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
                 const double qc = S1 * S1 - w2 * S2;
                 lam = (float)((-qb - sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa));
-                if (bstar >= 0) {
+                // no positive root: the constraint is inactive (s >= (|v|_1/|v|_2)^2, e.g. s >= n)
+                // and lambda = 0; the reference scan's quadratic degenerates there (NaN or a
+                // negative root), a branch no fixture pins
+                const bool clamped = !(lam > 0.f);
+                if (clamped) lam = 0.f;
+                if (bstar >= 0 && !clamped) {
                     // every element of the survivor set lies above lam: sum of (a - lam)^2 in closed form
                     const double lf = (double)lam;
                     norm2 = S2 - 2.0 * lf * S1 + c * lf * lf;
@@ -1151,6 +1158,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
             const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
             const double qc = S1 * S1 - w2 * S2;
             lam = (float)((-qb - sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa));
+            if (!(lam > 0.f)) lam = 0.f;  // inactive constraint, as in the fast path
             for (int q = 0; q < nq; q++) {
                 const float t2 = fabsf(LDV(q)) - lam;
                 if (t2 > 0.f) norm2 += (double)t2 * (double)t2;
@@ -1205,12 +1213,15 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 if (q >= nq) break;
                 const float v = vv[u];
                 float x;
-                if (f == 0 || f == 1) x = fabsf(v) < tau ? 0.f : v;
-                else if (f == 2 || f == 3 || f == 6 || f == 7) {
+                if (f < 4 && !l1c) x = fabsf(v) < tau ? 0.f : v;
+                else if (l1c || f == 6 || f == 7) {
                     const float t2 = fabsf(v) - lam;
                     x = t2 > 0.f ? t2 * sgnf(v) : 0.f;
                 } else {
-                    x = (v * v - lam) <= 0.f ? 0.f : v;  // L0 penalised: lam holds gamma
+                    // L0 penalised (lam holds gamma): kept iff v^2 - gamma > 0, so a NaN v (the
+                    // 0/0 after an SP was zeroed out) gives 0, as the reference CPU rule
+                    // (sparse_PCA_thresholding.h:67-102); equal to the GPU functor otherwise
+                    x = (v * v - lam) > 0.f ? v : 0.f;
                 }
                 if (f < 4) x = x * scale_out;
                 __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, tid * 4, q * kSelThreads * 4, 0);
