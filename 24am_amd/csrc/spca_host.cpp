@@ -58,7 +58,7 @@ struct Workspace {
     uint16_t* Zlo = nullptr;
     float* V = nullptr;
     float* fstate = nullptr;  // val, fz, err, errmax  (4 x bp floats)
-    int* istate = nullptr;    // iters, nnz, active, done (3 x bp + 4 ints)
+    int* istate = nullptr;    // iters, nnz, active (bp ints each), done (state_flag_ints(bp))
     double* ev = nullptr;
     uint64_t bytes = 0;
 };
@@ -478,7 +478,7 @@ void free_workspace(spca_ctx ctx, Engine& E) { free_ws(ctx, E.ws, E.stream); }
 uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
     const uint64_t xs = (uint64_t)np * bp, zs = (uint64_t)mp * bp;
     return xs * (4 + 2 + 2 + 4) + zs * splits * 4 + zs * (4 + 2 + 2) + (uint64_t)4 * bp * 4 +
-           ((uint64_t)3 * bp + 8) * 4 + (uint64_t)bp * 8;
+           ((uint64_t)3 * bp + state_flag_ints(bp)) * 4 + (uint64_t)bp * 8;
 }
 
 int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp, int splits,
@@ -496,7 +496,7 @@ int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp,
         (rc = dev_alloc(ctx, &w.Z, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.Zhi, zs, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.Zlo, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.V, xs, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.fstate, (size_t)4 * bp, &w.bytes, err)) ||
-        (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + 8, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + state_flag_ints(bp), &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err))) {
         free_ws(ctx, w, s);
         return rc;
@@ -640,7 +640,7 @@ int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& s
         HIP_TRY(ctx, launch_x_operand(xop, st.precision, w.np, batch, s));
     }
     HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
-    HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 8), s));
+    HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + state_flag_ints(w.bp)), s));
     std::vector<int> act(w.bp, 0);
     std::fill(act.begin(), act.begin() + batch, 1);
     HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
@@ -867,12 +867,12 @@ int tune_plan(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, Works
                 hipError_t e = hipEventRecord(e0, s);
                 if (e == hipSuccess)
                     e = launch_gemm_tn(prec, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart, mp,
-                                       zsplit, mp, bp, np, splits, bi, nullptr, s);
+                                       zsplit, mp, bp, np, splits, bi, nullptr, nullptr, s);
                 if (e == hipSuccess)
                     e = launch_prep_z(w.Zpart, splits, zsplit, mp, mp, zop, st.formulation, prec, st.batch, sp, s);
                 if (e == hipSuccess)
                     e = launch_gemm_tn(prec, P2_32, P2_hi, M->lo_cm, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V, np, 0, np, bp,
-                                       mp, 1, bi, nullptr, s);
+                                       mp, 1, bi, nullptr, nullptr, s);
                 if (e == hipSuccess) e = hipEventRecord(e1, s);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
                 float t = 0;
@@ -991,6 +991,8 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
     int rc = add_storage(ctx, M, need);
     if (rc) return rc;
     *nb = spca_settings_check(&st, (unsigned)M->n);
+    if (st.batch > kMaxSparseGroups * 256)
+        return fail(ctx, SPCA_EINVAL, "batch must be at most " + std::to_string(kMaxSparseGroups * 256) + " SPs");
     if (st.constraint < 1) st.constraint = 1;
     *st_out = st;
     return SPCA_OK;
@@ -1103,7 +1105,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipMemcpyAsync(ibuf, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice, s));
             HIP_TRY(ctx, hipMemcpyAsync(fbuf, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, s));
             HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
-            HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + 8), s));
+            HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + state_flag_ints(w.bp)), s));
             std::vector<int> act(w.bp, 0);
             std::fill(act.begin(), act.begin() + batch, 1);
             HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
@@ -1117,7 +1119,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
-                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, s));
+                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, nullptr, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1127,7 +1129,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V,
-                                        M->np, 0, M->np, bp, M->mp, 1, plan.big, sp.done, s));
+                                        M->np, 0, M->np, bp, M->mp, 1, plan.big, sp.done, nullptr, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1205,7 +1207,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e0, s));
             // one of the two A*X forms runs; the other sees the device flag and returns
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
-                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done + 3, s));
+                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done,
+                                        sp.done + kSparseFlags, s));
             HIP_TRY(ctx, launch_sparse_ax(M->a32_cm, M->mp, M->mp, n, w.X, w.np, w.Zpart, M->mp, batch, sp, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
@@ -1216,7 +1219,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V,
-                                        M->np, 0, M->np, bp, M->mp, 1, plan.big, sp.done, s));
+                                        M->np, 0, M->np, bp, M->mp, 1, plan.big, sp.done, nullptr, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));

@@ -24,6 +24,12 @@ constexpr int kSelMaxEPT = 64;  // elements per thread held in registers (n <= 6
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 
+// Per-SP state flag block (SpState::done): 8 counters/flags, then one sparse-A*X flag per
+// group of 256 SPs.  Batches are limited to kMaxSparseGroups * 256 SPs.
+constexpr int kSparseFlags = 8;
+constexpr int kMaxSparseGroups = 256;
+inline int state_flag_ints(int bp) { return kSparseFlags + (bp + 255) / 256; }
+
 // Philox4x32-10 block (Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as easy as
 // 1, 2, 3", SC 2011): ten rounds on the 128-bit counter c with the 64-bit key (k0, k1).
 __host__ __device__ inline void philox4x32_10(uint32_t c[4], uint32_t k0, uint32_t k1) {
@@ -50,9 +56,9 @@ struct SpState {
     int* iters;      // iterations performed
     int* nnz;        // nonzeros of the iterate
     int* active;     // 1 while the SP iterates
-    int* done;       // [0]: batch finished, [1]: iterations counted, [2]: next A*X is the sparse
-                     // gather, [3]: [0] | [2] (the dense A*X product stands down), [4]: sparse
-                     // A*X iterations counted, [5..7]: spare
+    int* done;       // [0]: batch finished, [1]: iterations counted, [2..3]: spare, [4]: iterations
+                     // with a gathered A*X counted, [5..7]: spare, [kSparseFlags + g]: the next
+                     // A*X of SPs 256g .. 256g+255 is the sparse gather (their dense tiles stand down)
     float* errmax;   // [0]: batch max error of the last iteration
 };
 
@@ -130,12 +136,13 @@ hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int preci
 // ldp / ldq rows, P(r, k) = P[((k / 32) * ldp + r) * 32 + k % 32] (likewise Q).
 // M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.  big = 1 runs the pipelined 256x256-tile
 // kernel, allowed for 16-bit products with M and N multiples of 256 (gemm_big_tiles); else
-// 128x128 tiles.
+// 128x128 tiles.  done (nullable): the launch returns when *done; skip (nullable): tiles
+// of 256-column group g return when skip[g].
 int gemm_big_tiles(int precision, int M, int N);
 hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo,
                           long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq,
                           float* C, long ldc, long split_stride, int M, int N, int K, int splits, int big,
-                          const int* done, hipStream_t st);
+                          const int* done, const int* skip, hipStream_t st);
 
 // Sum split partials of Z, per-column statistic and map (sign / normalise), write the
 // operand of the second product.

@@ -329,7 +329,8 @@ __global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restric
                                                           const float* __restrict__ Q, long ldq,
                                                           float* __restrict__ C, long ldc, long split_stride,
                                                           int tiles_m, int tiles_n, int splits, int ksplit,
-                                                          const int* __restrict__ done) {
+                                                          const int* __restrict__ done,
+    const int* __restrict__ skip) {
     if (done && *done) return;
     const int total = tiles_m * tiles_n * splits;
     const int t = xcd_remap(blockIdx.x, total);
@@ -337,6 +338,7 @@ __global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restric
     const int rest = t / tiles_n;
     const int tm = rest % tiles_m;
     const int sp = rest / tiles_m;
+    if (skip && skip[tn >> 1]) return;  // this 256-column group's A*X is the sparse gather
     gemm_f32_tile(P, ldp, Q, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM, (long)tn * kGemmBN,
                   (long)sp * ksplit, ksplit);
 }
@@ -450,7 +452,8 @@ __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restr
                                                          const uint16_t* __restrict__ Qlo, long ldq,
                                                          float* __restrict__ C, long ldc, long split_stride,
                                                          int tiles_m, int tiles_n, int splits, int ksplit,
-                                                         const int* __restrict__ done) {
+                                                         const int* __restrict__ done,
+    const int* __restrict__ skip) {
     if (done && *done) return;
     const int total = tiles_m * tiles_n * splits;
     const int t = xcd_remap(blockIdx.x, total);
@@ -458,6 +461,7 @@ __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restr
     const int rest = t / tiles_n;
     const int tm = rest % tiles_m;
     const int sp = rest / tiles_m;
+    if (skip && skip[tn >> 1]) return;  // this 256-column group's A*X is the sparse gather
     gemm16_tile<PREC>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM,
                       (long)tn * kGemmBN, (long)sp * ksplit, ksplit);
 }
@@ -496,7 +500,8 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                                                                  const uint16_t* __restrict__ Qlo, long ldq,
                                                                  float* __restrict__ C, long ldc, long split_stride,
                                                                  int tiles_m, int tiles_n, int splits, int ksplit,
-                                                                 const int* __restrict__ done) {
+                                                                 const int* __restrict__ done,
+    const int* __restrict__ skip) {
     if (done && *done) return;
     constexpr int NP = (PREC == 1) ? 2 : 1;
     constexpr int BM = 256, BN = 256, BK = kGemmBK;
@@ -510,6 +515,7 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
     const int rest = t / tiles_n;
     const int tm = rest % tiles_m;
     const int sp = rest / tiles_m;
+    if (skip && skip[tn]) return;  // this 256-column group's A*X is the sparse gather
     const long m0 = (long)tm * BM, n0 = (long)tn * BN, k0 = (long)sp * ksplit;
 
     __shared__ __attribute__((aligned(1024))) uint8_t lds[NSTAGE * STB];
@@ -630,7 +636,7 @@ int gemm_big_tiles(int precision, int M, int N) { return precision != 0 && M % 2
 hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo, long ldp,
                           const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq, float* C,
                           long ldc, long split_stride, int M, int N, int K, int splits, int big, const int* done,
-                          hipStream_t st) {
+                          const int* skip, hipStream_t st) {
     if (M % kGemmBM || N % kGemmBN || splits < 1 || K % (kGemmBK * splits)) return hipErrorInvalidValue;
     if (big && !gemm_big_tiles(precision, M, N)) return hipErrorInvalidValue;
     if (big) {
@@ -638,23 +644,23 @@ hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, 
         const int grid = tiles_m * tiles_n * splits;
         if (precision == 1)
             gemm_tn_16_pipe_kernel<1, 2><<<grid, 512, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride,
-                                                               tiles_m, tiles_n, splits, ksplit, done);
+                                                               tiles_m, tiles_n, splits, ksplit, done, skip);
         else
             gemm_tn_16_pipe_kernel<2, 4><<<grid, 512, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride,
-                                                               tiles_m, tiles_n, splits, ksplit, done);
+                                                               tiles_m, tiles_n, splits, ksplit, done, skip);
         return hipGetLastError();
     }
     const int tiles_m = M / kGemmBM, tiles_n = N / kGemmBN, ksplit = K / splits;
     const int grid = tiles_m * tiles_n * splits;
     if (precision == 0) {
         gemm_tn_f32_kernel<<<grid, 256, 0, st>>>(P32, ldp, Q32, ldq, C, ldc, split_stride, tiles_m, tiles_n, splits,
-                                                 ksplit, done);
+                                                 ksplit, done, skip);
     } else if (precision == 1) {
         gemm_tn_16_kernel<1><<<grid, 256, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
-                                                   tiles_n, splits, ksplit, done);
+                                                   tiles_n, splits, ksplit, done, skip);
     } else {
         gemm_tn_16_kernel<2><<<grid, 256, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
-                                                   tiles_n, splits, ksplit, done);
+                                                   tiles_n, splits, ksplit, done, skip);
     }
     return hipGetLastError();
 }
@@ -664,14 +670,15 @@ hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, 
 // penalised step): workgroup (row block, SP) walks the SP's nonzeros in index order (ballot
 // compaction of 256-index slabs) and accumulates its 1024 rows in FP64 from the FP32 master
 // A (column major, so each nonzero's column segment is one coalesced read; 4 rows per thread
-// as one 16-byte load, eight nonzeros' loads in flight).  Runs when the
-// previous iteration's convergence kernel set done[2]; the dense product stands down then.
+// as one 16-byte load, eight nonzeros' loads in flight).  Runs for the SPs of each group of
+// 256 whose flag the previous iteration's convergence kernel set; the dense product's tiles
+// of those groups stand down.
 // The FP64 sum in a fixed order is deterministic and more accurate than any product form.
 constexpr int kSpRows = kSparseRows;  // 4 rows per thread
 __device__ __forceinline__ void sparse_ax_body(int rb, int j, const float* __restrict__ A, int lda, int n,
                                                const float* __restrict__ X, int ldx, float* __restrict__ Z, int ldz,
                                                int mp, SpState sp) {
-    if (sp.done[0] || !sp.done[2] || !sp.active[j]) return;
+    if (sp.done[0] || !sp.done[kSparseFlags + (j >> 8)] || !sp.active[j]) return;
     __shared__ int idxs[256];
     __shared__ float vals[256];
     __shared__ int wcount[4];
@@ -766,7 +773,7 @@ __device__ __forceinline__ void prep_z_body(int j, const float* __restrict__ Zpa
                                             int m, int ldz, Operand zop, int formulation, int precision,
                                             SpState sp) {
     if (*sp.done || !sp.active[j]) return;
-    if (sp.done[2]) splits = 1;  // the sparse A*X product writes the whole of z into partial 0
+    if (sp.done[kSparseFlags + (j >> 8)]) splits = 1;  // the sparse gather wrote all of z into partial 0
     __shared__ double red[4];
     const float* zc = Zpart + (long)j * ldz;
     const bool l1v = is_l1_variance(formulation);
@@ -1265,18 +1272,21 @@ hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipS
 // Batch rule (gpu_sparse_PCA_solver.h:176-186, sparse_PCA_solver.h:203-207): error =
 // max_j err_j with NaN skipped by the '<' update; stop iff it > 0 and error < tol.
 // Per-SP rule: each active SP stops on its own error or at max_iter.
-// Also chooses the next iteration's A*X product: the sparse gather (done[2] = 1) when every
-// SP that keeps iterating has at most sparse_cap nonzeros (0: never); done[3] = done[0] |
-// done[2] is the flag the dense A*X product checks to stand down.
+// Also chooses the next iteration's A*X product per group of 256 SPs: the sparse gather
+// (done[kSparseFlags + g] = 1) when every SP of the group that keeps iterating has at most
+// sparse_cap nonzeros (0: never); the dense product's tiles of that group stand down.
 __device__ __forceinline__ void converge_body(SpState sp, int batch, int it, double tol, int max_iter, int per_sp,
                                               int sparse_cap, int* all_done) {
     if (*sp.done) return;
     if (it < 0) it = sp.done[1];  // fused launches read the iteration index from the device
     __shared__ float wmax[4];
     __shared__ int wany[4];
-    __shared__ int wnnz[4];
+    __shared__ int gdense[kMaxSparseGroups];
+    const int groups = (batch + 255) >> 8;
+    for (int g = threadIdx.x; g < groups; g += 256) gdense[g] = sparse_cap > 0 ? 0 : 1;
+    __syncthreads();
     float e = 0.f;
-    int any = 0, nzmax = 0;
+    int any = 0;
     for (int j = threadIdx.x; j < batch; j += 256) {
         if (!sp.active[j]) continue;
         const float ej = sp.err[j];
@@ -1286,42 +1296,42 @@ __device__ __forceinline__ void converge_body(SpState sp, int batch, int it, dou
             if ((it > 0 && (double)ej < tol) || sp.iters[j] >= max_iter) sp.active[j] = 0;
             else any = 1;
         }
-        if (sp.active[j]) nzmax = max(nzmax, sp.nnz[j]);
+        if (sp.active[j] && sp.nnz[j] > sparse_cap) gdense[j >> 8] = 1;  // same value from every writer
     }
     // max with the same NaN-skipping semantics: a NaN never wins a '<' comparison
     for (int o = 32; o > 0; o >>= 1) {
         const float t = __shfl_down(e, o, 64);
         if (e < t) e = t;
         any |= __shfl_down(any, o, 64);
-        nzmax = max(nzmax, __shfl_down(nzmax, o, 64));
     }
     if ((threadIdx.x & 63) == 0) {
         wmax[threadIdx.x >> 6] = e;
         wany[threadIdx.x >> 6] = any;
-        wnnz[threadIdx.x >> 6] = nzmax;
     }
     __syncthreads();
+    int nsparse = 0;
+    for (int g = threadIdx.x; g < groups; g += 256) {
+        sp.done[kSparseFlags + g] = !gdense[g];
+        nsparse += !gdense[g];
+    }
+    nsparse = __syncthreads_or(nsparse);
     if (threadIdx.x == 0) {
         float m = 0.f;
-        int a = 0, nz = 0;
+        int a = 0;
         for (int w = 0; w < 4; w++) {
             if (m < wmax[w]) m = wmax[w];
             a |= wany[w];
-            nz = max(nz, wnnz[w]);
         }
         sp.errmax[0] = m;
         sp.done[1] += 1;
         int stop;
         if (per_sp) stop = !a;
         else stop = it > 0 && (double)m < tol;
-        const int sparse = sparse_cap > 0 && nz <= sparse_cap;
-        sp.done[2] = sparse;
         if (stop || it + 1 >= max_iter) {
             sp.done[0] = 1;
             if (all_done) atomicAdd(all_done, 1);
         }
-        sp.done[3] = sp.done[0] | sparse;
-        if (sparse && !sp.done[0]) sp.done[4] += 1;  // iterations whose A*X is the sparse gather
+        if (nsparse && !sp.done[0]) sp.done[4] += 1;  // iterations whose A*X is gathered (some group)
     }
 }
 __global__ __launch_bounds__(256) void converge_kernel(SpState sp, int batch, int it, double tol, int max_iter,
@@ -1374,10 +1384,11 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
     const int t = xcd_window_remap(blockIdx.x, total);
     const int k = find_job<WHICH>(jobs, njobs, t);
     const FusedJob& J = jobs[k];
-    if (J.sp.done[WHICH == 0 ? 3 : 0]) return;  // A*X stands down for the sparse product too
+    if (*J.sp.done) return;
     const int lt = t - (WHICH == 0 ? J.t1 : J.t2);
     const int tiles_m = (WHICH == 0 ? J.mp : J.np) / kGemmBM, tiles_n = J.bp / kGemmBN;
     const int tn = lt % tiles_n;
+    if (WHICH == 0 && J.sp.done[kSparseFlags + (tn >> 1)]) return;  // A*X by the sparse gather
     const int rest = lt / tiles_n;
     const int tm = rest % tiles_m;
     const int sp = rest / tiles_m;
@@ -1589,13 +1600,13 @@ hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipS
 constexpr int kEvThreads = 256;
 constexpr int kEvRows = 4 * kEvThreads;  // rows per workgroup, 4 per thread
 
-__global__ __launch_bounds__(kEvThreads) void ev_norm_kernel(const float* __restrict__ X, int ldx, int n,
-                                                              double* __restrict__ invn) {
-    __shared__ double red[kEvThreads / 64];
+__global__ __launch_bounds__(1024) void ev_norm_kernel(const float* __restrict__ X, int ldx, int n,
+                                                       double* __restrict__ invn) {
+    __shared__ double red[1024 / 64];
     const float* x = X + (long)blockIdx.x * ldx;
     double nx = 0;
-    for (int k = threadIdx.x; k < n; k += kEvThreads) nx += (double)x[k] * (double)x[k];
-    nx = block_sum<kEvThreads>(nx, red);
+    for (int k = threadIdx.x; k < n; k += 1024) nx += (double)x[k] * (double)x[k];
+    nx = block_sum<1024>(nx, red);
     if (threadIdx.x == 0) invn[blockIdx.x] = 1.0 / sqrt(nx);
 }
 
@@ -1646,18 +1657,15 @@ __global__ __launch_bounds__(kEvThreads) void ev_partial_kernel(const float* __r
     for (int r = 0; r < 4; r++) pp[rb * kEvRows + threadIdx.x + r * kEvThreads] = acc[r];
 }
 
+// one row per thread: workgroup rb covers rows [256 rb, 256 rb + 256)
 __global__ __launch_bounds__(kEvThreads) void ev_reduce_kernel(const double* __restrict__ part, int chunks, int ldp,
                                                                 int m, int l1, double* __restrict__ rpart) {
     __shared__ double red[kEvThreads / 64];
     const int rb = blockIdx.x, j = blockIdx.y;
-    double s = 0;
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-        const int i = rb * kEvRows + threadIdx.x + r * kEvThreads;
-        double z = 0;
-        for (int c = 0; c < chunks; c++) z += part[((long)j * chunks + c) * ldp + i];
-        if (i < m) s += l1 ? fabs(z) : z * z;
-    }
+    const int i = rb * kEvThreads + threadIdx.x;
+    double z = 0;
+    for (int c = 0; c < chunks; c++) z += part[((long)j * chunks + c) * ldp + i];
+    double s = i < m ? (l1 ? fabs(z) : z * z) : 0.0;
     s = block_sum<kEvThreads>(s, red);
     if (threadIdx.x == 0) rpart[(long)j * gridDim.x + rb] = s;
 }
@@ -1672,16 +1680,16 @@ __global__ void ev_final_kernel(const double* __restrict__ rpart, int rbs, int b
 
 size_t ev_scratch_doubles(int m, int n, int batch, int* chunks_out, int* chunk_len_out) {
     const int rbs = (m + kEvRows - 1) / kEvRows, ldp = rbs * kEvRows;
-    // up to 256 chunks of at least 256 indices, within 64 MiB of partials
+    // up to 64 chunks of at least 256 indices, within 64 MiB of partials
     const long cap = (64l << 20) / 8 / ((long)batch * ldp);
-    int chunks = (int)std::max(1l, std::min(cap, 256l));
+    int chunks = (int)std::max(1l, std::min(cap, 64l));
     chunks = std::min(chunks, (n + kEvThreads - 1) / kEvThreads);
     int chunk_len = (n + chunks - 1) / chunks;
     chunk_len = (chunk_len + kEvThreads - 1) / kEvThreads * kEvThreads;
     chunks = (n + chunk_len - 1) / chunk_len;
     if (chunks_out) *chunks_out = chunks;
     if (chunk_len_out) *chunk_len_out = chunk_len;
-    return (size_t)batch * chunks * ldp + (size_t)batch * rbs + (size_t)batch;
+    return (size_t)batch * chunks * ldp + (size_t)batch * (ldp / kEvThreads) + (size_t)batch;
 }
 
 hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* X, int ldx, int batch, int l1,
@@ -1690,14 +1698,15 @@ hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* 
     int chunks = 0, chunk_len = 0;
     ev_scratch_doubles(m, n, batch, &chunks, &chunk_len);
     const int rbs = (m + kEvRows - 1) / kEvRows, ldp = rbs * kEvRows;
+    const int rrb = ldp / kEvThreads;  // 256-row blocks of the reduction
     double* part = scratch;
     double* rpart = part + (size_t)batch * chunks * ldp;
-    double* invn = rpart + (size_t)batch * rbs;
-    ev_norm_kernel<<<batch, kEvThreads, 0, st>>>(X, ldx, n, invn);
+    double* invn = rpart + (size_t)batch * rrb;
+    ev_norm_kernel<<<batch, 1024, 0, st>>>(X, ldx, n, invn);
     ev_partial_kernel<<<dim3(rbs, chunks, batch), kEvThreads, 0, st>>>(Acm, lda, m, n, X, ldx, invn, chunk_len, chunks,
                                                                        part, ldp);
-    ev_reduce_kernel<<<dim3(rbs, batch), kEvThreads, 0, st>>>(part, chunks, ldp, m, l1, rpart);
-    ev_final_kernel<<<(batch + 255) / 256, 256, 0, st>>>(rpart, rbs, batch, ev);
+    ev_reduce_kernel<<<dim3(rrb, batch), kEvThreads, 0, st>>>(part, chunks, ldp, m, l1, rpart);
+    ev_final_kernel<<<(batch + 255) / 256, 256, 0, st>>>(rpart, rrb, batch, ev);
     return hipGetLastError();
 }
 

@@ -511,3 +511,35 @@ def test_sparse_products_match_dense(small, small_mat, f):
     rel = np.abs(ss.values - sd.values) / np.abs(sd.values)
     assert np.mean(rel <= 1e-5) >= 0.95, np.sort(rel)[-4:]
     assert abs(ss.fval - sd.fval) <= 1e-5 * abs(sd.fval)
+
+
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3])
+def test_sparse_flags_per_group_of_256(ctx, prec):
+    """A*X switches between the dense product and the sparse gather per group of 256 SPs.
+    Batch of 512, L1 variance with the L1 constraint (dense iterates, never gathered), per-SP
+    termination: the first 256 SPs start at fixed points and stop after one check, which
+    leaves their group without an active SP, so its dense tiles stand down while the second
+    group keeps the dense product.  Results are bitwise those of the all-dense solve."""
+    A, _ = spiked(300, 1000, seed=11)
+    M = ctx.upload(A, storage=S.STORE_FP32 | S.STORE_BF16X3)
+    try:
+        def run(X0, sparse, it):
+            st = S.OptimizationSettings(formulation=3, constraintParameter=12, tolerance=1e-5, maximumIterations=it,
+                                        totalStartingPoints=X0.shape[1], batchSize=X0.shape[1], precision=prec)
+            st.termination = S.TERM_PER_SP
+            st.sparse_products = sparse
+            stats = S.OptimizationStatistics()
+            x, X = M.solve(st, X0=X0, stats=stats, want_X=True)
+            return X, stats
+        Xc, _ = run(unit_gaussian_starts(1000, 256, seed=12), 0, 80)
+        Xc = np.asfortranarray(Xc / np.linalg.norm(Xc, axis=0))
+        X0 = np.asfortranarray(np.concatenate([Xc, unit_gaussian_starts(1000, 256, seed=13)], axis=1))
+        Xd, sd = run(X0, 0, 60)
+        Xs, ss = run(X0, 1, 60)
+        print(f"prec {prec}: iterations {ss.it}, gathered {ss.sparse_iterations}", flush=True)
+        assert sd.sparse_iterations == 0 and ss.sparse_iterations > 0
+        np.testing.assert_array_equal(Xs, Xd)
+        np.testing.assert_array_equal(ss.values, sd.values)
+        assert ss.best_sp == sd.best_sp
+    finally:
+        M.release()
