@@ -43,6 +43,7 @@ struct Matrix {
     uint16_t* lo_rm = nullptr;
     uint16_t* h_cm = nullptr;
     uint16_t* h_rm = nullptr;
+    uint16_t* lo2_cm = nullptr;  // third piece of the exact split (with hi_cm, lo_cm)
     uint64_t bytes = 0;
     int refs = 0;
 };
@@ -251,7 +252,7 @@ bool valid_formulation(int f) { return f >= 0 && f <= 7; }
 
 void free_matrix(spca_ctx ctx, Matrix* M) {
     (void)hipStreamSynchronize(ctx->stream);
-    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm};
+    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm, M->lo2_cm};
     for (void* p : ptrs) raw_free(ctx, p);
     delete M;
 }
@@ -277,23 +278,33 @@ int add_storage(spca_ctx ctx, Matrix* M, unsigned storage) {
         HIP_TRY(ctx, launch_transpose_f32(M->a32_cm, M->mp, M->np, M->mp, M->a32_rm, M->np, st));
         M->storage |= SPCA_STORE_FP32;
     }
+    // hi_cm and lo_cm serve both the bf16x3 form and the exact split (the same two roundings)
+    if ((want & (SPCA_STORE_BF16X3 | SPCA_STORE_SPLIT3)) && !M->hi_cm) {
+        if ((rc = dev_alloc(ctx, &M->hi_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_cm, cnt, &M->bytes)))
+            return rc;
+        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 1, M->hi_cm, M->lo_cm, nullptr, st));
+    }
     if (want & SPCA_STORE_BF16X3) {
         if ((rc = need_rm())) return rc;
         const float* rm = M->a32_rm ? M->a32_rm : rm_tmp;
-        if ((rc = dev_alloc(ctx, &M->hi_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_cm, cnt, &M->bytes)) ||
-            (rc = dev_alloc(ctx, &M->hi_rm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_rm, cnt, &M->bytes)))
+        if ((rc = dev_alloc(ctx, &M->hi_rm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_rm, cnt, &M->bytes)))
             return rc;
-        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 1, M->hi_cm, M->lo_cm, st));
-        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 1, M->hi_rm, M->lo_rm, st));
+        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 1, M->hi_rm, M->lo_rm, nullptr, st));
         M->storage |= SPCA_STORE_BF16X3;
+    }
+    if (want & SPCA_STORE_SPLIT3) {
+        if ((rc = dev_alloc(ctx, &M->lo2_cm, cnt, &M->bytes))) return rc;
+        // rewrites hi_cm and lo_cm with the same values
+        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 3, M->hi_cm, M->lo_cm, M->lo2_cm, st));
+        M->storage |= SPCA_STORE_SPLIT3;
     }
     if (want & SPCA_STORE_FP16) {
         if ((rc = need_rm())) return rc;
         const float* rm = M->a32_rm ? M->a32_rm : rm_tmp;
         if ((rc = dev_alloc(ctx, &M->h_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->h_rm, cnt, &M->bytes)))
             return rc;
-        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 2, M->h_cm, nullptr, st));
-        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 2, M->h_rm, nullptr, st));
+        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 2, M->h_cm, nullptr, nullptr, st));
+        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 2, M->h_rm, nullptr, nullptr, st));
         M->storage |= SPCA_STORE_FP16;
     }
     HIP_TRY(ctx, hipStreamSynchronize(st));
@@ -602,8 +613,47 @@ int choose_splits(int precision, int mp, int np, int bp, bool allow_big = true) 
 Operand x_operand(Workspace& w, int precision) {
     return Operand{w.X, precision ? w.Xhi : nullptr, precision == 1 ? w.Xlo : nullptr, w.bp};
 }
+// precision 3: the bf16 sign vectors of the exact-split product, in Zhi alone
 Operand z_operand(Workspace& w, int precision) {
     return Operand{w.Z, precision ? w.Zhi : nullptr, precision == 1 ? w.Zlo : nullptr, w.bp};
+}
+
+// The second product, A^T*Z, of a solve: the precision's own operands, or, for the
+// L1-variance formulations with exact_sign_products (FP32 and bf16x3), the exact three-piece
+// split of A against Z = sgn(A x) in bf16: every product exact, FP32 accumulation.
+bool uses_sign_products(const spca_settings& st) {
+    return st.exact_sign_products && spca_is_l1_variance(st.formulation) && (st.precision == 0 || st.precision == 1);
+}
+struct AtzForm {
+    int prec;            // launch_gemm_tn precision
+    int zprec;           // form prep_z writes Z in
+    const float* p32;
+    const uint16_t* hi;
+    const uint16_t* lo;
+    const uint16_t* lo2;
+    long ldp, ldq;       // FP32 row strides or 16-bit row counts
+    int big;
+};
+AtzForm atz_form(const Matrix* M, const spca_settings& st, int bp, int plan_big) {
+    AtzForm a;
+    a.p32 = M->a32_cm;
+    if (uses_sign_products(st)) {
+        a.prec = a.zprec = 3;
+        a.hi = M->hi_cm;
+        a.lo = M->lo_cm;
+        a.lo2 = M->lo2_cm;
+        // FP32 plans never pick the big tiles for their own products; the split form does
+        a.big = gemm_big_tiles(3, M->np, bp) && (st.precision == 0 || plan_big);
+    } else {
+        a.prec = a.zprec = st.precision;
+        a.hi = st.precision == 1 ? M->hi_cm : M->h_cm;
+        a.lo = M->lo_cm;
+        a.lo2 = nullptr;
+        a.big = plan_big;
+    }
+    a.ldp = a.prec ? M->np : M->mp;
+    a.ldq = a.prec ? bp : M->mp;
+    return a;
 }
 
 // Starting points with global indices shift .. shift+cnt-1 (n floats each) into out.
@@ -866,13 +916,13 @@ int tune_plan(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, Works
             for (int r = 0; r < 4; r++) {
                 hipError_t e = hipEventRecord(e0, s);
                 if (e == hipSuccess)
-                    e = launch_gemm_tn(prec, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart, mp,
-                                       zsplit, mp, bp, np, splits, bi, nullptr, nullptr, s);
+                    e = launch_gemm_tn(prec, P1_32, P1_hi, M->lo_rm, nullptr, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
+                                       mp, zsplit, mp, bp, np, splits, bi, nullptr, nullptr, s);
                 if (e == hipSuccess)
                     e = launch_prep_z(w.Zpart, splits, zsplit, mp, mp, zop, st.formulation, prec, st.batch, sp, s);
                 if (e == hipSuccess)
-                    e = launch_gemm_tn(prec, P2_32, P2_hi, M->lo_cm, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V, np, 0, np, bp,
-                                       mp, 1, bi, nullptr, nullptr, s);
+                    e = launch_gemm_tn(prec, P2_32, P2_hi, M->lo_cm, nullptr, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V, np,
+                                       0, np, bp, mp, 1, bi, nullptr, nullptr, s);
                 if (e == hipSuccess) e = hipEventRecord(e1, s);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
                 float t = 0;
@@ -987,7 +1037,8 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
         return fail(ctx, SPCA_EINVAL, "start must be a spca_start value");
     if (st.start == SPCA_START_EXPLICIT && !X0) return fail(ctx, SPCA_EINVAL, "X0 is required for explicit starts");
     if (!x_out) return fail(ctx, SPCA_EINVAL, "x_out is required");
-    const unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
+    unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
+    if (uses_sign_products(st)) need |= SPCA_STORE_SPLIT3;
     int rc = add_storage(ctx, M, need);
     if (rc) return rc;
     *nb = spca_settings_check(&st, (unsigned)M->n);
@@ -1017,18 +1068,15 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     SpState sp = state_of(w);
     const int check_every = st.check_every > 0 ? st.check_every : 1;
 
+    const AtzForm atz = atz_form(M, st, bp, plan.big);
     const Operand xop = x_operand(w, st.precision);
-    const Operand zop = z_operand(w, st.precision);
+    const Operand zop = z_operand(w, atz.zprec);
     const float* P1_32 = M->a32_rm;
     const uint16_t* P1_hi = st.precision == 1 ? M->hi_rm : M->h_rm;
     const uint16_t* P1_lo = M->lo_rm;
-    const float* P2_32 = M->a32_cm;
-    const uint16_t* P2_hi = st.precision == 1 ? M->hi_cm : M->h_cm;
-    const uint16_t* P2_lo = M->lo_cm;
     const long zsplit = (long)M->mp * bp;
     // leading dimensions of the product operands: FP32 row strides, 16-bit (K-blocked) row counts
     const long ld1p = st.precision ? M->mp : M->np, ld1q = st.precision ? bp : M->np;
-    const long ld2p = st.precision ? M->np : M->mp, ld2q = st.precision ? bp : M->mp;
 
     StepArgs sa;
     sa.formulation = st.formulation;
@@ -1118,18 +1166,18 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, nullptr, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                         M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, nullptr, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, st.precision, batch,
+            HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, atz.zprec, batch,
                                        sp, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V,
-                                        M->np, 0, M->np, bp, M->mp, 1, plan.big, sp.done, nullptr, s));
+            HIP_TRY(ctx, launch_gemm_tn(atz.prec, atz.p32, atz.hi, atz.lo, atz.lo2, atz.ldp, w.Z, w.Zhi, w.Zlo, atz.ldq,
+                                        w.V, M->np, 0, M->np, bp, M->mp, 1, atz.big, sp.done, nullptr, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1206,20 +1254,20 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             // one of the two A*X forms runs; the other sees the device flag and returns
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, nullptr, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                         M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done,
                                         sp.done + kSparseFlags, s));
             HIP_TRY(ctx, launch_sparse_ax(M->a32_cm, M->mp, M->mp, n, w.X, w.np, w.Zpart, M->mp, batch, sp, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, st.precision, batch,
+            HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, atz.zprec, batch,
                                        sp, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P2_32, P2_hi, P2_lo, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V,
-                                        M->np, 0, M->np, bp, M->mp, 1, plan.big, sp.done, nullptr, s));
+            HIP_TRY(ctx, launch_gemm_tn(atz.prec, atz.p32, atz.hi, atz.lo, atz.lo2, atz.ldp, w.Z, w.Zhi, w.Zlo, atz.ldq,
+                                        w.V, M->np, 0, M->np, bp, M->mp, 1, atz.big, sp.done, nullptr, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1425,13 +1473,17 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         f.p1_32 = M->a32_rm;
         f.p1_hi = precision == 1 ? M->hi_rm : M->h_rm;
         f.p1_lo = M->lo_rm;
+        const AtzForm atz = atz_form(M, r.st, r.ws.bp, 0);
         f.p2_32 = M->a32_cm;
-        f.p2_hi = precision == 1 ? M->hi_cm : M->h_cm;
-        f.p2_lo = M->lo_cm;
+        f.p2_hi = atz.hi;
+        f.p2_lo = atz.lo;
+        f.p2_lo2 = atz.lo2;
+        f.p2_prec = atz.prec;
+        f.zprec = atz.zprec;
         f.Zpart = r.ws.Zpart;
         f.V = r.ws.V;
         f.xop = x_operand(r.ws, precision);
-        f.zop = z_operand(r.ws, precision);
+        f.zop = z_operand(r.ws, atz.zprec);
         f.sp = state_of(r.ws);
         f.sa.formulation = r.st.formulation;
         f.sa.precision = precision;
@@ -1688,6 +1740,7 @@ void spca_settings_default(spca_settings* st) {
     st->resolve_tol = 1e-12;
     st->seed = 0;
     st->sparse_products = 1;
+    st->exact_sign_products = 1;
 }
 
 int spca_settings_check(spca_settings* st, unsigned n) {

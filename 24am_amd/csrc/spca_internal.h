@@ -23,6 +23,7 @@ constexpr int kSelThreads = 1024;
 constexpr int kSelMaxEPT = 64;  // elements per thread held in registers (n <= 65536)
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
 
 // Per-SP state flag block (SpState::done): 8 counters/flags, then one sparse-A*X flag per
 // group of 256 SPs.  Batches are limited to kMaxSparseGroups * 256 SPs.
@@ -107,6 +108,7 @@ struct FusedJob {
     const float* p2_32;     // A column major (ld mp): A^T*Z operand
     const uint16_t* p2_hi;
     const uint16_t* p2_lo;
+    const uint16_t* p2_lo2;  // third piece of the exact split (p2_prec 3)
     float* Zpart;           // splits x (mp x bp) partial products of A*X
     float* V;               // np x bp
     Operand xop, zop;
@@ -119,6 +121,8 @@ struct FusedJob {
     int t3;                 // first (row block, SP) item of the sparse A*X product
     int max_iter, per_sp;
     int sparse_cap;         // largest nonzero count for the sparse A*X product (0: never)
+    int p2_prec;            // A^T*Z operand form: the job's precision, or 3 (exact split x sign vectors)
+    int zprec;              // form prep_z writes Z in (as p2_prec)
 };
 
 // ---- launchers (spca_kernels.hip); every one enqueues on `st` and returns a hip status
@@ -127,20 +131,24 @@ hipError_t launch_pad_copy_f32(const float* src, int m, int n, int lds, float* d
 hipError_t launch_transpose_f32(const float* src, int rows, int cols, int lds, float* dst, int ldd,
                                 hipStream_t st);
 // src: R rows x K (row r at src + r*lds) -> K-blocked 16-bit form (precision 1: bf16 hi + lo,
-// 2: fp16 in hi): element (r, k) at ((k / 32) * R + r) * 32 + k % 32.  K % 32 == 0.
+// 2: fp16 in hi, 3: bf16 hi + lo + lo2 summing to the FP32 value exactly): element (r, k) at
+// ((k / 32) * R + r) * 32 + k % 32.  K % 32 == 0.
 hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int precision, uint16_t* hi, uint16_t* lo,
-                             hipStream_t st);
+                             uint16_t* lo2, hipStream_t st);
 
 // C[r + c*ldc + split*split_stride] = sum_{k in split} P(r, k) * Q(c, k)
 // FP32: P(r, k) = P32[r*ldp + k], Q(c, k) = Q32[c*ldq + k].  16-bit: K-blocked operands with
-// ldp / ldq rows, P(r, k) = P[((k / 32) * ldp + r) * 32 + k % 32] (likewise Q).
+// ldp / ldq rows, P(r, k) = P[((k / 32) * ldp + r) * 32 + k % 32] (likewise Q).  precision
+// 1: bf16x3 (Phi, Plo, Qhi, Qlo), 2: fp16 (Phi, Qhi), 3: exact split (Phi + Plo + Plo2 == P)
+// against a bf16 Q in Qhi, whose products are exact when Q holds -1, 0 and +1.
 // M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.  big = 1 runs the pipelined 256x256-tile
 // kernel, allowed for 16-bit products with M and N multiples of 256 (gemm_big_tiles); else
 // 128x128 tiles.  done (nullable): the launch returns when *done; skip (nullable): tiles
 // of 256-column group g return when skip[g].
 int gemm_big_tiles(int precision, int M, int N);
 hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo,
-                          long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq,
+                          const uint16_t* Plo2, long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo,
+                          long ldq,
                           float* C, long ldc, long split_stride, int M, int N, int K, int splits, int big,
                           const int* done, const int* skip, hipStream_t st);
 

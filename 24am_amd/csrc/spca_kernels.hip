@@ -148,7 +148,7 @@ __device__ __forceinline__ long blk_index(int r, int k, int R) { return ((long)(
 // or fp16 (precision 2).  One thread per 32-element chunk; consecutive threads take
 // consecutive rows of one K block, so the 64-byte output rows are written contiguously.
 __global__ void to_blocked_kernel(const float* __restrict__ src, long lds, int R, int K, int precision,
-                                  uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
+                                  uint16_t* __restrict__ hi, uint16_t* __restrict__ lo, uint16_t* __restrict__ lo2) {
     const long chunks = (long)R * (K / 32);
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < chunks; c += (long)gridDim.x * blockDim.x) {
         const int r = (int)(c % R), kb = (int)(c / R);
@@ -158,31 +158,37 @@ __global__ void to_blocked_kernel(const float* __restrict__ src, long lds, int R
         for (int q = 0; q < 4; q++) {
             const f4 v0 = *(const f4*)(sp + 8 * q), v1 = *(const f4*)(sp + 8 * q + 4);
             const float x[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
-            u4 hv, lv;
+            u4 hv, lv, l2v;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                uint32_t h0, h1, l0 = 0, l1 = 0;
-                if (precision == 1) {
-                    h0 = f2bf(x[2 * e]);
-                    h1 = f2bf(x[2 * e + 1]);
-                    l0 = f2bf(x[2 * e] - bf2f((uint16_t)h0));
-                    l1 = f2bf(x[2 * e + 1] - bf2f((uint16_t)h1));
-                } else {
-                    h0 = f2h(x[2 * e]);
-                    h1 = f2h(x[2 * e + 1]);
+                uint32_t h[2], l[2] = {0, 0}, l2[2] = {0, 0};
+#pragma unroll
+                for (int u = 0; u < 2; u++) {
+                    const float v = x[2 * e + u];
+                    if (precision == 2) {
+                        h[u] = f2h(v);
+                        continue;
+                    }
+                    // v - hi and (v - hi) - lo are exact in FP32, so hi + lo + lo2 == v
+                    h[u] = f2bf(v);
+                    const float r1 = v - bf2f((uint16_t)h[u]);
+                    l[u] = f2bf(r1);
+                    l2[u] = f2bf(r1 - bf2f((uint16_t)l[u]));
                 }
-                hv[e] = h0 | (h1 << 16);
-                lv[e] = l0 | (l1 << 16);
+                hv[e] = h[0] | (h[1] << 16);
+                lv[e] = l[0] | (l[1] << 16);
+                l2v[e] = l2[0] | (l2[1] << 16);
             }
             *(u4*)(hi + o + 8 * q) = hv;
-            if (precision == 1) *(u4*)(lo + o + 8 * q) = lv;
+            if (precision != 2) *(u4*)(lo + o + 8 * q) = lv;
+            if (precision == 3) *(u4*)(lo2 + o + 8 * q) = l2v;
         }
     }
 }
 hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int precision, uint16_t* hi, uint16_t* lo,
-                             hipStream_t st) {
-    if (R <= 0 || K <= 0 || K % 32 || (precision != 1 && precision != 2)) return hipErrorInvalidValue;
-    to_blocked_kernel<<<8192, 256, 0, st>>>(src, lds, R, K, precision, hi, lo);
+                             uint16_t* lo2, hipStream_t st) {
+    if (R <= 0 || K <= 0 || K % 32 || precision < 1 || precision > 3) return hipErrorInvalidValue;
+    to_blocked_kernel<<<8192, 256, 0, st>>>(src, lds, R, K, precision, hi, lo, lo2);
     return hipGetLastError();
 }
 
@@ -261,11 +267,13 @@ __device__ __forceinline__ int xcd_remap(int bid, int total) {
 }
 
 // One 128x128 FP32 output tile: C[m0.., n0..] (+ split offset) over K range [k0, k0+ksplit).
+// smem: kF32TileLds bytes of LDS (16-byte aligned) from the calling kernel.
+constexpr int kF32TileLds = (kGemmBM + kGemmBN) * (kGemmBK + 4) * 4;
 __device__ __forceinline__ void gemm_f32_tile(const float* __restrict__ P, long ldp, const float* __restrict__ Q,
                                               long ldq, float* __restrict__ C, long ldc, long split_stride, int sp,
-                                              long m0, long n0, long k0, int ksplit) {
-    __shared__ __attribute__((aligned(16))) float Ps[kGemmBM][kGemmBK + 4];
-    __shared__ __attribute__((aligned(16))) float Qs[kGemmBN][kGemmBK + 4];
+                                              long m0, long n0, long k0, int ksplit, uint8_t* smem) {
+    auto Ps = (float(*)[kGemmBK + 4])smem;
+    auto Qs = (float(*)[kGemmBK + 4])(smem + kGemmBM * (kGemmBK + 4) * 4);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int lrow = tid >> 3, lc4 = (tid & 7) * 4;
@@ -339,46 +347,59 @@ __global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restric
     const int tm = rest % tiles_m;
     const int sp = rest / tiles_m;
     if (skip && skip[tn >> 1]) return;  // this 256-column group's A*X is the sparse gather
+    __shared__ __attribute__((aligned(16))) uint8_t smem[kF32TileLds];
     gemm_f32_tile(P, ldp, Q, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM, (long)tn * kGemmBN,
-                  (long)sp * ksplit, ksplit);
+                  (long)sp * ksplit, ksplit, smem);
 }
 
-// 16-bit operand products.  PREC 1: bf16x3 (Phi*Qhi + Phi*Qlo + Plo*Qhi), PREC 2: fp16.
-// One 16x16x32 MFMA covers a whole 32-deep K stage; lane l supplies row l%16 and
-// k = 8*(l/16) .. +7 of each operand.
+// 16-bit operand products (operand parts K-blocked, in bf16 unless noted):
+//   PREC 1  bf16x3: P = hi + lo, Q = hi + lo; Phi*Qlo + Plo*Qhi + Phi*Qhi
+//   PREC 2  fp16:   Phi*Qhi in fp16
+//   PREC 3  exact split against a bf16 Q: P = hi + lo + lo2 == A exactly (each piece the
+//           bf16 rounding of what the previous ones leave); lo2*Q + lo*Q + hi*Q.  With Q in
+//           {-1, 0, +1} (the sign vectors of the L1-variance iterations) every product is
+//           exact and only the FP32 accumulation rounds, as in the FP32 product.
+// Small terms are accumulated first.  One 16x16x32 MFMA covers a whole 32-deep K stage;
+// lane l supplies row l%16 and k = 8*(l/16) .. +7 of each operand.
+constexpr int tile16_parts_p(int prec) { return prec == 1 ? 2 : prec == 3 ? 3 : 1; }
+constexpr int tile16_parts_q(int prec) { return prec == 1 ? 2 : 1; }
+// LDS bytes of one 128x128 16-bit tile (80-byte rows)
+constexpr int tile16_lds(int prec) {
+    return (tile16_parts_p(prec) * kGemmBM + tile16_parts_q(prec) * kGemmBN) * (kGemmBK + 8) * 2;
+}
 template <int PREC>
 __device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, const uint16_t* __restrict__ Plo,
-                                            long ldp, const uint16_t* __restrict__ Qhi,
-                                            const uint16_t* __restrict__ Qlo, long ldq, float* __restrict__ C,
-                                            long ldc, long split_stride, int sp, long m0, long n0, long k0,
-                                            int ksplit) {
-    constexpr int NT = (PREC == 1) ? 2 : 1;  // number of parts per operand
-    constexpr int LDK = kGemmBK + 8;         // 80-byte rows: 16-byte aligned
+                                            const uint16_t* __restrict__ Plo2, long ldp,
+                                            const uint16_t* __restrict__ Qhi, const uint16_t* __restrict__ Qlo,
+                                            long ldq, float* __restrict__ C, long ldc, long split_stride, int sp,
+                                            long m0, long n0, long k0, int ksplit, uint8_t* smem) {
+    constexpr int NPP = tile16_parts_p(PREC), NPQ = tile16_parts_q(PREC);
+    constexpr int LDK = kGemmBK + 8;  // 80-byte rows: 16-byte aligned
 
-    __shared__ __attribute__((aligned(16))) uint16_t Ps[NT][kGemmBM][LDK];
-    __shared__ __attribute__((aligned(16))) uint16_t Qs[NT][kGemmBN][LDK];
+    auto Ps = (uint16_t(*)[kGemmBM][LDK])smem;
+    auto Qs = (uint16_t(*)[kGemmBN][LDK])(smem + NPP * kGemmBM * LDK * 2);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     // a 128 x 32 tile of 16-bit values is 512 chunks of 16 bytes: 2 per thread
     const int lrow = tid >> 2, lc8 = (tid & 3) * 8;
-    const uint16_t* Pg[NT];
-    const uint16_t* Qg[NT];
+    const uint16_t* Pg[NPP];
+    const uint16_t* Qg[NPQ];
     // K-blocked operands: ldp / ldq are their row counts; one K stage advances 32 rows' worth
     const long pstep = ldp * kGemmBK, qstep = ldq * kGemmBK;
-    Pg[0] = Phi + ((k0 >> 5) * ldp + m0 + lrow) * 32 + lc8;
-    Qg[0] = Qhi + ((k0 >> 5) * ldq + n0 + lrow) * 32 + lc8;
-    if constexpr (NT == 2) {
-        Pg[1] = Plo + ((k0 >> 5) * ldp + m0 + lrow) * 32 + lc8;
-        Qg[1] = Qlo + ((k0 >> 5) * ldq + n0 + lrow) * 32 + lc8;
+    const uint16_t* pbase[3] = {Phi, Plo, Plo2};
+    const uint16_t* qbase[2] = {Qhi, Qlo};
+#pragma unroll
+    for (int p = 0; p < NPP; p++) Pg[p] = pbase[p] + ((k0 >> 5) * ldp + m0 + lrow) * 32 + lc8;
+#pragma unroll
+    for (int p = 0; p < NPQ; p++) Qg[p] = qbase[p] + ((k0 >> 5) * ldq + n0 + lrow) * 32 + lc8;
+    u4 pr[NPP][2], qr[NPQ][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+        for (int p = 0; p < NPP; p++) pr[p][i] = *(const u4*)(Pg[p] + 64 * 32 * i);
+#pragma unroll
+        for (int p = 0; p < NPQ; p++) qr[p][i] = *(const u4*)(Qg[p] + 64 * 32 * i);
     }
-    u4 pr[NT][2], qr[NT][2];
-#pragma unroll
-    for (int p = 0; p < NT; p++)
-#pragma unroll
-        for (int i = 0; i < 2; i++) {
-            pr[p][i] = *(const u4*)(Pg[p] + 64 * 32 * i);
-            qr[p][i] = *(const u4*)(Qg[p] + 64 * 32 * i);
-        }
     f4 acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
@@ -388,39 +409,47 @@ __device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, co
     for (int kt = 0; kt < ksplit; kt += kGemmBK) {
         __syncthreads();
 #pragma unroll
-        for (int p = 0; p < NT; p++)
+        for (int i = 0; i < 2; i++) {
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                *(u4*)&Ps[p][lrow + 64 * i][lc8] = pr[p][i];
-                *(u4*)&Qs[p][lrow + 64 * i][lc8] = qr[p][i];
-            }
+            for (int p = 0; p < NPP; p++) *(u4*)&Ps[p][lrow + 64 * i][lc8] = pr[p][i];
+#pragma unroll
+            for (int p = 0; p < NPQ; p++) *(u4*)&Qs[p][lrow + 64 * i][lc8] = qr[p][i];
+        }
         __syncthreads();
         if (kt + kGemmBK < ksplit) {
 #pragma unroll
-            for (int p = 0; p < NT; p++) {
+            for (int p = 0; p < NPP; p++) {
                 Pg[p] += pstep;
+#pragma unroll
+                for (int i = 0; i < 2; i++) pr[p][i] = *(const u4*)(Pg[p] + 64 * 32 * i);
+            }
+#pragma unroll
+            for (int p = 0; p < NPQ; p++) {
                 Qg[p] += qstep;
 #pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    pr[p][i] = *(const u4*)(Pg[p] + 64 * 32 * i);
-                    qr[p][i] = *(const u4*)(Qg[p] + 64 * 32 * i);
-                }
+                for (int i = 0; i < 2; i++) qr[p][i] = *(const u4*)(Qg[p] + 64 * 32 * i);
             }
         }
         const int kof = 8 * (lane >> 4);
-        u4 a[NT][4], b[NT][4];
+        u4 a[NPP][4], b[NPQ][4];
 #pragma unroll
-        for (int p = 0; p < NT; p++) {
+        for (int p = 0; p < NPP; p++)
 #pragma unroll
             for (int i = 0; i < 4; i++) a[p][i] = *(const u4*)&Ps[p][wm * 64 + i * 16 + (lane & 15)][kof];
 #pragma unroll
+        for (int p = 0; p < NPQ; p++)
+#pragma unroll
             for (int j = 0; j < 4; j++) b[p][j] = *(const u4*)&Qs[p][wn * 64 + j * 16 + (lane & 15)][kof];
-        }
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                if constexpr (PREC == 1) {
+                if constexpr (PREC == 3) {
+#pragma unroll
+                    for (int p = 2; p >= 0; p--)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8, a[p][i]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+                } else if constexpr (PREC == 1) {
                     // small terms first, then the leading product
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         __builtin_bit_cast(bf8, a[0][i]), __builtin_bit_cast(bf8, b[1][j]), acc[i][j], 0, 0, 0);
@@ -447,7 +476,8 @@ __device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, co
 
 template <int PREC>
 __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restrict__ Phi,
-                                                         const uint16_t* __restrict__ Plo, long ldp,
+                                                         const uint16_t* __restrict__ Plo,
+                                                         const uint16_t* __restrict__ Plo2, long ldp,
                                                          const uint16_t* __restrict__ Qhi,
                                                          const uint16_t* __restrict__ Qlo, long ldq,
                                                          float* __restrict__ C, long ldc, long split_stride,
@@ -462,8 +492,9 @@ __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restr
     const int tm = rest % tiles_m;
     const int sp = rest / tiles_m;
     if (skip && skip[tn >> 1]) return;  // this 256-column group's A*X is the sparse gather
-    gemm16_tile<PREC>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM,
-                      (long)tn * kGemmBN, (long)sp * ksplit, ksplit);
+    __shared__ __attribute__((aligned(16))) uint8_t smem[tile16_lds(PREC)];
+    gemm16_tile<PREC>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM,
+                      (long)tn * kGemmBN, (long)sp * ksplit, ksplit, smem);
 }
 
 // Pipelined large-tile variant: 256x256 output tile per workgroup, 8 waves as 2 (M) x 4 (N),
@@ -495,7 +526,8 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
 
 template <int PREC, int NSTAGE>
 __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t* __restrict__ Phi,
-                                                                 const uint16_t* __restrict__ Plo, long ldp,
+                                                                 const uint16_t* __restrict__ Plo,
+                                                                 const uint16_t* __restrict__ Plo2, long ldp,
                                                                  const uint16_t* __restrict__ Qhi,
                                                                  const uint16_t* __restrict__ Qlo, long ldq,
                                                                  float* __restrict__ C, long ldc, long split_stride,
@@ -503,12 +535,12 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                                                                  const int* __restrict__ done,
     const int* __restrict__ skip) {
     if (done && *done) return;
-    constexpr int NP = (PREC == 1) ? 2 : 1;
+    constexpr int NPP = tile16_parts_p(PREC), NPQ = tile16_parts_q(PREC), NPT = NPP + NPQ;
     constexpr int BM = 256, BN = 256, BK = kGemmBK;
     constexpr int ROWB = BK * 2;          // bytes per LDS row
     constexpr int OPB = BM * ROWB;        // bytes per operand part per stage (16 KiB)
-    constexpr int STB = 2 * NP * OPB;     // bytes per stage
-    constexpr int LPS = 2 * NP * 2;       // global_load_lds instructions per wave per stage
+    constexpr int STB = NPT * OPB;        // bytes per stage: P parts, then Q parts
+    constexpr int LPS = NPT * 2;          // global_load_lds instructions per wave per stage
     const int total = tiles_m * tiles_n * splits;
     const int t = xcd_remap(blockIdx.x, total);
     const int tn = t % tiles_n;
@@ -526,12 +558,13 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
     // loader: wave w fills rows [32w, 32w+32) of every operand part, 16 rows per instruction
     const int lrow = lane >> 2;
     const int lslot = lane & 3;
-    const uint16_t* src[2][NP];
-    src[0][0] = Phi;
-    src[1][0] = Qhi;
-    if constexpr (NP == 2) {
-        src[0][1] = Plo;
-        src[1][1] = Qlo;
+    // part t < NPP is P part t, the rest Q parts
+    const uint16_t* src[NPT];
+    {
+        const uint16_t* pbase[3] = {Phi, Plo, Plo2};
+        const uint16_t* qbase[2] = {Qhi, Qlo};
+#pragma unroll
+        for (int t = 0; t < NPT; t++) src[t] = t < NPP ? pbase[t] : qbase[t - NPP];
     }
     // K-blocked operands with ldp / ldq rows: one K stage of an operand is rows*32 elements
     const long stage_elems[2] = {ldp * BK, ldq * BK};
@@ -545,16 +578,14 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
     }
     auto issue = [&](int ks, int buf) {
 #pragma unroll
-        for (int op = 0; op < 2; op++)
+        for (int t = 0; t < NPT; t++)
 #pragma unroll
-            for (int p = 0; p < NP; p++)
-#pragma unroll
-                for (int i = 0; i < 2; i++) {
-                    uint8_t* dst = lds + buf * STB + (op * NP + p) * OPB + (32 * wave + 16 * i) * ROWB;
-                    __builtin_amdgcn_global_load_lds(
-                        (const void*)(src[op][p] + rowoff[op][i] + (long)ks * stage_elems[op]),
-                        (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
-                }
+            for (int i = 0; i < 2; i++) {
+                const int op = t < NPP ? 0 : 1;
+                uint8_t* dst = lds + buf * STB + t * OPB + (32 * wave + 16 * i) * ROWB;
+                __builtin_amdgcn_global_load_lds((const void*)(src[t] + rowoff[op][i] + (long)ks * stage_elems[op]),
+                                                 (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+            }
     };
     f4 acc[8][4];
 #pragma unroll
@@ -572,53 +603,85 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
         __builtin_amdgcn_s_barrier();
         if (k + NSTAGE - 1 < nk) issue(k + NSTAGE - 1, (k + NSTAGE - 1) % NSTAGE);
         const uint8_t* sb = lds + (k % NSTAGE) * STB;
-        u4 b[NP][4];
+        u4 b[NPQ][4];
 #pragma unroll
-        for (int p = 0; p < NP; p++)
+        for (int p = 0; p < NPQ; p++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
                 const int r = wn * 64 + j * 16 + (lane & 15);
-                b[p][j] = *(const u4*)(sb + (NP + p) * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
+                b[p][j] = *(const u4*)(sb + (NPP + p) * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
             }
-        // every A fragment of the stage is read first; the MFMAs then issue in two groups kept
-        // apart by scheduling barriers, so the compiler neither sinks the reads between the
-        // MFMAs nor interleaves the groups
-        u4 afr[8][NP];
-        auto load_a = [&](int i) {
-            const int r = wm * 128 + i * 16 + (lane & 15);
+        if constexpr (PREC == 3) {
+            // one P piece at a time (lo2, lo, hi), each piece's eight fragments read while the
+            // previous piece's MFMAs issue: two fragment buffers instead of three keep the
+            // 128 accumulators and the fragments within 256 registers
+            u4 fa[2][8];
+            auto load_piece = [&](int p, u4* dst) {
 #pragma unroll
-            for (int p = 0; p < NP; p++) afr[i][p] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
-        };
-        auto mma = [&](int i) {
-            const u4* a = afr[i];
-            if constexpr (PREC == 1) {
+                for (int i = 0; i < 8; i++) {
+                    const int r = wm * 128 + i * 16 + (lane & 15);
+                    dst[i] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
+                }
+            };
+            auto mma_piece = [&](const u4* a) {
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[1][j]), acc[i][j], 0, 0, 0);
+                for (int i = 0; i < 8; i++)
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf8, a[1]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8, a[i]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+            };
+            load_piece(2, fa[0]);
+            __builtin_amdgcn_sched_barrier(0);
+            load_piece(1, fa[1]);
+            mma_piece(fa[0]);
+            __builtin_amdgcn_sched_barrier(0);
+            load_piece(0, fa[0]);
+            mma_piece(fa[1]);
+            __builtin_amdgcn_sched_barrier(0);
+            mma_piece(fa[0]);
+        } else {
+            // every A fragment of the stage is read first; the MFMAs then issue in two groups kept
+            // apart by scheduling barriers, so the compiler neither sinks the reads between the
+            // MFMAs nor interleaves the groups
+            u4 afr[8][NPP];
+            auto load_a = [&](int i) {
+                const int r = wm * 128 + i * 16 + (lane & 15);
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                        __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
-            } else {
+                for (int p = 0; p < NPP; p++)
+                    afr[i][p] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
+            };
+            auto mma = [&](int i) {
+                const u4* a = afr[i];
+                if constexpr (PREC == 1) {
 #pragma unroll
-                for (int j = 0; j < 4; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
-                        __builtin_bit_cast(h8, a[0]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
-            }
-        };
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[1][j]), acc[i][j], 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 8; i++) load_a(i);
-        __builtin_amdgcn_sched_barrier(0);
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8, a[1]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
 #pragma unroll
-        for (int i = 0; i < 4; i++) mma(i);
-        __builtin_amdgcn_sched_barrier(0);
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
+                            __builtin_bit_cast(bf8, a[0]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
+                } else {
 #pragma unroll
-        for (int i = 4; i < 8; i++) mma(i);
+                    for (int j = 0; j < 4; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_f16(
+                            __builtin_bit_cast(h8, a[0]), __builtin_bit_cast(h8, b[0][j]), acc[i][j], 0, 0, 0);
+                }
+            };
+#pragma unroll
+            for (int i = 0; i < 8; i++) load_a(i);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) mma(i);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 4; i < 8; i++) mma(i);
+        }
     }
     float* Cb = C + (long)sp * split_stride;
 #pragma unroll
@@ -633,21 +696,30 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
 
 int gemm_big_tiles(int precision, int M, int N) { return precision != 0 && M % 256 == 0 && N % 256 == 0; }
 
-hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo, long ldp,
-                          const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq, float* C,
+hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo,
+                          const uint16_t* Plo2, long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo,
+                          long ldq, float* C,
                           long ldc, long split_stride, int M, int N, int K, int splits, int big, const int* done,
                           const int* skip, hipStream_t st) {
     if (M % kGemmBM || N % kGemmBN || splits < 1 || K % (kGemmBK * splits)) return hipErrorInvalidValue;
+    if (precision < 0 || precision > 3 || (precision == 3 && (!Phi || !Plo || !Plo2 || !Qhi)))
+        return hipErrorInvalidValue;
     if (big && !gemm_big_tiles(precision, M, N)) return hipErrorInvalidValue;
     if (big) {
         const int tiles_m = M / 256, tiles_n = N / 256, ksplit = K / splits;
         const int grid = tiles_m * tiles_n * splits;
         if (precision == 1)
-            gemm_tn_16_pipe_kernel<1, 2><<<grid, 512, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride,
-                                                               tiles_m, tiles_n, splits, ksplit, done, skip);
+            gemm_tn_16_pipe_kernel<1, 2><<<grid, 512, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc,
+                                                               split_stride, tiles_m, tiles_n, splits, ksplit, done,
+                                                               skip);
+        else if (precision == 3)
+            gemm_tn_16_pipe_kernel<3, 2><<<grid, 512, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc,
+                                                               split_stride, tiles_m, tiles_n, splits, ksplit, done,
+                                                               skip);
         else
-            gemm_tn_16_pipe_kernel<2, 4><<<grid, 512, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride,
-                                                               tiles_m, tiles_n, splits, ksplit, done, skip);
+            gemm_tn_16_pipe_kernel<2, 4><<<grid, 512, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc,
+                                                               split_stride, tiles_m, tiles_n, splits, ksplit, done,
+                                                               skip);
         return hipGetLastError();
     }
     const int tiles_m = M / kGemmBM, tiles_n = N / kGemmBN, ksplit = K / splits;
@@ -656,10 +728,13 @@ hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, 
         gemm_tn_f32_kernel<<<grid, 256, 0, st>>>(P32, ldp, Q32, ldq, C, ldc, split_stride, tiles_m, tiles_n, splits,
                                                  ksplit, done, skip);
     } else if (precision == 1) {
-        gemm_tn_16_kernel<1><<<grid, 256, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
+        gemm_tn_16_kernel<1><<<grid, 256, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
+                                                   tiles_n, splits, ksplit, done, skip);
+    } else if (precision == 3) {
+        gemm_tn_16_kernel<3><<<grid, 256, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
                                                    tiles_n, splits, ksplit, done, skip);
     } else {
-        gemm_tn_16_kernel<2><<<grid, 256, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
+        gemm_tn_16_kernel<2><<<grid, 256, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
                                                    tiles_n, splits, ksplit, done, skip);
     }
     return hipGetLastError();
@@ -763,6 +838,8 @@ __device__ __forceinline__ void write_operand(Operand op, int precision, long ld
         op.lo[b] = f2bf(v - bf2f(h));
     } else if (precision == 2) {
         op.hi[blk_index(j, i, op.bp)] = f2h(v);
+    } else if (precision == 3) {
+        op.hi[blk_index(j, i, op.bp)] = f2bf(v);  // sign vectors: exact
     }
 }
 
@@ -1394,17 +1471,28 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
     const int sp = rest / tiles_m;
     const int ksplit = WHICH == 0 ? J.np / J.splits : J.mp;
     const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
+    // one LDS block serves whichever tile form the job's product takes
+    constexpr int kLds = cmax(cmax(kF32TileLds, tile16_lds(PREC)), WHICH == 1 ? tile16_lds(3) : 0);
+    __shared__ __attribute__((aligned(16))) uint8_t smem[kLds];
+    if constexpr (WHICH == 1) {
+        if (J.p2_prec == 3) {
+            gemm16_tile<3>(J.p2_hi, J.p2_lo, J.p2_lo2, J.np, J.zop.hi, nullptr, J.bp, J.V, J.np, 0, 0, m0, n0, k0,
+                           ksplit, smem);
+            return;
+        }
+    }
     if constexpr (PREC == 0) {
         if constexpr (WHICH == 0)
-            gemm_f32_tile(J.p1_32, J.np, J.xop.f32, J.np, J.Zpart, J.mp, J.zsplit, sp, m0, n0, k0, ksplit);
+            gemm_f32_tile(J.p1_32, J.np, J.xop.f32, J.np, J.Zpart, J.mp, J.zsplit, sp, m0, n0, k0, ksplit, smem);
         else
-            gemm_f32_tile(J.p2_32, J.mp, J.zop.f32, J.mp, J.V, J.np, 0, 0, m0, n0, k0, ksplit);
+            gemm_f32_tile(J.p2_32, J.mp, J.zop.f32, J.mp, J.V, J.np, 0, 0, m0, n0, k0, ksplit, smem);
     } else {
         if constexpr (WHICH == 0)
-            gemm16_tile<PREC>(J.p1_hi, J.p1_lo, J.mp, J.xop.hi, J.xop.lo, J.bp, J.Zpart, J.mp, J.zsplit, sp, m0, n0,
-                              k0, ksplit);
+            gemm16_tile<PREC>(J.p1_hi, J.p1_lo, nullptr, J.mp, J.xop.hi, J.xop.lo, J.bp, J.Zpart, J.mp, J.zsplit, sp,
+                              m0, n0, k0, ksplit, smem);
         else
-            gemm16_tile<PREC>(J.p2_hi, J.p2_lo, J.np, J.zop.hi, J.zop.lo, J.bp, J.V, J.np, 0, 0, m0, n0, k0, ksplit);
+            gemm16_tile<PREC>(J.p2_hi, J.p2_lo, nullptr, J.np, J.zop.hi, J.zop.lo, J.bp, J.V, J.np, 0, 0, m0, n0, k0,
+                              ksplit, smem);
     }
 }
 
@@ -1419,8 +1507,7 @@ __global__ __launch_bounds__(256) void fused_sparse_ax_kernel(const FusedJob* __
 __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __restrict__ jobs, int njobs) {
     const int k = find_job<2>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
-    prep_z_body(blockIdx.x - J.c0, J.Zpart, J.splits, J.zsplit, J.mp, J.mp, J.zop, J.sa.formulation, J.sa.precision,
-                J.sp);
+    prep_z_body(blockIdx.x - J.c0, J.Zpart, J.splits, J.zsplit, J.mp, J.mp, J.zop, J.sa.formulation, J.zprec, J.sp);
 }
 
 __global__ __launch_bounds__(kSelThreads) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {

@@ -23,7 +23,7 @@ SPCA_OK, SPCA_EINVAL, SPCA_ENOMEM, SPCA_EBUDGET, SPCA_EDEVICE, SPCA_ECANCELLED, 
 STATUS_NAMES = {0: "SPCA_OK", 1: "SPCA_EINVAL", 2: "SPCA_ENOMEM", 3: "SPCA_EBUDGET", 4: "SPCA_EDEVICE",
                 5: "SPCA_ECANCELLED", 6: "SPCA_ENOTFOUND", 7: "SPCA_EAGAIN"}
 
-STORE_FP32, STORE_BF16X3, STORE_FP16 = 1, 2, 4
+STORE_FP32, STORE_BF16X3, STORE_FP16, STORE_SPLIT3 = 1, 2, 4, 8
 PREC_FP32, PREC_BF16X3, PREC_FP16 = 0, 1, 2
 TERM_BATCH_MAX, TERM_PER_SP = 0, 1
 START_EXPLICIT, START_REFERENCE, START_REFERENCE_F64, START_GAUSSIAN = 0, 1, 2, 3
@@ -67,6 +67,7 @@ class _Settings(ctypes.Structure):
         ("precision", ctypes.c_int), ("start", ctypes.c_int), ("check_every", ctypes.c_int),
         ("resolve_iters", ctypes.c_int), ("resolve_tol", ctypes.c_double), ("seed", ctypes.c_uint64),
         ("sparse_products", ctypes.c_int),
+        ("exact_sign_products", ctypes.c_int),
     ]
 
 
@@ -176,6 +177,8 @@ class OptimizationSettings:
         self.resolve_tol = 1e-12
         self.seed = 0  # START_GAUSSIAN: key of the device-generated starting points
         self.sparse_products = 1  # gather A*X over the nonzeros of sparse iterates (0: always dense)
+        # L1 variance, FP32 and bf16x3: A^T sgn(z) from the exact bf16 split of A (0: the precision's product)
+        self.exact_sign_products = 1
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -202,7 +205,7 @@ class OptimizationSettings:
                          float(self.tolerance), int(self.maximumIterations), int(self.totalStartingPoints),
                          int(self.batchSize), int(bool(self.useOTF)), int(self.termination), int(self.precision),
                          int(self.start), int(self.check_every), int(self.resolve_iters), float(self.resolve_tol),
-                         int(self.seed), int(self.sparse_products))
+                         int(self.seed), int(self.sparse_products), int(self.exact_sign_products))
 
 
 class OptimizationStatistics:

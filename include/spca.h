@@ -83,6 +83,10 @@ typedef enum spca_start {
 #define SPCA_STORE_FP32 (1u << 0)
 #define SPCA_STORE_BF16X3 (1u << 1)
 #define SPCA_STORE_FP16 (1u << 2)
+/* three bf16 pieces of A's column-major orientation that sum to A exactly (each the bf16
+   rounding of what the previous ones leave): the operand of A^T * sgn(z), the L1-variance
+   formulations' second product, on bf16 MFMA with every product exact (exact_sign_products) */
+#define SPCA_STORE_SPLIT3 (1u << 3)
 
 /* ---- settings / statistics ---------------------------------------------- */
 /* mirrors OptimizationSettings (c++/src/class/optimization_settings.h:71-123) */
@@ -104,9 +108,15 @@ typedef struct spca_settings {
     double resolve_tol;       /* relative objective change that stops the re-solve */
     uint64_t seed;            /* key of SPCA_START_GAUSSIAN starting points */
     int sparse_products;      /* 1 (default): A*X is gathered over the nonzeros of the iterates
-                                 (FP64 accumulation) once every iterating SP has at most n/64
-                                 (FP32), n/192 (bf16x3) or n/384 (fp16) of them; 0: always the
-                                 dense product of `precision` */
+                                 (FP64 accumulation) for each group of 256 SPs of a batch once
+                                 every iterating SP of the group has at most n/64 (FP32), n/192
+                                 (bf16x3) or n/384 (fp16) of them; 0: always the dense product
+                                 of `precision` */
+    int exact_sign_products;  /* 1 (default): for the L1-variance formulations (z = sgn(A x) in
+                                 {-1, 0, +1}), FP32 and bf16x3 solves compute A^T z from the
+                                 exact three-piece bf16 split of A (SPCA_STORE_SPLIT3, added on
+                                 first use): every product exact, FP32 accumulation, at bf16
+                                 MFMA rate; 0: the product of `precision` */
 } spca_settings;
 
 /* mirrors OptimizationStatistics (c++/src/class/optimization_statistics.h:27-46) */

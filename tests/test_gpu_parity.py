@@ -543,3 +543,66 @@ def test_sparse_flags_per_group_of_256(ctx, prec):
         assert ss.best_sp == sd.best_sp
     finally:
         M.release()
+
+
+@pytest.mark.parametrize("B", [64, 256])
+@pytest.mark.parametrize("f", [1, 3, 5, 7])
+def test_exact_split_sign_products_bitwise(ctx, f, B):
+    """A^T sgn(z) from the three-piece bf16 split of A is exact: on a matrix whose FP32 sums
+    are all exact (entries k * 2^-19 with |k| < 2^19, 32 rows, so every partial sum is a
+    multiple of 2^-19 below 2^5), the split product and the FP32 product agree bitwise, and
+    so do whole L1-variance trajectories.  The entries carry up to 20 significant bits, so all
+    three pieces are non-zero.  B = 64 runs the 128x128-tile kernel, B = 256 the pipelined
+    256x256-tile kernel.  Penalties keep about 5% of the coordinates ((A^T sgn z)_i has
+    standard deviation sqrt(32/3))."""
+    rng = np.random.default_rng(30 + f)
+    m, n = 32, 1024
+    A = (rng.integers(-2**19, 2**19, size=(m, n)) * 2.0**-19).astype(np.float32)
+    X0 = unit_gaussian_starts(n, B, seed=31)
+    gamma = {5: 43.0, 7: 6.5}.get(f, 0.0)
+    M = ctx.upload(A)
+    try:
+        out, nbytes = {}, {}
+        for exact in (0, 1):
+            st = S.OptimizationSettings(formulation=f, constraintParameter=24, penaltyParameter=gamma,
+                                        tolerance=0.0, maximumIterations=6, totalStartingPoints=B, batchSize=B,
+                                        exact_sign_products=exact)
+            stats = S.OptimizationStatistics()
+            _, X = M.solve(st, X0=X0, stats=stats, want_X=True)
+            out[exact] = (X, stats)
+            nbytes[exact] = M.nbytes()
+        nnz = np.count_nonzero(out[1][0], axis=0)
+        print(f"case f={f} B={B}: nnz per SP min {nnz.min()} max {nnz.max()}", flush=True)
+        assert nnz.min() > 0 and np.all(np.isfinite(out[1][0]))
+        np.testing.assert_array_equal(out[1][0], out[0][0])
+        np.testing.assert_array_equal(out[1][1].values, out[0][1].values)
+        assert nbytes[1] > nbytes[0]  # the split pieces were added on first use
+    finally:
+        M.release()
+
+
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3])
+def test_exact_split_sign_products_against_fp64(small, small_mat, prec):
+    """L1 variance with the L1 constraint, 40 iterations: the exact split and the precision's
+    own A^T*Z product both stay within the precision's EV bound of the FP64 oracle, and the
+    exact split's A^T*Z error (seen through the final iterates) is no larger than that of the
+    product it replaces."""
+    A, X0, _ = small
+    f, B, it = 3, X0.shape[1], 40
+    o = O.solve(A.astype(np.float64), f, s=10, gamma=0.0, tol=0.0, max_iter=it, total_sp=B, X0=X0.astype(np.float64),
+                want_X=True, lambda_rule=O.LAMBDA_GPU)
+    ev_o = explained_variance(A, o["X"], f)
+    errs = {}
+    for exact in (0, 1):
+        st = _settings(f, it, prec, B)
+        st.exact_sign_products = exact
+        stats = S.OptimizationStatistics()
+        _, Xg = small_mat.solve(st, X0=X0, stats=stats, want_X=True)
+        same = np.all((Xg != 0) == (o["X"] != 0), axis=0)
+        rel = np.abs(stats.sp_ev_fp64 - ev_o) / ev_o
+        errs[exact] = np.median(rel[same])
+        print(f"prec {prec} exact {exact}: supports equal {same.mean():.3f} EV rel median {errs[exact]:.2e} "
+              f"max {rel[same].max():.2e}", flush=True)
+        assert same.mean() >= 0.9
+        assert np.mean(rel[same] <= TOL[prec][0]) >= 0.95
+    assert errs[1] <= 2 * errs[0] + 1e-9
