@@ -43,7 +43,8 @@ struct Matrix {
     uint16_t* lo_rm = nullptr;
     uint16_t* h_cm = nullptr;
     uint16_t* h_rm = nullptr;
-    uint16_t* lo2_cm = nullptr;  // third piece of the exact split (with hi_cm, lo_cm)
+    int8_t* q_cm = nullptr;      // fixed-point pieces of A for the exact A^T * sgn(z) (4 x mp x np bytes)
+    float* qscale = nullptr;     // their per-column power-of-two scales (np)
     uint64_t bytes = 0;
     int refs = 0;
 };
@@ -252,7 +253,7 @@ bool valid_formulation(int f) { return f >= 0 && f <= 7; }
 
 void free_matrix(spca_ctx ctx, Matrix* M) {
     (void)hipStreamSynchronize(ctx->stream);
-    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm, M->lo2_cm};
+    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm, M->q_cm, M->qscale};
     for (void* p : ptrs) raw_free(ctx, p);
     delete M;
 }
@@ -278,33 +279,30 @@ int add_storage(spca_ctx ctx, Matrix* M, unsigned storage) {
         HIP_TRY(ctx, launch_transpose_f32(M->a32_cm, M->mp, M->np, M->mp, M->a32_rm, M->np, st));
         M->storage |= SPCA_STORE_FP32;
     }
-    // hi_cm and lo_cm serve both the bf16x3 form and the exact split (the same two roundings)
-    if ((want & (SPCA_STORE_BF16X3 | SPCA_STORE_SPLIT3)) && !M->hi_cm) {
+    if (want & SPCA_STORE_BF16X3) {
         if ((rc = dev_alloc(ctx, &M->hi_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_cm, cnt, &M->bytes)))
             return rc;
-        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 1, M->hi_cm, M->lo_cm, nullptr, st));
-    }
-    if (want & SPCA_STORE_BF16X3) {
+        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 1, M->hi_cm, M->lo_cm, st));
         if ((rc = need_rm())) return rc;
         const float* rm = M->a32_rm ? M->a32_rm : rm_tmp;
         if ((rc = dev_alloc(ctx, &M->hi_rm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->lo_rm, cnt, &M->bytes)))
             return rc;
-        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 1, M->hi_rm, M->lo_rm, nullptr, st));
+        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 1, M->hi_rm, M->lo_rm, st));
         M->storage |= SPCA_STORE_BF16X3;
     }
-    if (want & SPCA_STORE_SPLIT3) {
-        if ((rc = dev_alloc(ctx, &M->lo2_cm, cnt, &M->bytes))) return rc;
-        // rewrites hi_cm and lo_cm with the same values
-        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 3, M->hi_cm, M->lo_cm, M->lo2_cm, st));
-        M->storage |= SPCA_STORE_SPLIT3;
+    if (want & SPCA_STORE_FIX8) {
+        if ((rc = dev_alloc(ctx, &M->q_cm, 4 * cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->qscale, (size_t)M->np, &M->bytes)))
+            return rc;
+        HIP_TRY(ctx, launch_to_fix8(M->a32_cm, M->mp, M->np, M->q_cm, M->qscale, st));
+        M->storage |= SPCA_STORE_FIX8;
     }
     if (want & SPCA_STORE_FP16) {
         if ((rc = need_rm())) return rc;
         const float* rm = M->a32_rm ? M->a32_rm : rm_tmp;
         if ((rc = dev_alloc(ctx, &M->h_cm, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->h_rm, cnt, &M->bytes)))
             return rc;
-        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 2, M->h_cm, nullptr, nullptr, st));
-        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 2, M->h_rm, nullptr, nullptr, st));
+        HIP_TRY(ctx, launch_to_blocked(M->a32_cm, M->mp, M->np, M->mp, 2, M->h_cm, nullptr, st));
+        HIP_TRY(ctx, launch_to_blocked(rm, M->np, M->mp, M->np, 2, M->h_rm, nullptr, st));
         M->storage |= SPCA_STORE_FP16;
     }
     HIP_TRY(ctx, hipStreamSynchronize(st));
@@ -613,47 +611,53 @@ int choose_splits(int precision, int mp, int np, int bp, bool allow_big = true) 
 Operand x_operand(Workspace& w, int precision) {
     return Operand{w.X, precision ? w.Xhi : nullptr, precision == 1 ? w.Xlo : nullptr, w.bp};
 }
-// precision 3: the bf16 sign vectors of the exact-split product, in Zhi alone
+// precision 4: the int8 sign vectors of the fixed-point product, in Zhi alone
 Operand z_operand(Workspace& w, int precision) {
     return Operand{w.Z, precision ? w.Zhi : nullptr, precision == 1 ? w.Zlo : nullptr, w.bp};
 }
 
 // The second product, A^T*Z, of a solve: the precision's own operands, or, for the
-// L1-variance formulations with exact_sign_products (FP32 and bf16x3), the exact three-piece
-// split of A against Z = sgn(A x) in bf16: every product exact, FP32 accumulation.
+// L1-variance formulations with exact_sign_products (FP32 and bf16x3), the fixed-point pieces
+// of A against Z = sgn(A x) in int8 on the i8 MFMA: exact integer sums, one rounding per v_j.
 bool uses_sign_products(const spca_settings& st) {
     return st.exact_sign_products && spca_is_l1_variance(st.formulation) && (st.precision == 0 || st.precision == 1);
 }
 struct AtzForm {
-    int prec;            // launch_gemm_tn precision
+    int prec;            // launch_gemm_tn precision, or 4: the fixed-point product (launch_gemm_fix8)
     int zprec;           // form prep_z writes Z in
     const float* p32;
     const uint16_t* hi;
     const uint16_t* lo;
-    const uint16_t* lo2;
+    const int8_t* q;     // fixed-point pieces (prec 4)
+    const float* qscale;
     long ldp, ldq;       // FP32 row strides or 16-bit row counts
     int big;
 };
 AtzForm atz_form(const Matrix* M, const spca_settings& st, int bp, int plan_big) {
     AtzForm a;
     a.p32 = M->a32_cm;
+    a.q = M->q_cm;
+    a.qscale = M->qscale;
     if (uses_sign_products(st)) {
-        a.prec = a.zprec = 3;
-        a.hi = M->hi_cm;
-        a.lo = M->lo_cm;
-        a.lo2 = M->lo2_cm;
-        // FP32 plans never pick the big tiles for their own products; the split form does
-        a.big = gemm_big_tiles(3, M->np, bp) && (st.precision == 0 || plan_big);
+        a.prec = a.zprec = 4;
+        a.hi = a.lo = nullptr;
+        a.big = 0;
     } else {
         a.prec = a.zprec = st.precision;
         a.hi = st.precision == 1 ? M->hi_cm : M->h_cm;
         a.lo = M->lo_cm;
-        a.lo2 = nullptr;
         a.big = plan_big;
     }
     a.ldp = a.prec ? M->np : M->mp;
     a.ldq = a.prec ? bp : M->mp;
     return a;
+}
+// V = A^T * Z of one iteration in the solve's A^T*Z form.
+hipError_t launch_atz(const AtzForm& a, const Matrix* M, Workspace& w, int bp, const int* done, hipStream_t s) {
+    if (a.prec == 4)
+        return launch_gemm_fix8(a.q, a.qscale, M->np, (const int8_t*)w.Zhi, bp, w.V, M->np, M->np, bp, M->mp, done, s);
+    return launch_gemm_tn(a.prec, a.p32, a.hi, a.lo, a.ldp, w.Z, w.Zhi, w.Zlo, a.ldq, w.V, M->np, 0, M->np, bp, M->mp, 1,
+                          a.big, done, nullptr, s);
 }
 
 // Starting points with global indices shift .. shift+cnt-1 (n floats each) into out.
@@ -916,12 +920,12 @@ int tune_plan(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, Works
             for (int r = 0; r < 4; r++) {
                 hipError_t e = hipEventRecord(e0, s);
                 if (e == hipSuccess)
-                    e = launch_gemm_tn(prec, P1_32, P1_hi, M->lo_rm, nullptr, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
+                    e = launch_gemm_tn(prec, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                        mp, zsplit, mp, bp, np, splits, bi, nullptr, nullptr, s);
                 if (e == hipSuccess)
                     e = launch_prep_z(w.Zpart, splits, zsplit, mp, mp, zop, st.formulation, prec, st.batch, sp, s);
                 if (e == hipSuccess)
-                    e = launch_gemm_tn(prec, P2_32, P2_hi, M->lo_cm, nullptr, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V, np,
+                    e = launch_gemm_tn(prec, P2_32, P2_hi, M->lo_cm, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V, np,
                                        0, np, bp, mp, 1, bi, nullptr, nullptr, s);
                 if (e == hipSuccess) e = hipEventRecord(e1, s);
                 if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1038,7 +1042,7 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
     if (st.start == SPCA_START_EXPLICIT && !X0) return fail(ctx, SPCA_EINVAL, "X0 is required for explicit starts");
     if (!x_out) return fail(ctx, SPCA_EINVAL, "x_out is required");
     unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
-    if (uses_sign_products(st)) need |= SPCA_STORE_SPLIT3;
+    if (uses_sign_products(st)) need |= SPCA_STORE_FIX8;
     int rc = add_storage(ctx, M, need);
     if (rc) return rc;
     *nb = spca_settings_check(&st, (unsigned)M->n);
@@ -1166,7 +1170,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, nullptr, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                         M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, nullptr, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
@@ -1176,8 +1180,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(atz.prec, atz.p32, atz.hi, atz.lo, atz.lo2, atz.ldp, w.Z, w.Zhi, w.Zlo, atz.ldq,
-                                        w.V, M->np, 0, M->np, bp, M->mp, 1, atz.big, sp.done, nullptr, s));
+            HIP_TRY(ctx, launch_atz(atz, M, w, bp, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1254,7 +1257,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             // one of the two A*X forms runs; the other sees the device flag and returns
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, nullptr, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
+            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                         M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done,
                                         sp.done + kSparseFlags, s));
             HIP_TRY(ctx, launch_sparse_ax(M->a32_cm, M->mp, M->mp, n, w.X, w.np, w.Zpart, M->mp, batch, sp, s));
@@ -1266,8 +1269,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_gemm_tn(atz.prec, atz.p32, atz.hi, atz.lo, atz.lo2, atz.ldp, w.Z, w.Zhi, w.Zlo, atz.ldq,
-                                        w.V, M->np, 0, M->np, bp, M->mp, 1, atz.big, sp.done, nullptr, s));
+            HIP_TRY(ctx, launch_atz(atz, M, w, bp, sp.done, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1465,7 +1467,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     }
     // ---- job table
     std::vector<FusedJob> tab(njobs);
-    int t1 = 0, t2 = 0, c0 = 0, t3 = 0;
+    int t1 = 0, t2 = 0, c0 = 0, t3 = 0, any_fix8 = 0;
     for (int k = 0; k < njobs; k++) {
         FusedRun& r = run[k];
         Matrix* M = r.M;
@@ -1477,9 +1479,11 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         f.p2_32 = M->a32_cm;
         f.p2_hi = atz.hi;
         f.p2_lo = atz.lo;
-        f.p2_lo2 = atz.lo2;
+        f.p2_q = atz.q;
+        f.p2_qscale = atz.qscale;
         f.p2_prec = atz.prec;
         f.zprec = atz.zprec;
+        any_fix8 |= atz.prec == 4;
         f.Zpart = r.ws.Zpart;
         f.V = r.ws.V;
         f.xop = x_operand(r.ws, precision);
@@ -1519,7 +1523,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     if (flags & SPCA_FUSED_GRAPH) {
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         for (int i = 0; i < chunk && e == hipSuccess; i++)
-            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, d_done, s);
+            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, any_fix8, d_done, s);
         hipError_t e2 = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = e2;
         if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
@@ -1543,7 +1547,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
             e = hipGraphLaunch(gexec, s);
         else
             for (int i = 0; i < chunk && e == hipSuccess; i++)
-                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, d_done, s);
+                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, any_fix8, d_done, s);
         hipEvent_t ev;
         if (e == hipSuccess && !new_event(&ev)) e = hipErrorOutOfMemory;
         if (e == hipSuccess) e = hipEventRecord(ev, s);

@@ -68,7 +68,7 @@ struct SpState {
 // column j at ((i / 32) * bp + j) * 32 + i % 32.
 struct Operand {
     float* f32;          // FP32 values (also the master copy of iterates)
-    uint16_t* hi;        // bf16 hi part, or fp16 values
+    uint16_t* hi;        // bf16 hi part, or fp16 values, or (Z of the fixed-point product) int8 signs
     uint16_t* lo;        // bf16 lo part (bf16x3 only)
     int bp;              // columns (padded batch)
 };
@@ -108,7 +108,8 @@ struct FusedJob {
     const float* p2_32;     // A column major (ld mp): A^T*Z operand
     const uint16_t* p2_hi;
     const uint16_t* p2_lo;
-    const uint16_t* p2_lo2;  // third piece of the exact split (p2_prec 3)
+    const int8_t* p2_q;      // fixed-point pieces of A (p2_prec 4)
+    const float* p2_qscale;  // their per-column scales
     float* Zpart;           // splits x (mp x bp) partial products of A*X
     float* V;               // np x bp
     Operand xop, zop;
@@ -121,7 +122,7 @@ struct FusedJob {
     int t3;                 // first (row block, SP) item of the sparse A*X product
     int max_iter, per_sp;
     int sparse_cap;         // largest nonzero count for the sparse A*X product (0: never)
-    int p2_prec;            // A^T*Z operand form: the job's precision, or 3 (exact split x sign vectors)
+    int p2_prec;            // A^T*Z operand form: the job's precision, or 4 (fixed point x sign vectors)
     int zprec;              // form prep_z writes Z in (as p2_prec)
 };
 
@@ -131,26 +132,34 @@ hipError_t launch_pad_copy_f32(const float* src, int m, int n, int lds, float* d
 hipError_t launch_transpose_f32(const float* src, int rows, int cols, int lds, float* dst, int ldd,
                                 hipStream_t st);
 // src: R rows x K (row r at src + r*lds) -> K-blocked 16-bit form (precision 1: bf16 hi + lo,
-// 2: fp16 in hi, 3: bf16 hi + lo + lo2 summing to the FP32 value exactly): element (r, k) at
-// ((k / 32) * R + r) * 32 + k % 32.  K % 32 == 0.
+// 2: fp16 in hi): element (r, k) at ((k / 32) * R + r) * 32 + k % 32.  K % 32 == 0.
 hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int precision, uint16_t* hi, uint16_t* lo,
-                             uint16_t* lo2, hipStream_t st);
+                             hipStream_t st);
+// A (mp x np column major) -> the fixed-point form of the exact A^T * sgn(z) product: per
+// column a power-of-two scale s_j (qscale) and four int8 pieces with
+// A_ij = s_j (2^-7 q0 + 2^-13 q1 + 2^-19 q2 + 2^-25 q3) + e, |e| <= 2^-26 s_j; piece p at
+// q + p * mp * np, element (j, i) K-blocked by 64 over np rows: ((i / 64) * np + j) * 64 + i % 64.
+hipError_t launch_to_fix8(const float* A, int mp, int np, int8_t* q, float* qscale, hipStream_t st);
 
 // C[r + c*ldc + split*split_stride] = sum_{k in split} P(r, k) * Q(c, k)
 // FP32: P(r, k) = P32[r*ldp + k], Q(c, k) = Q32[c*ldq + k].  16-bit: K-blocked operands with
 // ldp / ldq rows, P(r, k) = P[((k / 32) * ldp + r) * 32 + k % 32] (likewise Q).  precision
-// 1: bf16x3 (Phi, Plo, Qhi, Qlo), 2: fp16 (Phi, Qhi), 3: exact split (Phi + Plo + Plo2 == P)
-// against a bf16 Q in Qhi, whose products are exact when Q holds -1, 0 and +1.
+// 1: bf16x3 (Phi, Plo, Qhi, Qlo), 2: fp16 (Phi, Qhi).
 // M % 128 == 0, N % 128 == 0, K % (32*splits) == 0.  big = 1 runs the pipelined 256x256-tile
 // kernel, allowed for 16-bit products with M and N multiples of 256 (gemm_big_tiles); else
 // 128x128 tiles.  done (nullable): the launch returns when *done; skip (nullable): tiles
 // of 256-column group g return when skip[g].
 int gemm_big_tiles(int precision, int M, int N);
-hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo,
-                          const uint16_t* Plo2, long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo,
-                          long ldq,
-                          float* C, long ldc, long split_stride, int M, int N, int K, int splits, int big,
-                          const int* done, const int* skip, hipStream_t st);
+hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo, long ldp,
+                          const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq, float* C, long ldc,
+                          long split_stride, int M, int N, int K, int splits, int big, const int* done,
+                          const int* skip, hipStream_t st);
+// C[r + c*ldc] = s_r (2^-13 accA + 2^-25 accB): the exact A^T * sgn(z) from the fixed-point
+// pieces P (launch_to_fix8 with np = ldp rows, mp = K) against int8 signs Q (K-blocked by 64
+// over ldq rows, entries -1, 0, +1); int32 accumulation, so the result does not depend on
+// tiling or order.  M % 128 == 0, N % 128 == 0, K % 64 == 0, K < 2^18.
+hipError_t launch_gemm_fix8(const int8_t* P, const float* qscale, long ldp, const int8_t* Q, long ldq, float* C,
+                            long ldc, int M, int N, int K, const int* done, hipStream_t st);
 
 // Sum split partials of Z, per-column statistic and map (sign / normalise), write the
 // operand of the second product.
@@ -183,12 +192,13 @@ hipError_t launch_plant(float* A, int m, int mp, const int* idx, const float* co
                         hipStream_t st);
 
 // One AM iteration of every job in `jobs` (device array, sorted by t1/t2/c0) as five
-// launches: A*X tiles of all jobs, the z epilogue of all columns, A^T*Z tiles, the
+// launches: A*X tiles of all jobs, the z epilogue of all columns, A^T*Z tiles (any_fix8: plus
+// a launch over the same tile list for the jobs on the fixed-point product), the
 // formulation step of all columns and the per-job convergence (a finished job adds 1 to
 // *all_done).  128x128 tiles; per job the same tile order, K split and kernel bodies as a
 // solo solve, so a job's FP32 results do not depend on the other jobs.
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
-                                  int items3, int* all_done, hipStream_t st);
+                                  int items3, int any_fix8, int* all_done, hipStream_t st);
 
 // z_j = A x_j over the nonzeros of each active x_j (FP64 accumulation, fixed order) into Z
 // (ld ldz), when sp.done[2] is set; else it returns at once.  A: FP32 column major, ld lda.

@@ -148,7 +148,7 @@ __device__ __forceinline__ long blk_index(int r, int k, int R) { return ((long)(
 // or fp16 (precision 2).  One thread per 32-element chunk; consecutive threads take
 // consecutive rows of one K block, so the 64-byte output rows are written contiguously.
 __global__ void to_blocked_kernel(const float* __restrict__ src, long lds, int R, int K, int precision,
-                                  uint16_t* __restrict__ hi, uint16_t* __restrict__ lo, uint16_t* __restrict__ lo2) {
+                                  uint16_t* __restrict__ hi, uint16_t* __restrict__ lo) {
     const long chunks = (long)R * (K / 32);
     for (long c = blockIdx.x * (long)blockDim.x + threadIdx.x; c < chunks; c += (long)gridDim.x * blockDim.x) {
         const int r = (int)(c % R), kb = (int)(c / R);
@@ -158,10 +158,10 @@ __global__ void to_blocked_kernel(const float* __restrict__ src, long lds, int R
         for (int q = 0; q < 4; q++) {
             const f4 v0 = *(const f4*)(sp + 8 * q), v1 = *(const f4*)(sp + 8 * q + 4);
             const float x[8] = {v0[0], v0[1], v0[2], v0[3], v1[0], v1[1], v1[2], v1[3]};
-            u4 hv, lv, l2v;
+            u4 hv, lv;
 #pragma unroll
             for (int e = 0; e < 4; e++) {
-                uint32_t h[2], l[2] = {0, 0}, l2[2] = {0, 0};
+                uint32_t h[2], l[2] = {0, 0};
 #pragma unroll
                 for (int u = 0; u < 2; u++) {
                     const float v = x[2 * e + u];
@@ -169,26 +169,22 @@ __global__ void to_blocked_kernel(const float* __restrict__ src, long lds, int R
                         h[u] = f2h(v);
                         continue;
                     }
-                    // v - hi and (v - hi) - lo are exact in FP32, so hi + lo + lo2 == v
+                    // v - hi is exact in FP32; lo is its bf16 rounding
                     h[u] = f2bf(v);
-                    const float r1 = v - bf2f((uint16_t)h[u]);
-                    l[u] = f2bf(r1);
-                    l2[u] = f2bf(r1 - bf2f((uint16_t)l[u]));
+                    l[u] = f2bf(v - bf2f((uint16_t)h[u]));
                 }
                 hv[e] = h[0] | (h[1] << 16);
                 lv[e] = l[0] | (l[1] << 16);
-                l2v[e] = l2[0] | (l2[1] << 16);
             }
             *(u4*)(hi + o + 8 * q) = hv;
             if (precision != 2) *(u4*)(lo + o + 8 * q) = lv;
-            if (precision == 3) *(u4*)(lo2 + o + 8 * q) = l2v;
         }
     }
 }
 hipError_t launch_to_blocked(const float* src, long lds, int R, int K, int precision, uint16_t* hi, uint16_t* lo,
-                             uint16_t* lo2, hipStream_t st) {
-    if (R <= 0 || K <= 0 || K % 32 || precision < 1 || precision > 3) return hipErrorInvalidValue;
-    to_blocked_kernel<<<8192, 256, 0, st>>>(src, lds, R, K, precision, hi, lo, lo2);
+                             hipStream_t st) {
+    if (R <= 0 || K <= 0 || K % 32 || precision < 1 || precision > 2) return hipErrorInvalidValue;
+    to_blocked_kernel<<<8192, 256, 0, st>>>(src, lds, R, K, precision, hi, lo);
     return hipGetLastError();
 }
 
@@ -355,21 +351,16 @@ __global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restric
 // 16-bit operand products (operand parts K-blocked, in bf16 unless noted):
 //   PREC 1  bf16x3: P = hi + lo, Q = hi + lo; Phi*Qlo + Plo*Qhi + Phi*Qhi
 //   PREC 2  fp16:   Phi*Qhi in fp16
-//   PREC 3  exact split against a bf16 Q: P = hi + lo + lo2 == A exactly (each piece the
-//           bf16 rounding of what the previous ones leave); lo2*Q + lo*Q + hi*Q.  With Q in
-//           {-1, 0, +1} (the sign vectors of the L1-variance iterations) every product is
-//           exact and only the FP32 accumulation rounds, as in the FP32 product.
 // Small terms are accumulated first.  One 16x16x32 MFMA covers a whole 32-deep K stage;
 // lane l supplies row l%16 and k = 8*(l/16) .. +7 of each operand.
-constexpr int tile16_parts_p(int prec) { return prec == 1 ? 2 : prec == 3 ? 3 : 1; }
+constexpr int tile16_parts_p(int prec) { return prec == 1 ? 2 : 1; }
 constexpr int tile16_parts_q(int prec) { return prec == 1 ? 2 : 1; }
 // LDS bytes of one 128x128 16-bit tile (80-byte rows)
 constexpr int tile16_lds(int prec) {
     return (tile16_parts_p(prec) * kGemmBM + tile16_parts_q(prec) * kGemmBN) * (kGemmBK + 8) * 2;
 }
 template <int PREC>
-__device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, const uint16_t* __restrict__ Plo,
-                                            const uint16_t* __restrict__ Plo2, long ldp,
+__device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, const uint16_t* __restrict__ Plo, long ldp,
                                             const uint16_t* __restrict__ Qhi, const uint16_t* __restrict__ Qlo,
                                             long ldq, float* __restrict__ C, long ldc, long split_stride, int sp,
                                             long m0, long n0, long k0, int ksplit, uint8_t* smem) {
@@ -386,7 +377,7 @@ __device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, co
     const uint16_t* Qg[NPQ];
     // K-blocked operands: ldp / ldq are their row counts; one K stage advances 32 rows' worth
     const long pstep = ldp * kGemmBK, qstep = ldq * kGemmBK;
-    const uint16_t* pbase[3] = {Phi, Plo, Plo2};
+    const uint16_t* pbase[2] = {Phi, Plo};
     const uint16_t* qbase[2] = {Qhi, Qlo};
 #pragma unroll
     for (int p = 0; p < NPP; p++) Pg[p] = pbase[p] + ((k0 >> 5) * ldp + m0 + lrow) * 32 + lc8;
@@ -444,12 +435,7 @@ __device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, co
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                if constexpr (PREC == 3) {
-#pragma unroll
-                    for (int p = 2; p >= 0; p--)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf8, a[p][i]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
-                } else if constexpr (PREC == 1) {
+                if constexpr (PREC == 1) {
                     // small terms first, then the leading product
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
                         __builtin_bit_cast(bf8, a[0][i]), __builtin_bit_cast(bf8, b[1][j]), acc[i][j], 0, 0, 0);
@@ -476,8 +462,7 @@ __device__ __forceinline__ void gemm16_tile(const uint16_t* __restrict__ Phi, co
 
 template <int PREC>
 __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restrict__ Phi,
-                                                         const uint16_t* __restrict__ Plo,
-                                                         const uint16_t* __restrict__ Plo2, long ldp,
+                                                         const uint16_t* __restrict__ Plo, long ldp,
                                                          const uint16_t* __restrict__ Qhi,
                                                          const uint16_t* __restrict__ Qlo, long ldq,
                                                          float* __restrict__ C, long ldc, long split_stride,
@@ -493,7 +478,7 @@ __global__ __launch_bounds__(256) void gemm_tn_16_kernel(const uint16_t* __restr
     const int sp = rest / tiles_m;
     if (skip && skip[tn >> 1]) return;  // this 256-column group's A*X is the sparse gather
     __shared__ __attribute__((aligned(16))) uint8_t smem[tile16_lds(PREC)];
-    gemm16_tile<PREC>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM,
+    gemm16_tile<PREC>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM,
                       (long)tn * kGemmBN, (long)sp * ksplit, ksplit, smem);
 }
 
@@ -526,8 +511,7 @@ __device__ __forceinline__ void wait_vmcnt(int n) {
 
 template <int PREC, int NSTAGE>
 __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t* __restrict__ Phi,
-                                                                 const uint16_t* __restrict__ Plo,
-                                                                 const uint16_t* __restrict__ Plo2, long ldp,
+                                                                 const uint16_t* __restrict__ Plo, long ldp,
                                                                  const uint16_t* __restrict__ Qhi,
                                                                  const uint16_t* __restrict__ Qlo, long ldq,
                                                                  float* __restrict__ C, long ldc, long split_stride,
@@ -561,7 +545,7 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
     // part t < NPP is P part t, the rest Q parts
     const uint16_t* src[NPT];
     {
-        const uint16_t* pbase[3] = {Phi, Plo, Plo2};
+        const uint16_t* pbase[2] = {Phi, Plo};
         const uint16_t* qbase[2] = {Qhi, Qlo};
 #pragma unroll
         for (int t = 0; t < NPT; t++) src[t] = t < NPP ? pbase[t] : qbase[t - NPP];
@@ -611,36 +595,7 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
                 const int r = wn * 64 + j * 16 + (lane & 15);
                 b[p][j] = *(const u4*)(sb + (NPP + p) * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
             }
-        if constexpr (PREC == 3) {
-            // one P piece at a time (lo2, lo, hi), each piece's eight fragments read while the
-            // previous piece's MFMAs issue: two fragment buffers instead of three keep the
-            // 128 accumulators and the fragments within 256 registers
-            u4 fa[2][8];
-            auto load_piece = [&](int p, u4* dst) {
-#pragma unroll
-                for (int i = 0; i < 8; i++) {
-                    const int r = wm * 128 + i * 16 + (lane & 15);
-                    dst[i] = *(const u4*)(sb + p * OPB + r * ROWB + ((fchunk ^ ((r >> 1) & 3)) << 4));
-                }
-            };
-            auto mma_piece = [&](const u4* a) {
-#pragma unroll
-                for (int i = 0; i < 8; i++)
-#pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(
-                            __builtin_bit_cast(bf8, a[i]), __builtin_bit_cast(bf8, b[0][j]), acc[i][j], 0, 0, 0);
-            };
-            load_piece(2, fa[0]);
-            __builtin_amdgcn_sched_barrier(0);
-            load_piece(1, fa[1]);
-            mma_piece(fa[0]);
-            __builtin_amdgcn_sched_barrier(0);
-            load_piece(0, fa[0]);
-            mma_piece(fa[1]);
-            __builtin_amdgcn_sched_barrier(0);
-            mma_piece(fa[0]);
-        } else {
+        {
             // every A fragment of the stage is read first; the MFMAs then issue in two groups kept
             // apart by scheduling barriers, so the compiler neither sinks the reads between the
             // MFMAs nor interleaves the groups
@@ -696,30 +651,22 @@ __global__ __launch_bounds__(512, 1) void gemm_tn_16_pipe_kernel(const uint16_t*
 
 int gemm_big_tiles(int precision, int M, int N) { return precision != 0 && M % 256 == 0 && N % 256 == 0; }
 
-hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo,
-                          const uint16_t* Plo2, long ldp, const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo,
-                          long ldq, float* C,
-                          long ldc, long split_stride, int M, int N, int K, int splits, int big, const int* done,
+hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, const uint16_t* Plo, long ldp,
+                          const float* Q32, const uint16_t* Qhi, const uint16_t* Qlo, long ldq, float* C, long ldc,
+                          long split_stride, int M, int N, int K, int splits, int big, const int* done,
                           const int* skip, hipStream_t st) {
     if (M % kGemmBM || N % kGemmBN || splits < 1 || K % (kGemmBK * splits)) return hipErrorInvalidValue;
-    if (precision < 0 || precision > 3 || (precision == 3 && (!Phi || !Plo || !Plo2 || !Qhi)))
-        return hipErrorInvalidValue;
+    if (precision < 0 || precision > 2) return hipErrorInvalidValue;
     if (big && !gemm_big_tiles(precision, M, N)) return hipErrorInvalidValue;
     if (big) {
         const int tiles_m = M / 256, tiles_n = N / 256, ksplit = K / splits;
         const int grid = tiles_m * tiles_n * splits;
         if (precision == 1)
-            gemm_tn_16_pipe_kernel<1, 2><<<grid, 512, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc,
-                                                               split_stride, tiles_m, tiles_n, splits, ksplit, done,
-                                                               skip);
-        else if (precision == 3)
-            gemm_tn_16_pipe_kernel<3, 2><<<grid, 512, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc,
-                                                               split_stride, tiles_m, tiles_n, splits, ksplit, done,
-                                                               skip);
+            gemm_tn_16_pipe_kernel<1, 2><<<grid, 512, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride,
+                                                               tiles_m, tiles_n, splits, ksplit, done, skip);
         else
-            gemm_tn_16_pipe_kernel<2, 4><<<grid, 512, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc,
-                                                               split_stride, tiles_m, tiles_n, splits, ksplit, done,
-                                                               skip);
+            gemm_tn_16_pipe_kernel<2, 4><<<grid, 512, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride,
+                                                               tiles_m, tiles_n, splits, ksplit, done, skip);
         return hipGetLastError();
     }
     const int tiles_m = M / kGemmBM, tiles_n = N / kGemmBN, ksplit = K / splits;
@@ -728,15 +675,192 @@ hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, 
         gemm_tn_f32_kernel<<<grid, 256, 0, st>>>(P32, ldp, Q32, ldq, C, ldc, split_stride, tiles_m, tiles_n, splits,
                                                  ksplit, done, skip);
     } else if (precision == 1) {
-        gemm_tn_16_kernel<1><<<grid, 256, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
-                                                   tiles_n, splits, ksplit, done, skip);
-    } else if (precision == 3) {
-        gemm_tn_16_kernel<3><<<grid, 256, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
+        gemm_tn_16_kernel<1><<<grid, 256, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
                                                    tiles_n, splits, ksplit, done, skip);
     } else {
-        gemm_tn_16_kernel<2><<<grid, 256, 0, st>>>(Phi, Plo, Plo2, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
+        gemm_tn_16_kernel<2><<<grid, 256, 0, st>>>(Phi, Plo, ldp, Qhi, Qlo, ldq, C, ldc, split_stride, tiles_m,
                                                    tiles_n, splits, ksplit, done, skip);
     }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- exact A^T * sgn(z)
+// In the L1-variance formulations z = sgn(A x) holds -1, 0 and +1, so v = A^T z is a signed
+// sum of entries of A.  Column j of A is held as a 25-bit fixed-point number under a
+// power-of-two scale s_j with max_i |A_ij| <= (127/128) s_j:
+//   A_ij = s_j (2^-7 q0 + 2^-13 q1 + 2^-19 q2 + 2^-25 q3) + e,   |e| <= 2^-26 s_j,
+// with int8 pieces q0 in [-127, 127] and q1, q2, q3 in [-32, 32], each the rounding of what
+// the previous ones leave.  The i8 MFMA multiplies and accumulates in int32 exactly; two
+// accumulators take
+//   accA = sum_i (64 q0 + q1) z_i   and   accB = sum_i (64 q2 + q3) z_i,
+// the factor 64 entering through a second sign operand 64 z (z's bytes shifted in registers),
+// and v_j = s_j (2^-13 accA + 2^-25 accB) is formed in FP64 and rounded once to FP32.
+// |accA| <= 8160 m < 2^31 for m < 2^18.  Integer sums do not depend on their order, so every
+// tiling (solo, fused) gives the same bits; the only rounding left against the FP32 master
+// is e, a 2^-26 s_j per entry, where an FP32 accumulation of the same sum rounds at
+// 2^-24 |partial sum| per addition.
+//
+// int8 operands are K-blocked by 64: element (r, k) of an operand with R rows sits at byte
+// ((k / 64) * R + r) * 64 + k % 64, so one 64-deep K stage of a row range is one contiguous
+// run of 64-byte rows (the 16-bit operands' layout, byte for byte).
+typedef int i4 __attribute__((ext_vector_type(4)));
+constexpr int kFixBK = 64;
+__device__ __forceinline__ long fix8_index(int r, int k, int R) { return ((long)(k >> 6) * R + r) * 64 + (k & 63); }
+
+// One workgroup per column j of A (column major, mp rows, zero padded): s_j, then the four
+// pieces of every entry into q (piece p at q + p * mp * np, K-blocked with np rows).
+__global__ __launch_bounds__(256) void to_fix8_kernel(const float* __restrict__ A, int mp, int np,
+                                                      int8_t* __restrict__ q, float* __restrict__ qscale) {
+    __shared__ double red[4];
+    const int j = blockIdx.x;
+    const float* col = A + (long)j * mp;
+    float mx = 0.f;
+    for (int i = threadIdx.x; i < mp; i += 256) mx = fmaxf(mx, fabsf(col[i]));
+    mx = block_max<256>(mx, red);
+    double sc = 1.0;
+    if (mx > 0.f) {
+        int e;
+        frexp((double)mx * (128.0 / 127.0), &e);  // 2^e > mx * 128/127
+        sc = ldexp(1.0, e);
+    }
+    if (threadIdx.x == 0) qscale[j] = (float)sc;
+    const double inv = 1.0 / sc;  // a power of two: every step below is exact in FP64
+    const long piece = (long)mp * np;
+    for (int i = threadIdx.x; i < mp; i += 256) {
+        double u = (double)col[i] * inv * 128.0;
+        const long o = fix8_index(j, i, np);
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const double r = rint(u);
+            q[p * piece + o] = (int8_t)(int)r;
+            u = (u - r) * 64.0;
+        }
+    }
+}
+hipError_t launch_to_fix8(const float* A, int mp, int np, int8_t* q, float* qscale, hipStream_t st) {
+    if (mp <= 0 || np <= 0 || mp % kFixBK) return hipErrorInvalidValue;
+    to_fix8_kernel<<<np, 256, 0, st>>>(A, mp, np, q, qscale);
+    return hipGetLastError();
+}
+
+// LDS of one 128x128 fixed-point tile: the four pieces and the sign operand, 128 rows of 64
+// bytes each, the 16-byte chunks of row r XOR-swizzled by (r >> 1) & 3.
+constexpr int kFixPart = kGemmBM * kFixBK;
+constexpr int kFixLds = 5 * kFixPart;
+__device__ __forceinline__ int fix8_lds(int r, int chunk) { return r * kFixBK + ((chunk ^ ((r >> 1) & 3)) << 4); }
+
+// V[r + c*ldc] = s_r (2^-13 accA + 2^-25 accB) over the 128x128 tile at (m0, n0).  P: the
+// pieces (stride `piece` bytes, K-blocked with ldp rows), Q: int8 signs (K-blocked with ldq
+// rows).  256 threads as 2x2 waves of 64x64; one 16x16x64 MFMA covers a whole K stage of a
+// 16x16 block, lane l supplying row l%16 and the 16 bytes of chunk l/16 of each operand (the
+// same chunk of both operands, so the products pair equal k).  Each stage is loaded between
+// two barriers and then computed, so no load registers stay live beside the 128 accumulator
+// registers; two workgroups per CU overlap one's loads with the other's MFMAs.
+__device__ __forceinline__ void gemm_fix8_tile(const int8_t* __restrict__ P, long piece,
+                                               const float* __restrict__ qscale, long ldp,
+                                               const int8_t* __restrict__ Q, long ldq, float* __restrict__ C,
+                                               long ldc, long m0, long n0, int K, uint8_t* smem) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // loader: a 128-row stage of one part is 512 chunks of 16 bytes, 2 per thread
+    const int lrow = tid >> 2, lslot = tid & 3;
+    const long pst = ldp * kFixBK, qst = ldq * kFixBK;  // bytes per K stage
+    const int8_t* pg = P + (m0 + lrow) * kFixBK + lslot * 16;
+    const int8_t* qg = Q + (n0 + lrow) * kFixBK + lslot * 16;
+    i4 accA[4][4], accB[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            accA[i][j] = (i4){0, 0, 0, 0};
+            accB[i][j] = (i4){0, 0, 0, 0};
+        }
+    const int nk = K / kFixBK;
+    const int fc = lane >> 4;
+    for (int ks = 0; ks < nk; ks++) {
+        __syncthreads();
+        {
+            u4 pr[4][2], qr[2];
+#pragma unroll
+            for (int p = 0; p < 4; p++)
+#pragma unroll
+                for (int i = 0; i < 2; i++) pr[p][i] = *(const u4*)(pg + p * piece + ks * pst + 64 * kFixBK * i);
+#pragma unroll
+            for (int i = 0; i < 2; i++) qr[i] = *(const u4*)(qg + ks * qst + 64 * kFixBK * i);
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int r = lrow + 64 * i;
+#pragma unroll
+                for (int p = 0; p < 4; p++) *(u4*)(smem + p * kFixPart + fix8_lds(r, lslot)) = pr[p][i];
+                *(u4*)(smem + 4 * kFixPart + fix8_lds(r, lslot)) = qr[i];
+            }
+        }
+        __syncthreads();
+        i4 b[4], b64[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int r = wn * 64 + j * 16 + (lane & 15);
+            const u4 v = *(const u4*)(smem + 4 * kFixPart + fix8_lds(r, fc));
+            // 64 z per byte: bits 0-1 of each byte (z in {-1, 0, 1}) moved to bits 6-7
+            const u4 v64 = (v << 6u) & 0xC0C0C0C0u;
+            b[j] = __builtin_bit_cast(i4, v);
+            b64[j] = __builtin_bit_cast(i4, v64);
+        }
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            i4 a[4];
+            __builtin_amdgcn_sched_barrier(0);  // one piece's fragments at a time
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = wm * 64 + i * 16 + (lane & 15);
+                a[i] = __builtin_bit_cast(i4, *(const u4*)(smem + p * kFixPart + fix8_lds(r, fc)));
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (p == 0) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b64[j], accA[i][j], 0, 0, 0);
+                    else if (p == 1) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], accA[i][j], 0, 0, 0);
+                    else if (p == 2) accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b64[j], accB[i][j], 0, 0, 0);
+                    else accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], accB[i][j], 0, 0, 0);
+                }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const long row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+        const f4 sc = *(const f4*)(qscale + row);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+            f4 o;
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                o[e] = (float)((double)sc[e] * ((double)accA[i][j][e] * 0x1p-13 + (double)accB[i][j][e] * 0x1p-25));
+            *(f4*)(C + row + col * ldc) = o;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void gemm_tn_fix8_kernel(const int8_t* __restrict__ P, long piece,
+                                                           const float* __restrict__ qscale, long ldp,
+                                                           const int8_t* __restrict__ Q, long ldq,
+                                                           float* __restrict__ C, long ldc, int tiles_m, int tiles_n,
+                                                           int K, const int* __restrict__ done) {
+    if (done && *done) return;
+    const int t = xcd_remap(blockIdx.x, tiles_m * tiles_n);
+    const int tn = t % tiles_n, tm = t / tiles_n;
+    __shared__ __attribute__((aligned(16))) uint8_t smem[kFixLds];
+    gemm_fix8_tile(P, piece, qscale, ldp, Q, ldq, C, ldc, (long)tm * kGemmBM, (long)tn * kGemmBN, K, smem);
+}
+hipError_t launch_gemm_fix8(const int8_t* P, const float* qscale, long ldp, const int8_t* Q, long ldq, float* C,
+                            long ldc, int M, int N, int K, const int* done, hipStream_t st) {
+    if (!P || !qscale || !Q || M % kGemmBM || N % kGemmBN || K % kFixBK || K <= 0 || K >= (1 << 18) || ldp < M ||
+        ldq < N)
+        return hipErrorInvalidValue;
+    const int tiles_m = M / kGemmBM, tiles_n = N / kGemmBN;
+    gemm_tn_fix8_kernel<<<tiles_m * tiles_n, 256, 0, st>>>(P, ldp * (long)K, qscale, ldp, Q, ldq, C, ldc, tiles_m,
+                                                           tiles_n, K, done);
     return hipGetLastError();
 }
 
@@ -828,7 +952,8 @@ __device__ __forceinline__ bool is_constrained(int f) { return f < 4; }
 __device__ __forceinline__ bool is_l1_variance(int f) { return f == 1 || f == 3 || f == 5 || f == 7; }
 __device__ __forceinline__ float sgnf(float v) { return (float)((v > 0.f) - (v < 0.f)); }
 
-// Element i of column j: FP32 column major (ld), 16-bit forms K-blocked over op.bp columns.
+// Element i of column j: FP32 column major (ld), 16-bit forms K-blocked over op.bp columns,
+// precision 4: the int8 sign operand of the fixed-point product (K-blocked by 64).
 __device__ __forceinline__ void write_operand(Operand op, int precision, long ld, int i, int j, float v) {
     op.f32[(long)j * ld + i] = v;
     if (precision == 1) {
@@ -838,8 +963,8 @@ __device__ __forceinline__ void write_operand(Operand op, int precision, long ld
         op.lo[b] = f2bf(v - bf2f(h));
     } else if (precision == 2) {
         op.hi[blk_index(j, i, op.bp)] = f2h(v);
-    } else if (precision == 3) {
-        op.hi[blk_index(j, i, op.bp)] = f2bf(v);  // sign vectors: exact
+    } else if (precision == 4) {
+        ((int8_t*)op.hi)[fix8_index(j, i, op.bp)] = (int8_t)(int)v;  // sign vectors of the fixed-point product
     }
 }
 
@@ -1472,15 +1597,9 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
     const int ksplit = WHICH == 0 ? J.np / J.splits : J.mp;
     const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN, k0 = (long)sp * ksplit;
     // one LDS block serves whichever tile form the job's product takes
-    constexpr int kLds = cmax(cmax(kF32TileLds, tile16_lds(PREC)), WHICH == 1 ? tile16_lds(3) : 0);
+    if (WHICH == 1 && J.p2_prec == 4) return;  // fused_fix8_kernel's tile
+    constexpr int kLds = cmax(kF32TileLds, tile16_lds(PREC));
     __shared__ __attribute__((aligned(16))) uint8_t smem[kLds];
-    if constexpr (WHICH == 1) {
-        if (J.p2_prec == 3) {
-            gemm16_tile<3>(J.p2_hi, J.p2_lo, J.p2_lo2, J.np, J.zop.hi, nullptr, J.bp, J.V, J.np, 0, 0, m0, n0, k0,
-                           ksplit, smem);
-            return;
-        }
-    }
     if constexpr (PREC == 0) {
         if constexpr (WHICH == 0)
             gemm_f32_tile(J.p1_32, J.np, J.xop.f32, J.np, J.Zpart, J.mp, J.zsplit, sp, m0, n0, k0, ksplit, smem);
@@ -1488,12 +1607,25 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
             gemm_f32_tile(J.p2_32, J.mp, J.zop.f32, J.mp, J.V, J.np, 0, 0, m0, n0, k0, ksplit, smem);
     } else {
         if constexpr (WHICH == 0)
-            gemm16_tile<PREC>(J.p1_hi, J.p1_lo, nullptr, J.mp, J.xop.hi, J.xop.lo, J.bp, J.Zpart, J.mp, J.zsplit, sp,
+            gemm16_tile<PREC>(J.p1_hi, J.p1_lo, J.mp, J.xop.hi, J.xop.lo, J.bp, J.Zpart, J.mp, J.zsplit, sp,
                               m0, n0, k0, ksplit, smem);
         else
-            gemm16_tile<PREC>(J.p2_hi, J.p2_lo, nullptr, J.np, J.zop.hi, J.zop.lo, J.bp, J.V, J.np, 0, 0, m0, n0, k0,
+            gemm16_tile<PREC>(J.p2_hi, J.p2_lo, J.np, J.zop.hi, J.zop.lo, J.bp, J.V, J.np, 0, 0, m0, n0, k0,
                               ksplit, smem);
     }
+}
+
+// The A^T*Z tiles of the jobs whose second product is the fixed-point one (p2_prec 4), over
+// the same tile list as fused_gemm_kernel<PREC, 1>, which returns for them; a launch of its
+// own keeps the fixed-point tile's 180 registers out of the other tiles' occupancy.
+__global__ __launch_bounds__(256, 2) void fused_fix8_kernel(const FusedJob* __restrict__ jobs, int njobs, int total) {
+    const int t = xcd_window_remap(blockIdx.x, total);
+    const FusedJob& J = jobs[find_job<1>(jobs, njobs, t)];
+    if (*J.sp.done || J.p2_prec != 4) return;
+    const int lt = t - J.t2, tiles_n = J.bp / kGemmBN;
+    __shared__ __attribute__((aligned(16))) uint8_t smem[kFixLds];
+    gemm_fix8_tile(J.p2_q, (long)J.np * J.mp, J.p2_qscale, J.np, (const int8_t*)J.zop.hi, J.bp, J.V, J.np,
+                   (long)(lt / tiles_n) * kGemmBM, (long)(lt % tiles_n) * kGemmBN, J.mp, smem);
 }
 
 // sparse A*X items: (row block, SP) of each job, row block fastest
@@ -1522,23 +1654,24 @@ __global__ __launch_bounds__(256) void fused_converge_kernel(const FusedJob* __r
 }
 
 template <int PREC>
-hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int items3, int* all_done,
-                        hipStream_t st) {
+hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int items3, int any_fix8,
+                        int* all_done, hipStream_t st) {
     fused_gemm_kernel<PREC, 0><<<tiles1, 256, 0, st>>>(jobs, njobs, tiles1);
     fused_sparse_ax_kernel<<<items3, 256, 0, st>>>(jobs, njobs);
     fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
     fused_gemm_kernel<PREC, 1><<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
+    if (any_fix8) fused_fix8_kernel<<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
     fused_step_kernel<<<cols, kSelThreads, 0, st>>>(jobs, njobs);
     fused_converge_kernel<<<njobs, 256, 0, st>>>(jobs, all_done);
     return hipGetLastError();
 }
 
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
-                                  int items3, int* all_done, hipStream_t st) {
+                                  int items3, int any_fix8, int* all_done, hipStream_t st) {
     if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1 || items3 < 1) return hipErrorInvalidValue;
-    if (precision == 0) return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, items3, all_done, st);
-    if (precision == 1) return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, items3, all_done, st);
-    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, items3, all_done, st);
+    if (precision == 0) return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, items3, any_fix8, all_done, st);
+    if (precision == 1) return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, items3, any_fix8, all_done, st);
+    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, items3, any_fix8, all_done, st);
 }
 
 // ---------------------------------------------------------------- OTF slot refill

@@ -23,7 +23,7 @@ SPCA_OK, SPCA_EINVAL, SPCA_ENOMEM, SPCA_EBUDGET, SPCA_EDEVICE, SPCA_ECANCELLED, 
 STATUS_NAMES = {0: "SPCA_OK", 1: "SPCA_EINVAL", 2: "SPCA_ENOMEM", 3: "SPCA_EBUDGET", 4: "SPCA_EDEVICE",
                 5: "SPCA_ECANCELLED", 6: "SPCA_ENOTFOUND", 7: "SPCA_EAGAIN"}
 
-STORE_FP32, STORE_BF16X3, STORE_FP16, STORE_SPLIT3 = 1, 2, 4, 8
+STORE_FP32, STORE_BF16X3, STORE_FP16, STORE_FIX8 = 1, 2, 4, 8
 PREC_FP32, PREC_BF16X3, PREC_FP16 = 0, 1, 2
 TERM_BATCH_MAX, TERM_PER_SP = 0, 1
 START_EXPLICIT, START_REFERENCE, START_REFERENCE_F64, START_GAUSSIAN = 0, 1, 2, 3

@@ -83,10 +83,10 @@ typedef enum spca_start {
 #define SPCA_STORE_FP32 (1u << 0)
 #define SPCA_STORE_BF16X3 (1u << 1)
 #define SPCA_STORE_FP16 (1u << 2)
-/* three bf16 pieces of A's column-major orientation that sum to A exactly (each the bf16
-   rounding of what the previous ones leave): the operand of A^T * sgn(z), the L1-variance
-   formulations' second product, on bf16 MFMA with every product exact (exact_sign_products) */
-#define SPCA_STORE_SPLIT3 (1u << 3)
+/* A in 25-bit fixed point per column (a power-of-two scale and four int8 pieces): the operand
+   of A^T * sgn(z), the L1-variance formulations' second product, on the int8 MFMA with exact
+   integer sums (exact_sign_products) */
+#define SPCA_STORE_FIX8 (1u << 3)
 
 /* ---- settings / statistics ---------------------------------------------- */
 /* mirrors OptimizationSettings (c++/src/class/optimization_settings.h:71-123) */
@@ -113,10 +113,11 @@ typedef struct spca_settings {
                                  (bf16x3) or n/384 (fp16) of them; 0: always the dense product
                                  of `precision` */
     int exact_sign_products;  /* 1 (default): for the L1-variance formulations (z = sgn(A x) in
-                                 {-1, 0, +1}), FP32 and bf16x3 solves compute A^T z from the
-                                 exact three-piece bf16 split of A (SPCA_STORE_SPLIT3, added on
-                                 first use): every product exact, FP32 accumulation, at bf16
-                                 MFMA rate; 0: the product of `precision` */
+                                 {-1, 0, +1}), FP32 and bf16x3 solves compute A^T z from A in
+                                 25-bit fixed point (SPCA_STORE_FIX8, added on first use) on the
+                                 int8 MFMA: exact integer sums, one rounding of each v_j to FP32,
+                                 |error| <= 2^-26 * max_i |A_ij| * sum_i |z_i| (+ that rounding);
+                                 0: the product of `precision` */
 } spca_settings;
 
 /* mirrors OptimizationStatistics (c++/src/class/optimization_statistics.h:27-46) */

@@ -59,7 +59,7 @@ def lib():
             getattr(L, f"oracle_solve_{sfx}").restype = real
             getattr(L, f"oracle_solve_{sfx}").argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                                           ctypes.POINTER(Settings), P, P,
-                                                          ctypes.POINTER(Stats), P, P, P, P]
+                                                          ctypes.POINTER(Stats), P, P, P, P, P]
             getattr(L, f"oracle_hard_threshold_{sfx}").restype = real
             getattr(L, f"oracle_hard_threshold_{sfx}").argtypes = [P, ctypes.c_uint, ctypes.c_uint,
                                                                    ctypes.c_int, P]
@@ -137,7 +137,7 @@ def solve(A, formulation, s=10, gamma=0.0, tol=0.01, max_iter=20, total_sp=64, b
           termination=TERM_BATCH_MAX, hard_rule=HARD_SORT, X0=None, threads=8, gemm_backend=0,
           want_X=False, lambda_rule=LAMBDA_CPU):
     """Solve with the oracle.  A: (m x n) array (any order; copied to column-major).
-    Returns dict(fval, x, it, best_sp, sp_val, sp_iters, sp_nnz, X (optional), loop_s, wall_s)."""
+    Returns dict(fval, x, it, best_sp, sp_val, sp_iters, sp_nnz, sp_err, X (optional), loop_s, wall_s)."""
     A = np.asfortranarray(A)
     dtype = A.dtype
     m, n = A.shape
@@ -150,6 +150,7 @@ def solve(A, formulation, s=10, gamma=0.0, tol=0.01, max_iter=20, total_sp=64, b
     sp_val = np.zeros(tot, dtype=dtype)
     sp_iters = np.zeros(tot, dtype=np.int32)
     sp_nnz = np.zeros(tot, dtype=np.int32)
+    sp_err = np.zeros(tot, dtype=dtype)
     X = np.zeros((tot, n), dtype=dtype) if want_X else None
     if X0 is not None:
         X0 = np.ascontiguousarray(np.asarray(X0, dtype=dtype).T)  # (B x n) rows = columns
@@ -158,9 +159,9 @@ def solve(A, formulation, s=10, gamma=0.0, tol=0.01, max_iter=20, total_sp=64, b
     fval = getattr(lib(), f"oracle_solve_{_sfx(dtype)}")(_ptr(A), A.strides[1] // A.itemsize, m, n,
                                                        ctypes.byref(st), _ptr(X0), _ptr(x),
                                                        ctypes.byref(stats), _ptr(sp_val), _ptr(sp_iters),
-                                                       _ptr(sp_nnz), _ptr(X))
+                                                       _ptr(sp_nnz), _ptr(sp_err), _ptr(X))
     out = dict(fval=float(fval), x=x, it=stats.it, best_sp=stats.best_sp, sp_val=sp_val, sp_iters=sp_iters,
-               sp_nnz=sp_nnz, loop_s=stats.loop_s, wall_s=stats.wall_s, threads=stats.threads_used,
+               sp_nnz=sp_nnz, sp_err=sp_err, loop_s=stats.loop_s, wall_s=stats.wall_s, threads=stats.threads_used,
                total_sp=stats.total_sp, constraint=stats.constraint)
     if want_X:
         out["X"] = X.T

@@ -127,16 +127,17 @@ float  oracle_l1_pen_threshold_f32(float* x, unsigned n, double gamma);
  *   sp_val   final reference-native value f
  *   sp_iters iterations this SP performed
  *   sp_nnz   nonzeros of its final iterate
+ *   sp_err   its last relative change |f - f_prev| / f_prev (reference termination_criteria.h:31-36)
  *   X_final  n x total_sp final iterates (unnormalised as the reference holds them)
  * x_out (length n) receives the best iterate normalised to unit 2-norm.
  * Returns the best value (as the reference's denseDataSolver does).
  */
 double oracle_solve_f64(const double* A, int ldA, int m, int n, const oracle_settings* st,
                         const double* X0, double* x_out, oracle_stats* stats,
-                        double* sp_val, int* sp_iters, int* sp_nnz, double* X_final);
+                        double* sp_val, int* sp_iters, int* sp_nnz, double* sp_err, double* X_final);
 float oracle_solve_f32(const float* A, int ldA, int m, int n, const oracle_settings* st,
                        const float* X0, float* x_out, oracle_stats* stats,
-                       float* sp_val, int* sp_iters, int* sp_nnz, float* X_final);
+                       float* sp_val, int* sp_iters, int* sp_nnz, float* sp_err, float* X_final);
 
 /* Explained variance of a unit vector x in double: |A x|_2^2 (l2=1) or |A x|_1. */
 double oracle_explained_variance_f64(const double* A, int ldA, int m, int n, const double* x, int l2);

@@ -431,7 +431,7 @@ static int SFX(terminated)(R err, int it, double tol) { return it > 0 && err < t
 /* ---- drivers --------------------------------------------------------------- */
 
 R SFX(oracle_solve)(const R* A, int ldA, int m, int n, const oracle_settings* st_in, const R* X0,
-                    R* x_out, oracle_stats* stats, R* sp_val, int* sp_iters, int* sp_nnz, R* X_final) {
+                    R* x_out, oracle_stats* stats, R* sp_val, int* sp_iters, int* sp_nnz, R* sp_err, R* X_final) {
     oracle_settings st = *st_in;
     double t_wall0 = oracle_now();
     if (st.gemm_backend == 1) oracle_blas_threads(st.threads);
@@ -488,6 +488,7 @@ R SFX(oracle_solve)(const R* A, int ldA, int m, int n, const oracle_settings* st
                     if (sp_val) sp_val[order[i]] = vals[i].val;
                     if (sp_iters) sp_iters[order[i]] = cur_it[i];
                     if (sp_nnz) sp_nnz[order[i]] = (int)SFX(nnz)(V + (long)n * i, n);
+                    if (sp_err) sp_err[order[i]] = vals[i].err;
                     if (X_final) memcpy(X_final + (long)n * order[i], V + (long)n * i, sizeof(R) * n);
                     if (generated < total) {
                         vals[i].val = 0; vals[i].tmp = 0; vals[i].err = 0;
@@ -546,6 +547,7 @@ R SFX(oracle_solve)(const R* A, int ldA, int m, int n, const oracle_settings* st
                 if (sp_val) sp_val[shift + i] = vals[i].val;
                 if (sp_iters) sp_iters[shift + i] = cur_it[i];
                 if (sp_nnz) sp_nnz[shift + i] = (int)SFX(nnz)(V + (long)n * i, n);
+                if (sp_err) sp_err[shift + i] = vals[i].err;
             }
             if (X_final) memcpy(X_final + (long)n * shift, V, sizeof(R) * (size_t)n * batch);
             if (the_best < best_value) {

@@ -547,14 +547,14 @@ def test_sparse_flags_per_group_of_256(ctx, prec):
 
 @pytest.mark.parametrize("B", [64, 256])
 @pytest.mark.parametrize("f", [1, 3, 5, 7])
-def test_exact_split_sign_products_bitwise(ctx, f, B):
-    """A^T sgn(z) from the three-piece bf16 split of A is exact: on a matrix whose FP32 sums
-    are all exact (entries k * 2^-19 with |k| < 2^19, 32 rows, so every partial sum is a
-    multiple of 2^-19 below 2^5), the split product and the FP32 product agree bitwise, and
-    so do whole L1-variance trajectories.  The entries carry up to 20 significant bits, so all
-    three pieces are non-zero.  B = 64 runs the 128x128-tile kernel, B = 256 the pipelined
-    256x256-tile kernel.  Penalties keep about 5% of the coordinates ((A^T sgn z)_i has
-    standard deviation sqrt(32/3))."""
+def test_fixed_point_sign_products_bitwise(ctx, f, B):
+    """A^T sgn(z) from the 25-bit fixed-point form of A is exact when A is: on a matrix whose
+    entries are k * 2^-19 with |k| < 2^19 (so each column's scale is 1 and every entry is a
+    whole number of 2^-25 units) and whose FP32 sums are all exact (32 rows: every partial sum
+    is a multiple of 2^-19 below 2^5), the fixed-point product and the FP32 product agree
+    bitwise, and so do whole L1-variance trajectories.  The entries carry up to 20 significant
+    bits, so all four pieces are used.  Penalties keep about 5% of the coordinates
+    ((A^T sgn z)_i has standard deviation sqrt(32/3))."""
     rng = np.random.default_rng(30 + f)
     m, n = 32, 1024
     A = (rng.integers(-2**19, 2**19, size=(m, n)) * 2.0**-19).astype(np.float32)
@@ -576,17 +576,17 @@ def test_exact_split_sign_products_bitwise(ctx, f, B):
         assert nnz.min() > 0 and np.all(np.isfinite(out[1][0]))
         np.testing.assert_array_equal(out[1][0], out[0][0])
         np.testing.assert_array_equal(out[1][1].values, out[0][1].values)
-        assert nbytes[1] > nbytes[0]  # the split pieces were added on first use
+        assert nbytes[1] > nbytes[0]  # the fixed-point pieces were added on first use
     finally:
         M.release()
 
 
 @pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3])
-def test_exact_split_sign_products_against_fp64(small, small_mat, prec):
-    """L1 variance with the L1 constraint, 40 iterations: the exact split and the precision's
-    own A^T*Z product both stay within the precision's EV bound of the FP64 oracle, and the
-    exact split's A^T*Z error (seen through the final iterates) is no larger than that of the
-    product it replaces."""
+def test_fixed_point_sign_products_against_fp64(small, small_mat, prec):
+    """L1 variance with the L1 constraint, 40 iterations: the fixed-point product and the
+    precision's own A^T*Z product both stay within the precision's EV bound of the FP64
+    oracle, and the fixed-point product's error (seen through the final iterates) is no
+    larger than that of the product it replaces."""
     A, X0, _ = small
     f, B, it = 3, X0.shape[1], 40
     o = O.solve(A.astype(np.float64), f, s=10, gamma=0.0, tol=0.0, max_iter=it, total_sp=B, X0=X0.astype(np.float64),

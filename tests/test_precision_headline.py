@@ -1,19 +1,27 @@
-"""bf16x3 against FP64 at the headline shape.  Needs an MI355X; slow (the FP64 oracle solves
-on the host, about half a minute).
+"""FP32 and bf16x3 against FP64 at the headline shape, all 8 formulations.  Needs an MI355X;
+slow (the FP64 oracle solves on the host, about 10-20 s per formulation on 16 threads).
 
-4096 x 65536 spiked model (bench.make_problem), s = 256, tol = 1e-6, at most 200
-iterations, the two formulations of the precision workload (L2 variance / L0 constraint and
-L1 variance / L1 constraint), on the first 128 of the bench's unit Gaussian starting points.
-Per starting point: the explained variance of the GPU's final iterate (FP64, on the device)
-against that of the FP64 oracle's final iterate, and whether the supports are identical.
+4096 x 65536 spiked model (bench.make_problem), s = 256, gamma per penalised formulation as
+the bench calibrates it, tol = 1e-6, at most 200 iterations, on the first 128 of the bench's
+unit Gaussian starting points.  Per starting point: the explained variance of the GPU's final
+iterate (FP64, on the device) against that of the FP64 oracle's final iterate, and whether the
+supports are identical.  The explained variance is compared over the starting points the
+oracle has converged (its last relative change below tol); the L1 penalty with L2 variance
+leaves some points still moving after 200 iterations, where the two trajectories are compared
+mid-way and differ by up to ~1e-4 without a change of support.
 
-The solve is nonconvex: a starting point whose trajectory passes near a tie (two candidates
-for the s-th place within the products' rounding, or for L1 variance an entry of z = A x
-within rounding of zero) can settle on another fixed point.  FP32 does so for 1 of 128
-points here, bf16x3 (~2^-17 relative products) for 2-4 of 128.  Bars: FP32 EV within 1e-6 and
-identical supports on >= 99% of points; bf16x3 EV within 1e-5 on >= 97% and identical
-supports on >= 96% (measured 99.2% / 98.4% for L2 variance with the L0 constraint, 97.7% /
-96.9% for L1 variance with the L1 constraint).
+The solve is nonconvex: a starting point whose trajectory passes near a tie can settle on
+another fixed point.  Two kinds of tie occur:
+  * two candidates for the s-th place (or for the gamma / lambda threshold) within the
+    rounding of the product that computed them;
+  * for L1 variance, an entry of z = A x within rounding of zero, whose sign flips.  One flip
+    moves v = A^T sgn(z) by 2 |A_i| (about 1.5% of |v| here), and the trajectory can leave
+    for another fixed point.  Such entries arise at about 1e-6 of the entries' scale, so
+    A^T sgn(z) runs in exact fixed point: an FP32 accumulation of it put ~4e-7 of error into
+    x, enough to flip such an entry at the next iteration.
+Measured (identical supports): FP32 100% for six formulations and 127/128 for the L0
+constraint / L2 variance and the L1 penalty / L2 variance; bf16x3 126/128 to 128/128.  The
+bars allow two (FP32) and three (bf16x3) starting points below 100%.
 """
 import os
 
@@ -26,41 +34,49 @@ import spca_mi355x as S
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
 NSP = 128
+# (EV within tol on at least this share, identical supports on at least this share)
+BARS = {S.PREC_FP32: (1e-6, 0.98, 0.98), S.PREC_BF16X3: (1e-5, 0.97, 0.97)}
 
 
 @pytest.fixture(scope="module")
 def headline(ctx):
     import bench
-    A, X0, _ = bench.make_problem()
+    A, X0, loadings = bench.make_problem()
+    gam = bench.calibrate_gamma(A, loadings[0])
     M = ctx.upload(A, storage=S.STORE_FP32 | S.STORE_BF16X3)
-    yield A, np.asfortranarray(X0[:, :NSP]), M
+    yield A, A.astype(np.float64), np.asfortranarray(X0[:, :NSP]), gam, M
     M.release()
 
 
-@pytest.mark.parametrize("f", [0, 3])
+@pytest.mark.parametrize("f", range(8))
 def test_headline_precision_against_fp64(headline, f):
-    A, X0, M = headline
+    A, A64, X0, gam, M = headline
     threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
-    A64 = A.astype(np.float64)
-    o = O.solve(A64, f, s=256, tol=1e-6, max_iter=200, total_sp=NSP, batch=NSP, X0=X0.astype(np.float64),
-                threads=threads, gemm_backend=1, want_X=True, lambda_rule=O.LAMBDA_GPU)
+    gamma = gam.get(f, 0.0)
+    o = O.solve(A64, f, s=256, gamma=gamma, tol=1e-6, max_iter=200, total_sp=NSP, batch=NSP,
+                X0=X0.astype(np.float64), threads=threads, gemm_backend=1, want_X=True, lambda_rule=O.LAMBDA_GPU)
     Xo = o["X"]
-    Zo = A64 @ (Xo / np.linalg.norm(Xo, axis=0))
-    ev_o = np.sum(Zo * Zo, axis=0) if f == 0 else np.sum(np.abs(Zo), axis=0)
+    nrm = np.linalg.norm(Xo, axis=0)
+    Zo = A64 @ (Xo / np.where(nrm > 0, nrm, 1.0))
+    l2 = f in (0, 2, 4, 6)
+    ev_o = np.sum(Zo * Zo, axis=0) if l2 else np.sum(np.abs(Zo), axis=0)
+    conv = o["sp_err"] < 1e-6
     res = {}
-    for prec, ev_tol in ((S.PREC_BF16X3, 1e-5), (S.PREC_FP32, 1e-6)):
-        st = S.OptimizationSettings(formulation=f, constraintParameter=256, tolerance=1e-6, maximumIterations=200,
-                                    totalStartingPoints=NSP, batchSize=NSP, precision=prec)
+    for prec in (S.PREC_FP32, S.PREC_BF16X3):
+        st = S.OptimizationSettings(formulation=f, constraintParameter=256, penaltyParameter=gamma, tolerance=1e-6,
+                                    maximumIterations=200, totalStartingPoints=NSP, batchSize=NSP, precision=prec)
         stats = S.OptimizationStatistics()
         _, Xg = M.solve(st, X0=X0, stats=stats, want_X=True)
-        rel = np.abs(stats.sp_ev_fp64 - ev_o) / ev_o
+        rel_all = np.abs(stats.sp_ev_fp64 - ev_o) / ev_o
+        rel = rel_all[conv]
         same = np.all((Xg != 0) == (Xo != 0), axis=0)
-        res[prec] = (rel, same, ev_tol)
-        print(f"f={f} prec={prec}: iterations gpu {stats.it} oracle {o['it']}; EV rel max {rel.max():.2e} "
-              f"median {np.median(rel):.2e}; within tol {np.mean(rel <= ev_tol):.3f}; identical supports "
+        res[prec] = (rel, same)
+        print(f"\nf={f} prec={prec}: iterations gpu {stats.it} oracle {o['it']}; median nnz {np.median(o['sp_nnz'])}; "
+              f"oracle converged {conv.sum()}/{NSP}; EV rel (converged) max {rel.max():.2e} median "
+              f"{np.median(rel):.2e}, within {BARS[prec][0]:.0e}: {np.mean(rel <= BARS[prec][0]):.3f}, within 1e-5: "
+              f"{np.mean(rel <= 1e-5):.3f}; EV rel (all) max {rel_all.max():.2e}; identical supports "
               f"{same.mean():.3f}; off-support SPs {np.nonzero(~same)[0].tolist()}", flush=True)
-    bars = {S.PREC_FP32: (0.99, 0.99), S.PREC_BF16X3: (0.97, 0.96)}
-    for prec, (rel, same, ev_tol) in res.items():
-        ev_bar, supp_bar = bars[prec]
+    for prec, (rel, same) in res.items():
+        ev_tol, ev_bar, supp_bar = BARS[prec]
         assert np.mean(rel <= ev_tol) >= ev_bar, (prec, np.sort(rel)[-4:])
         assert same.mean() >= supp_bar, (prec, np.nonzero(~same)[0])
