@@ -45,6 +45,7 @@ struct Matrix {
     uint16_t* h_rm = nullptr;
     int8_t* q_cm = nullptr;      // fixed-point pieces of A for the exact A^T * sgn(z) (4 x mp x np bytes)
     float* qscale = nullptr;     // their per-column power-of-two scales (np)
+    float* rowmax = nullptr;     // max_k |A_ik| per row (mp): the sign refinement's error bound
     uint64_t bytes = 0;
     int refs = 0;
 };
@@ -253,7 +254,7 @@ bool valid_formulation(int f) { return f >= 0 && f <= 7; }
 
 void free_matrix(spca_ctx ctx, Matrix* M) {
     (void)hipStreamSynchronize(ctx->stream);
-    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm, M->q_cm, M->qscale};
+    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm, M->q_cm, M->qscale, M->rowmax};
     for (void* p : ptrs) raw_free(ctx, p);
     delete M;
 }
@@ -294,6 +295,8 @@ int add_storage(spca_ctx ctx, Matrix* M, unsigned storage) {
         if ((rc = dev_alloc(ctx, &M->q_cm, 4 * cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->qscale, (size_t)M->np, &M->bytes)))
             return rc;
         HIP_TRY(ctx, launch_to_fix8(M->a32_cm, M->mp, M->np, M->q_cm, M->qscale, st));
+        if ((rc = dev_alloc(ctx, &M->rowmax, (size_t)M->mp, &M->bytes))) return rc;
+        HIP_TRY(ctx, launch_row_max(M->a32_cm, M->mp, M->np, M->rowmax, st));
         M->storage |= SPCA_STORE_FIX8;
     }
     if (want & SPCA_STORE_FP16) {
@@ -652,6 +655,17 @@ AtzForm atz_form(const Matrix* M, const spca_settings& st, int bp, int plan_big)
     a.ldq = a.prec ? bp : M->mp;
     return a;
 }
+// Exact signs of z after a dense A*X (L1 variance with exact_sign_products): entries within
+// eps * rowmax_i * |x|_2 of zero are recomputed in FP64.  The bound is about a dozen standard
+// deviations of the product's rounding error: FP32 MFMA accumulation over K/splits terms
+// (~2^-18.5 |z|), bf16x3 products (~2^-17 per term); with rowmax_i >= the rms of the row's
+// terms it flags ~1e-4 (FP32) to ~2e-4 (bf16x3) of the entries of Gaussian data.
+SignRefine sign_refine(const Matrix* M, const spca_settings& st, Workspace& w) {
+    SignRefine r{M->a32_rm, M->rowmax, w.X, M->np, 0.f};
+    if (uses_sign_products(st) && M->a32_rm && M->rowmax) r.eps = st.precision == 0 ? 0x1p-17f : 0x1p-14f;
+    return r;
+}
+
 // V = A^T * Z of one iteration in the solve's A^T*Z form.
 hipError_t launch_atz(const AtzForm& a, const Matrix* M, Workspace& w, int bp, const int* done, hipStream_t s) {
     if (a.prec == 4)
@@ -923,7 +937,8 @@ int tune_plan(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, Works
                     e = launch_gemm_tn(prec, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                        mp, zsplit, mp, bp, np, splits, bi, nullptr, nullptr, s);
                 if (e == hipSuccess)
-                    e = launch_prep_z(w.Zpart, splits, zsplit, mp, mp, zop, st.formulation, prec, st.batch, sp, s);
+                    e = launch_prep_z(w.Zpart, splits, zsplit, mp, mp, zop, st.formulation, prec, st.batch, sp,
+                                      SignRefine{}, s);
                 if (e == hipSuccess)
                     e = launch_gemm_tn(prec, P2_32, P2_hi, M->lo_cm, ld2p, w.Z, w.Zhi, w.Zlo, ld2q, w.V, np,
                                        0, np, bp, mp, 1, bi, nullptr, nullptr, s);
@@ -1042,7 +1057,8 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
     if (st.start == SPCA_START_EXPLICIT && !X0) return fail(ctx, SPCA_EINVAL, "X0 is required for explicit starts");
     if (!x_out) return fail(ctx, SPCA_EINVAL, "x_out is required");
     unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
-    if (uses_sign_products(st)) need |= SPCA_STORE_FIX8;
+    // the fixed-point product, and the FP32 row-major copy for the exact signs of z
+    if (uses_sign_products(st)) need |= SPCA_STORE_FIX8 | SPCA_STORE_FP32;
     int rc = add_storage(ctx, M, need);
     if (rc) return rc;
     *nb = spca_settings_check(&st, (unsigned)M->n);
@@ -1073,6 +1089,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     const int check_every = st.check_every > 0 ? st.check_every : 1;
 
     const AtzForm atz = atz_form(M, st, bp, plan.big);
+    const SignRefine rf = sign_refine(M, st, w);
     const Operand xop = x_operand(w, st.precision);
     const Operand zop = z_operand(w, atz.zprec);
     const float* P1_32 = M->a32_rm;
@@ -1176,7 +1193,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, atz.zprec, batch,
-                                       sp, s));
+                                       sp, rf, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1265,7 +1282,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
             HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, atz.zprec, batch,
-                                       sp, s));
+                                       sp, rf, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1488,6 +1505,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         f.V = r.ws.V;
         f.xop = x_operand(r.ws, precision);
         f.zop = z_operand(r.ws, atz.zprec);
+        f.rf = sign_refine(M, r.st, r.ws);
         f.sp = state_of(r.ws);
         f.sa.formulation = r.st.formulation;
         f.sa.precision = precision;

@@ -88,6 +88,18 @@ struct OtfState {
     float* xbest;      // iterate of the best SP (padded n)
 };
 
+// Exact signs of z = A x for the L1-variance formulations after a dense A*X product: entries
+// with |z_i| <= eps * rowmax_i * |x|_2 (inside the product's rounding of zero) are recomputed
+// in FP64 from the FP32 master.  eps = 0: off.
+struct SignRefine {
+    const float* arm;     // A row major (mp x np, ld np)
+    const float* rowmax;  // max_k |A_ik| per row (mp; zero rows and padding 0)
+    const float* x;       // the iterates the product read (FP32, ld np)
+    int np;
+    float eps;
+};
+constexpr int kRefineMax = 256;  // flagged entries of one SP and iteration (more: none refined)
+
 struct StepArgs {
     int formulation;
     int precision;       // spca_precision
@@ -115,6 +127,7 @@ struct FusedJob {
     Operand xop, zop;
     SpState sp;
     StepArgs sa;
+    SignRefine rf;
     long zsplit;            // mp * bp
     double tol;
     int mp, np, bp, splits;
@@ -165,7 +178,9 @@ hipError_t launch_gemm_fix8(const int8_t* P, const float* qscale, long ldp, cons
 // operand of the second product.
 hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int m, int ldz,
                          Operand zop, int formulation, int precision, int batch, SpState sp,
-                         hipStream_t st);
+                         SignRefine rf, hipStream_t st);
+// rowmax[i] = max_k |A_ik| of A (mp x np column major).
+hipError_t launch_row_max(const float* Acm, int mp, int np, float* out, hipStream_t st);
 
 // Formulation step on V (A^T Z) for each active SP, writes X (+operand), updates state.
 hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st);

@@ -971,42 +971,106 @@ __device__ __forceinline__ void write_operand(Operand op, int precision, long ld
 // One workgroup per SP column.  z = sum of split partials (fixed order).  Constrained:
 // L1 variance f = |z|_1 then z = sgn z; L2 variance f = |z|_2 (z unchanged)
 // (c++/src/gpower/gpower_commons.h:75-101).  Penalised: z = sgn z or z/|z|_2 (:150-166).
+// Exact signs (rf.eps > 0, L1 variance, z from the dense product): an entry with
+// |z_i| <= eps * max_k |A_ik| * |x|_2, inside the product's rounding of zero, is recomputed as
+// an FP64 dot product of row i of the FP32 master with x, so sgn z_i is the one exact
+// arithmetic gives.  The flagged set does not depend on thread order; more than kRefineMax
+// flags (degenerate data) leave the column as the product gave it.
 __device__ __forceinline__ void prep_z_body(int j, const float* __restrict__ Zpart, int splits, long split_stride,
                                             int m, int ldz, Operand zop, int formulation, int precision,
-                                            SpState sp) {
+                                            SpState sp, SignRefine rf) {
     if (*sp.done || !sp.active[j]) return;
-    if (sp.done[kSparseFlags + (j >> 8)]) splits = 1;  // the sparse gather wrote all of z into partial 0
+    const bool gathered = sp.done[kSparseFlags + (j >> 8)] != 0;
+    if (gathered) splits = 1;  // the sparse gather wrote all of z (FP64 sums) into partial 0
     __shared__ double red[4];
+    __shared__ int nflag;
+    __shared__ int flags[kRefineMax];
+    const int tid = threadIdx.x;
     const float* zc = Zpart + (long)j * ldz;
     const bool l1v = is_l1_variance(formulation);
+    const bool refine = l1v && rf.eps > 0.f && !gathered;
+    const float* xj = rf.x + (long)j * rf.np;
+    float thr = 0.f;
+    if (refine) {
+        double xn = 0;
+        for (int k = tid; k < rf.np; k += 256) xn += (double)xj[k] * (double)xj[k];
+        thr = rf.eps * (float)sqrt(block_sum<256>(xn, red));
+        if (tid == 0) nflag = 0;
+        __syncthreads();
+    }
     double acc = 0;
-    for (int i = threadIdx.x; i < m; i += 256) {
+    for (int i = tid; i < m; i += 256) {
         float z = zc[i];
         for (int s = 1; s < splits; s++) z += zc[(long)s * split_stride + i];
         zop.f32[(long)j * ldz + i] = z;
-        acc += l1v ? (double)fabsf(z) : (double)z * (double)z;
+        if (!l1v) acc += (double)z * (double)z;
+        if (refine && fabsf(z) <= thr * rf.rowmax[i] && rf.rowmax[i] > 0.f) {
+            const int slot = atomicAdd(&nflag, 1);
+            if (slot < kRefineMax) flags[slot] = i;
+        }
     }
-    const double tot = block_sum<256>(acc, red);
-    float stat = l1v ? (float)tot : (float)sqrt(tot);
-    if (threadIdx.x == 0) sp.fz[j] = stat;
+    if (refine) {
+        __syncthreads();
+        const int nf = nflag;
+        if (nf <= kRefineMax) {
+            for (int f = 0; f < nf; f++) {
+                const int i = flags[f];
+                const float* ai = rf.arm + (long)i * rf.np;
+                double d = 0;
+                for (int k = tid; k < rf.np; k += 256) d += (double)ai[k] * (double)xj[k];
+                d = block_sum<256>(d, red);
+                if (tid == 0) zop.f32[(long)j * ldz + i] = (float)d;
+            }
+            __syncthreads();
+        }
+    }
     const bool constrained = is_constrained(formulation);
-    const float inv = 1.0f / stat;
-    for (int i = threadIdx.x; i < m; i += 256) {
+    float inv = 0.f;
+    if (!l1v) {
+        const float stat = (float)sqrt(block_sum<256>(acc, red));
+        if (tid == 0) sp.fz[j] = stat;
+        inv = 1.0f / stat;
+    }
+    for (int i = tid; i < m; i += 256) {
         const long idx = (long)j * ldz + i;
         float z = zop.f32[idx];
-        if (l1v) z = sgnf(z);
-        else if (!constrained) z = z * inv;
+        if (l1v) {
+            acc += (double)fabsf(z);
+            z = sgnf(z);
+        } else if (!constrained) {
+            z = z * inv;
+        }
         write_operand(zop, precision, ldz, i, j, z);
+    }
+    if (l1v) {
+        const double tot = block_sum<256>(acc, red);
+        if (tid == 0) sp.fz[j] = (float)tot;
     }
 }
 __global__ __launch_bounds__(256) void prep_z_kernel(const float* __restrict__ Zpart, int splits,
                                                      long split_stride, int m, int ldz, Operand zop,
-                                                     int formulation, int precision, SpState sp) {
-    prep_z_body(blockIdx.x, Zpart, splits, split_stride, m, ldz, zop, formulation, precision, sp);
+                                                     int formulation, int precision, SpState sp, SignRefine rf) {
+    prep_z_body(blockIdx.x, Zpart, splits, split_stride, m, ldz, zop, formulation, precision, sp, rf);
 }
 hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int m, int ldz, Operand zop,
-                         int formulation, int precision, int batch, SpState sp, hipStream_t st) {
-    prep_z_kernel<<<batch, 256, 0, st>>>(Zpart, splits, split_stride, m, ldz, zop, formulation, precision, sp);
+                         int formulation, int precision, int batch, SpState sp, SignRefine rf, hipStream_t st) {
+    if (rf.eps > 0.f && (!rf.arm || !rf.rowmax || !rf.x || rf.np <= 0)) return hipErrorInvalidValue;
+    prep_z_kernel<<<batch, 256, 0, st>>>(Zpart, splits, split_stride, m, ldz, zop, formulation, precision, sp, rf);
+    return hipGetLastError();
+}
+
+// max_k |A_ik| of each row of A (mp x np column major): thread per row, columns in order.
+__global__ __launch_bounds__(256) void row_max_kernel(const float* __restrict__ A, int mp, int np,
+                                                      float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= mp) return;
+    float mx = 0.f;
+    for (long k = 0; k < np; k++) mx = fmaxf(mx, fabsf(A[k * mp + i]));
+    out[i] = mx;
+}
+hipError_t launch_row_max(const float* Acm, int mp, int np, float* out, hipStream_t st) {
+    if (mp <= 0 || np <= 0) return hipErrorInvalidValue;
+    row_max_kernel<<<(mp + 255) / 256, 256, 0, st>>>(Acm, mp, np, out);
     return hipGetLastError();
 }
 
@@ -1639,7 +1703,8 @@ __global__ __launch_bounds__(256) void fused_sparse_ax_kernel(const FusedJob* __
 __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __restrict__ jobs, int njobs) {
     const int k = find_job<2>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
-    prep_z_body(blockIdx.x - J.c0, J.Zpart, J.splits, J.zsplit, J.mp, J.mp, J.zop, J.sa.formulation, J.zprec, J.sp);
+    prep_z_body(blockIdx.x - J.c0, J.Zpart, J.splits, J.zsplit, J.mp, J.mp, J.zop, J.sa.formulation, J.zprec, J.sp,
+                J.rf);
 }
 
 __global__ __launch_bounds__(kSelThreads) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {

@@ -606,3 +606,61 @@ def test_fixed_point_sign_products_against_fp64(small, small_mat, prec):
         assert same.mean() >= 0.9
         assert np.mean(rel[same] <= TOL[prec][0]) >= 0.95
     assert errs[1] <= 2 * errs[0] + 1e-9
+
+
+def _planted_near_zero(m=512, n=4096, B=128, planted_sps=8, rows_per_sp=8, delta=1e-8, seed=41):
+    """A (FP32) and starting points where, for SPs 0..planted_sps-1, rows_per_sp rows of
+    z = A x_j sit at +-delta (FP64 of the FP32 data, alternating signs): far outside FP64's
+    rounding, far inside that of the FP32 and bf16x3 products (~1e-7 to 1e-5 here).  Per
+    planted row, A[i, k] moves at the coordinate k where |x_j| is largest, then A[i, k2] at the
+    one where it is smallest takes up what FP32 rounding of the first left."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, n)).astype(np.float32)
+    X0 = unit_gaussian_starts(n, B, seed=seed + 1)
+    planted = []
+    for j in range(planted_sps):
+        x = X0[:, j].astype(np.float64)
+        k, k2 = int(np.argmax(np.abs(x))), int(np.argmin(np.abs(x)))
+        for r in range(rows_per_sp):
+            i = j * rows_per_sp + r
+            target = delta if r % 2 == 0 else -delta
+            for kk in (k, k2):
+                z = A[i].astype(np.float64) @ x
+                A[i, kk] = np.float32(A[i, kk] - (z - target) / x[kk])
+            planted.append((i, j))
+    Z = A.astype(np.float64) @ X0.astype(np.float64)
+    for i, j in planted:
+        assert 0.5 * delta < abs(Z[i, j]) < 2 * delta, Z[i, j]
+    return A, X0, planted
+
+
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3])
+def test_exact_signs_of_near_zero_z(ctx, prec):
+    """L1 variance, L0 constraint, one step from dense starting points (the dense A*X
+    product): entries of z planted at +-1e-8, inside the product's rounding, get the sign of
+    exact arithmetic (the FP64 recompute of flagged entries), so the step equals the FP64
+    oracle's: identical supports and iterates within 1e-5.  With exact_sign_products = 0
+    (the precision's own products, no recompute) some planted signs come out wrong, which
+    shows up as iterates off by far more than rounding."""
+    A, X0, planted = _planted_near_zero()
+    B, f, s = X0.shape[1], 1, 64
+    o = O.solve(A.astype(np.float64), f, s=s, tol=0.0, max_iter=1, total_sp=B, batch=B, X0=X0.astype(np.float64),
+                want_X=True, lambda_rule=O.LAMBDA_GPU)
+    Xo = o["X"]
+    M = ctx.upload(A, storage=S.STORE_FP32 | S.STORE_BF16X3)
+    try:
+        out = {}
+        for exact in (1, 0):
+            st = S.OptimizationSettings(formulation=f, constraintParameter=s, tolerance=0.0, maximumIterations=1,
+                                        totalStartingPoints=B, batchSize=B, precision=prec, exact_sign_products=exact)
+            _, X = M.solve(st, X0=X0, want_X=True)
+            out[exact] = X
+        dev = {e: np.abs(out[e] - Xo).max(axis=0) for e in out}
+        pj = sorted({j for _, j in planted})
+        print(f"case prec={prec}: max |x - x_fp64| exact {dev[1].max():.2e}, plain on planted SPs "
+              f"{np.round(dev[0][pj], 6).tolist()}", flush=True)
+        assert np.all((out[1] != 0) == (Xo != 0))
+        assert dev[1].max() <= 1e-5
+        assert dev[0][pj].max() > 1e-4  # the plain products mis-sign planted entries
+    finally:
+        M.release()

@@ -19,9 +19,10 @@ another fixed point.  Two kinds of tie occur:
     for another fixed point.  Such entries arise at about 1e-6 of the entries' scale, so
     A^T sgn(z) runs in exact fixed point: an FP32 accumulation of it put ~4e-7 of error into
     x, enough to flip such an entry at the next iteration.
-Measured (identical supports): FP32 100% for six formulations and 127/128 for the L0
-constraint / L2 variance and the L1 penalty / L2 variance; bf16x3 126/128 to 128/128.  The
-bars allow two (FP32) and three (bf16x3) starting points below 100%.
+and the entries of z within the dense product's rounding of zero are recomputed in FP64.
+Measured (identical supports): all four L1-variance formulations 128/128 in FP32 and
+bf16x3; L2 variance FP32 127/128 (L0 constraint, L1 penalty) or 128/128, bf16x3 126/128 to
+128/128.  The bars allow two (FP32) and three (bf16x3) starting points below 100%.
 """
 import os
 
