@@ -968,6 +968,30 @@ __device__ __forceinline__ void write_operand(Operand op, int precision, long ld
     }
 }
 
+// sum_k a[k] * b[k] in FP64 over k < n (n % 4 == 0, 16-byte aligned rows) by a 256-thread
+// block: 16-byte loads, eight in flight per thread, fixed summation order.
+__device__ __forceinline__ double block_dot_f64(const float* __restrict__ a, const float* __restrict__ b, int n,
+                                                double* red) {
+    const int nq = n >> 2;
+    const f4* a4 = (const f4*)a;
+    const f4* b4 = (const f4*)b;
+    double acc = 0;
+    for (int q0 = threadIdx.x; q0 < nq; q0 += 256 * 8) {
+        f4 va[8], vb[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int q = q0 + 256 * u;
+            va[u] = q < nq ? a4[q] : (f4){0.f, 0.f, 0.f, 0.f};
+            vb[u] = q < nq ? b4[q] : (f4){0.f, 0.f, 0.f, 0.f};
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) acc += (double)va[u][e] * (double)vb[u][e];
+    }
+    return block_sum<256>(acc, red);
+}
+
 // One workgroup per SP column.  z = sum of split partials (fixed order).  Constrained:
 // L1 variance f = |z|_1 then z = sgn z; L2 variance f = |z|_2 (z unchanged)
 // (c++/src/gpower/gpower_commons.h:75-101).  Penalised: z = sgn z or z/|z|_2 (:150-166).
@@ -992,9 +1016,7 @@ __device__ __forceinline__ void prep_z_body(int j, const float* __restrict__ Zpa
     const float* xj = rf.x + (long)j * rf.np;
     float thr = 0.f;
     if (refine) {
-        double xn = 0;
-        for (int k = tid; k < rf.np; k += 256) xn += (double)xj[k] * (double)xj[k];
-        thr = rf.eps * (float)sqrt(block_sum<256>(xn, red));
+        thr = rf.eps * (float)sqrt(block_dot_f64(xj, xj, rf.np, red));
         if (tid == 0) nflag = 0;
         __syncthreads();
     }
@@ -1015,10 +1037,7 @@ __device__ __forceinline__ void prep_z_body(int j, const float* __restrict__ Zpa
         if (nf <= kRefineMax) {
             for (int f = 0; f < nf; f++) {
                 const int i = flags[f];
-                const float* ai = rf.arm + (long)i * rf.np;
-                double d = 0;
-                for (int k = tid; k < rf.np; k += 256) d += (double)ai[k] * (double)xj[k];
-                d = block_sum<256>(d, red);
+                const double d = block_dot_f64(rf.arm + (long)i * rf.np, xj, rf.np, red);
                 if (tid == 0) zop.f32[(long)j * ldz + i] = (float)d;
             }
             __syncthreads();
@@ -1059,18 +1078,25 @@ hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int 
     return hipGetLastError();
 }
 
-// max_k |A_ik| of each row of A (mp x np column major): thread per row, columns in order.
+// max_k |A_ik| of each row of A (mp x np column major): block (row block, column chunk),
+// thread per row over the chunk's columns, then an integer atomic max of the bit patterns
+// (monotone for non-negative floats), so the result does not depend on order.  out must
+// hold zeros.
+constexpr int kRowMaxChunk = 256;
 __global__ __launch_bounds__(256) void row_max_kernel(const float* __restrict__ A, int mp, int np,
                                                       float* __restrict__ out) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= mp) return;
+    const long k0 = (long)blockIdx.y * kRowMaxChunk, k1 = min((long)np, k0 + kRowMaxChunk);
     float mx = 0.f;
-    for (long k = 0; k < np; k++) mx = fmaxf(mx, fabsf(A[k * mp + i]));
-    out[i] = mx;
+    for (long k = k0; k < k1; k++) mx = fmaxf(mx, fabsf(A[k * mp + i]));
+    atomicMax((unsigned int*)&out[i], __float_as_uint(mx));
 }
 hipError_t launch_row_max(const float* Acm, int mp, int np, float* out, hipStream_t st) {
     if (mp <= 0 || np <= 0) return hipErrorInvalidValue;
-    row_max_kernel<<<(mp + 255) / 256, 256, 0, st>>>(Acm, mp, np, out);
+    hipError_t e = hipMemsetAsync(out, 0, sizeof(float) * (size_t)mp, st);
+    if (e != hipSuccess) return e;
+    row_max_kernel<<<dim3((mp + 255) / 256, (np + kRowMaxChunk - 1) / kRowMaxChunk), 256, 0, st>>>(Acm, mp, np, out);
     return hipGetLastError();
 }
 

@@ -29,7 +29,7 @@ HBM_PEAK = 8000.0  # GB/s
 # HBM bytes per launch of the product kernels at this workload, from rocprofv3 --pmc
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs, scaled by the counters' reading of a 1 GiB
 # copy kernel (profiles/gemm_products_spiked4096x65536_B1024_hbm_counters.md).
-MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (4.77 + 0.25) * 2**30, ("fp32", "gemm_ax"): (7.18 + 0.125) * 2**30,
+MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (6.34 + 0.25) * 2**30, ("fp32", "gemm_ax"): (8.16 + 0.125) * 2**30,
                     ("bf16x3", "gemm_atz"): (1.50 + 0.26) * 2**30, ("bf16x3", "gemm_ax"): (1.50 + 0.13) * 2**30}
 
 
