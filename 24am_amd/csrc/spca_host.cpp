@@ -1069,8 +1069,16 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
     return SPCA_OK;
 }
 
+// Earlier components a solve deflates by (device arrays; see launch_deflate).
+struct Deflation {
+    const int* idx = nullptr;
+    const float* val = nullptr;
+    const int* off = nullptr;
+    int ncomp = 0;
+};
+
 int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, const float* X0, float* x_out,
-               spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
+               spca_stats* stats, spca_sp_result* per_sp, float* X_final, const Deflation* defl = nullptr) {
     const auto t_wall0 = std::chrono::steady_clock::now();
     spca_settings st;
     int total_batches = 0;
@@ -1187,6 +1195,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
+            if (defl) HIP_TRY(ctx, launch_deflate(xop, st.precision, w.np, defl->idx, defl->val, defl->off, defl->ncomp,
+                                                  batch, sp, s));
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                         M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, nullptr, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
@@ -1201,6 +1211,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
+            if (defl) HIP_TRY(ctx, launch_deflate(Operand{w.V, nullptr, nullptr, w.bp}, 0, w.np, defl->idx, defl->val,
+                                                  defl->off, defl->ncomp, batch, sp, s));
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
             HIP_TRY(ctx, launch_otf_converge(sp, o, batch, st.tolerance, st.max_iter, total, s));
             HIP_TRY(ctx, launch_otf_refill(sp, o, xop, st.precision, n, w.np, batch, s));
@@ -1273,6 +1285,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
             HIP_TRY(ctx, hipEventRecord(e0, s));
+            if (defl) HIP_TRY(ctx, launch_deflate(xop, st.precision, w.np, defl->idx, defl->val, defl->off, defl->ncomp,
+                                                  batch, sp, s));
             // one of the two A*X forms runs; the other sees the device flag and returns
             HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
                                         M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done,
@@ -1290,6 +1304,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
+            if (defl) HIP_TRY(ctx, launch_deflate(Operand{w.V, nullptr, nullptr, w.bp}, 0, w.np, defl->idx, defl->val,
+                                                  defl->off, defl->ncomp, batch, sp, s));
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
             HIP_TRY(ctx, launch_converge(sp, batch, it, st.tolerance, st.max_iter,
                                          st.termination == SPCA_TERM_PER_SP, cap, s));
@@ -1386,8 +1402,9 @@ int resolve_impl(spca_ctx ctx, Engine& E, Matrix* M, int formulation, const floa
 }
 
 int solve_and_resolve(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* settings, const float* X0,
-                      float* x_out, spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
-    int rc = solve_impl(ctx, E, M, settings, X0, x_out, stats, per_sp, X_final);
+                      float* x_out, spca_stats* stats, spca_sp_result* per_sp, float* X_final,
+                      const Deflation* defl = nullptr) {
+    int rc = solve_impl(ctx, E, M, settings, X0, x_out, stats, per_sp, X_final, defl);
     if (rc == SPCA_OK && settings->resolve_iters > 0 && stats) {
         int its = 0;
         rc = resolve_impl(ctx, E, M, settings->formulation, x_out, settings->resolve_iters, settings->resolve_tol,
@@ -1908,6 +1925,65 @@ int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, con
     return solve_and_resolve(ctx, ctx->eng, itm->second, settings, X0, x_out, stats, per_sp, X_final);
 }
 
+
+int spca_solve_components(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, int k,
+                          float* X_out, spca_stats* stats) {
+    if (!ctx) return SPCA_EINVAL;
+    if (!settings || !X_out || k < 1) return fail(ctx, SPCA_EINVAL, "settings, X_out and k >= 1 are required");
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto itm = ctx->mats.find(mat);
+    if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+    Matrix* M = itm->second;
+    const int n = M->n;
+    if (k > n) return fail(ctx, SPCA_EINVAL, "k must be at most n");
+    (void)hipSetDevice(ctx->device);
+    if (stats) std::memset(stats, 0, sizeof(*stats) * (size_t)k);
+    // components 0 .. c-1 as a sparse list: nonzeros idx/val, component c at off[c] .. off[c+1]
+    int* d_idx = nullptr;
+    float* d_val = nullptr;
+    int* d_off = nullptr;
+    uint64_t db = 0;
+    const size_t cap = (size_t)n * (size_t)(k - 1) + 1;
+    int rc = SPCA_OK;
+    if ((rc = dev_alloc(ctx, &d_idx, cap, &db)) || (rc = dev_alloc(ctx, &d_val, cap, &db)) ||
+        (rc = dev_alloc(ctx, &d_off, (size_t)k + 1, &db))) {
+        dev_free(ctx, d_idx, 0);
+        dev_free(ctx, d_val, 0);
+        dev_free(ctx, d_off, 0);
+        return rc;
+    }
+    std::vector<int> h_idx, h_off(1, 0);
+    std::vector<float> h_val;
+    for (int c = 0; c < k && rc == SPCA_OK; c++) {
+        Deflation d;
+        if (c > 0) {
+            const float* p = X_out + (size_t)(c - 1) * n;
+            for (int i = 0; i < n; i++)
+                if (p[i] != 0.f) {
+                    h_idx.push_back(i);
+                    h_val.push_back(p[i]);
+                }
+            h_off.push_back((int)h_idx.size());
+            if (hipMemcpy(d_idx, h_idx.data(), sizeof(int) * h_idx.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_val, h_val.data(), sizeof(float) * h_val.size(), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(d_off, h_off.data(), sizeof(int) * h_off.size(), hipMemcpyHostToDevice) != hipSuccess) {
+                rc = fail(ctx, SPCA_EDEVICE, "deflation upload failed");
+                break;
+            }
+            d.idx = d_idx;
+            d.val = d_val;
+            d.off = d_off;
+            d.ncomp = c;
+        }
+        rc = solve_and_resolve(ctx, ctx->eng, M, settings, X0, X_out + (size_t)c * n, stats ? stats + c : nullptr,
+                               nullptr, nullptr, c > 0 ? &d : nullptr);
+    }
+    (void)hipStreamSynchronize(ctx->eng.stream);
+    dev_free(ctx, d_idx, 0);
+    dev_free(ctx, d_val, 0);
+    dev_free(ctx, d_off, 0);
+    return rc;
+}
 
 int spca_ctx_memory(spca_ctx ctx, uint64_t* used, uint64_t* cached, uint64_t* budget) {
     if (!ctx) return SPCA_EINVAL;

@@ -221,6 +221,14 @@ constexpr int kSparseRows = 1024;  // rows per workgroup of the sparse product
 hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z, int ldz,
                             int batch, SpState sp, hipStream_t st);
 
+// Deflation by earlier components (reference SparseDeflationCollection,
+// c++/src/gpower/sparse_PCA_solver_for_CSC.h:56-106): for c = 0 .. ncomp-1 in order, each
+// active column of op (ld rows) loses its projection on component c, whose nonzeros are
+// idx[off[c] .. off[c+1]) with values val; the operand forms of `precision` are rewritten at
+// the changed entries.
+hipError_t launch_deflate(Operand op, int precision, long ld, const int* idx, const float* val, const int* off,
+                          int ncomp, int batch, SpState sp, hipStream_t st);
+
 // Write the operand forms of X from its FP32 values (for starting points).
 hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st);
 

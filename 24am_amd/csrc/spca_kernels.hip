@@ -1902,6 +1902,39 @@ hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipS
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- deflation
+// Reference SparseDeflationCollection::deflateV (c++/src/gpower/sparse_PCA_solver_for_CSC.h:
+// 69-92): for each earlier component p in order (its nonzeros idx[off[c] .. off[c+1]) with
+// values val), c = p . col and col -= c p.  One workgroup per active column; the dot product
+// is summed in FP64 in a fixed order and each entry is updated once, through write_operand so
+// the operand forms of X follow (precision 0 writes FP32 only).
+__global__ __launch_bounds__(256) void deflate_kernel(Operand op, int precision, long ld, const int* __restrict__ idx,
+                                                      const float* __restrict__ val, const int* __restrict__ off,
+                                                      int ncomp, SpState sp) {
+    const int j = blockIdx.x;
+    if (*sp.done || !sp.active[j]) return;
+    __shared__ double red[4];
+    const float* col = op.f32 + (long)j * ld;
+    for (int c = 0; c < ncomp; c++) {
+        const int t0 = off[c], t1 = off[c + 1];
+        double d = 0;
+        for (int t = t0 + threadIdx.x; t < t1; t += 256) d += (double)col[idx[t]] * (double)val[t];
+        d = block_sum<256>(d, red);
+        for (int t = t0 + threadIdx.x; t < t1; t += 256) {
+            const int i = idx[t];
+            write_operand(op, precision, ld, i, j, (float)((double)col[i] - d * (double)val[t]));
+        }
+        __syncthreads();
+    }
+}
+hipError_t launch_deflate(Operand op, int precision, long ld, const int* idx, const float* val, const int* off,
+                          int ncomp, int batch, SpState sp, hipStream_t st) {
+    if (ncomp <= 0 || batch <= 0) return hipSuccess;
+    if (!idx || !val || !off || precision < 0 || precision > 2) return hipErrorInvalidValue;
+    deflate_kernel<<<batch, 256, 0, st>>>(op, precision, ld, idx, val, off, ncomp, sp);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- FP64 explained variance
 // ev_j = |A xhat_j|_2^2 (L2 variance) or |A xhat_j|_1 (L1 variance), xhat = x/|x|_2, all in
 // FP64 from the FP32 master copy of A.  Work is split over (row block, index chunk of x, SP):

@@ -36,7 +36,7 @@ EXPORTS = [
     "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
     "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan", "spca_matrix_generate",
-    "spca_matrix_download", "spca_ctx_memory",
+    "spca_matrix_download", "spca_ctx_memory", "spca_solve_components",
 ]
 FUSED_GRAPH = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
@@ -120,6 +120,7 @@ def lib():
             "spca_matrix_release": (I, [P, U64]),
             "spca_matrix_bytes": (I, [P, U64, ctypes.POINTER(U64)]),
             "spca_solve": (I, [P, U64, ctypes.POINTER(_Settings), P, P, ctypes.POINTER(_Stats), P, P]),
+            "spca_solve_components": (I, [P, U64, ctypes.POINTER(_Settings), P, I, P, P]),
             "spca_dense_solver_device": (I, [P, U, U, P, P, ctypes.POINTER(_Settings), ctypes.POINTER(_Stats), U, U]),
             "spca_dense_solver_host": (I, [P, P, I, P, U, U, ctypes.POINTER(_Settings), ctypes.POINTER(_Stats),
                                            ctypes.POINTER(ctypes.c_double)]),
@@ -404,6 +405,33 @@ class Matrix:
         if self.handle:
             _check(self.ctx.ptr, lib().spca_matrix_release(self.ctx.ptr, self.handle))
             self.handle = 0
+
+    def solve_components(self, settings, k, X0=None):
+        """k sparse components by deflation (reference SparseDeflationCollection,
+        c++/src/gpower/sparse_PCA_solver_for_CSC.h:56-106): component c is solved with every
+        iterate projected off components 0 .. c-1.  Returns X (n x k, unit columns) and a list
+        of k OptimizationStatistics."""
+        c = settings._c()
+        total = _total_after_check(c, self.n)
+        X0p = None
+        if c.start == START_EXPLICIT:
+            if X0 is None:
+                raise ValueError("X0 required for START_EXPLICIT")
+            X0 = np.asfortranarray(X0, dtype=np.float32)
+            if X0.shape != (self.n, total):
+                raise ValueError(f"X0 must be {self.n} x {total}")
+            X0p = X0.ctypes.data_as(ctypes.c_void_p)
+        X = np.zeros((self.n, k), dtype=np.float32, order="F")
+        st = (_Stats * k)()
+        _check(self.ctx.ptr, lib().spca_solve_components(self.ctx.ptr, self.handle, ctypes.byref(c), X0p, int(k),
+                                                         X.ctypes.data_as(ctypes.c_void_p),
+                                                         ctypes.cast(st, ctypes.c_void_p)))
+        out = []
+        for i in range(k):
+            S = OptimizationStatistics()
+            S._fill(st[i])
+            out.append(S)
+        return X, out
 
     def solve(self, settings, X0=None, stats=None, want_X=False, per_sp=True):
         """Returns x (unit norm, length n); fills `stats`; with want_X also returns X (n x total)."""

@@ -1,4 +1,5 @@
-"""Builds libspca_mi355x.so (HIP kernels + host driver + C ABI) for gfx950 in-tree.
+"""Builds libspca_mi355x.so (HIP kernels + host driver + C ABI) for gfx950 in-tree, and the
+spca_console command-line front end linked against it.
 
 Run as ``python 24am_amd/spca_mi355x/build.py``; __graft_entry__.build() calls build().
 """
@@ -11,6 +12,7 @@ PKG = os.path.dirname(HERE)
 REPO = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 LIB = os.path.join(HERE, "libspca_mi355x.so")
+CONSOLE = os.path.join(HERE, "spca_console")
 SOURCES = ["spca_kernels.hip", "spca_host.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -31,6 +33,14 @@ def build(verbose=False):
         print(" ".join(cmd), flush=True)
     subprocess.check_call(cmd)
     os.replace(tmp, LIB)
+    ctmp = CONSOLE + ".tmp"
+    ccmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(REPO, "include"),
+            os.path.join(CSRC, "spca_console.cpp"), "-o", ctmp, "-L" + HERE, "-l:libspca_mi355x.so",
+            "-Wl,-rpath,$ORIGIN"]
+    if verbose:
+        print(" ".join(ccmd), flush=True)
+    subprocess.check_call(ccmd)
+    os.replace(ctmp, CONSOLE)
     return LIB
 
 

@@ -261,6 +261,16 @@ int spca_dense_solver_device(spca_ctx ctx, unsigned m, unsigned n, const float* 
 int spca_dense_solver_host(spca_ctx ctx, const float* B, int ldB, float* x, unsigned m, unsigned n,
                            const spca_settings* settings, spca_stats* stats, double* fval_out);
 
+/* Sparse principal components one after another, by deflation (reference
+ * SparseDeflationCollection, c++/src/gpower/sparse_PCA_solver_for_CSC.h:56-106, used by its
+ * sparse solver at :207, :311, :425, :438).  Component c is solved as spca_solve solves,
+ * except that every iterate is projected off the components before it, x <- x - (p . x) p
+ * for p = x_0 .. x_{c-1} in order, before each A*X and on v right after each A^T*Z; its
+ * unit-norm result then joins the list.  The same settings and starting points serve every
+ * component.  X_out: n x k column major; stats: k entries (may be NULL). */
+int spca_solve_components(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, int k,
+                          float* X_out, spca_stats* stats);
+
 /* ---- request server (continuous batching) ---------------------------------- */
 /* No reference counterpart: the reference solves one request per call.  A server runs up to
  * `workers` jobs at once, each on its own HIP stream with its own workspace; queued jobs are
