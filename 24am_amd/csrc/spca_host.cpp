@@ -666,6 +666,20 @@ SignRefine sign_refine(const Matrix* M, const spca_settings& st, Workspace& w) {
     return r;
 }
 
+// Exact ties for the L0 constraint with L2 variance (FP32, bf16x3): the band half-width is
+// 2 * eps * tau, eps a few standard deviations of the A^T*Z product's relative rounding at
+// the top of |v| (FP32 accumulation over m terms ~1e-6; bf16x3 products ~2e-6 more).
+void set_exact_ties(StepArgs& sa, const Matrix* M, const spca_settings& st, Workspace& w) {
+    sa.acm = M->a32_cm;
+    sa.zf = w.Z;
+    sa.mp = M->mp;
+    sa.band_eps = 0.f;
+    const bool l2_thresholded = st.formulation == L0_constrained_L2_PCA || st.formulation == L0_penalized_L2_PCA ||
+                                st.formulation == L1_penalized_L2_PCA;
+    if (st.exact_ties && l2_thresholded && (st.precision == 0 || st.precision == 1))
+        sa.band_eps = st.precision == 0 ? 0x1p-17f : 0x1p-15f;
+}
+
 // V = A^T * Z of one iteration in the solve's A^T*Z form.
 hipError_t launch_atz(const AtzForm& a, const Matrix* M, Workspace& w, int bp, const int* done, hipStream_t s) {
     if (a.prec == 4)
@@ -1115,6 +1129,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     sa.s = st.constraint;
     sa.gamma = st.penalty;
     sa.batch = batch;
+    set_exact_ties(sa, M, st, w);
 
     std::vector<float> hX(st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)n * batch);
     Results R;
@@ -1531,6 +1546,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         f.sa.s = r.st.constraint;
         f.sa.gamma = r.st.penalty;
         f.sa.batch = r.st.batch;
+        set_exact_ties(f.sa, M, r.st, r.ws);
         f.mp = M->mp;
         f.np = M->np;
         f.bp = r.ws.bp;
@@ -1780,6 +1796,7 @@ void spca_settings_default(spca_settings* st) {
     st->seed = 0;
     st->sparse_products = 1;
     st->exact_sign_products = 1;
+    st->exact_ties = 1;
 }
 
 int spca_settings_check(spca_settings* st, unsigned n) {

@@ -108,7 +108,14 @@ struct StepArgs {
     unsigned s;          // constraint after clamp
     double gamma;
     int batch;           // true number of SPs in this batch
+    // Exact ties (L0 constraint, L2 variance): candidates for the s-th place within
+    // 2 * band_eps * tau of tau are recomputed as FP64 dot products of A's column with z.
+    const float* acm;    // A column major (ld mp)
+    const float* zf;     // z of this iteration, FP32 (ld mp)
+    int mp;
+    float band_eps;      // 0: off
 };
+constexpr int kBandMax = 16;  // most candidates one tie band re-evaluates (more: none)
 
 // One job of a fused multi-job iteration: every field the five per-iteration kernels of a
 // solo solve take, plus the job's first tile in each product's tile list and its first

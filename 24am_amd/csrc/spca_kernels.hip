@@ -749,13 +749,70 @@ constexpr int kFixPart = kGemmBM * kFixBK;
 constexpr int kFixLds = 5 * kFixPart;
 __device__ __forceinline__ int fix8_lds(int r, int chunk) { return r * kFixBK + ((chunk ^ ((r >> 1) & 3)) << 4); }
 
-// V[r + c*ldc] = s_r (2^-13 accA + 2^-25 accB) over the 128x128 tile at (m0, n0).  P: the
-// pieces (stride `piece` bytes, K-blocked with ldp rows), Q: int8 signs (K-blocked with ldq
-// rows).  256 threads as 2x2 waves of 64x64; one 16x16x64 MFMA covers a whole K stage of a
-// 16x16 block, lane l supplying row l%16 and the 16 bytes of chunk l/16 of each operand (the
-// same chunk of both operands, so the products pair equal k).  Each stage is loaded between
-// two barriers and then computed, so no load registers stay live beside the 128 accumulator
-// registers; two workgroups per CU overlap one's loads with the other's MFMAs.
+// The MFMAs of one 64-deep K stage held in LDS (the four pieces, then the signs; fix8_lds
+// layout): wave (wm, wn) of the 128x128 tile accumulates its 64x64 block.  One 16x16x64 MFMA
+// covers a whole stage of a 16x16 block, lane l supplying row l%16 and the 16 bytes of chunk
+// l/16 of each operand (the same chunk of both operands, so the products pair equal k).
+__device__ __forceinline__ void fix8_stage(const uint8_t* sb, i4 (&accA)[4][4], i4 (&accB)[4][4], int wm, int wn,
+                                           int lane) {
+    const int fc = lane >> 4;
+    i4 b[4], b64[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const int r = wn * 64 + j * 16 + (lane & 15);
+        const u4 v = *(const u4*)(sb + 4 * kFixPart + fix8_lds(r, fc));
+        // 64 z per byte: bits 0-1 of each byte (z in {-1, 0, 1}) moved to bits 6-7
+        const u4 v64 = (v << 6u) & 0xC0C0C0C0u;
+        b[j] = __builtin_bit_cast(i4, v);
+        b64[j] = __builtin_bit_cast(i4, v64);
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        i4 a[4];
+        __builtin_amdgcn_sched_barrier(0);  // one piece's fragments at a time
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = wm * 64 + i * 16 + (lane & 15);
+            a[i] = __builtin_bit_cast(i4, *(const u4*)(sb + p * kFixPart + fix8_lds(r, fc)));
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (p == 0) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b64[j], accA[i][j], 0, 0, 0);
+                else if (p == 1) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], accA[i][j], 0, 0, 0);
+                else if (p == 2) accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b64[j], accB[i][j], 0, 0, 0);
+                else accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], accB[i][j], 0, 0, 0);
+            }
+    }
+}
+
+// V[r + c*ldc] = s_r (2^-13 accA + 2^-25 accB) for the wave's 64x64 block of the tile at (m0, n0).
+__device__ __forceinline__ void fix8_store(const i4 (&accA)[4][4], const i4 (&accB)[4][4],
+                                           const float* __restrict__ qscale, float* __restrict__ C, long ldc,
+                                           long m0, long n0, int wm, int wn, int lane) {
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const long row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
+        const f4 sc = *(const f4*)(qscale + row);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const long col = n0 + wn * 64 + j * 16 + (lane & 15);
+            f4 o;
+#pragma unroll
+            for (int e = 0; e < 4; e++)
+                o[e] = (float)((double)sc[e] * ((double)accA[i][j][e] * 0x1p-13 + (double)accB[i][j][e] * 0x1p-25));
+            *(f4*)(C + row + col * ldc) = o;
+        }
+    }
+}
+
+// One 128x128 tile of V = s (2^-13 accA + 2^-25 accB).  P: the pieces (stride `piece` bytes,
+// K-blocked with ldp rows), Q: int8 signs (K-blocked with ldq rows).  256 threads as 2x2 waves
+// of 64x64.  Each stage is loaded between two barriers and then computed, so no load
+// registers stay live beside the 128 accumulator registers; two workgroups per CU overlap
+// one's loads with the other's MFMAs.  The fused iteration's tiles; solo solves run the
+// pipelined kernel below (same sums, so the same bits).
 __device__ __forceinline__ void gemm_fix8_tile(const int8_t* __restrict__ P, long piece,
                                                const float* __restrict__ qscale, long ldp,
                                                const int8_t* __restrict__ Q, long ldq, float* __restrict__ C,
@@ -776,7 +833,6 @@ __device__ __forceinline__ void gemm_fix8_tile(const int8_t* __restrict__ P, lon
             accB[i][j] = (i4){0, 0, 0, 0};
         }
     const int nk = K / kFixBK;
-    const int fc = lane >> 4;
     for (int ks = 0; ks < nk; ks++) {
         __syncthreads();
         {
@@ -796,52 +852,18 @@ __device__ __forceinline__ void gemm_fix8_tile(const int8_t* __restrict__ P, lon
             }
         }
         __syncthreads();
-        i4 b[4], b64[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const int r = wn * 64 + j * 16 + (lane & 15);
-            const u4 v = *(const u4*)(smem + 4 * kFixPart + fix8_lds(r, fc));
-            // 64 z per byte: bits 0-1 of each byte (z in {-1, 0, 1}) moved to bits 6-7
-            const u4 v64 = (v << 6u) & 0xC0C0C0C0u;
-            b[j] = __builtin_bit_cast(i4, v);
-            b64[j] = __builtin_bit_cast(i4, v64);
-        }
-#pragma unroll
-        for (int p = 0; p < 4; p++) {
-            i4 a[4];
-            __builtin_amdgcn_sched_barrier(0);  // one piece's fragments at a time
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const int r = wm * 64 + i * 16 + (lane & 15);
-                a[i] = __builtin_bit_cast(i4, *(const u4*)(smem + p * kFixPart + fix8_lds(r, fc)));
-            }
-#pragma unroll
-            for (int i = 0; i < 4; i++)
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    if (p == 0) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b64[j], accA[i][j], 0, 0, 0);
-                    else if (p == 1) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], accA[i][j], 0, 0, 0);
-                    else if (p == 2) accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b64[j], accB[i][j], 0, 0, 0);
-                    else accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], accB[i][j], 0, 0, 0);
-                }
-        }
+        fix8_stage(smem, accA, accB, wm, wn, lane);
     }
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const long row = m0 + wm * 64 + i * 16 + 4 * (lane >> 4);
-        const f4 sc = *(const f4*)(qscale + row);
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            const long col = n0 + wn * 64 + j * 16 + (lane & 15);
-            f4 o;
-#pragma unroll
-            for (int e = 0; e < 4; e++)
-                o[e] = (float)((double)sc[e] * ((double)accA[i][j][e] * 0x1p-13 + (double)accB[i][j][e] * 0x1p-25));
-            *(f4*)(C + row + col * ldc) = o;
-        }
-    }
+    fix8_store(accA, accB, qscale, C, ldc, m0, n0, wm, wn, lane);
 }
 
+// Solo solves: the same 128x128 tile with a two-stage LDS ring filled by direct global->LDS
+// loads (global_load_lds_dwordx4: lane l's 16 bytes land at the wave's LDS base + 16 l), so
+// the loads of stage k+1 are in flight while stage k is computed.  Wave w loads rows
+// [32w, 32w+32) of every part, 16 rows per instruction; each lane's source chunk is chosen so
+// that the LDS rows come out in the fix8_lds (XOR-swizzled) layout.  Per stage: each wave
+// waits for its own loads, then one barrier; after it every wave's loads of stage k have
+// landed and every wave has finished reading stage k-1, whose buffer stage k+1 then reuses.
 __global__ __launch_bounds__(256, 2) void gemm_tn_fix8_kernel(const int8_t* __restrict__ P, long piece,
                                                            const float* __restrict__ qscale, long ldp,
                                                            const int8_t* __restrict__ Q, long ldq,
@@ -850,8 +872,47 @@ __global__ __launch_bounds__(256, 2) void gemm_tn_fix8_kernel(const int8_t* __re
     if (done && *done) return;
     const int t = xcd_remap(blockIdx.x, tiles_m * tiles_n);
     const int tn = t % tiles_n, tm = t / tiles_n;
-    __shared__ __attribute__((aligned(16))) uint8_t smem[kFixLds];
-    gemm_fix8_tile(P, piece, qscale, ldp, Q, ldq, C, ldc, (long)tm * kGemmBM, (long)tn * kGemmBN, K, smem);
+    const long m0 = (long)tm * kGemmBM, n0 = (long)tn * kGemmBN;
+    __shared__ __attribute__((aligned(1024))) uint8_t lds[2 * kFixLds];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const long pst = ldp * kFixBK, qst = ldq * kFixBK;  // bytes per K stage
+    long src[2][2];                                   // [P, Q][16-row group]: this lane's bytes in stage 0
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int row = 32 * wave + 16 * i + (lane >> 2);
+        const int g = (lane & 3) ^ ((row >> 1) & 3);
+        src[0][i] = (m0 + row) * kFixBK + g * 16;
+        src[1][i] = (n0 + row) * kFixBK + g * 16;
+    }
+    auto issue = [&](int ks, int buf) {
+#pragma unroll
+        for (int part = 0; part < 5; part++)
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const int8_t* g = part < 4 ? P + part * piece + ks * pst + src[0][i] : Q + ks * qst + src[1][i];
+                uint8_t* dst = lds + buf * kFixLds + part * kFixPart + (32 * wave + 16 * i) * kFixBK;
+                __builtin_amdgcn_global_load_lds((const void*)g, (__attribute__((address_space(3))) void*)dst, 16, 0,
+                                                 0);
+            }
+    };
+    i4 accA[4][4], accB[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            accA[i][j] = (i4){0, 0, 0, 0};
+            accB[i][j] = (i4){0, 0, 0, 0};
+        }
+    const int nk = K / kFixBK;
+    issue(0, 0);
+    for (int ks = 0; ks < nk; ks++) {
+        wait_vmcnt(0);
+        __builtin_amdgcn_s_barrier();
+        if (ks + 1 < nk) issue(ks + 1, (ks + 1) & 1);
+        fix8_stage(lds + (ks & 1) * kFixLds, accA, accB, wm, wn, lane);
+    }
+    fix8_store(accA, accB, qscale, C, ldc, m0, n0, wm, wn, lane);
 }
 hipError_t launch_gemm_fix8(const int8_t* P, const float* qscale, long ldp, const int8_t* Q, long ldq, float* C,
                             long ldc, int M, int N, int K, const int* done, hipStream_t st) {
@@ -1124,6 +1185,12 @@ struct SelShared {
     uint64_t ures[2];
     int ncand;
     double dres[4];
+    // exact ties: the re-evaluated band (sorted descending), its value range and keep flags
+    int nband;
+    float band_lo, band_hi;
+    int band_idx[kBandMax];
+    float band_val[kBandMax];
+    int band_keep[kBandMax];
 };
 
 __device__ __forceinline__ int lin_bin(float a, float scale) {
@@ -1201,6 +1268,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     // element transform applied when the iterate is written: keep |v| >= tau (L0),
     // soft map with lam (L1), penalised maps; then multiply by `scale_out`
     float tau = 0.f, lam = 0.f, scale_out = 1.f;
+    if (tid == 0) sh.nband = 0;  // no re-evaluated tie band unless the L0 fast path makes one
 
     // L1 constraint with s <= 1 (|x|_1 <= |x|_2) admits only 1-sparse x: the top-1 hard
     // threshold, which the lambda scan cannot produce (its quadratic degenerates)
@@ -1310,11 +1378,76 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 // ---- L0 constrained: tau = the (s - above)-th largest candidate (counting ties)
                 const uint32_t r = a.s - cA_hist;
                 tau = (r >= 1 && (int)r <= nc) ? __uint_as_float((uint32_t)(sh.cand[r - 1] >> 32)) : 0.f;
+                // exact ties: candidates within the product's rounding of tau could be in either
+                // order; when the band [lo, hi] lies inside this bin (so every such value is a
+                // candidate) and holds 2..kBandMax of them, they are recomputed in FP64 and the
+                // s-th place is decided on the exact values
+                const float lo = tau * (1.f - 2.f * a.band_eps), hi = tau * (1.f + 2.f * a.band_eps);
+                const bool try_band = a.band_eps > 0.f && tau > 0.f && lin_bin(lo, scale) == bstar &&
+                                      lin_bin(hi, scale) == bstar;
+                int i0 = 0, i1 = 0;  // band = sorted candidates [i0, i1)
+                if (try_band) {
+                    double above = 0, inband = 0;
+                    for (int i = tid; i < nc; i += kSelThreads) {
+                        const float av = __uint_as_float((uint32_t)(sh.cand[i] >> 32));
+                        above += av > hi ? 1.0 : 0.0;
+                        inband += (av <= hi && av >= lo) ? 1.0 : 0.0;
+                    }
+                    i0 = (int)block_sum<kSelThreads>(above, sh.red);
+                    i1 = i0 + (int)block_sum<kSelThreads>(inband, sh.red);
+                }
+                const int nb = i1 - i0;
+                if (tid == 0) sh.nband = 0;
+                if (try_band && nb >= 2 && nb <= kBandMax) {
+                    const int w = tid >> 6, lane = tid & 63;
+                    if (w < nb) {
+                        const int idx = (int)(uint32_t)sh.cand[i0 + w];
+                        const f4* ac = (const f4*)(a.acm + (long)idx * a.mp);
+                        const f4* zc = (const f4*)(a.zf + (long)j * a.mp);
+                        double d = 0;
+                        for (int q = lane; q < (a.mp >> 2); q += 64) {
+                            const f4 x = ac[q], y = zc[q];
+                            d += (double)x[0] * y[0] + (double)x[1] * y[1] + (double)x[2] * y[2] + (double)x[3] * y[3];
+                        }
+                        d = wave_sum(d);
+                        if (lane == 0) {
+                            sh.band_idx[w] = idx;
+                            sh.band_val[w] = (float)d;
+                        }
+                    }
+                    __syncthreads();
+                    if (tid == 0) {
+                        // descending by (|v|, index), the candidates' own order
+                        for (int p = 1; p < nb; p++) {
+                            const float v = sh.band_val[p];
+                            const int ix = sh.band_idx[p];
+                            const uint64_t key = ((uint64_t)__float_as_uint(fabsf(v)) << 32) | (uint32_t)ix;
+                            int q = p - 1;
+                            while (q >= 0 && (((uint64_t)__float_as_uint(fabsf(sh.band_val[q])) << 32) |
+                                              (uint32_t)sh.band_idx[q]) < key) {
+                                sh.band_val[q + 1] = sh.band_val[q];
+                                sh.band_idx[q + 1] = sh.band_idx[q];
+                                q--;
+                            }
+                            sh.band_val[q + 1] = v;
+                            sh.band_idx[q + 1] = ix;
+                        }
+                        // the band supplies places i0+1 .. of the bin: r - i0 of them
+                        const float tb = fabsf(sh.band_val[(int)r - 1 - i0]);
+                        for (int p = 0; p < nb; p++) sh.band_keep[p] = !(fabsf(sh.band_val[p]) < tb);
+                        sh.band_lo = lo;
+                        sh.band_hi = hi;
+                        sh.nband = nb;
+                    }
+                }
+                __syncthreads();
+                const int nband = sh.nband;
                 double cs = 0;
                 for (int i = tid; i < nc; i += kSelThreads) {
                     const float av = __uint_as_float((uint32_t)(sh.cand[i] >> 32));
-                    if (!(av < tau)) cs += (double)av * (double)av;
+                    if (nband ? i < i0 : !(av < tau)) cs += (double)av * (double)av;
                 }
+                if (tid < nband && sh.band_keep[tid]) cs += (double)sh.band_val[tid] * (double)sh.band_val[tid];
                 norm2 = S2A + block_sum<kSelThreads>(cs, sh.red);
             } else {
                 // ---- L1 constrained: the first candidate (descending) where the reference scan
@@ -1471,24 +1604,88 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     } else {
         // ---- penalised (my_cublas_wrapper.h:170-238,636-710), gamma in the iterate's precision
         const float g = (float)a.gamma;
-        double acc = 0;
+        double acc = 0, acc_band = 0;
         const bool l0p = (f == 4 || f == 5);
+        auto contrib = [&](float v) -> double {
+            if (l0p) {
+                const float t = v * v - g;
+                return t > 0.f ? (double)t : 0.0;
+            }
+            const float t = fabsf(v) - g;
+            return t > 0.f ? (double)t * (double)t : 0.0;
+        };
+        // exact ties (L2 variance): entries whose |v| lies within 2 eps of the threshold (sqrt(gamma)
+        // for the L0 penalty, gamma for the L1 penalty) are listed, recomputed in FP64 below, and
+        // their contributions replaced; more than kBandMax of them: none
+        const float thr = l0p ? sqrtf(g) : g;
+        const float lo = thr * (1.f - 2.f * a.band_eps), hi = thr * (1.f + 2.f * a.band_eps);
+        const bool band = a.band_eps > 0.f && thr > 0.f;
+        if (band) {
+            if (tid == 0) sh.ncand = 0;
+            __syncthreads();
+        }
         for (int q0 = 0; q0 < nq; q0 += 8) {
             float vv[8];
 #pragma unroll
             for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);  // zeros past n contribute nothing
 #pragma unroll
             for (int u = 0; u < 8; u++) {
-                if (l0p) {
-                    const float t = vv[u] * vv[u] - g;
-                    if (t > 0.f) acc += (double)t;
-                } else {
-                    const float t = fabsf(vv[u]) - g;
-                    if (t > 0.f) acc += (double)t * (double)t;
+                const double c = contrib(vv[u]);
+                acc += c;
+                const float av = fabsf(vv[u]);
+                if (band && av >= lo && av <= hi) {
+                    acc_band += c;
+                    const int slot = atomicAdd(&sh.ncand, 1);
+                    if (slot < kBandMax) sh.band_idx[slot] = tid + (q0 + u) * kSelThreads;
                 }
             }
         }
-        fval = l0p ? (float)block_sum<kSelThreads>(acc, sh.red) : (float)sqrt(block_sum<kSelThreads>(acc, sh.red));
+        double tot = block_sum<kSelThreads>(acc, sh.red);
+        if (band) {
+            const double tb = block_sum<kSelThreads>(acc_band, sh.red);  // its barriers publish ncand
+            const int nb = sh.ncand;
+            if (nb >= 1 && nb <= kBandMax) {
+                const int w = tid >> 6, lane = tid & 63;
+                if (tid == 0)  // index order, so the replaced sum below has a fixed order
+                    for (int p = 1; p < nb; p++) {
+                        const int ix = sh.band_idx[p];
+                        int q = p - 1;
+                        while (q >= 0 && sh.band_idx[q] > ix) {
+                            sh.band_idx[q + 1] = sh.band_idx[q];
+                            q--;
+                        }
+                        sh.band_idx[q + 1] = ix;
+                    }
+                __syncthreads();
+                if (w < nb) {
+                    const f4* ac = (const f4*)(a.acm + (long)sh.band_idx[w] * a.mp);
+                    const f4* zc = (const f4*)(a.zf + (long)j * a.mp);
+                    double d = 0;
+                    for (int q = lane; q < (a.mp >> 2); q += 64) {
+                        const f4 x = ac[q], y = zc[q];
+                        d += (double)x[0] * y[0] + (double)x[1] * y[1] + (double)x[2] * y[2] + (double)x[3] * y[3];
+                    }
+                    d = wave_sum(d);
+                    if (lane == 0) sh.band_val[w] = (float)d;
+                }
+                __syncthreads();
+                if (tid == 0) {
+                    double tr = 0;
+                    for (int p = 0; p < nb; p++) {
+                        const float v = sh.band_val[p];
+                        sh.band_keep[p] = contrib(v) > 0.0;
+                        tr += contrib(v);
+                    }
+                    sh.dres[3] = tr;
+                    sh.band_lo = lo;
+                    sh.band_hi = hi;
+                    sh.nband = nb;
+                }
+                __syncthreads();
+                tot = tot - tb + sh.dres[3];
+            }
+        }
+        fval = l0p ? (float)tot : (float)sqrt(tot);
         lam = g;
     }
     // ---- write the iterate (+ operand forms), count nonzeros, update the SP record
@@ -1502,6 +1699,10 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
         const __amdgpu_buffer_rsrc_t lr =
             col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, a.precision == 1 ? opb : 0u);
+        // exact ties (L0 fast path, penalised L2 variance): values inside the re-evaluated band
+        // take its decision and value
+        const int nband = ((f < 4 && !l1c) || f == 4 || f == 6) ? sh.nband : 0;
+        const float blo = nband ? sh.band_lo : 0.f, bhi = nband ? sh.band_hi : -1.f;
         for (int q0 = 0; q0 < nq; q0 += 8) {
             float vv[8];
 #pragma unroll
@@ -1512,8 +1713,27 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 if (q >= nq) break;
                 const float v = vv[u];
                 float x;
-                if (f < 4 && !l1c) x = fabsf(v) < tau ? 0.f : v;
-                else if (l1c || f == 6 || f == 7) {
+                if (f < 4 && !l1c) {
+                    const float av = fabsf(v);
+                    if (av >= blo && av <= bhi) {
+                        const int i = tid + q * kSelThreads;
+                        x = 0.f;
+                        for (int p = 0; p < nband; p++)
+                            if (sh.band_idx[p] == i) x = sh.band_keep[p] ? sh.band_val[p] : 0.f;
+                    } else {
+                        x = av < tau ? 0.f : v;
+                    }
+                }
+                else if (nband && fabsf(v) >= blo && fabsf(v) <= bhi) {
+                    // penalised, inside the re-evaluated band: the exact value decides
+                    const int i = tid + q * kSelThreads;
+                    x = 0.f;
+                    for (int p = 0; p < nband; p++)
+                        if (sh.band_idx[p] == i && sh.band_keep[p]) {
+                            const float ve = sh.band_val[p];
+                            x = f == 4 ? ve : (fabsf(ve) - lam) * sgnf(ve);
+                        }
+                } else if (l1c || f == 6 || f == 7) {
                     const float t2 = fabsf(v) - lam;
                     x = t2 > 0.f ? t2 * sgnf(v) : 0.f;
                 } else {

@@ -68,6 +68,7 @@ class _Settings(ctypes.Structure):
         ("resolve_iters", ctypes.c_int), ("resolve_tol", ctypes.c_double), ("seed", ctypes.c_uint64),
         ("sparse_products", ctypes.c_int),
         ("exact_sign_products", ctypes.c_int),
+        ("exact_ties", ctypes.c_int),
     ]
 
 
@@ -180,6 +181,7 @@ class OptimizationSettings:
         self.sparse_products = 1  # gather A*X over the nonzeros of sparse iterates (0: always dense)
         # L1 variance, FP32 and bf16x3: A^T sgn(z) from the exact bf16 split of A (0: the precision's product)
         self.exact_sign_products = 1
+        self.exact_ties = 1
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -206,7 +208,8 @@ class OptimizationSettings:
                          float(self.tolerance), int(self.maximumIterations), int(self.totalStartingPoints),
                          int(self.batchSize), int(bool(self.useOTF)), int(self.termination), int(self.precision),
                          int(self.start), int(self.check_every), int(self.resolve_iters), float(self.resolve_tol),
-                         int(self.seed), int(self.sparse_products), int(self.exact_sign_products))
+                         int(self.seed), int(self.sparse_products), int(self.exact_sign_products),
+                         int(self.exact_ties))
 
 
 class OptimizationStatistics:

@@ -118,6 +118,11 @@ typedef struct spca_settings {
                                  int8 MFMA: exact integer sums, one rounding of each v_j to FP32,
                                  |error| <= 2^-26 * max_i |A_ij| * sum_i |z_i| (+ that rounding);
                                  0: the product of `precision` */
+    int exact_ties;           /* 1 (default): L0 constraint with L2 variance in FP32 and bf16x3:
+                                 when candidates for the s-th place lie within the A^T*Z
+                                 product's rounding of each other (2 to 16 of them), their
+                                 entries of v are recomputed in FP64 from the FP32 master and z
+                                 and the place is decided on those; 0: on the product's values */
 } spca_settings;
 
 /* mirrors OptimizationStatistics (c++/src/class/optimization_statistics.h:27-46) */

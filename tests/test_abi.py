@@ -39,7 +39,7 @@ def test_settings_defaults_match_reference():
     assert (c.tolerance, c.constraint, c.penalty, c.total_sp, c.batch, c.max_iter, c.use_otf) == \
         (0.01, 10, 0.0, 64, 64, 20, 0)
     assert c.formulation == S.SPCA_Formulation.L0_constrained_L2_PCA
-    assert (c.sparse_products, c.exact_sign_products) == (1, 1)
+    assert (c.sparse_products, c.exact_sign_products, c.exact_ties) == (1, 1, 1)
 
 
 @pytest.mark.parametrize("s,n,total,batch,exp", [

@@ -664,3 +664,112 @@ def test_exact_signs_of_near_zero_z(ctx, prec):
         assert dev[0][pj].max() > 1e-4  # the plain products mis-sign planted entries
     finally:
         M.release()
+
+
+def _planted_top_s_ties(m=2048, n=4096, B=32, s=32, gap=1.5e-7, seed=51):
+    """FP32 A and one-coordinate starting points e_c (so z = A[:, c] exactly), with, for half
+    the SPs, the s-th and (s+1)-th largest |v| of v = A^T z moved to within `gap` (relative) of
+    each other: above FP32's resolution of v, below the FP32 product's rounding (~1e-6 at this
+    m).  The (s+1)-th column is moved at the row where |z| is largest.  Returns A, X0 and the
+    SPs whose planted gap holds in FP64 after all the moves."""
+    rng = np.random.default_rng(seed)
+    A = rng.standard_normal((m, n)).astype(np.float32)
+    cols = rng.choice(n, B, replace=False)
+    X0 = np.zeros((n, B), dtype=np.float32)
+    X0[cols, np.arange(B)] = 1.0
+    for j in range(B // 2):
+        c = cols[j]
+        z = A[:, c].astype(np.float64)
+        v = A.T.astype(np.float64) @ z
+        order = np.argsort(-np.abs(v), kind="stable")
+        i_s, i_t = order[s - 1], order[s]
+        if c in (i_s, i_t):
+            continue
+        r = int(np.argmax(np.abs(z)))
+        target = np.sign(v[i_t]) * abs(v[i_s]) * (1 - gap)
+        A[r, i_t] = np.float32(A[r, i_t] + (target - v[i_t]) / z[r])
+    V = A.T.astype(np.float64) @ A[:, cols].astype(np.float64)
+    planted = []
+    for j in range(B // 2):
+        a = np.sort(np.abs(V[:, j]))[::-1]
+        rel = (a[s - 1] - a[s]) / a[s - 1]
+        if 0.5 * gap < rel < 2 * gap:
+            planted.append(j)
+    return A, X0, planted
+
+
+def test_exact_ties_at_the_s_th_place(ctx):
+    """L0 constraint, L2 variance, FP32, one step: candidates for the s-th place within the
+    product's rounding of each other are re-evaluated in FP64 (exact_ties), so the step equals
+    the FP64 oracle's on every SP, planted near-ties included; with exact_ties = 0 the
+    product's own order decides and some planted SPs keep the wrong coordinate."""
+    A, X0, planted = _planted_top_s_ties()
+    B, s = X0.shape[1], 32
+    assert len(planted) >= 12, planted
+    o = O.solve(A.astype(np.float64), 0, s=s, tol=0.0, max_iter=1, total_sp=B, batch=B, X0=X0.astype(np.float64),
+                want_X=True, lambda_rule=O.LAMBDA_GPU)
+    Xo = o["X"]
+    M = ctx.upload(A)
+    try:
+        out = {}
+        for ties in (1, 0):
+            st = S.OptimizationSettings(formulation=0, constraintParameter=s, tolerance=0.0, maximumIterations=1,
+                                        totalStartingPoints=B, batchSize=B, exact_ties=ties)
+            _, X = M.solve(st, X0=X0, want_X=True)
+            out[ties] = X
+    finally:
+        M.release()
+    off = {t: np.nonzero(~np.all((out[t] != 0) == (Xo != 0), axis=0))[0] for t in out}
+    print(f"case planted {len(planted)}: SPs off the FP64 support, exact ties {off[1].tolist()}, "
+          f"plain {off[0].tolist()}", flush=True)
+    assert off[1].size == 0
+    assert np.abs(out[1] - Xo).max() <= 1e-5
+    assert np.intersect1d(off[0], planted).size >= 1  # the plain product misorders some planted ties
+
+
+@pytest.mark.parametrize("f", [4, 6])
+def test_exact_ties_at_the_penalty_threshold(ctx, f):
+    """Penalised L2 variance, FP32, one step from e_c: gamma (an FP32 value) is set so that one
+    coordinate's |v| sits 1.5e-7 (relative, 1.2e-7 to 2.4e-7 after rounding gamma) above or
+    below the threshold (sqrt(gamma) for the L0 penalty, gamma for the L1 penalty): outside the
+    rounding that FP32 z leaves in v (at most 9e-8 here), inside the product's (median 1.7e-7).
+    With exact_ties the band entry is recomputed in FP64 and the step equals the FP64 oracle's;
+    with exact_ties = 0 some of the 24 planted cases come out the other way."""
+    rng = np.random.default_rng(61)
+    m, n, g_rel = 2048, 4096, 1.5e-7
+    A = rng.standard_normal((m, n)).astype(np.float32)
+    A64 = A.astype(np.float64)
+    M = ctx.upload(A)
+    wrong = {0: 0, 1: 0}
+    try:
+        for case, c in enumerate(rng.choice(n, 24, replace=False)):
+            z = A64[:, c] / np.linalg.norm(A64[:, c])
+            v = A64.T @ z
+            sgn = 1 if case % 2 == 0 else -1  # coordinate just above / just below the threshold
+            for rank in range(40, 80):
+                a = abs(v[np.argsort(-np.abs(v))[rank]])
+                t = a / (1 + sgn * g_rel)
+                gamma = float(np.float32(t * t if f == 4 else t))
+                thr = np.sqrt(gamma) if f == 4 else gamma
+                if 0.8 * g_rel <= sgn * (a - thr) / thr <= 1.6 * g_rel:
+                    break
+            else:
+                pytest.fail("no coordinate to plant")
+            X0 = np.zeros((n, 1), dtype=np.float32)
+            X0[c, 0] = 1.0
+            o = O.solve(A64, f, gamma=gamma, tol=0.0, max_iter=1, total_sp=1, batch=1, X0=X0.astype(np.float64),
+                        want_X=True, lambda_rule=O.LAMBDA_GPU)
+            for ties in (1, 0):
+                st = S.OptimizationSettings(formulation=f, penaltyParameter=gamma, tolerance=0.0, maximumIterations=1,
+                                            totalStartingPoints=1, batchSize=1, exact_ties=ties)
+                _, X = M.solve(st, X0=X0, want_X=True)
+                same = np.array_equal(X[:, 0] != 0, o["X"][:, 0] != 0)
+                if ties:
+                    assert same, (case, c)
+                    np.testing.assert_allclose(X[:, 0], o["X"][:, 0], rtol=1e-5, atol=1e-6 * np.abs(o["X"]).max())
+                wrong[ties] += 0 if same else 1
+    finally:
+        M.release()
+    print(f"case f={f}: planted threshold ties decided off FP64: exact ties {wrong[1]}, plain {wrong[0]} of 24",
+          flush=True)
+    assert wrong[0] >= 1
