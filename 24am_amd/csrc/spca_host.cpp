@@ -46,6 +46,18 @@ struct Matrix {
     int8_t* q_cm = nullptr;      // fixed-point pieces of A for the exact A^T * sgn(z) (4 x mp x np bytes)
     float* qscale = nullptr;     // their per-column power-of-two scales (np)
     float* rowmax = nullptr;     // max_k |A_ik| per row (mp): the sign refinement's error bound
+    // sparse matrices (spca_matrix_upload_csc): CSC and CSR arrays, optional centring means;
+    // no dense storage exists for them
+    bool sparse = false;
+    int nnz = 0;
+    int* cptr = nullptr;
+    int* ridx = nullptr;
+    float* cval = nullptr;
+    int* rptr = nullptr;
+    int* cidx = nullptr;
+    float* rval = nullptr;
+    float* mu = nullptr;
+    float* rmu = nullptr;
     uint64_t bytes = 0;
     int refs = 0;
 };
@@ -63,6 +75,7 @@ struct Workspace {
     float* fstate = nullptr;  // val, fz, err, errmax  (4 x bp floats)
     int* istate = nullptr;    // iters, nnz, active (bp ints each), done (state_flag_ints(bp))
     double* ev = nullptr;
+    double* corr = nullptr;  // per-SP centring scalars of sparse products (4 x bp)
     uint64_t bytes = 0;
 };
 
@@ -254,7 +267,8 @@ bool valid_formulation(int f) { return f >= 0 && f <= 7; }
 
 void free_matrix(spca_ctx ctx, Matrix* M) {
     (void)hipStreamSynchronize(ctx->stream);
-    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm, M->q_cm, M->qscale, M->rowmax};
+    void* ptrs[] = {M->a32_cm, M->a32_rm, M->hi_cm, M->lo_cm, M->hi_rm, M->lo_rm, M->h_cm, M->h_rm, M->q_cm, M->qscale, M->rowmax,
+                    M->cptr, M->ridx, M->cval, M->rptr, M->cidx, M->rval, M->mu, M->rmu};
     for (void* p : ptrs) raw_free(ctx, p);
     delete M;
 }
@@ -480,7 +494,7 @@ int make_generated(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const
 
 void free_ws(spca_ctx ctx, Workspace& w, hipStream_t s) {
     (void)hipStreamSynchronize(s);
-    void* ptrs[] = {w.X, w.Xhi, w.Xlo, w.Zpart, w.Z, w.Zhi, w.Zlo, w.V, w.fstate, w.istate, w.ev};
+    void* ptrs[] = {w.X, w.Xhi, w.Xlo, w.Zpart, w.Z, w.Zhi, w.Zlo, w.V, w.fstate, w.istate, w.ev, w.corr};
     for (void* p : ptrs) raw_free(ctx, p);
     w = Workspace();
 }
@@ -490,7 +504,7 @@ void free_workspace(spca_ctx ctx, Engine& E) { free_ws(ctx, E.ws, E.stream); }
 uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
     const uint64_t xs = (uint64_t)np * bp, zs = (uint64_t)mp * bp;
     return xs * (4 + 2 + 2 + 4) + zs * splits * 4 + zs * (4 + 2 + 2) + (uint64_t)4 * bp * 4 +
-           ((uint64_t)3 * bp + state_flag_ints(bp)) * 4 + (uint64_t)bp * 8;
+           ((uint64_t)3 * bp + state_flag_ints(bp)) * 4 + (uint64_t)bp * 8 * 5;
 }
 
 int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp, int splits,
@@ -509,7 +523,8 @@ int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp,
         (rc = dev_alloc(ctx, &w.Zlo, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.V, xs, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.fstate, (size_t)4 * bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + state_flag_ints(bp), &w.bytes, err)) ||
-        (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err))) {
+        (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.corr, (size_t)4 * bp, &w.bytes, err))) {
         free_ws(ctx, w, s);
         return rc;
     }
@@ -547,6 +562,10 @@ void engine_destroy(spca_ctx ctx, Engine& E) {
     E.h_done = nullptr;
     if (E.stream) (void)hipStreamDestroy(E.stream);
     E.stream = nullptr;
+}
+
+SparseMat sparse_of(const Matrix* M) {
+    return SparseMat{M->cptr, M->ridx, M->cval, M->rptr, M->cidx, M->rval, M->mu, M->rmu, M->m, M->n};
 }
 
 SpState state_of(Workspace& w) {
@@ -674,18 +693,47 @@ void set_exact_ties(StepArgs& sa, const Matrix* M, const spca_settings& st, Work
     sa.zf = w.Z;
     sa.mp = M->mp;
     sa.band_eps = 0.f;
+    if (!M->a32_cm) return;  // sparse matrices: the products already sum in FP64
     const bool l2_thresholded = st.formulation == L0_constrained_L2_PCA || st.formulation == L0_penalized_L2_PCA ||
                                 st.formulation == L1_penalized_L2_PCA;
     if (st.exact_ties && l2_thresholded && (st.precision == 0 || st.precision == 1))
         sa.band_eps = st.precision == 0 ? 0x1p-17f : 0x1p-15f;
 }
 
-// V = A^T * Z of one iteration in the solve's A^T*Z form.
-hipError_t launch_atz(const AtzForm& a, const Matrix* M, Workspace& w, int bp, const int* done, hipStream_t s) {
+// V = A^T * Z of one iteration in the solve's A^T*Z form (sparse matrices: the CSC gather with
+// the centring terms).
+hipError_t launch_atz(const AtzForm& a, const Matrix* M, Workspace& w, int bp, int batch, SpState sp, hipStream_t s) {
+    if (M->sparse) {
+        hipError_t e = launch_sp_scalars(w.Z, M->mp, M->m, M->rmu, w.corr + 2 * bp, bp, batch, sp, s);
+        if (e != hipSuccess) return e;
+        return launch_csc_atz(sparse_of(M), w.Z, M->mp, w.V, M->np, M->np, w.corr + 2 * bp, bp, batch, sp, s);
+    }
     if (a.prec == 4)
-        return launch_gemm_fix8(a.q, a.qscale, M->np, (const int8_t*)w.Zhi, bp, w.V, M->np, M->np, bp, M->mp, done, s);
+        return launch_gemm_fix8(a.q, a.qscale, M->np, (const int8_t*)w.Zhi, bp, w.V, M->np, M->np, bp, M->mp, sp.done,
+                                s);
     return launch_gemm_tn(a.prec, a.p32, a.hi, a.lo, a.ldp, w.Z, w.Zhi, w.Zlo, a.ldq, w.V, M->np, 0, M->np, bp, M->mp, 1,
-                          a.big, done, nullptr, s);
+                          a.big, sp.done, nullptr, s);
+}
+
+// Zpart = A * X of one iteration: the dense product (K split over `splits`) and, with
+// `gather`, the sparse gather beside it (one of them runs, on the device flag); sparse
+// matrices: the CSR gather with the centring terms, into partial 0.
+hipError_t launch_ax(const Matrix* M, const spca_settings& st, Workspace& w, int splits, int big, int bp, int batch,
+                     long zsplit, SpState sp, bool gather, hipStream_t s) {
+    if (M->sparse) {
+        hipError_t e = launch_sp_scalars(w.X, M->np, M->n, M->mu, w.corr, bp, batch, sp, s);
+        if (e != hipSuccess) return e;
+        return launch_csr_ax(sparse_of(M), w.X, M->np, w.Zpart, M->mp, M->mp, w.corr, bp, batch, sp, s);
+    }
+    const float* P1_32 = M->a32_rm;
+    const uint16_t* P1_hi = st.precision == 1 ? M->hi_rm : M->h_rm;
+    const long ld1p = st.precision ? M->mp : M->np, ld1q = st.precision ? bp : M->np;
+    hipError_t e = launch_gemm_tn(st.precision, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart, M->mp,
+                                  zsplit, M->mp, bp, M->np, splits, big, sp.done,
+                                  gather ? sp.done + kSparseFlags : nullptr, s);
+    if (e == hipSuccess && gather)
+        e = launch_sparse_ax(M->a32_cm, M->mp, M->mp, M->n, w.X, w.np, w.Zpart, M->mp, batch, sp, s);
+    return e;
 }
 
 // Starting points with global indices shift .. shift+cnt-1 (n floats each) into out.
@@ -735,10 +783,12 @@ int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& s
 int run_ev(spca_ctx ctx, hipStream_t s, Matrix* M, const float* X, int ldx, int batch, int formulation, double* ev) {
     double* scratch = nullptr;
     uint64_t sb = 0;
-    int rc = dev_alloc(ctx, &scratch, ev_scratch_doubles(M->m, M->n, batch), &sb);
+    const int l1 = spca_is_l1_variance(formulation);
+    int rc = dev_alloc(ctx, &scratch, M->sparse ? csr_ev_scratch_doubles(M->m, batch) : ev_scratch_doubles(M->m, M->n, batch),
+                       &sb);
     if (rc) return rc;
-    hipError_t e = launch_ev_fp64(M->a32_cm, M->mp, M->m, M->n, X, ldx, batch, spca_is_l1_variance(formulation), ev,
-                                  scratch, s);
+    hipError_t e = M->sparse ? launch_csr_ev(sparse_of(M), X, ldx, batch, l1, ev, scratch, s)
+                             : launch_ev_fp64(M->a32_cm, M->mp, M->m, M->n, X, ldx, batch, l1, ev, scratch, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     dev_free(ctx, scratch, sb);
     if (e != hipSuccess) return fail(ctx, SPCA_EDEVICE, std::string("explained variance: ") + hipGetErrorString(e));
@@ -997,6 +1047,10 @@ Plan peek_plan(spca_ctx ctx, int precision, int mp, int np, int bp, bool allow_b
 
 int plan_for(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, int bp, bool allow_big, const float* X0,
              Plan* out, Workspace* wsp = nullptr) {
+    if (M->sparse) {  // sparse products: no K split, no tiles to choose
+        *out = Plan();
+        return SPCA_OK;
+    }
     const PlanKey key = plan_key(st.precision, M->mp, M->np, bp, allow_big);
     allow_big = key.allow_big != 0;
     {
@@ -1070,10 +1124,14 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
         return fail(ctx, SPCA_EINVAL, "start must be a spca_start value");
     if (st.start == SPCA_START_EXPLICIT && !X0) return fail(ctx, SPCA_EINVAL, "X0 is required for explicit starts");
     if (!x_out) return fail(ctx, SPCA_EINVAL, "x_out is required");
+    if (M->sparse && st.precision != SPCA_PREC_FP32)
+        return fail(ctx, SPCA_EINVAL, "a sparse (CSC) matrix is solved in FP32 only");
+    if (M->sparse && st.resolve_iters > 0)
+        return fail(ctx, SPCA_EINVAL, "the FP64 support re-solve needs a dense matrix");
     unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
     // the fixed-point product, and the FP32 row-major copy for the exact signs of z
     if (uses_sign_products(st)) need |= SPCA_STORE_FIX8 | SPCA_STORE_FP32;
-    int rc = add_storage(ctx, M, need);
+    int rc = M->sparse ? SPCA_OK : add_storage(ctx, M, need);
     if (rc) return rc;
     *nb = spca_settings_check(&st, (unsigned)M->n);
     if (st.batch > kMaxSparseGroups * 256)
@@ -1114,12 +1172,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     const SignRefine rf = sign_refine(M, st, w);
     const Operand xop = x_operand(w, st.precision);
     const Operand zop = z_operand(w, atz.zprec);
-    const float* P1_32 = M->a32_rm;
-    const uint16_t* P1_hi = st.precision == 1 ? M->hi_rm : M->h_rm;
-    const uint16_t* P1_lo = M->lo_rm;
     const long zsplit = (long)M->mp * bp;
-    // leading dimensions of the product operands: FP32 row strides, 16-bit (K-blocked) row counts
-    const long ld1p = st.precision ? M->mp : M->np, ld1q = st.precision ? bp : M->np;
 
     StepArgs sa;
     sa.formulation = st.formulation;
@@ -1212,8 +1265,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(xop, st.precision, w.np, defl->idx, defl->val, defl->off, defl->ncomp,
                                                   batch, sp, s));
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
-                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done, nullptr, s));
+            HIP_TRY(ctx, launch_ax(M, st, w, splits, plan.big, bp, batch, zsplit, sp, false, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1222,7 +1274,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_atz(atz, M, w, bp, sp.done, s));
+            HIP_TRY(ctx, launch_atz(atz, M, w, bp, batch, sp, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1292,7 +1344,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     } else
     for (int bt = 0; bt < total_batches; bt++) {
         const int shift = bt * batch;
-        const int cap = sparse_cap(st, n);
+        const int cap = M->sparse ? 0 : sparse_cap(st, n);
         make_starts(st, n, X0, shift, batch, hX.data());
         if ((rc = load_batch(ctx, s, w, st, n, batch, hX.data(), shift))) return rc;
         HIP_TRY(ctx, hipEventRecord(loop_ev[0], s));
@@ -1302,11 +1354,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(xop, st.precision, w.np, defl->idx, defl->val, defl->off, defl->ncomp,
                                                   batch, sp, s));
-            // one of the two A*X forms runs; the other sees the device flag and returns
-            HIP_TRY(ctx, launch_gemm_tn(st.precision, P1_32, P1_hi, P1_lo, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart,
-                                        M->mp, zsplit, M->mp, bp, M->np, splits, plan.big, sp.done,
-                                        sp.done + kSparseFlags, s));
-            HIP_TRY(ctx, launch_sparse_ax(M->a32_cm, M->mp, M->mp, n, w.X, w.np, w.Zpart, M->mp, batch, sp, s));
+            // one of the two dense A*X forms runs; the other sees the device flag and returns
+            HIP_TRY(ctx, launch_ax(M, st, w, splits, plan.big, bp, batch, zsplit, sp, true, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1315,7 +1364,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_atz(atz, M, w, bp, sp.done, s));
+            HIP_TRY(ctx, launch_atz(atz, M, w, bp, batch, sp, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1353,6 +1402,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
 int resolve_impl(spca_ctx ctx, Engine& E, Matrix* M, int formulation, const float* x, int max_iter, double tol,
                  double* y_out, double* ev_out, int* iters_out) {
     if (!valid_formulation(formulation) || !x || max_iter < 1) return fail(ctx, SPCA_EINVAL, "bad re-solve arguments");
+    if (M->sparse) return fail(ctx, SPCA_EINVAL, "the FP64 support re-solve needs a dense matrix");
     const int n = M->n, m = M->m;
     std::vector<int> idx;
     double nx = 0;
@@ -1481,6 +1531,10 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         if (!r.M || !J.settings) {
             J.status = SPCA_EINVAL;
             return cleanup(fail(ctx, SPCA_EINVAL, "fused job " + std::to_string(k) + ": bad matrix or settings"));
+        }
+        if (r.M->sparse) {
+            J.status = SPCA_EINVAL;
+            return cleanup(fail(ctx, SPCA_EINVAL, "fused job " + std::to_string(k) + ": sparse matrices solve alone"));
         }
         int nb = 0;
         if ((rc = prepare_settings(ctx, r.M, J.settings, J.X0, J.x_out, &r.st, &nb))) {
@@ -1892,11 +1946,77 @@ int spca_matrix_download(spca_ctx ctx, spca_matrix mat, float* A, int ld) {
     auto it = ctx->mats.find(mat);
     if (it == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
     Matrix* M = it->second;
+    if (M->sparse) return fail(ctx, SPCA_EINVAL, "a sparse matrix has no dense copy to download");
     if (ld < M->m) return fail(ctx, SPCA_EINVAL, "ld < m");
     (void)hipSetDevice(ctx->device);
     HIP_TRY(ctx, hipMemcpy2DAsync(A, sizeof(float) * ld, M->a32_cm, sizeof(float) * M->mp, sizeof(float) * M->m,
                                   M->n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    return SPCA_OK;
+}
+
+int spca_matrix_upload_csc(spca_ctx ctx, const int* col_ptr, const int* row_idx, const float* vals, int m, int n,
+                           const float* col_means, const float* row_means, uint64_t key, spca_matrix* out) {
+    if (!ctx) return SPCA_EINVAL;
+    if (!col_ptr || !out || m <= 0 || n <= 0) return fail(ctx, SPCA_EINVAL, "bad sparse matrix arguments");
+    const int nnz = col_ptr[n];
+    if (col_ptr[0] != 0 || nnz < 0 || (nnz > 0 && (!row_idx || !vals)))
+        return fail(ctx, SPCA_EINVAL, "col_ptr must start at 0 and end at nnz");
+    for (int j = 0; j < n; j++)
+        if (col_ptr[j + 1] < col_ptr[j]) return fail(ctx, SPCA_EINVAL, "col_ptr must be non-decreasing");
+    for (int t = 0; t < nnz; t++)
+        if (row_idx[t] < 0 || row_idx[t] >= m) return fail(ctx, SPCA_EINVAL, "row index out of range");
+    // CSR copy for A*X: counting sort by row, columns in order within a row
+    std::vector<int> rp((size_t)m + 1, 0), ci((size_t)std::max(nnz, 1));
+    std::vector<float> rv((size_t)std::max(nnz, 1));
+    for (int t = 0; t < nnz; t++) rp[(size_t)row_idx[t] + 1]++;
+    for (int i = 0; i < m; i++) rp[(size_t)i + 1] += rp[i];
+    {
+        std::vector<int> fill(rp.begin(), rp.end() - 1);
+        for (int j = 0; j < n; j++)
+            for (int t = col_ptr[j]; t < col_ptr[j + 1]; t++) {
+                const int d = fill[row_idx[t]]++;
+                ci[d] = j;
+                rv[d] = vals[t];
+            }
+    }
+    std::lock_guard<std::mutex> g(ctx->mu);
+    (void)hipSetDevice(ctx->device);
+    Matrix* M = new Matrix();
+    M->sparse = true;
+    M->m = m;
+    M->n = n;
+    M->mp = round_up(m, kPadMN);
+    M->np = round_up(n, kPadMN);
+    M->nnz = nnz;
+    M->key = key;
+    M->refs = 1;
+    M->storage = SPCA_STORE_FP32;
+    const size_t cnt = (size_t)std::max(nnz, 1);
+    int rc;
+    if ((rc = dev_alloc(ctx, &M->cptr, (size_t)n + 1, &M->bytes)) || (rc = dev_alloc(ctx, &M->ridx, cnt, &M->bytes)) ||
+        (rc = dev_alloc(ctx, &M->cval, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->rptr, (size_t)m + 1, &M->bytes)) ||
+        (rc = dev_alloc(ctx, &M->cidx, cnt, &M->bytes)) || (rc = dev_alloc(ctx, &M->rval, cnt, &M->bytes)) ||
+        (col_means && (rc = dev_alloc(ctx, &M->mu, (size_t)n, &M->bytes))) ||
+        (row_means && (rc = dev_alloc(ctx, &M->rmu, (size_t)m, &M->bytes)))) {
+        free_matrix(ctx, M);
+        return rc;
+    }
+    hipError_t e = hipMemcpy(M->cptr, col_ptr, sizeof(int) * ((size_t)n + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz > 0) e = hipMemcpy(M->ridx, row_idx, sizeof(int) * nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz > 0) e = hipMemcpy(M->cval, vals, sizeof(float) * nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(M->rptr, rp.data(), sizeof(int) * ((size_t)m + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz > 0) e = hipMemcpy(M->cidx, ci.data(), sizeof(int) * nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess && nnz > 0) e = hipMemcpy(M->rval, rv.data(), sizeof(float) * nnz, hipMemcpyHostToDevice);
+    if (e == hipSuccess && col_means) e = hipMemcpy(M->mu, col_means, sizeof(float) * n, hipMemcpyHostToDevice);
+    if (e == hipSuccess && row_means) e = hipMemcpy(M->rmu, row_means, sizeof(float) * m, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        free_matrix(ctx, M);
+        return fail(ctx, SPCA_EDEVICE, std::string("sparse matrix upload: ") + hipGetErrorString(e));
+    }
+    const uint64_t h = ctx->next_handle++;
+    ctx->mats[h] = M;
+    *out = h;
     return SPCA_OK;
 }
 
@@ -1906,6 +2026,7 @@ int spca_matrix_add_storage(spca_ctx ctx, spca_matrix mat, unsigned storage) {
     auto itm = ctx->mats.find(mat);
     if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
     (void)hipSetDevice(ctx->device);
+    if (itm->second->sparse) return fail(ctx, SPCA_EINVAL, "a sparse matrix takes no dense storage forms");
     return add_storage(ctx, itm->second, storage);
 }
 

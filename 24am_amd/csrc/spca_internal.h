@@ -236,6 +236,36 @@ hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const floa
 hipError_t launch_deflate(Operand op, int precision, long ld, const int* idx, const float* val, const int* off,
                           int ncomp, int batch, SpState sp, hipStream_t st);
 
+// Sparse A (reference sparse_PCA_solver_CSC, c++/src/gpower/sparse_PCA_solver_for_CSC.h:112-495):
+// CSC arrays and their CSR copy (zero-based), with optional implicit centring
+// A_c = A - 1 mu^T - r 1^T (mu: column means, the reference's doMean; r: row means, doRowMean).
+struct SparseMat {
+    const int* cptr;   // n + 1 column pointers
+    const int* ridx;   // row of each nonzero, columns in order
+    const float* cval;
+    const int* rptr;   // m + 1 row pointers
+    const int* cidx;   // column of each nonzero, rows in order
+    const float* rval;
+    const float* mu;   // n column means or null
+    const float* rmu;  // m row means or null
+    int m, n;
+};
+// Per-SP scalars of a batch of columns (ld ldx, len entries): out[b] = w . x_b (0 when w is
+// null), out[bp + b] = 1 . x_b, in FP64 in a fixed order; inactive SPs are skipped.
+hipError_t launch_sp_scalars(const float* X, long ldx, int len, const float* w, double* out, int bp, int batch,
+                             SpState sp, hipStream_t st);
+// Z[:, b] = A_c x_b (rows >= m zero) with xs = launch_sp_scalars(X, mu): FP64 sums per row.
+hipError_t launch_csr_ax(SparseMat A, const float* X, int ldx, float* Z, int ldz, int mp, const double* xs, int bp,
+                         int batch, SpState sp, hipStream_t st);
+// V[:, b] = A_c^T z_b (rows >= n zero) with zs = launch_sp_scalars(Z, r): FP64 sums per column.
+hipError_t launch_csc_atz(SparseMat A, const float* Z, int ldz, float* V, int ldv, int np, const double* zs, int bp,
+                          int batch, SpState sp, hipStream_t st);
+// FP64 explained variance |A_c xhat|_2^2 or |A_c xhat|_1 of `batch` columns of X (ld ldx).
+// scratch: csr_ev_scratch_doubles(m, batch) doubles.
+size_t csr_ev_scratch_doubles(int m, int batch);
+hipError_t launch_csr_ev(SparseMat A, const float* X, int ldx, int batch, int l1, double* ev, double* scratch,
+                         hipStream_t st);
+
 // Write the operand forms of X from its FP32 values (for starting points).
 hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st);
 

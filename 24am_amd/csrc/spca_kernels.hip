@@ -2122,6 +2122,107 @@ hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipS
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- sparse (CSC) matrices
+// Reference sparse_PCA_solver_CSC (c++/src/gpower/sparse_PCA_solver_for_CSC.h:112-495) with
+// its implicit centring: z = A x - (mu . x) 1 - (1 . x) r (:215-250) and v = A^T z - (1 . z) mu -
+// (r . z) 1 (:287-316).  Each product is a gather per SP: thread per row (CSR) or per column
+// (CSC), the SP's column of X or Z read where the nonzeros point, FP64 sums in the stored
+// order, one rounding to FP32.
+__global__ __launch_bounds__(256) void sp_scalars_kernel(const float* __restrict__ X, long ldx, int len,
+                                                         const float* __restrict__ w, double* __restrict__ out, int bp,
+                                                         SpState sp) {
+    const int b = blockIdx.x;
+    if (*sp.done || !sp.active[b]) return;
+    __shared__ double red[4];
+    const float* x = X + (long)b * ldx;
+    double d = 0, s = 0;
+    for (int k = threadIdx.x; k < len; k += 256) {
+        const double v = (double)x[k];
+        s += v;
+        if (w) d += (double)w[k] * v;
+    }
+    d = block_sum<256>(d, red);
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) {
+        out[b] = d;
+        out[bp + b] = s;
+    }
+}
+hipError_t launch_sp_scalars(const float* X, long ldx, int len, const float* w, double* out, int bp, int batch,
+                             SpState sp, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    sp_scalars_kernel<<<batch, 256, 0, st>>>(X, ldx, len, w, out, bp, sp);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void csr_ax_kernel(SparseMat A, const float* __restrict__ X, int ldx,
+                                                     float* __restrict__ Z, int ldz, int mp,
+                                                     const double* __restrict__ xs, int bp, SpState sp) {
+    const int b = blockIdx.y;
+    if (*sp.done || !sp.active[b]) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= mp) return;
+    double acc = 0;
+    if (i < A.m) {
+        const float* x = X + (long)b * ldx;
+        for (int t = A.rptr[i]; t < A.rptr[i + 1]; t++) acc += (double)A.rval[t] * (double)x[A.cidx[t]];
+        acc -= xs[b];
+        if (A.rmu) acc -= (double)A.rmu[i] * xs[bp + b];
+    }
+    Z[(long)b * ldz + i] = (float)acc;
+}
+hipError_t launch_csr_ax(SparseMat A, const float* X, int ldx, float* Z, int ldz, int mp, const double* xs, int bp,
+                         int batch, SpState sp, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    csr_ax_kernel<<<dim3((mp + 255) / 256, batch), 256, 0, st>>>(A, X, ldx, Z, ldz, mp, xs, bp, sp);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void csc_atz_kernel(SparseMat A, const float* __restrict__ Z, int ldz,
+                                                      float* __restrict__ V, int ldv, int np,
+                                                      const double* __restrict__ zs, int bp, SpState sp) {
+    const int b = blockIdx.y;
+    if (*sp.done || !sp.active[b]) return;
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= np) return;
+    double acc = 0;
+    if (j < A.n) {
+        const float* z = Z + (long)b * ldz;
+        for (int t = A.cptr[j]; t < A.cptr[j + 1]; t++) acc += (double)A.cval[t] * (double)z[A.ridx[t]];
+        if (A.mu) acc -= (double)A.mu[j] * zs[bp + b];
+        acc -= zs[b];
+    }
+    V[(long)b * ldv + j] = (float)acc;
+}
+hipError_t launch_csc_atz(SparseMat A, const float* Z, int ldz, float* V, int ldv, int np, const double* zs, int bp,
+                          int batch, SpState sp, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    csc_atz_kernel<<<dim3((np + 255) / 256, batch), 256, 0, st>>>(A, Z, ldz, V, ldv, np, zs, bp, sp);
+    return hipGetLastError();
+}
+
+// FP64 explained variance over the CSR rows: row block partials per SP, summed in block order.
+__global__ __launch_bounds__(256) void csr_ev_partial_kernel(SparseMat A, const float* __restrict__ X, int ldx,
+                                                             const double* __restrict__ xs, int bp,
+                                                             const double* __restrict__ invn, int l1,
+                                                             double* __restrict__ part, int nblk) {
+    __shared__ double red[4];
+    const int b = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+    double e = 0;
+    if (i < A.m) {
+        const float* x = X + (long)b * ldx;
+        double acc = 0;
+        for (int t = A.rptr[i]; t < A.rptr[i + 1]; t++) acc += (double)A.rval[t] * (double)x[A.cidx[t]];
+        acc -= xs[b];
+        if (A.rmu) acc -= (double)A.rmu[i] * xs[bp + b];
+        acc *= invn[b];
+        e = l1 ? fabs(acc) : acc * acc;
+    }
+    e = block_sum<256>(e, red);
+    if (threadIdx.x == 0) part[(long)b * nblk + blockIdx.x] = e;
+}
+size_t csr_ev_scratch_doubles(int m, int batch) { return (size_t)batch * (3 + (m + 255) / 256); }
+
 // ---------------------------------------------------------------- deflation
 // Reference SparseDeflationCollection::deflateV (c++/src/gpower/sparse_PCA_solver_for_CSC.h:
 // 69-92): for each earlier component p in order (its nonzeros idx[off[c] .. off[c+1]) with
@@ -2271,6 +2372,40 @@ hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* 
                                                                        part, ldp);
     ev_reduce_kernel<<<dim3(rrb, batch), kEvThreads, 0, st>>>(part, chunks, ldp, m, l1, rpart);
     ev_final_kernel<<<(batch + 255) / 256, 256, 0, st>>>(rpart, rrb, batch, ev);
+    return hipGetLastError();
+}
+
+// mu . x_b and 1 . x_b for the explained variance (every column, FP64, fixed order)
+__global__ __launch_bounds__(256) void ev_scalars_kernel(const float* __restrict__ X, int ldx, int n,
+                                                         const float* __restrict__ w, double* __restrict__ out,
+                                                         int batch) {
+    __shared__ double red[4];
+    const int b = blockIdx.x;
+    const float* x = X + (long)b * ldx;
+    double d = 0, s = 0;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const double v = (double)x[k];
+        s += v;
+        if (w) d += (double)w[k] * v;
+    }
+    d = block_sum<256>(d, red);
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) {
+        out[b] = d;
+        out[batch + b] = s;
+    }
+}
+hipError_t launch_csr_ev(SparseMat A, const float* X, int ldx, int batch, int l1, double* ev, double* scratch,
+                         hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    const int nblk = (A.m + 255) / 256;
+    double* xs = scratch;               // 2 * batch
+    double* invn = xs + 2 * batch;      // batch
+    double* part = invn + batch;        // nblk * batch
+    ev_scalars_kernel<<<batch, 256, 0, st>>>(X, ldx, A.n, A.mu, xs, batch);
+    ev_norm_kernel<<<batch, 1024, 0, st>>>(X, ldx, A.n, invn);
+    csr_ev_partial_kernel<<<dim3(nblk, batch), 256, 0, st>>>(A, X, ldx, xs, batch, invn, l1, part, nblk);
+    ev_final_kernel<<<(batch + 255) / 256, 256, 0, st>>>(part, nblk, batch, ev);
     return hipGetLastError();
 }
 

@@ -36,7 +36,7 @@ EXPORTS = [
     "spca_is_constrained", "spca_is_l1_constrained", "spca_is_l1_penalized", "spca_is_l1_variance",
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
     "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan", "spca_matrix_generate",
-    "spca_matrix_download", "spca_ctx_memory", "spca_solve_components",
+    "spca_matrix_download", "spca_ctx_memory", "spca_solve_components", "spca_matrix_upload_csc",
 ]
 FUSED_GRAPH = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
@@ -122,6 +122,7 @@ def lib():
             "spca_matrix_bytes": (I, [P, U64, ctypes.POINTER(U64)]),
             "spca_solve": (I, [P, U64, ctypes.POINTER(_Settings), P, P, ctypes.POINTER(_Stats), P, P]),
             "spca_solve_components": (I, [P, U64, ctypes.POINTER(_Settings), P, I, P, P]),
+            "spca_matrix_upload_csc": (I, [P, P, P, P, I, I, P, P, U64, ctypes.POINTER(U64)]),
             "spca_dense_solver_device": (I, [P, U, U, P, P, ctypes.POINTER(_Settings), ctypes.POINTER(_Stats), U, U]),
             "spca_dense_solver_host": (I, [P, P, I, P, U, U, ctypes.POINTER(_Settings), ctypes.POINTER(_Stats),
                                            ctypes.POINTER(ctypes.c_double)]),
@@ -370,6 +371,23 @@ class Context:
         h = ctypes.c_uint64()
         _check(self._p, lib().spca_matrix_upload(self._p, A.ctypes.data_as(ctypes.c_void_p), m, n, m, storage, key,
                                                  ctypes.byref(h)))
+        return Matrix(self, h.value, m, n)
+
+    def upload_csc(self, col_ptr, row_idx, vals, m, n, col_means=None, row_means=None, key=0):
+        """Sparse A from CSC arrays (zero-based), centred implicitly by the optional column and
+        row means: A_c = A - 1 col_means^T - row_means 1^T (the reference sparse solver's
+        doMean / doRowMean).  Solves run in FP32 on the CSR/CSC copies."""
+        cp = np.ascontiguousarray(col_ptr, dtype=np.int32)
+        ri = np.ascontiguousarray(row_idx, dtype=np.int32)
+        va = np.ascontiguousarray(vals, dtype=np.float32)
+        if cp.size != n + 1 or ri.size != cp[-1] or va.size != cp[-1]:
+            raise ValueError("col_ptr must have n+1 entries ending at len(row_idx) == len(vals)")
+        cm = None if col_means is None else np.ascontiguousarray(col_means, dtype=np.float32)
+        rm = None if row_means is None else np.ascontiguousarray(row_means, dtype=np.float32)
+        ptr = lambda a: None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+        h = ctypes.c_uint64()
+        _check(self._p, lib().spca_matrix_upload_csc(self._p, ptr(cp), ptr(ri), ptr(va), int(m), int(n), ptr(cm),
+                                                     ptr(rm), int(key), ctypes.byref(h)))
         return Matrix(self, h.value, m, n)
 
 

@@ -174,6 +174,16 @@ const char* spca_last_error(spca_ctx ctx);
 const char* spca_version(void);
 
 /* ---- resident matrices ---------------------------------------------------- */
+/* A sparse A (host CSC arrays, zero-based: col_ptr n+1, row_idx and vals col_ptr[n]), the
+ * input of the reference's sparse solver (sparse_PCA_solver_CSC,
+ * c++/src/gpower/sparse_PCA_solver_for_CSC.h:112-495).  col_means (n) and row_means (m), each
+ * optional (NULL), centre A implicitly, A_c = A - 1 col_means^T - row_means 1^T, as its doMean
+ * and doRowMean do (:215-250, :287-316): A is never densified.  The device keeps the CSC
+ * arrays and a CSR copy; products are gathers with FP64 sums.  Solves of a sparse matrix run
+ * in FP32 (spca_solve, spca_solve_components, the request server); the FP64 re-solve, fused
+ * solves, dense storage forms and download return SPCA_EINVAL.  `key` is kept, not shared. */
+int spca_matrix_upload_csc(spca_ctx ctx, const int* col_ptr, const int* row_idx, const float* vals, int m, int n,
+                           const float* col_means, const float* row_means, uint64_t key, spca_matrix* out);
 /* Copy A (host, FP32, m x n, column major, leading dimension ld) to the device and build
  * the operand layouts for the storage forms in `storage` (SPCA_STORE_* mask).  `key` lets
  * later uploads of the same data share the resident copy (0 = no sharing).
