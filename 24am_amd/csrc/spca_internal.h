@@ -19,8 +19,8 @@ constexpr int kGemmBN = 128;
 constexpr int kGemmBK = 32;
 
 // Select/formulation-step kernel: one workgroup of kSelThreads per starting point.
-constexpr int kSelThreads = 1024;
-constexpr int kSelMaxEPT = 64;  // elements per thread held in registers (n <= 65536)
+constexpr int kSelThreads = 512;
+constexpr int kSelDepth = 16;  // loads in flight per thread in each of the step kernel's column passes
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }

@@ -928,7 +928,8 @@ hipError_t launch_gemm_fix8(const int8_t* P, const float* qscale, long ldp, cons
 // ---------------------------------------------------------------- sparse A*X
 // z_j = A x_j over the nonzeros of x_j, for iterates that are sparse (after a constrained or
 // penalised step): workgroup (row block, SP) walks the SP's nonzeros in index order (ballot
-// compaction of 256-index slabs) and accumulates its 1024 rows in FP64 from the FP32 master
+// compaction of 2048-index slabs, eight loads of x in flight per thread) and accumulates its
+// 1024 rows in FP64 from the FP32 master
 // A (column major, so each nonzero's column segment is one coalesced read; 4 rows per thread
 // as one 16-byte load, eight nonzeros' loads in flight).  Runs for the SPs of each group of
 // 256 whose flag the previous iteration's convergence kernel set; the dense product's tiles
@@ -939,31 +940,47 @@ __device__ __forceinline__ void sparse_ax_body(int rb, int j, const float* __res
                                                const float* __restrict__ X, int ldx, float* __restrict__ Z, int ldz,
                                                int mp, SpState sp) {
     if (sp.done[0] || !sp.done[kSparseFlags + (j >> 8)] || !sp.active[j]) return;
-    __shared__ int idxs[256];
-    __shared__ float vals[256];
-    __shared__ int wcount[4];
-    const float* x = X + (long)j * ldx;
+    constexpr int kE = 8;                 // x entries per thread per slab, loads in flight
+    constexpr int kSlab = 256 * kE;       // indices per slab
+    __shared__ int idxs[kSlab];
+    __shared__ float vals[kSlab];
+    __shared__ int wcount[kE][4];
+    const __amdgpu_buffer_rsrc_t xr = col_rsrc(X + (long)j * ldx, (uint32_t)n * 4u);  // reads 0 past n
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // each thread owns 4 consecutive rows: one 16-byte load per nonzero column
     const int row0 = rb * kSpRows + 4 * threadIdx.x;
     const bool in = row0 < mp;  // mp and the row block are multiples of 4 rows
     double acc[4] = {0, 0, 0, 0};
-    for (int k0 = 0; k0 < n; k0 += 256) {
-        const int k = k0 + threadIdx.x;
-        const float xv = k < n ? x[k] : 0.f;
-        const bool nzf = xv != 0.f;
-        const uint64_t bal = __ballot(nzf);
-        if (lane == 0) wcount[w] = __popcll(bal);
-        __syncthreads();
-        int off = 0, cnt = 0;
-        for (int i = 0; i < 4; i++) {
-            if (i < w) off += wcount[i];
-            cnt += wcount[i];
+    for (int k0 = 0; k0 < n; k0 += kSlab) {
+        // entry e of this thread is index k0 + e*256 + tid: coalesced loads, and the slab's index
+        // order is (e, wave, lane), so a nonzero's list position is the count of nonzeros in
+        // earlier rows e, then earlier waves of row e, then earlier lanes
+        float xv[kE];
+#pragma unroll
+        for (int e = 0; e < kE; e++)
+            xv[e] = __uint_as_float(
+                __builtin_amdgcn_raw_buffer_load_b32(xr, (uint32_t)(k0 + e * 256 + (int)threadIdx.x) * 4u, 0, 0));
+        uint64_t bal[kE];
+#pragma unroll
+        for (int e = 0; e < kE; e++) {
+            bal[e] = __ballot(xv[e] != 0.f);
+            if (lane == 0) wcount[e][w] = __popcll(bal[e]);
         }
-        if (nzf) {
-            const int at = off + __popcll(bal & ((1ull << lane) - 1ull));
-            idxs[at] = k;
-            vals[at] = xv;
+        __syncthreads();
+        int cnt = 0;
+#pragma unroll
+        for (int e = 0; e < kE; e++) {
+            int before = cnt;  // nonzeros ahead of (e, w, lane 0)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                before += i < w ? wcount[e][i] : 0;
+                cnt += wcount[e][i];
+            }
+            if (xv[e] != 0.f) {
+                const int at = before + __popcll(bal[e] & ((1ull << lane) - 1ull));
+                idxs[at] = k0 + e * 256 + (int)threadIdx.x;
+                vals[at] = xv[e];
+            }
         }
         __syncthreads();
         if (in) {
@@ -1004,6 +1021,7 @@ __global__ __launch_bounds__(256) void sparse_ax_kernel(const float* __restrict_
 hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z, int ldz,
                             int batch, SpState sp, hipStream_t st) {
     if (batch <= 0) return hipSuccess;
+    if ((long)n * 4 + 8192 > 0xffffffffL) return hipErrorInvalidValue;  // 32-bit offsets into x
     sparse_ax_kernel<<<dim3((mp + kSpRows - 1) / kSpRows, batch), 256, 0, st>>>(Acm, lda, n, X, ldx, Z, ldz, mp, sp);
     return hipGetLastError();
 }
@@ -1256,13 +1274,29 @@ __device__ void scan_hist(SelShared& sh, bool l1c, uint32_t rank, uint32_t cA, u
     if (lane == 0 && fmax < 0) sh.ires[0] = -1;
 }
 
+// One workgroup of NT threads per starting point.  Each pass streams the column through the
+// bounded view, DEPTH loads in flight per thread; every pass visits a thread's elements in
+// the same order, and reductions have a fixed order, so results are bitwise reproducible.
+template <int NT, int DEPTH>
 __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Operand xop, StepArgs a, SpState sp) {
     if (*sp.done || !sp.active[j]) return;
     __shared__ SelShared sh;
     const int tid = threadIdx.x;
-    const int nq = (a.n + kSelThreads - 1) / kSelThreads;
+    const int nq = (a.n + NT - 1) / NT;
     const __amdgpu_buffer_rsrc_t vr = col_rsrc(V + (long)j * a.ldv, (uint32_t)a.n * 4u);
-#define LDV(q) __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vr, tid * 4, (q) * kSelThreads * 4, 0))
+#define LDV(q) __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vr, tid * 4 + (q) * NT * 4, 0, 0))
+    // fn(q, v) for this thread's elements q = 0 .. nq-1 (element index tid + q * NT; the last
+    // row's entries past n read 0)
+    auto sweep = [&](auto&& fn) {
+        for (int q0 = 0; q0 < nq; q0 += DEPTH) {
+            float vv[DEPTH];
+#pragma unroll
+            for (int u = 0; u < DEPTH; u++) vv[u] = LDV(q0 + u);
+#pragma unroll
+            for (int u = 0; u < DEPTH; u++)
+                if (q0 + u < nq) fn(q0 + u, vv[u]);
+        }
+    };
     const int f = a.formulation;
     float fval;
     // element transform applied when the iterate is written: keep |v| >= tau (L0),
@@ -1276,40 +1310,27 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     if (f < 4) {
         // ---- constrained: select, then x <- x/|x|_2
         float mloc = 0.f;
-        for (int q0 = 0; q0 < nq; q0 += 8) {
-            float vv[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);  // past n the bounded view reads 0
-#pragma unroll
-            for (int u = 0; u < 8; u++) mloc = fmaxf(mloc, fabsf(vv[u]));
-        }
-        const float M = block_max<kSelThreads>(mloc, sh.red);
+        sweep([&](int, float v) { mloc = fmaxf(mloc, fabsf(v)); });
+        const float M = block_max<NT>(mloc, sh.red);
         const float scale = M > 0.f ? (float)kBins / M : 0.f;
         const double invM = M > 0.f ? 1.0 / (double)M : 0.0;
         const double Md = (double)M;
-        for (int b = tid; b < kBins; b += kSelThreads) {
+        for (int b = tid; b < kBins; b += NT) {
             sh.cnt[b] = 0;
             sh.s1[b] = 0;
             sh.s2[b] = 0;
         }
         __syncthreads();
-        for (int q0 = 0; q0 < nq; q0 += 8) {
-            float vv[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                if (q0 + u >= nq) break;
-                const float av = fabsf(vv[u]);
-                const int b = lin_bin(av, scale);
-                atomicAdd(&sh.cnt[b], 1u);
-                if (l1c && av > 0.f) {
-                    const double w = (double)av * invM;
-                    atomicAdd((unsigned long long*)&sh.s1[b], (unsigned long long)(w * kFix + 0.5));
-                    atomicAdd((unsigned long long*)&sh.s2[b], (unsigned long long)(w * w * kFix + 0.5));
-                }
+        sweep([&](int, float v) {
+            const float av = fabsf(v);
+            const int b = lin_bin(av, scale);
+            atomicAdd(&sh.cnt[b], 1u);
+            if (l1c && av > 0.f) {
+                const double w = (double)av * invM;
+                atomicAdd((unsigned long long*)&sh.s1[b], (unsigned long long)(w * kFix + 0.5));
+                atomicAdd((unsigned long long*)&sh.s2[b], (unsigned long long)(w * w * kFix + 0.5));
             }
-        }
+        });
         __syncthreads();
         const double wd = (double)(float)sqrt((double)a.s);
         if (tid < 64)
@@ -1330,38 +1351,32 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
             if (tid == 0) sh.ncand = 0;
             __syncthreads();
             double c_loc = 0, s1_loc = 0, s2_loc = 0;
-            for (int q0 = 0; q0 < nq; q0 += 8) {
-                float vv[8];
-#pragma unroll
-                for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);
-#pragma unroll
-                for (int u = 0; u < 8; u++) {
-                    const int idx = tid + (q0 + u) * kSelThreads;
-                    if (idx >= a.n) continue;
-                    const float av = fabsf(vv[u]);
-                    const int b = lin_bin(av, scale);
-                    if (b > bstar) {
-                        c_loc += 1.0;
-                        s1_loc += (double)av;
-                        s2_loc += (double)av * (double)av;
-                    } else if (b == bstar) {
-                        const int slot = atomicAdd(&sh.ncand, 1);
-                        if (slot < kCand) sh.cand[slot] = ((uint64_t)__float_as_uint(av) << 32) | (uint32_t)idx;
-                    }
+            sweep([&](int q, float v) {
+                const int idx = tid + q * NT;
+                if (idx >= a.n) return;
+                const float av = fabsf(v);
+                const int b = lin_bin(av, scale);
+                if (b > bstar) {
+                    c_loc += 1.0;
+                    s1_loc += (double)av;
+                    s2_loc += (double)av * (double)av;
+                } else if (b == bstar) {
+                    const int slot = atomicAdd(&sh.ncand, 1);
+                    if (slot < kCand) sh.cand[slot] = ((uint64_t)__float_as_uint(av) << 32) | (uint32_t)idx;
                 }
-            }
-            const double cAd = block_sum<kSelThreads>(c_loc, sh.red);
-            const double S1A = block_sum<kSelThreads>(s1_loc, sh.red);
-            const double S2A = block_sum<kSelThreads>(s2_loc, sh.red);
+            });
+            const double cAd = block_sum<NT>(c_loc, sh.red);
+            const double S1A = block_sum<NT>(s1_loc, sh.red);
+            const double S2A = block_sum<NT>(s2_loc, sh.red);
             const int nc = min(sh.ncand, kCand);
             // bitonic sort of the candidates, descending by (|v|, index): a fixed order
             int npow = 1;
             while (npow < nc) npow <<= 1;
-            for (int i = nc + tid; i < npow; i += kSelThreads) sh.cand[i] = 0ull;
+            for (int i = nc + tid; i < npow; i += NT) sh.cand[i] = 0ull;
             __syncthreads();
             for (int k = 2; k <= npow; k <<= 1)
                 for (int jj = k >> 1; jj > 0; jj >>= 1) {
-                    for (int i = tid; i < npow; i += kSelThreads) {
+                    for (int i = tid; i < npow; i += NT) {
                         const int ixj = i ^ jj;
                         if (ixj > i) {
                             const uint64_t x0 = sh.cand[i], x1 = sh.cand[ixj];
@@ -1388,19 +1403,19 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 int i0 = 0, i1 = 0;  // band = sorted candidates [i0, i1)
                 if (try_band) {
                     double above = 0, inband = 0;
-                    for (int i = tid; i < nc; i += kSelThreads) {
+                    for (int i = tid; i < nc; i += NT) {
                         const float av = __uint_as_float((uint32_t)(sh.cand[i] >> 32));
                         above += av > hi ? 1.0 : 0.0;
                         inband += (av <= hi && av >= lo) ? 1.0 : 0.0;
                     }
-                    i0 = (int)block_sum<kSelThreads>(above, sh.red);
-                    i1 = i0 + (int)block_sum<kSelThreads>(inband, sh.red);
+                    i0 = (int)block_sum<NT>(above, sh.red);
+                    i1 = i0 + (int)block_sum<NT>(inband, sh.red);
                 }
                 const int nb = i1 - i0;
                 if (tid == 0) sh.nband = 0;
                 if (try_band && nb >= 2 && nb <= kBandMax) {
-                    const int w = tid >> 6, lane = tid & 63;
-                    if (w < nb) {
+                    const int lane = tid & 63;
+                    for (int w = tid >> 6; w < nb; w += NT / 64) {
                         const int idx = (int)(uint32_t)sh.cand[i0 + w];
                         const f4* ac = (const f4*)(a.acm + (long)idx * a.mp);
                         const f4* zc = (const f4*)(a.zf + (long)j * a.mp);
@@ -1443,12 +1458,12 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 __syncthreads();
                 const int nband = sh.nband;
                 double cs = 0;
-                for (int i = tid; i < nc; i += kSelThreads) {
+                for (int i = tid; i < nc; i += NT) {
                     const float av = __uint_as_float((uint32_t)(sh.cand[i] >> 32));
                     if (nband ? i < i0 : !(av < tau)) cs += (double)av * (double)av;
                 }
                 if (tid < nband && sh.band_keep[tid]) cs += (double)sh.band_val[tid] * (double)sh.band_val[tid];
-                norm2 = S2A + block_sum<kSelThreads>(cs, sh.red);
+                norm2 = S2A + block_sum<NT>(cs, sh.red);
             } else {
                 // ---- L1 constrained: the first candidate (descending) where the reference scan
                 // triggers, r(a) >= w over the elements above it; one wave walks 64 at a time
@@ -1507,11 +1522,11 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                     norm2 = S2 - 2.0 * lf * S1 + c * lf * lf;
                 } else {
                     double acc = 0;
-                    for (int q = 0; q < nq; q++) {
-                        const float t2 = fabsf(LDV(q)) - lam;
+                    sweep([&](int, float v) {
+                        const float t2 = fabsf(v) - lam;
                         if (t2 > 0.f) acc += (double)t2 * (double)t2;
-                    }
-                    norm2 = block_sum<kSelThreads>(acc, sh.red);
+                    });
+                    norm2 = block_sum<NT>(acc, sh.red);
                 }
             }
         } else {
@@ -1530,8 +1545,8 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                     sh.s2[tid] = 0;
                 }
                 __syncthreads();
-                for (int q = 0; q < nq; q++) {
-                    const float av = fabsf(LDV(q));
+                sweep([&](int, float v) {
+                    const float av = fabsf(v);
                     const uint32_t key = __float_as_uint(av);
                     if (lin_bin(av, scale) == bstar && (pass == 0 || (key >> (shift + 8)) == prefix)) {
                         const int d = (key >> shift) & 255;
@@ -1542,7 +1557,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                             atomicAdd((unsigned long long*)&sh.s2[d], (unsigned long long)(u * u * kFix + 0.5));
                         }
                     }
-                }
+                });
                 __syncthreads();
                 if (tid < 64)
                     scan_hist<256>(sh, l1c, rank, cA, s1A, s2A, Md, wd, [&](int d, double& l) {
@@ -1566,37 +1581,36 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         __syncthreads();
         if (!l1c) {
             tau = bstar >= 0 ? __uint_as_float(prefix) : 0.f;
-            for (int q = 0; q < nq; q++) {
-                const float v = LDV(q);
+            sweep([&](int, float v) {
                 if (!(fabsf(v) < tau)) norm2 += (double)v * (double)v;
-            }
+            });
         } else {
             double c = 0, S1 = 0, S2 = 0;
-            for (int q = 0; q < nq; q++) {
-                const float av = fabsf(LDV(q));
+            sweep([&](int q, float v) {
+                const float av = fabsf(v);
                 const int b = lin_bin(av, scale);
                 const bool in = b > bstar || (b == bstar && __float_as_uint(av) > prefix);
-                if (in && (tid + q * kSelThreads) < a.n) {
+                if (in && (tid + q * NT) < a.n) {
                     c += 1.0;
                     S1 += (double)av;
                     S2 += (double)av * (double)av;
                 }
-            }
-            c = block_sum<kSelThreads>(c, sh.red);
-            S1 = block_sum<kSelThreads>(S1, sh.red);
-            S2 = block_sum<kSelThreads>(S2, sh.red);
+            });
+            c = block_sum<NT>(c, sh.red);
+            S1 = block_sum<NT>(S1, sh.red);
+            S2 = block_sum<NT>(S2, sh.red);
             const double w2 = wd * wd;
             const double qa = c * (c - w2);
             const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
             const double qc = S1 * S1 - w2 * S2;
             lam = (float)((-qb - sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa));
             if (!(lam > 0.f)) lam = 0.f;  // inactive constraint, as in the fast path
-            for (int q = 0; q < nq; q++) {
-                const float t2 = fabsf(LDV(q)) - lam;
+            sweep([&](int, float v) {
+                const float t2 = fabsf(v) - lam;
                 if (t2 > 0.f) norm2 += (double)t2 * (double)t2;
-            }
+            });
         }
-        norm2 = block_sum<kSelThreads>(norm2, sh.red);
+        norm2 = block_sum<NT>(norm2, sh.red);
         }
         const float nf = (float)sqrt(norm2);
         scale_out = 1.0f / nf;
@@ -1624,28 +1638,23 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
             if (tid == 0) sh.ncand = 0;
             __syncthreads();
         }
-        for (int q0 = 0; q0 < nq; q0 += 8) {
-            float vv[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);  // zeros past n contribute nothing
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const double c = contrib(vv[u]);
-                acc += c;
-                const float av = fabsf(vv[u]);
-                if (band && av >= lo && av <= hi) {
-                    acc_band += c;
-                    const int slot = atomicAdd(&sh.ncand, 1);
-                    if (slot < kBandMax) sh.band_idx[slot] = tid + (q0 + u) * kSelThreads;
-                }
+        sweep([&](int q, float v) {
+            if (tid + q * NT >= a.n) return;  // the last row's entries past n
+            const double c = contrib(v);
+            acc += c;
+            const float av = fabsf(v);
+            if (band && av >= lo && av <= hi) {
+                acc_band += c;
+                const int slot = atomicAdd(&sh.ncand, 1);
+                if (slot < kBandMax) sh.band_idx[slot] = tid + q * NT;
             }
-        }
-        double tot = block_sum<kSelThreads>(acc, sh.red);
+        });
+        double tot = block_sum<NT>(acc, sh.red);
         if (band) {
-            const double tb = block_sum<kSelThreads>(acc_band, sh.red);  // its barriers publish ncand
+            const double tb = block_sum<NT>(acc_band, sh.red);  // its barriers publish ncand
             const int nb = sh.ncand;
             if (nb >= 1 && nb <= kBandMax) {
-                const int w = tid >> 6, lane = tid & 63;
+                const int lane = tid & 63;
                 if (tid == 0)  // index order, so the replaced sum below has a fixed order
                     for (int p = 1; p < nb; p++) {
                         const int ix = sh.band_idx[p];
@@ -1657,7 +1666,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                         sh.band_idx[q + 1] = ix;
                     }
                 __syncthreads();
-                if (w < nb) {
+                for (int w = tid >> 6; w < nb; w += NT / 64) {
                     const f4* ac = (const f4*)(a.acm + (long)sh.band_idx[w] * a.mp);
                     const f4* zc = (const f4*)(a.zf + (long)j * a.mp);
                     double d = 0;
@@ -1703,20 +1712,12 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         // take its decision and value
         const int nband = ((f < 4 && !l1c) || f == 4 || f == 6) ? sh.nband : 0;
         const float blo = nband ? sh.band_lo : 0.f, bhi = nband ? sh.band_hi : -1.f;
-        for (int q0 = 0; q0 < nq; q0 += 8) {
-            float vv[8];
-#pragma unroll
-            for (int u = 0; u < 8; u++) vv[u] = LDV(q0 + u);
-#pragma unroll
-            for (int u = 0; u < 8; u++) {
-                const int q = q0 + u;
-                if (q >= nq) break;
-                const float v = vv[u];
+        sweep([&](int q, float v) {
                 float x;
                 if (f < 4 && !l1c) {
                     const float av = fabsf(v);
                     if (av >= blo && av <= bhi) {
-                        const int i = tid + q * kSelThreads;
+                        const int i = tid + q * NT;
                         x = 0.f;
                         for (int p = 0; p < nband; p++)
                             if (sh.band_idx[p] == i) x = sh.band_keep[p] ? sh.band_val[p] : 0.f;
@@ -1726,7 +1727,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 }
                 else if (nband && fabsf(v) >= blo && fabsf(v) <= bhi) {
                     // penalised, inside the re-evaluated band: the exact value decides
-                    const int i = tid + q * kSelThreads;
+                    const int i = tid + q * NT;
                     x = 0.f;
                     for (int p = 0; p < nband; p++)
                         if (sh.band_idx[p] == i && sh.band_keep[p]) {
@@ -1743,9 +1744,9 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                     x = (v * v - lam) > 0.f ? v : 0.f;
                 }
                 if (f < 4) x = x * scale_out;
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, tid * 4, q * kSelThreads * 4, 0);
-                nz += (x != 0.f && (tid + q * kSelThreads) < a.n) ? 1.0 : 0.0;
-                const int i = tid + q * kSelThreads;
+                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, tid * 4 + q * NT * 4, 0, 0);
+                nz += (x != 0.f && (tid + q * NT) < a.n) ? 1.0 : 0.0;
+                const int i = tid + q * NT;
                 if (a.precision && i < a.n) {
                     const uint32_t bo = (uint32_t)(blk_index(j, i, xop.bp) * 2);
                     if (a.precision == 1) {
@@ -1756,11 +1757,10 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                         __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, bo, 0, 0);
                     }
                 }
-            }
-        }
+        });
     }
 #undef LDV
-    nz = block_sum<kSelThreads>(nz, sh.red);
+    nz = block_sum<NT>(nz, sh.red);
     if (tid == 0) {
         const float prev = sp.val[j];
         sp.err[j] = fabsf(fval - prev) / prev;  // computeTheError (termination_criteria.h:29-36)
@@ -1769,14 +1769,14 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     }
 }
 
-__global__ __launch_bounds__(kSelThreads) void step_kernel(const float* __restrict__ V, Operand xop, StepArgs a,
-                                                           SpState sp) {
-    step_body(blockIdx.x, V, xop, a, sp);
+template <int NT, int DEPTH>
+__global__ __launch_bounds__(NT) void step_kernel(const float* __restrict__ V, Operand xop, StepArgs a, SpState sp) {
+    step_body<NT, DEPTH>(blockIdx.x, V, xop, a, sp);
 }
 hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st) {
     if (a.n <= 0 || (long)a.n * 4 > 0x7fffffffL) return hipErrorInvalidValue;
     if (a.precision && (long)a.ldv * xop.bp * 2 > 0xffffffffL) return hipErrorInvalidValue;  // 32-bit store offsets
-    step_kernel<<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp);
+    step_kernel<kSelThreads, kSelDepth><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp);
     return hipGetLastError();
 }
 
@@ -1956,7 +1956,7 @@ __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __res
 __global__ __launch_bounds__(kSelThreads) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {
     const int k = find_job<2>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
-    step_body(blockIdx.x - J.c0, J.V, J.xop, J.sa, J.sp);
+    step_body<kSelThreads, kSelDepth>(blockIdx.x - J.c0, J.V, J.xop, J.sa, J.sp);
 }
 
 __global__ __launch_bounds__(256) void fused_converge_kernel(const FusedJob* __restrict__ jobs, int* all_done) {
