@@ -76,6 +76,14 @@ struct Workspace {
     int* istate = nullptr;    // iters, nnz, active (bp ints each), done (state_flag_ints(bp))
     double* ev = nullptr;
     double* corr = nullptr;  // per-SP centring scalars of sparse products (4 x bp)
+    // union A*X (sparse_products = 2): index flags and list (np ints each), stat (4 ints),
+    // Au = A[:, U] (mp x ldu, row major) and Xu = X[U, :] (bp x ldu); ldu = 0: not allocated
+    int* uflag = nullptr;
+    int* uidx = nullptr;
+    int* ustat = nullptr;
+    float* Au = nullptr;
+    float* Xu = nullptr;
+    int ldu = 0;
     uint64_t bytes = 0;
 };
 
@@ -494,7 +502,8 @@ int make_generated(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const
 
 void free_ws(spca_ctx ctx, Workspace& w, hipStream_t s) {
     (void)hipStreamSynchronize(s);
-    void* ptrs[] = {w.X, w.Xhi, w.Xlo, w.Zpart, w.Z, w.Zhi, w.Zlo, w.V, w.fstate, w.istate, w.ev, w.corr};
+    void* ptrs[] = {w.X,      w.Xhi,   w.Xlo,    w.Zpart, w.Z,  w.Zhi, w.Zlo, w.V,
+                    w.fstate, w.istate, w.ev,    w.corr,  w.uflag, w.uidx, w.ustat, w.Au, w.Xu};
     for (void* p : ptrs) raw_free(ctx, p);
     w = Workspace();
 }
@@ -536,13 +545,34 @@ int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp,
     }
     return SPCA_OK;
 }
+// The union A*X buffers of a workspace, for union sizes up to ldu (a multiple of 32 * splits).
+int ensure_union(spca_ctx ctx, Workspace& w, hipStream_t s, int ldu) {
+    if (w.ldu >= ldu) return SPCA_OK;
+    if (w.ldu > 0 && hipStreamSynchronize(s) != hipSuccess) return SPCA_EDEVICE;  // old buffers idle
+    int* ptrs_i[] = {w.uflag, w.uidx, w.ustat};
+    for (int* p : ptrs_i) raw_free(ctx, p);
+    raw_free(ctx, w.Au);
+    raw_free(ctx, w.Xu);
+    w.uflag = w.uidx = w.ustat = nullptr;
+    w.Au = w.Xu = nullptr;
+    w.ldu = 0;
+    int rc;
+    if ((rc = dev_alloc(ctx, &w.uflag, (size_t)w.np, &w.bytes)) || (rc = dev_alloc(ctx, &w.uidx, (size_t)w.np, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.ustat, (size_t)4, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.Au, (size_t)w.mp * ldu, &w.bytes)) ||
+        (rc = dev_alloc(ctx, &w.Xu, (size_t)w.bp * ldu, &w.bytes)))
+        return rc;
+    w.ldu = ldu;
+    return SPCA_OK;
+}
+
 int ensure_workspace(spca_ctx ctx, Engine& E, int np, int mp, int bp, int splits, std::string* err = nullptr) {
     return ensure_ws(ctx, E.ws, E.stream, np, mp, bp, splits, err);
 }
 
 int engine_init(Engine& E) {
     if (hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking) != hipSuccess) return SPCA_EDEVICE;
-    if (hipHostMalloc((void**)&E.h_done, sizeof(int) * 4, hipHostMallocDefault) != hipSuccess) return SPCA_ENOMEM;
+    if (hipHostMalloc((void**)&E.h_done, sizeof(int) * 8, hipHostMallocDefault) != hipSuccess) return SPCA_ENOMEM;
     if (hipEventCreate(&E.loop_ev[0]) != hipSuccess || hipEventCreate(&E.loop_ev[1]) != hipSuccess)
         return SPCA_EDEVICE;
     return SPCA_OK;
@@ -616,6 +646,15 @@ void reference_starting_point(float* out, unsigned n, unsigned s, bool constrain
 int sparse_cap(const spca_settings& st, int n) {
     if (!st.sparse_products) return 0;
     const int div = st.precision == SPCA_PREC_FP32 ? 64 : st.precision == SPCA_PREC_BF16X3 ? 192 : 384;
+    return std::max(1, n / div);
+}
+
+// Largest union of supports |U| at which A*X is the union product (sparse_products = 2): its
+// FP32 product costs ~|U|/n of the dense FP32 one, so up to n/4 in FP32 solves, and n/12 and
+// n/24 against the ~3x and ~6x faster dense bf16x3 and fp16 products.
+int union_cap(const spca_settings& st, int n) {
+    if (st.sparse_products != 2) return 0;
+    const int div = st.precision == SPCA_PREC_FP32 ? 4 : st.precision == SPCA_PREC_BF16X3 ? 12 : 24;
     return std::max(1, n / div);
 }
 
@@ -801,6 +840,7 @@ struct Results {
     int best_sp = -1;
     unsigned total_it = 0;
     unsigned sparse_it = 0;
+    unsigned union_it = 0;
     unsigned long long sp_its = 0;
     double last_err = 0;
     double ev_best = 0;
@@ -899,6 +939,7 @@ void finish_solve(const Results& R, int n, const spca_settings& st, float* x_out
         stats->sp_iterations = R.sp_its;
         stats->ev_fp64 = R.ev_best;
         stats->sparse_iterations = R.sparse_it;
+        stats->union_iterations = R.union_it;
     }
 }
 
@@ -1341,12 +1382,17 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
                 per_sp[k].ev_fp64 = (k == best_sp) ? evb : std::nan("");
             }
         (void)X_final;  // slots are reused: final iterates per SP are not kept in OTF mode
-    } else
+    } else {
+    // union A*X (sparse_products = 2, dense matrix, no deflation): after each iteration the
+    // host reads |U| and the next A*X is the FP32 product over U when |U| <= ucap
+    const int ucap = (M->sparse || defl) ? 0 : union_cap(st, n);
+    if (ucap > 0 && (rc = ensure_union(ctx, w, s, round_up(ucap, kGemmBK * splits)))) return rc;
     for (int bt = 0; bt < total_batches; bt++) {
         const int shift = bt * batch;
         const int cap = M->sparse ? 0 : sparse_cap(st, n);
         make_starts(st, n, X0, shift, batch, hX.data());
         if ((rc = load_batch(ctx, s, w, st, n, batch, hX.data(), shift))) return rc;
+        int uK = 0;  // |U| for the next iteration's A*X (0: not known or empty; the dense product)
         HIP_TRY(ctx, hipEventRecord(loop_ev[0], s));
         for (int it = 0; it < st.max_iter; it++) {
             hipEvent_t e0, e1;
@@ -1354,12 +1400,27 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(xop, st.precision, w.np, defl->idx, defl->val, defl->off, defl->ncomp,
                                                   batch, sp, s));
-            // one of the two dense A*X forms runs; the other sees the device flag and returns
-            HIP_TRY(ctx, launch_ax(M, st, w, splits, plan.big, bp, batch, zsplit, sp, true, s));
+            int ax_splits = splits;
+            if (uK > 0 && uK <= ucap) {
+                // K split so each split is at least 512 deep; the sparse gather keeps its groups
+                int us = 1;
+                while (us * 2 <= splits && round_up(std::max(uK, 1), kGemmBK * us * 2) / (us * 2) >= 512) us *= 2;
+                const int kpad = round_up(std::max(uK, 1), kGemmBK * us);
+                HIP_TRY(ctx, launch_union_gather(M->a32_cm, M->mp, M->mp, w.X, w.np, bp, w.uidx, uK, kpad, w.Au, w.Xu,
+                                                 w.ldu, s));
+                HIP_TRY(ctx, launch_gemm_tn(0, w.Au, nullptr, nullptr, w.ldu, w.Xu, nullptr, nullptr, w.ldu, w.Zpart,
+                                            M->mp, zsplit, M->mp, bp, kpad, us, 0, sp.done, sp.done + kSparseFlags, s));
+                HIP_TRY(ctx, launch_sparse_ax(M->a32_cm, M->mp, M->mp, M->n, w.X, w.np, w.Zpart, M->mp, batch, sp, s));
+                ax_splits = us;
+                R.union_it++;
+            } else {
+                // one of the two dense A*X forms runs; the other sees the device flag and returns
+                HIP_TRY(ctx, launch_ax(M, st, w, splits, plan.big, bp, batch, zsplit, sp, true, s));
+            }
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
-            HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, atz.zprec, batch,
+            HIP_TRY(ctx, launch_prep_z(w.Zpart, ax_splits, zsplit, M->mp, M->mp, zop, st.formulation, atz.zprec, batch,
                                        sp, rf, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
@@ -1373,8 +1434,18 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
             HIP_TRY(ctx, launch_converge(sp, batch, it, st.tolerance, st.max_iter,
                                          st.termination == SPCA_TERM_PER_SP, cap, s));
+            if (ucap > 0 && it + 1 < st.max_iter)
+                HIP_TRY(ctx, launch_union_list(w.X, w.np, n, batch, sp, w.uflag, w.uidx, w.ustat, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
-            if ((it + 1) % check_every == 0 || it + 1 == st.max_iter) {
+            if (ucap > 0 && it + 1 < st.max_iter) {
+                // stat = {|U|, done[0], done[1]}: the host decides the next A*X form
+                HIP_TRY(ctx, hipMemcpyAsync(E.h_done + 4, w.ustat, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
+                HIP_TRY(ctx, hipStreamSynchronize(s));
+                pt.flush();
+                if (E.h_done[5]) break;
+                if (E.cancel && E.cancel->load()) return fail(ctx, SPCA_ECANCELLED, "cancelled");
+                uK = E.h_done[4];
+            } else if ((it + 1) % check_every == 0 || it + 1 == st.max_iter) {
                 HIP_TRY(ctx, hipMemcpyAsync(E.h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
                 HIP_TRY(ctx, hipStreamSynchronize(s));
                 pt.flush();
@@ -1393,6 +1464,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         if ((rc = collect_batch(ctx, s, M, w, st, shift, batch, per_sp != nullptr || stats != nullptr, per_sp, X_final,
                                 R)))
             return rc;
+    }
     }
     finish_solve(R, n, st, x_out, stats, loop_ms, pt.ms, t_wall0);
     return SPCA_OK;

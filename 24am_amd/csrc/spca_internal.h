@@ -225,6 +225,13 @@ hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs
 // z_j = A x_j over the nonzeros of each active x_j (FP64 accumulation, fixed order) into Z
 // (ld ldz), when sp.done[2] is set; else it returns at once.  A: FP32 column major, ld lda.
 constexpr int kSparseRows = 1024;  // rows per workgroup of the sparse product
+// union A*X: flags and the ascending list U of the indices nonzero in any iterate whose A*X is
+// dense next iteration, stat = {|U|, done[0], done[1]}; then Au = A[:, U] (row major) and
+// Xu = X[U, :] (row j = SP j), both zero-padded to kpad columns, for the FP32 product
+hipError_t launch_union_list(const float* X, int ldx, int n, int batch, SpState sp, int* flag, int* U, int* stat,
+                             hipStream_t st);
+hipError_t launch_union_gather(const float* Acm, int lda, int mp, const float* X, int ldx, int rows, const int* U,
+                               int K, int kpad, float* Au, float* Xu, int ldu, hipStream_t st);
 hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z, int ldz,
                             int batch, SpState sp, hipStream_t st);
 

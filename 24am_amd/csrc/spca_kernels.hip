@@ -1179,6 +1179,106 @@ hipError_t launch_row_max(const float* Acm, int mp, int np, float* out, hipStrea
     return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- union A*X
+// z_j = A[:, U] x_j[U] for every SP of a batch, where U is the union of the supports of the
+// iterates whose A*X would otherwise be the dense product: the K dimension of the dense FP32
+// product shrinks from n to |U|.  Iterates that converge to a few supports share most of U
+// (L1-constrained iterates hold thousands of nonzeros each, too many for the per-SP gather).
+// Sums over U in FP32 on the MFMA, like the dense FP32 product, over the nonzero terms only.
+
+// flag[k] = 1 iff x_j[k] != 0 for an SP j that iterates next and whose group is not gathered
+__global__ __launch_bounds__(256) void union_flags_kernel(const float* __restrict__ X, int ldx, int n, int batch,
+                                                          SpState sp, int* __restrict__ flag) {
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    // 16 SPs' loads in flight per step (a loop that waited on each load would serialise them)
+    int any = 0;
+    for (int j0 = 0; j0 < batch && !any; j0 += 16) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; u++) v[u] = j0 + u < batch ? X[(long)(j0 + u) * ldx + k] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 16; u++) {
+            const int j = j0 + u;
+            // NaN counts as nonzero, as in the dense product
+            if (j < batch && v[u] != 0.f && sp.active[j] && !sp.done[kSparseFlags + (j >> 8)]) any = 1;
+        }
+    }
+    flag[k] = any;
+}
+
+// U = the flagged indices in ascending order; stat = {|U|, done[0], done[1]} for the host.
+// One workgroup: thread t counts and then lists the flags of its own contiguous index range.
+__global__ __launch_bounds__(1024) void union_list_kernel(const int* __restrict__ flag, int n, int* __restrict__ U,
+                                                          const int* __restrict__ done, int* __restrict__ stat) {
+    __shared__ int wsum[16];
+    const int per = (n + 1023) / 1024;
+    const int k0 = threadIdx.x * per, k1 = min(n, k0 + per);
+    int c = 0;
+    for (int k = k0; k < k1; k++) c += flag[k];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int incl = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) wsum[w] = incl;
+    __syncthreads();
+    int pos = incl - c, tot = 0;
+    for (int i = 0; i < 16; i++) {
+        pos += i < w ? wsum[i] : 0;
+        tot += wsum[i];
+    }
+    for (int k = k0; k < k1; k++)
+        if (flag[k]) U[pos++] = k;
+    if (threadIdx.x == 0) {
+        stat[0] = tot;
+        stat[1] = done[0];
+        stat[2] = done[1];
+    }
+}
+
+// Xu row j (ld ldu) = x_j[U[0..K)], zero on [K, kpad); rows j < rows
+__global__ __launch_bounds__(256) void union_gather_x_kernel(const float* __restrict__ X, int ldx,
+                                                             const int* __restrict__ U, int K, int kpad,
+                                                             float* __restrict__ Xu, int ldu) {
+    const int j = blockIdx.y;
+    for (int k = blockIdx.x * 256 + threadIdx.x; k < kpad; k += gridDim.x * 256)
+        Xu[(long)j * ldu + k] = k < K ? X[(long)j * ldx + U[k]] : 0.f;
+}
+
+// Au (row major, ld ldu) = A[:, U[0..K)], zero on columns [K, kpad): 32x32 tiles transposed
+// through LDS, so both the column reads of A (column major, ld lda) and the row writes coalesce
+__global__ __launch_bounds__(256) void union_gather_a_kernel(const float* __restrict__ Acm, int lda,
+                                                             const int* __restrict__ U, int K,
+                                                             float* __restrict__ Au, int ldu) {
+    __shared__ float t[32][33];
+    const int k0 = blockIdx.x * 32, i0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int k = k0 + r;
+        t[r][tx] = k < K ? Acm[(long)U[k] * lda + i0 + tx] : 0.f;
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) Au[(long)(i0 + r) * ldu + k0 + tx] = t[tx][r];
+}
+
+hipError_t launch_union_list(const float* X, int ldx, int n, int batch, SpState sp, int* flag, int* U, int* stat,
+                             hipStream_t st) {
+    if (n <= 0 || batch <= 0) return hipErrorInvalidValue;
+    union_flags_kernel<<<(n + 255) / 256, 256, 0, st>>>(X, ldx, n, batch, sp, flag);
+    union_list_kernel<<<1, 1024, 0, st>>>(flag, n, U, sp.done, stat);
+    return hipGetLastError();
+}
+
+hipError_t launch_union_gather(const float* Acm, int lda, int mp, const float* X, int ldx, int rows, const int* U,
+                               int K, int kpad, float* Au, float* Xu, int ldu, hipStream_t st) {
+    if (K < 0 || K > kpad || kpad % 32 || kpad > ldu || mp % 32 || rows <= 0) return hipErrorInvalidValue;
+    union_gather_x_kernel<<<dim3(std::min((kpad + 255) / 256, 64), rows), 256, 0, st>>>(X, ldx, U, K, kpad, Xu, ldu);
+    union_gather_a_kernel<<<dim3(kpad / 32, mp / 32), 256, 0, st>>>(Acm, lda, U, K, Au, ldu);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- formulation step
 // Exact selection without sorting.  A first pass bins |v| linearly over [0, max] (2048
 // bins, LDS histogram), which spreads the LDS atomics; the bin holding the answer is then

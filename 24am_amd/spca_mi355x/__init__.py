@@ -81,6 +81,7 @@ class _Stats(ctypes.Structure):
         ("step_ms", ctypes.c_double), ("sp_iterations", ctypes.c_ulonglong),
         ("ev_resolve_fp64", ctypes.c_double), ("resolve_iterations", ctypes.c_int),
         ("sparse_iterations", ctypes.c_uint),
+        ("union_iterations", ctypes.c_uint),
     ]
 
 
@@ -179,7 +180,9 @@ class OptimizationSettings:
         self.resolve_iters = 0  # > 0: FP64 re-solve on the support of the result
         self.resolve_tol = 1e-12
         self.seed = 0  # START_GAUSSIAN: key of the device-generated starting points
-        self.sparse_products = 1  # gather A*X over the nonzeros of sparse iterates (0: always dense)
+        # 1: gather A*X over the nonzeros of sparse iterates; 2: also the FP32 product over the union
+        # of the other iterates' supports when it is small (batch mode); 0: always dense
+        self.sparse_products = 1
         # L1 variance, FP32 and bf16x3: A^T sgn(z) from the exact bf16 split of A (0: the precision's product)
         self.exact_sign_products = 1
         self.exact_ties = 1
@@ -237,6 +240,7 @@ class OptimizationStatistics:
         self.ev_resolve_fp64 = None
         self.resolve_iterations = 0
         self.sparse_iterations = 0
+        self.union_iterations = 0
 
     def _fill(self, s, per_sp=None):
         self.it, self.fval, self.error = s.it, s.fval, s.error
@@ -247,6 +251,7 @@ class OptimizationStatistics:
         self.sp_iterations = s.sp_iterations
         self.ev_resolve_fp64, self.resolve_iterations = s.ev_resolve_fp64, s.resolve_iterations
         self.sparse_iterations = s.sparse_iterations
+        self.union_iterations = s.union_iterations
         if per_sp is not None:
             self.values = np.array([r.value for r in per_sp], dtype=np.float32)
             self.iters = np.array([r.iters for r in per_sp], dtype=np.int32)

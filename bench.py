@@ -29,7 +29,7 @@ HBM_PEAK = 8000.0  # GB/s
 # HBM bytes per launch of the product kernels at this workload, from rocprofv3 --pmc
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs, scaled by the counters' reading of a 1 GiB
 # copy kernel (profiles/gemm_products_spiked4096x65536_B1024_hbm_counters.md).
-MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (6.34 + 0.25) * 2**30, ("fp32", "gemm_ax"): (8.16 + 0.125) * 2**30,
+MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (6.33 + 0.25) * 2**30, ("fp32", "gemm_ax"): (8.16 + 0.125) * 2**30,
                     ("bf16x3", "gemm_atz"): (1.50 + 0.26) * 2**30, ("bf16x3", "gemm_ax"): (1.50 + 0.13) * 2**30}
 
 
@@ -266,6 +266,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--formulations", default="0,1,2,3,4,5,6,7")
     ap.add_argument("--precisions", default="fp32,fp32_dense,bf16x3,fp16")
+    ap.add_argument("--sparse-products", type=int, default=1,
+                    help="A*X over sparse iterates in the fp32/bf16x3/fp16 forms: 1 per-SP gather, 2 also the "
+                         "product over the union of supports, 0 dense")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-jobs", type=int, default=64, help="uneven fused-job workload size (0: skip)")
     ap.add_argument("--uneven-iters", type=int, default=20)
@@ -306,7 +309,7 @@ def main():
                 st = S.OptimizationSettings(formulation=f, constraintParameter=SPARSITY, penaltyParameter=gam.get(f, 0.0),
                                             tolerance=0.0, maximumIterations=args.warmup, totalStartingPoints=BATCH,
                                             batchSize=BATCH, precision=pcode[p], check_every=1000,
-                                            sparse_products=0 if p == "fp32_dense" else 1,
+                                            sparse_products=0 if p == "fp32_dense" else args.sparse_products,
                                             exact_sign_products=0 if p == "fp32_dense" else 1)
                 if args.warmup > 0:
                     M.solve(st, X0=X0, per_sp=False)
@@ -316,6 +319,7 @@ def main():
                 sp_it = BATCH * stats.it
                 per[str(f)] = {"sp_it_per_s": sp_it / stats.totalTrueComputationTime,
                                "sparse_iterations": stats.sparse_iterations,
+                               "union_iterations": stats.union_iterations,
                                "ms_per_step": stats.totalTrueComputationTime * 1e3 / stats.it,
                                "gemm_ax_ms": stats.gemm_ax_ms / stats.it, "gemm_atz_ms": stats.gemm_atz_ms / stats.it,
                                "step_ms": stats.step_ms / stats.it, "fval": stats.fval, "ev_fp64": stats.ev_fp64}

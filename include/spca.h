@@ -111,7 +111,11 @@ typedef struct spca_settings {
                                  (FP64 accumulation) for each group of 256 SPs of a batch once
                                  every iterating SP of the group has at most n/64 (FP32), n/192
                                  (bf16x3) or n/384 (fp16) of them; 0: always the dense product
-                                 of `precision` */
+                                 of `precision`; 2: as 1, and in batch mode (use_otf = 0) on a
+                                 dense matrix without deflation the other groups' A*X is the FP32
+                                 product over U, the union of their iterates' supports, whenever
+                                 |U| <= n/4 (FP32), n/12 (bf16x3) or n/24 (fp16): the host reads
+                                 |U| after every iteration.  Fused solves treat 2 as 1. */
     int exact_sign_products;  /* 1 (default): for the L1-variance formulations (z = sgn(A x) in
                                  {-1, 0, +1}), FP32 and bf16x3 solves compute A^T z from A in
                                  25-bit fixed point (SPCA_STORE_FIX8, added on first use) on the
@@ -144,6 +148,8 @@ typedef struct spca_stats {
     double ev_resolve_fp64;   /* explained variance after the FP64 support re-solve (0 if not run) */
     int resolve_iterations;   /* iterations the re-solve ran */
     unsigned sparse_iterations; /* iterations whose A*X was gathered over the iterates' nonzeros */
+    unsigned union_iterations;  /* iterations whose A*X was the product over the union of supports
+                                   (sparse_products = 2) */
 } spca_stats;
 
 /* per starting point result (reference values / iters / cardinalities) */

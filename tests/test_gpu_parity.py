@@ -773,3 +773,37 @@ def test_exact_ties_at_the_penalty_threshold(ctx, f):
     print(f"case f={f}: planted threshold ties decided off FP64: exact ties {wrong[1]}, plain {wrong[0]} of 24",
           flush=True)
     assert wrong[0] >= 1
+
+
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3])
+@pytest.mark.parametrize("f", range(8))
+def test_union_products_match_dense(ctx, f, prec):
+    """sparse_products = 2: once the union U of the iterates' supports is small, A*X is the FP32
+    product over U (the per-SP gather keeps the groups it covers).  The solve agrees with the
+    all-dense solve of the same precision; with a strong planted loading the L1-constrained
+    iterates (too many nonzeros for the gather) share a small U, so the union product runs."""
+    A, _ = spiked(200, 1000, k=20, strength=8.0, seed=41)
+    X0 = unit_gaussian_starts(1000, 64, seed=42)
+    M = ctx.upload(A, storage=S.STORE_FP32 | S.STORE_BF16X3)
+    try:
+        out = {}
+        for sparse in (0, 2):
+            st = _settings(f, 30, prec, 64)
+            st.sparse_products = sparse
+            stats = S.OptimizationStatistics()
+            x = M.solve(st, X0=X0, stats=stats)
+            out[sparse] = (x, stats)
+        (xd, sd), (xu, su) = out[0], out[2]
+        print(f"f={f} prec={prec}: iterations {su.it}, union {su.union_iterations}, gathered "
+              f"{su.sparse_iterations}", flush=True)
+        assert sd.union_iterations == 0 and sd.sparse_iterations == 0
+        assert su.union_iterations + su.sparse_iterations <= su.it - 1  # the first A*X is dense
+        if f in (2, 3):
+            assert su.union_iterations > 0
+        rel = np.abs(su.values - sd.values) / np.abs(sd.values)
+        tol = 1e-5 if prec == S.PREC_FP32 else 1e-4
+        assert np.mean(rel <= tol) >= 0.95, np.sort(rel)[-4:]
+        assert abs(su.fval - sd.fval) <= tol * abs(sd.fval)
+        assert set(np.nonzero(xu)[0]) == set(np.nonzero(xd)[0])
+    finally:
+        M.release()
