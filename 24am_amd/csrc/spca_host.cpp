@@ -2292,7 +2292,12 @@ int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, co
         J->mp = M->mp;
         J->bp = round_up(chk.batch, kPadB);
         J->splits = peek_plan(ctx, J->st.precision, M->mp, M->np, J->bp, true).splits;
-        const uint64_t need_bytes = workspace_bytes(J->np, J->mp, J->bp, J->splits);
+        uint64_t need_bytes = workspace_bytes(J->np, J->mp, J->bp, J->splits);
+        const int ucap = (M->sparse || chk.use_otf) ? 0 : union_cap(chk, M->n);
+        if (ucap > 0) {  // the union A*X buffers (flags, list, Au, Xu)
+            const uint64_t ldu = (uint64_t)round_up(ucap, kGemmBK * J->splits);
+            need_bytes += ((uint64_t)2 * J->np + 4 + ((uint64_t)J->mp + J->bp) * ldu) * 4;
+        }
         if (need_bytes + M->bytes > ctx->budget) {
             delete J;
             return fail(ctx, SPCA_EBUDGET, "job workspace exceeds the HBM budget");
