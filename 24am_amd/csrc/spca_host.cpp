@@ -750,6 +750,9 @@ hipError_t launch_atz(const AtzForm& a, const Matrix* M, Workspace& w, int bp, i
     if (a.prec == 4)
         return launch_gemm_fix8(a.q, a.qscale, M->np, (const int8_t*)w.Zhi, bp, w.V, M->np, M->np, bp, M->mp, sp.done,
                                 s);
+    if (a.prec == 0 && !a.big && batch <= 64)  // narrow tile: no padding of the batch to 128 columns
+        return launch_gemm_f32_narrow(a.p32, a.ldp, w.Z, a.ldq, w.V, M->np, 0, M->np, batch, M->mp, 1, sp.done, nullptr,
+                                      s);
     return launch_gemm_tn(a.prec, a.p32, a.hi, a.lo, a.ldp, w.Z, w.Zhi, w.Zlo, a.ldq, w.V, M->np, 0, M->np, bp, M->mp, 1,
                           a.big, sp.done, nullptr, s);
 }
@@ -767,9 +770,13 @@ hipError_t launch_ax(const Matrix* M, const spca_settings& st, Workspace& w, int
     const float* P1_32 = M->a32_rm;
     const uint16_t* P1_hi = st.precision == 1 ? M->hi_rm : M->h_rm;
     const long ld1p = st.precision ? M->mp : M->np, ld1q = st.precision ? bp : M->np;
-    hipError_t e = launch_gemm_tn(st.precision, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart, M->mp,
-                                  zsplit, M->mp, bp, M->np, splits, big, sp.done,
-                                  gather ? sp.done + kSparseFlags : nullptr, s);
+    hipError_t e;
+    if (st.precision == 0 && !big && batch <= 64)  // narrow tile, same splits and bits
+        e = launch_gemm_f32_narrow(P1_32, ld1p, w.X, ld1q, w.Zpart, M->mp, zsplit, M->mp, batch, M->np, splits, sp.done,
+                                   gather ? sp.done + kSparseFlags : nullptr, s);
+    else
+        e = launch_gemm_tn(st.precision, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart, M->mp, zsplit,
+                           M->mp, bp, M->np, splits, big, sp.done, gather ? sp.done + kSparseFlags : nullptr, s);
     if (e == hipSuccess && gather)
         e = launch_sparse_ax(M->a32_cm, M->mp, M->mp, M->n, w.X, w.np, w.Zpart, M->mp, batch, sp, s);
     return e;

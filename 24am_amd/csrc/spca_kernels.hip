@@ -348,6 +348,108 @@ __global__ __launch_bounds__(256) void gemm_tn_f32_kernel(const float* __restric
                   (long)sp * ksplit, ksplit, smem);
 }
 
+// Narrow FP32 tile for batches of at most BN (32 or 64) SPs: 128 x BN outputs, wave w owns
+// rows 32w .. 32w+31 and all BN columns.  A wide 128x128 tile would pad the batch to 128
+// columns and multiply mostly zeros; at BN = 32 the product reads A once at 2*BN/4 = 16
+// flop/byte, under the FP32 ridge, so it is bound by HBM.  Every output element sees the
+// same 16x16x4 MFMA sequence over the same K range as in the wide tile, so both give the
+// same bits.
+template <int BN>
+__device__ __forceinline__ void gemm_f32_narrow_tile(const float* __restrict__ P, long ldp,
+                                                     const float* __restrict__ Q, long ldq, float* __restrict__ C,
+                                                     long ldc, long split_stride, int sp, long m0, long k0, int ksplit,
+                                                     uint8_t* smem) {
+    constexpr int QI = BN / 32;  // 16-byte Q loads per thread per stage
+    constexpr int NJ = BN / 16;  // 16-column MFMA blocks per wave
+    auto Ps = (float(*)[kGemmBK + 4])smem;
+    auto Qs = (float(*)[kGemmBK + 4])(smem + kGemmBM * (kGemmBK + 4) * 4);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int lrow = tid >> 3, lc4 = (tid & 7) * 4;
+    const float* Pg = P + (m0 + lrow) * ldp + k0 + lc4;
+    const float* Qg = Q + (long)lrow * ldq + k0 + lc4;
+    f4 pr[4], qr[QI];
+#pragma unroll
+    for (int i = 0; i < 4; i++) pr[i] = *(const f4*)(Pg + (long)(32 * i) * ldp);
+#pragma unroll
+    for (int i = 0; i < QI; i++) qr[i] = *(const f4*)(Qg + (long)(32 * i) * ldq);
+    f4 acc[2][NJ];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++) acc[i][j] = (f4){0.f, 0.f, 0.f, 0.f};
+
+    for (int kt = 0; kt < ksplit; kt += kGemmBK) {
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; i++) *(f4*)&Ps[lrow + 32 * i][lc4] = pr[i];
+#pragma unroll
+        for (int i = 0; i < QI; i++) *(f4*)&Qs[lrow + 32 * i][lc4] = qr[i];
+        __syncthreads();
+        if (kt + kGemmBK < ksplit) {
+            Pg += kGemmBK;
+            Qg += kGemmBK;
+#pragma unroll
+            for (int i = 0; i < 4; i++) pr[i] = *(const f4*)(Pg + (long)(32 * i) * ldp);
+#pragma unroll
+            for (int i = 0; i < QI; i++) qr[i] = *(const f4*)(Qg + (long)(32 * i) * ldq);
+        }
+#pragma unroll
+        for (int kk = 0; kk < kGemmBK; kk += 4) {
+            float a[2], b[NJ];
+#pragma unroll
+            for (int i = 0; i < 2; i++) a[i] = Ps[wave * 32 + i * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int j = 0; j < NJ; j++) b[j] = Qs[j * 16 + (lane & 15)][kk + (lane >> 4)];
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < NJ; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    float* Cb = C + (long)sp * split_stride;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const long row = m0 + wave * 32 + i * 16 + 4 * (lane >> 4);
+            const long col = j * 16 + (lane & 15);
+            *(f4*)(Cb + row + col * ldc) = acc[i][j];
+        }
+}
+
+template <int BN>
+__global__ __launch_bounds__(256) void gemm_tn_f32_narrow_kernel(const float* __restrict__ P, long ldp,
+                                                                 const float* __restrict__ Q, long ldq,
+                                                                 float* __restrict__ C, long ldc, long split_stride,
+                                                                 int tiles_m, int splits, int ksplit,
+                                                                 const int* __restrict__ done,
+                                                                 const int* __restrict__ skip) {
+    if (done && *done) return;
+    if (skip && skip[0]) return;  // the batch (one 256-SP group) is the sparse gather's
+    const int t = xcd_remap(blockIdx.x, tiles_m * splits);
+    const int tm = t % tiles_m, sp = t / tiles_m;
+    __shared__ __attribute__((aligned(16))) uint8_t smem[(kGemmBM + BN) * (kGemmBK + 4) * 4];
+    gemm_f32_narrow_tile<BN>(P, ldp, Q, ldq, C, ldc, split_stride, sp, (long)tm * kGemmBM, (long)sp * ksplit, ksplit,
+                             smem);
+}
+
+// FP32 C[:, 0..N) (+ split offsets) = P^T Q for N <= 64 columns: the narrow tile
+hipError_t launch_gemm_f32_narrow(const float* P, long ldp, const float* Q, long ldq, float* C, long ldc,
+                                  long split_stride, int M, int N, int K, int splits, const int* done,
+                                  const int* skip, hipStream_t st) {
+    if (M % kGemmBM || N < 1 || N > 64 || splits < 1 || K % (kGemmBK * splits)) return hipErrorInvalidValue;
+    const int tiles_m = M / kGemmBM, ksplit = K / splits;
+    const int grid = tiles_m * splits;
+    if (N <= 32)
+        gemm_tn_f32_narrow_kernel<32><<<grid, 256, 0, st>>>(P, ldp, Q, ldq, C, ldc, split_stride, tiles_m, splits,
+                                                            ksplit, done, skip);
+    else
+        gemm_tn_f32_narrow_kernel<64><<<grid, 256, 0, st>>>(P, ldp, Q, ldq, C, ldc, split_stride, tiles_m, splits,
+                                                            ksplit, done, skip);
+    return hipGetLastError();
+}
+
 // 16-bit operand products (operand parts K-blocked, in bf16 unless noted):
 //   PREC 1  bf16x3: P = hi + lo, Q = hi + lo; Phi*Qlo + Plo*Qhi + Phi*Qhi
 //   PREC 2  fp16:   Phi*Qhi in fp16

@@ -95,6 +95,37 @@ def noise_gamma(f, s, m, n):
     return t * t if f in (4, 5) else t
 
 
+def run_small_batches(S, M, X0, batches, precs, pcode, steps):
+    """Dense products of the headline matrix at small batches, where A is read once per product
+    at 2B/e_A flop/byte, below every form's ridge: the products are HBM-bound, so each is
+    reported as achieved GB/s (bytes of A it must read / its hipEvent time) against 8 TB/s.
+    L0 constraint, L2 variance; sparse_products = 0 keeps every iteration's A*X dense."""
+    a_bytes = {S.PREC_FP32: 4.0, S.PREC_BF16X3: 4.0, S.PREC_FP16: 2.0}  # hi + lo for bf16x3
+    out = {}
+    for B in batches:
+        row = {}
+        for p in precs:
+            if p == "fp32_dense":
+                continue
+            st = S.OptimizationSettings(formulation=0, constraintParameter=SPARSITY, tolerance=0.0,
+                                        maximumIterations=steps, totalStartingPoints=B, batchSize=B,
+                                        precision=pcode[p], check_every=1000, sparse_products=0)
+            X = np.asfortranarray(X0[:, :B])
+            M.solve(st, X0=X, per_sp=False)  # warm-up
+            stats = S.OptimizationStatistics()
+            M.solve(st, X0=X, stats=stats, per_sp=False)
+            ax, atz = stats.gemm_ax_ms / stats.it, stats.gemm_atz_ms / stats.it
+            gb = a_bytes[pcode[p]] * M_ROWS * N_COLS / 1e9
+            row[p] = {"sp_it_per_s": B * stats.it / stats.totalTrueComputationTime,
+                      "gemm_ax_ms": ax, "gemm_atz_ms": atz,
+                      "ax_GBps": gb / (ax / 1e3), "atz_GBps": gb / (atz / 1e3),
+                      "ax_frac_hbm": gb / (ax / 1e3) / HBM_PEAK, "atz_frac_hbm": gb / (atz / 1e3) / HBM_PEAK}
+            print(f"# small batch B={B} {p}: A*X {ax:.3f} ms ({row[p]['ax_GBps']:.0f} GB/s), "
+                  f"A^T*Z {atz:.3f} ms ({row[p]['atz_GBps']:.0f} GB/s)", file=sys.stderr, flush=True)
+        out[str(B)] = row
+    return out
+
+
 def run_uneven(S, ctx, count, iters, graph=True):
     """The uneven jobs solved two ways on one context: one fused solve (one launch per phase
     per iteration for all jobs, replayed from a hipGraph) and one solo solve per job.  tol = 0
@@ -269,6 +300,8 @@ def main():
     ap.add_argument("--sparse-products", type=int, default=1,
                     help="A*X over sparse iterates in the fp32/bf16x3/fp16 forms: 1 per-SP gather, 2 also the "
                          "product over the union of supports, 0 dense")
+    ap.add_argument("--small-batches", default="32,64",
+                    help="batch sizes at which the dense products are timed as HBM-bound kernels ('' skips)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-jobs", type=int, default=64, help="uneven fused-job workload size (0: skip)")
     ap.add_argument("--uneven-iters", type=int, default=20)
@@ -345,6 +378,8 @@ def main():
                           "steps": launches, "per_formulation": per,
                           "kernel_ms_per_step": {"gemm_ax": ax_avg, "gemm_atz": atz_avg, "step": step_ms / launches},
                           "dominant": dom, "dominant_tflops": flops / (dom_ms / 1e3) / 1e12}
+        small = run_small_batches(S, M, X0, [int(b) for b in args.small_batches.split(",") if b], precs,
+                                  pcode, args.steps) if args.small_batches else None
         M.release()
         uneven = (run_uneven(S, ctx, args.uneven_jobs, args.uneven_iters, graph=bool(args.uneven_graph))
                   if args.uneven_jobs > 0 else None)
@@ -381,6 +416,7 @@ def main():
         "forms": {p: {k: v for k, v in r.items() if k != "per_formulation"} for p, r in results.items()},
         "per_formulation": {p: r["per_formulation"] for p, r in results.items()},
     }
+    line["small_batches"] = small
     line["uneven_jobs"] = uneven
     line["precision_variant"] = variant
     line["serving"] = serving

@@ -807,3 +807,28 @@ def test_union_products_match_dense(ctx, f, prec):
         assert set(np.nonzero(xu)[0]) == set(np.nonzero(xd)[0])
     finally:
         M.release()
+
+
+@pytest.mark.parametrize("f", range(8))
+def test_narrow_fp32_tiles_bitwise(ctx, f):
+    """FP32 batches of at most 64 SPs run the products in 128 x 32 / 128 x 64 tiles instead of
+    padding the batch to 128 columns.  Every output element sees the same MFMA sequence over
+    the same K split, so with dense products, tol = 0 and a fixed iteration count (no coupling
+    between SPs) a batch of 32 (narrow, 32 wide), 64 (narrow, 64 wide) and 96 (the 128 x 128
+    tile) give bitwise the same iterates and values for the starting points they share."""
+    A, _ = spiked(300, 2000, seed=51)
+    M = ctx.upload(A)
+    X0 = unit_gaussian_starts(2000, 96, seed=52)
+    try:
+        res = {}
+        for B in (32, 64, 96):
+            st = _settings(f, 12, S.PREC_FP32, B, m=300)
+            st.sparse_products = 0
+            stats = S.OptimizationStatistics()
+            _, X = M.solve(st, X0=np.asfortranarray(X0[:, :B]), stats=stats, want_X=True)
+            res[B] = (X, stats.values)
+        for B in (32, 64):
+            np.testing.assert_array_equal(res[B][0], res[96][0][:, :B])
+            np.testing.assert_array_equal(res[B][1], res[96][1][:B])
+    finally:
+        M.release()
