@@ -1693,8 +1693,12 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         f.c0 = c0;
         f.t3 = t3;
         f.sparse_cap = sparse_cap(r.st, M->n);
-        t1 += (f.mp / kGemmBM) * (f.bp / kGemmBN) * f.splits;
-        t2 += (f.np / kGemmBM) * (f.bp / kGemmBN);
+        // FP32 batches of at most 64 SPs: narrow product tiles, as in a solo solve
+        const int nb = (precision == 0 && r.st.batch <= 64) ? (r.st.batch <= 32 ? 32 : 64) : 0;
+        f.nb1 = nb;
+        f.nb2 = f.p2_prec == 0 ? nb : 0;
+        t1 += (f.mp / kGemmBM) * (f.nb1 ? 1 : f.bp / kGemmBN) * f.splits;
+        t2 += (f.np / kGemmBM) * (f.nb2 ? 1 : f.bp / kGemmBN);
         c0 += r.st.batch;
         t3 += (f.mp + kSparseRows - 1) / kSparseRows * r.st.batch;
     }

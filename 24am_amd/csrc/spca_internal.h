@@ -144,6 +144,7 @@ struct FusedJob {
     int sparse_cap;         // largest nonzero count for the sparse A*X product (0: never)
     int p2_prec;            // A^T*Z operand form: the job's precision, or 4 (fixed point x sign vectors)
     int zprec;              // form prep_z writes Z in (as p2_prec)
+    int nb1, nb2;           // FP32 A*X / A^T*Z in the narrow 128 x nb tile (32 or 64; 0: 128 x 128 tiles)
 };
 
 // ---- launchers (spca_kernels.hip); every one enqueues on `st` and returns a hip status

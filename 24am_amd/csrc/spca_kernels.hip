@@ -2100,7 +2100,8 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
     const FusedJob& J = jobs[k];
     if (*J.sp.done) return;
     const int lt = t - (WHICH == 0 ? J.t1 : J.t2);
-    const int tiles_m = (WHICH == 0 ? J.mp : J.np) / kGemmBM, tiles_n = J.bp / kGemmBN;
+    const int nb = WHICH == 0 ? J.nb1 : J.nb2;  // narrow tiles: one column tile
+    const int tiles_m = (WHICH == 0 ? J.mp : J.np) / kGemmBM, tiles_n = nb ? 1 : J.bp / kGemmBN;
     const int tn = lt % tiles_n;
     if (WHICH == 0 && J.sp.done[kSparseFlags + (tn >> 1)]) return;  // A*X by the sparse gather
     const int rest = lt / tiles_n;
@@ -2113,7 +2114,17 @@ __global__ __launch_bounds__(256) void fused_gemm_kernel(const FusedJob* __restr
     constexpr int kLds = cmax(kF32TileLds, tile16_lds(PREC));
     __shared__ __attribute__((aligned(16))) uint8_t smem[kLds];
     if constexpr (PREC == 0) {
-        if constexpr (WHICH == 0)
+        if (nb) {  // the solo solve's narrow tile, the same bits as the 128 x 128 one
+            const float* P = WHICH == 0 ? J.p1_32 : J.p2_32;
+            const float* Q = WHICH == 0 ? J.xop.f32 : J.zop.f32;
+            const long ld = WHICH == 0 ? J.np : J.mp;
+            float* C = WHICH == 0 ? J.Zpart : J.V;
+            const long ldc = WHICH == 0 ? J.mp : J.np, sstride = WHICH == 0 ? J.zsplit : 0;
+            if (nb == 32)
+                gemm_f32_narrow_tile<32>(P, ld, Q, ld, C, ldc, sstride, sp, m0, k0, ksplit, smem);
+            else
+                gemm_f32_narrow_tile<64>(P, ld, Q, ld, C, ldc, sstride, sp, m0, k0, ksplit, smem);
+        } else if constexpr (WHICH == 0)
             gemm_f32_tile(J.p1_32, J.np, J.xop.f32, J.np, J.Zpart, J.mp, J.zsplit, sp, m0, n0, k0, ksplit, smem);
         else
             gemm_f32_tile(J.p2_32, J.mp, J.zop.f32, J.mp, J.V, J.np, 0, 0, m0, n0, k0, ksplit, smem);
