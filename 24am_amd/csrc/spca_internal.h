@@ -182,14 +182,16 @@ hipError_t launch_gemm_tn(int precision, const float* P32, const uint16_t* Phi, 
 hipError_t launch_gemm_fix8(const int8_t* P, const float* qscale, long ldp, const int8_t* Q, long ldq, float* C,
                             long ldc, int M, int N, int K, const int* done, hipStream_t st);
 
-// Sum split partials of Z, per-column statistic and map (sign / normalise), write the
-// operand of the second product.
 // FP32 product for batches of at most 64 SPs (columns 0 .. N-1 of C written; N <= 64): the
 // 128 x 32 or 128 x 64 tile, bitwise equal to launch_gemm_tn's FP32 product with the same split
 hipError_t launch_gemm_f32_narrow(const float* P, long ldp, const float* Q, long ldq, float* C, long ldc,
                                   long split_stride, int M, int N, int K, int splits, const int* done,
                                   const int* skip, hipStream_t st);
-hipError_t launch_prep_z(const float* Zpart, int splits, long split_stride, int m, int ldz,
+
+// Sum split partials of Z, per-column statistic and map (sign / normalise), write the
+// operand of the second product.  Batches of at most 256 SPs first sum the partials into
+// partial 0 over (row block, SP) workgroups, in the same order.
+hipError_t launch_prep_z(float* Zpart, int splits, long split_stride, int m, int ldz,
                          Operand zop, int formulation, int precision, int batch, SpState sp,
                          SignRefine rf, hipStream_t st);
 // rowmax[i] = max_k |A_ik| of A (mp x np column major).
