@@ -1561,12 +1561,12 @@ int solve_and_resolve(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* s
 
 // ---------------------------------------------------------------- fused multi-job solve
 // Jobs of different shapes and formulations iterate in lockstep: each AM iteration of all
-// of them is five launches (launch_fused_iteration), and a chunk of check_every
+// of them is one launch per phase (launch_fused_iteration), and a chunk of check_every
 // iterations is captured once into a hipGraph and replayed.  Each job keeps its own
 // workspace, state and termination; a finished job's work items return at once, and the
-// host stops when every job has finished.  Per job the tile order, K split and kernel
-// bodies are those of a solo solve with 128x128 tiles, so FP32 results equal a solo
-// solve's bit for bit.
+// host stops when every job has finished.  Per job the tile order, K split, partial-sum
+// order and kernel bodies are those of a solo solve, so FP32 results equal a solo solve's
+// bit for bit.
 struct FusedRun {
     Matrix* M = nullptr;
     spca_settings st{};
@@ -1649,7 +1649,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     }
     // ---- job table
     std::vector<FusedJob> tab(njobs);
-    int t1 = 0, t2 = 0, c0 = 0, t3 = 0, any_fix8 = 0;
+    int t1 = 0, t2 = 0, c0 = 0, t3 = 0, t4 = 0, any_fix8 = 0;
     for (int k = 0; k < njobs; k++) {
         FusedRun& r = run[k];
         Matrix* M = r.M;
@@ -1701,6 +1701,11 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         t2 += (f.np / kGemmBM) * (f.nb2 ? 1 : f.bp / kGemmBN);
         c0 += r.st.batch;
         t3 += (f.mp + kSparseRows - 1) / kSparseRows * r.st.batch;
+        // small batches: the K-split partials are summed over (row block, SP) items first, as
+        // in a solo solve (the same order, so the same bits)
+        f.t4 = t4;
+        f.psum = f.splits > 1 && r.st.batch <= 256;
+        if (f.psum) t4 += (f.mp + 255) / 256 * r.st.batch;
     }
     if ((rc = dev_alloc(ctx, &d_jobs, (size_t)njobs, &tb)) || (rc = dev_alloc(ctx, &d_done, 4, &tb)))
         return cleanup(rc);
@@ -1711,7 +1716,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     if (flags & SPCA_FUSED_GRAPH) {
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         for (int i = 0; i < chunk && e == hipSuccess; i++)
-            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, any_fix8, d_done, s);
+            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, t4, any_fix8, d_done, s);
         hipError_t e2 = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = e2;
         if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
@@ -1735,7 +1740,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
             e = hipGraphLaunch(gexec, s);
         else
             for (int i = 0; i < chunk && e == hipSuccess; i++)
-                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, any_fix8, d_done, s);
+                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, t4, any_fix8, d_done, s);
         hipEvent_t ev;
         if (e == hipSuccess && !new_event(&ev)) e = hipErrorOutOfMemory;
         if (e == hipSuccess) e = hipEventRecord(ev, s);

@@ -145,6 +145,8 @@ struct FusedJob {
     int p2_prec;            // A^T*Z operand form: the job's precision, or 4 (fixed point x sign vectors)
     int zprec;              // form prep_z writes Z in (as p2_prec)
     int nb1, nb2;           // FP32 A*X / A^T*Z in the narrow 128 x nb tile (32 or 64; 0: 128 x 128 tiles)
+    int t4;                 // first (row block, SP) item of the partial-sum kernel
+    int psum;               // 1: the partial-sum kernel adds the K splits into partial 0 before prep_z
 };
 
 // ---- launchers (spca_kernels.hip); every one enqueues on `st` and returns a hip status
@@ -221,14 +223,15 @@ hipError_t launch_gen_starts(float* X, int ldx, int n, int batch, int shift, uin
 hipError_t launch_plant(float* A, int m, int mp, const int* idx, const float* coef, int cnt, const float* g,
                         hipStream_t st);
 
-// One AM iteration of every job in `jobs` (device array, sorted by t1/t2/c0) as five
-// launches: A*X tiles of all jobs, the z epilogue of all columns, A^T*Z tiles (any_fix8: plus
-// a launch over the same tile list for the jobs on the fixed-point product), the
-// formulation step of all columns and the per-job convergence (a finished job adds 1 to
-// *all_done).  128x128 tiles; per job the same tile order, K split and kernel bodies as a
-// solo solve, so a job's FP32 results do not depend on the other jobs.
+// One AM iteration of every job in `jobs` (device array, sorted by t1/t2/c0/t3/t4): A*X tiles
+// of all jobs, the sparse gather items, the partial-sum items of small-batch jobs (items4, may
+// be 0), the z epilogue of all columns, A^T*Z tiles (any_fix8: plus a launch over the same
+// tile list for the jobs on the fixed-point product), the formulation step of all columns and
+// the per-job convergence (a finished job adds 1 to *all_done).  128x128 tiles (128 x 32 / 64
+// for FP32 jobs of at most 64 SPs); per job the same tile order, K split, sum order and
+// kernel bodies as a solo solve, so a job's FP32 results do not depend on the other jobs.
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
-                                  int items3, int any_fix8, int* all_done, hipStream_t st);
+                                  int items3, int items4, int any_fix8, int* all_done, hipStream_t st);
 
 // z_j = A x_j over the nonzeros of each active x_j (FP64 accumulation, fixed order) into Z
 // (ld ldz), when sp.done[2] is set; else it returns at once.  A: FP32 column major, ld lda.

@@ -1256,17 +1256,20 @@ __global__ __launch_bounds__(256) void prep_z_kernel(const float* __restrict__ Z
 // 2, ...: the same bits), over workgroups (row block, SP).  For small batches, where prep_z's
 // one workgroup per SP would walk every partial of every row alone (a batch of 32 with 64
 // splits: 0.35 ms, against ~0.03 ms here).  Gathered groups hold all of z in partial 0.
-__global__ __launch_bounds__(256) void sum_partials_kernel(float* __restrict__ Zpart, int splits, long split_stride,
-                                                           int m, int ldz, SpState sp) {
-    const int j = blockIdx.y;
+__device__ __forceinline__ void sum_partials_body(int rb, int j, float* __restrict__ Zpart, int splits,
+                                                  long split_stride, int m, int ldz, SpState sp) {
     if (*sp.done || !sp.active[j] || sp.done[kSparseFlags + (j >> 8)]) return;
     float* zc = Zpart + (long)j * ldz;
-    const int i = blockIdx.x * 256 + threadIdx.x;
+    const int i = rb * 256 + threadIdx.x;
     if (i >= m) return;
     float z = zc[i];
 #pragma unroll 8
     for (int s = 1; s < splits; s++) z += zc[(long)s * split_stride + i];
     zc[i] = z;
+}
+__global__ __launch_bounds__(256) void sum_partials_kernel(float* __restrict__ Zpart, int splits, long split_stride,
+                                                           int m, int ldz, SpState sp) {
+    sum_partials_body(blockIdx.x, blockIdx.y, Zpart, splits, split_stride, m, ldz, sp);
 }
 
 hipError_t launch_prep_z(float* Zpart, int splits, long split_stride, int m, int ldz, Operand zop, int formulation,
@@ -2085,7 +2088,8 @@ hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_it
 // body as the solo kernel on the job's own buffers.  Finished jobs return at once.
 template <int FIELD>
 __device__ __forceinline__ int job_first(const FusedJob* __restrict__ jobs, int k) {
-    return FIELD == 0 ? jobs[k].t1 : FIELD == 1 ? jobs[k].t2 : FIELD == 2 ? jobs[k].c0 : jobs[k].t3;
+    return FIELD == 0 ? jobs[k].t1 : FIELD == 1 ? jobs[k].t2 : FIELD == 2 ? jobs[k].c0 : FIELD == 3 ? jobs[k].t3
+                                                                                            : jobs[k].t4;
 }
 template <int FIELD>
 __device__ __forceinline__ int find_job(const FusedJob* __restrict__ jobs, int njobs, int item) {
@@ -2180,11 +2184,16 @@ __global__ __launch_bounds__(256) void fused_sparse_ax_kernel(const FusedJob* __
     sparse_ax_body(li % rbs, li / rbs, J.p2_32, J.mp, J.sa.n, J.xop.f32, J.np, J.Zpart, J.mp, J.mp, J.sp);
 }
 
+__global__ __launch_bounds__(256) void fused_sum_partials_kernel(const FusedJob* __restrict__ jobs, int njobs) {
+    const FusedJob& J = jobs[find_job<4>(jobs, njobs, blockIdx.x)];
+    const int li = blockIdx.x - J.t4, rbs = (J.mp + 255) / 256;
+    sum_partials_body(li % rbs, li / rbs, J.Zpart, J.splits, J.zsplit, J.mp, J.mp, J.sp);
+}
 __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __restrict__ jobs, int njobs) {
     const int k = find_job<2>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
-    prep_z_body(blockIdx.x - J.c0, J.Zpart, J.splits, J.zsplit, J.mp, J.mp, J.zop, J.sa.formulation, J.zprec, J.sp,
-                J.rf);
+    prep_z_body(blockIdx.x - J.c0, J.Zpart, J.psum ? 1 : J.splits, J.zsplit, J.mp, J.mp, J.zop, J.sa.formulation,
+                J.zprec, J.sp, J.rf);
 }
 
 __global__ __launch_bounds__(kSelThreads) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {
@@ -2199,10 +2208,11 @@ __global__ __launch_bounds__(256) void fused_converge_kernel(const FusedJob* __r
 }
 
 template <int PREC>
-hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int items3, int any_fix8,
-                        int* all_done, hipStream_t st) {
+hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int items3, int items4,
+                        int any_fix8, int* all_done, hipStream_t st) {
     fused_gemm_kernel<PREC, 0><<<tiles1, 256, 0, st>>>(jobs, njobs, tiles1);
     fused_sparse_ax_kernel<<<items3, 256, 0, st>>>(jobs, njobs);
+    if (items4 > 0) fused_sum_partials_kernel<<<items4, 256, 0, st>>>(jobs, njobs);
     fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
     fused_gemm_kernel<PREC, 1><<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
     if (any_fix8) fused_fix8_kernel<<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
@@ -2212,11 +2222,13 @@ hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2,
 }
 
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
-                                  int items3, int any_fix8, int* all_done, hipStream_t st) {
-    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1 || items3 < 1) return hipErrorInvalidValue;
-    if (precision == 0) return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, items3, any_fix8, all_done, st);
-    if (precision == 1) return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, items3, any_fix8, all_done, st);
-    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, items3, any_fix8, all_done, st);
+                                  int items3, int items4, int any_fix8, int* all_done, hipStream_t st) {
+    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1 || items3 < 1 || items4 < 0) return hipErrorInvalidValue;
+    if (precision == 0)
+        return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, items3, items4, any_fix8, all_done, st);
+    if (precision == 1)
+        return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, items3, items4, any_fix8, all_done, st);
+    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, items3, items4, any_fix8, all_done, st);
 }
 
 // ---------------------------------------------------------------- OTF slot refill
