@@ -1239,6 +1239,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     hipEvent_t* loop_ev = E.loop_ev;
 
     if (st.use_otf) {
+        // the sparse gather runs for groups whose active slots (refills included) are all sparse
+        const int otf_cap = M->sparse ? 0 : sparse_cap(st, n);
         // ---- on-the-fly refill (reference OTF driver, sparse_PCA_solver.h:83-169): every SP is
         // resident in a device pool; finished slots take the next SP on the device.
         const int total = st.total_sp;
@@ -1313,7 +1315,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventRecord(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(xop, st.precision, w.np, defl->idx, defl->val, defl->off, defl->ncomp,
                                                   batch, sp, s));
-            HIP_TRY(ctx, launch_ax(M, st, w, splits, plan.big, bp, batch, zsplit, sp, false, s));
+            HIP_TRY(ctx, launch_ax(M, st, w, splits, plan.big, bp, batch, zsplit, sp, otf_cap > 0, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
             HIP_TRY(ctx, hipEventRecord(e0, s));
@@ -1330,7 +1332,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
                                                   defl->off, defl->ncomp, batch, sp, s));
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
             HIP_TRY(ctx, launch_otf_converge(sp, o, batch, st.tolerance, st.max_iter, total, s));
-            HIP_TRY(ctx, launch_otf_refill(sp, o, xop, st.precision, n, w.np, batch, s));
+            HIP_TRY(ctx, launch_otf_refill(sp, o, xop, st.precision, n, w.np, batch, otf_cap, s));
             HIP_TRY(ctx, hipEventRecord(e1, s));
             if ((it + 1) % check_every == 0 || it + 1 == cap) {
                 HIP_TRY(ctx, hipMemcpyAsync(E.h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
@@ -1351,11 +1353,11 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, hipEventElapsedTime(&t, loop_ev[0], loop_ev[1]));
             loop_ms += t;
         }
-        int done_h[2], gen_h[3];
+        int done_h[5], gen_h[3];
         float bestv;
         std::vector<float> spv(total);
         std::vector<int> spi(total), spn(total);
-        HIP_TRY(ctx, hipMemcpyAsync(done_h, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
+        HIP_TRY(ctx, hipMemcpyAsync(done_h, sp.done, sizeof(int) * 5, hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipMemcpyAsync(gen_h, o.generated, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipMemcpyAsync(&bestv, o.best_val, sizeof(float), hipMemcpyDeviceToHost, s));
         HIP_TRY(ctx, hipMemcpyAsync(spv.data(), o.sp_val, sizeof(float) * total, hipMemcpyDeviceToHost, s));
@@ -1374,6 +1376,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         HIP_TRY(ctx, hipStreamSynchronize(s));
         release();
         R.total_it = (unsigned)done_h[1];
+        R.sparse_it = (unsigned)done_h[4];
         R.best = bestv;
         R.best_sp = gen_h[2];
         R.last_err = emax;

@@ -211,8 +211,10 @@ hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_it
 // order; then the refill kernel saves the best column and loads the new SPs.
 hipError_t launch_otf_converge(SpState sp, OtfState o, int batch, double tol, int max_iter, int total_sp,
                                hipStream_t st);
+// refilled slots take their next SP; sparse_cap > 0: then each group of 256 slots gets its
+// sparse-gather flag for the next A*X (every active slot at most sparse_cap nonzeros)
 hipError_t launch_otf_refill(SpState sp, OtfState o, Operand xop, int precision, int n, int ldx, int batch,
-                             hipStream_t st);
+                             int sparse_cap, hipStream_t st);
 
 // Synthetic spiked matrices: A (mp x np column major) <- N(0,1) noise from Philox4x32-10
 // counters (r, c, 0) keyed by seed (padding zero); then A[:, idx[t]] += coef[t] * g, one

@@ -2341,19 +2341,44 @@ __global__ __launch_bounds__(256) void otf_refill_kernel(SpState sp, OtfState o,
     if (o.generated[1] == i)
         for (int k = threadIdx.x; k < n; k += 256) o.xbest[k] = xop.f32[base + k];
     if (!o.refill[i]) return;
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
     __syncthreads();
     const float* src = o.pool + (long)o.order[i] * ldx;
-    for (int k = threadIdx.x; k < n; k += 256) write_operand(xop, precision, ldx, k, i, src[k]);
+    int c = 0;
+    for (int k = threadIdx.x; k < n; k += 256) {
+        const float v = src[k];
+        write_operand(xop, precision, ldx, k, i, v);
+        c += v != 0.f;
+    }
+    atomicAdd(&cnt, c);  // integer sum: order-free
+    __syncthreads();
     if (threadIdx.x == 0) {
         sp.val[i] = 0.f;
         sp.err[i] = 0.f;
-        sp.nnz[i] = 0;
+        sp.nnz[i] = cnt;  // the new iterate's nonzeros, for the sparse A*X decision
         o.refill[i] = 0;
     }
 }
+
+// OTF: the next A*X of each group of 256 slots is the sparse gather iff every active slot of
+// the group (refilled ones included) has at most sparse_cap nonzeros; counted in done[4]
+__global__ __launch_bounds__(256) void otf_sparse_flags_kernel(SpState sp, int batch, int sparse_cap) {
+    const int groups = (batch + 255) >> 8;
+    int nsparse = 0;
+    for (int g = 0; g < groups; g++) {  // uniform trip count: every thread reaches each barrier
+        const int j = g * 256 + threadIdx.x;
+        const int ok = j >= batch || !sp.active[j] || sp.nnz[j] <= sparse_cap;
+        const int flag = __syncthreads_and(ok) && sparse_cap > 0 && !sp.done[0];
+        if (threadIdx.x == 0) sp.done[kSparseFlags + g] = flag;
+        nsparse += flag;
+    }
+    if (threadIdx.x == 0 && nsparse) sp.done[4] += 1;  // iterations whose A*X is gathered (some group)
+}
 hipError_t launch_otf_refill(SpState sp, OtfState o, Operand xop, int precision, int n, int ldx, int batch,
-                             hipStream_t st) {
+                             int sparse_cap, hipStream_t st) {
     otf_refill_kernel<<<batch, 256, 0, st>>>(sp, o, xop, precision, n, ldx, batch);
+    if (sparse_cap > 0) otf_sparse_flags_kernel<<<1, 256, 0, st>>>(sp, batch, sparse_cap);
     return hipGetLastError();
 }
 
