@@ -126,6 +126,30 @@ def run_small_batches(S, M, X0, batches, precs, pcode, steps):
     return out
 
 
+def run_otf(S, M, total):
+    """The reference's OTF experiment on the headline matrix: the same `total` reference
+    starting points (constrained: s nonzeros each) to convergence (tol 1e-6, at most 200
+    iterations per SP), L0 constraint / L2 variance, FP32, in 1024-SP batches (every batch
+    runs until its slowest SP converges) and on the fly (1024 slots, each refilled as its SP
+    converges).  Whole-solve loop time and SP-iterations done."""
+    out = {}
+    for otf in (False, True):
+        st = S.OptimizationSettings(formulation=0, constraintParameter=SPARSITY, tolerance=1e-6, maximumIterations=200,
+                                    totalStartingPoints=total, batchSize=BATCH, useOTF=otf, start=S.START_REFERENCE,
+                                    check_every=8)
+        stats = S.OptimizationStatistics()
+        M.solve(st, stats=stats, per_sp=False)
+        key = "otf" if otf else "batch"
+        out[key] = {"loop_s": stats.totalTrueComputationTime, "iterations": stats.it,
+                    "sp_iterations": stats.sp_iterations, "gathered_iterations": stats.sparse_iterations,
+                    "fval": stats.fval}
+        print(f"# {key}: {total} SPs in {stats.totalTrueComputationTime:.3f} s, {stats.it} iterations, "
+              f"fval {stats.fval:.6g}", file=sys.stderr, flush=True)
+    out["otf_speedup"] = out["batch"]["loop_s"] / out["otf"]["loop_s"]
+    out["workload"] = f"{total} reference starting points, batches/slots of {BATCH}, tol 1e-6, max 200 iterations"
+    return out
+
+
 def run_uneven(S, ctx, count, iters, graph=True):
     """The uneven jobs solved two ways on one context: one fused solve (one launch per phase
     per iteration for all jobs, replayed from a hipGraph) and one solo solve per job.  tol = 0
@@ -302,6 +326,8 @@ def main():
                          "product over the union of supports, 0 dense")
     ap.add_argument("--small-batches", default="32,64",
                     help="batch sizes at which the dense products are timed as HBM-bound kernels ('' skips)")
+    ap.add_argument("--otf-sps", type=int, default=4096,
+                    help="starting points of the batch-vs-OTF comparison (0 skips)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--uneven-jobs", type=int, default=64, help="uneven fused-job workload size (0: skip)")
     ap.add_argument("--uneven-iters", type=int, default=20)
@@ -378,6 +404,7 @@ def main():
                           "steps": launches, "per_formulation": per,
                           "kernel_ms_per_step": {"gemm_ax": ax_avg, "gemm_atz": atz_avg, "step": step_ms / launches},
                           "dominant": dom, "dominant_tflops": flops / (dom_ms / 1e3) / 1e12}
+        otf = run_otf(S, M, args.otf_sps) if args.otf_sps > 0 and "fp32" in precs else None
         small = run_small_batches(S, M, X0, [int(b) for b in args.small_batches.split(",") if b], precs,
                                   pcode, args.steps) if args.small_batches else None
         M.release()
@@ -417,6 +444,7 @@ def main():
         "per_formulation": {p: r["per_formulation"] for p, r in results.items()},
     }
     line["small_batches"] = small
+    line["otf"] = otf
     line["uneven_jobs"] = uneven
     line["precision_variant"] = variant
     line["serving"] = serving
