@@ -24,7 +24,7 @@ sys.path.insert(0, os.path.join(REPO, "24am_amd"))
 
 M_ROWS, N_COLS, BATCH, SPARSITY = 4096, 65536, 1024, 256
 THETA = (20.0, 15.0, 10.0, 8.0, 6.0)
-PEAK = {"fp32": 157.3, "bf16x3": 2500.0 / 3.0, "fp16": 2500.0}  # TFLOP/s (bf16x3: bf16 peak / 3 products)
+PEAK = {"fp32": 157.3, "fp32_dense": 157.3, "bf16x3": 2500.0 / 3.0, "fp16": 2500.0}  # TFLOP/s (bf16x3: bf16 peak / 3)
 HBM_PEAK = 8000.0  # GB/s
 # HBM bytes per launch of the product kernels at this workload, from rocprofv3 --pmc
 # FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs, scaled by the counters' reading of a 1 GiB
@@ -435,7 +435,7 @@ def main():
         "metric": "start-point iterations/sec + p99 SPCA solve latency at fixed expl-var tolerance",
         "value": head["sp_it_per_s"], "unit": "SP-it/s", "n_gpus": 1, "steps": head["steps"],
         "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f32" if hp == "fp32" else hp, "data": "synthetic",
+        "vs_baseline": None, "dtype": "f32" if hp in ("fp32", "fp32_dense") else hp, "data": "synthetic",
         "config": {"workload": f"spiked {M_ROWS}x{N_COLS} FP32, B={BATCH}, s={SPARSITY}, tol=0 "
                                f"(fixed {args.steps} iterations), formulations {args.formulations}",
                    "global_batch": BATCH, "precision": hp, "gamma": gam},
