@@ -3,8 +3,8 @@
 // c++/src/frontends/multicore_console.cpp).  Same options, CSV input and output files; the
 // solve runs on the GPU through the C ABI (include/spca.h).  Starting points are the
 // reference CPU method's (rand_r per starting point, c++/src/gpower/gpower_commons.h:21-47),
-// drawn in float, or in double with -d, as multicore_console draws them; gpu_console's
-// unseeded rand() draws are not reproduced.
+// drawn in float, as multicore_console draws them without -d; gpu_console's unseeded rand()
+// draws are not reproduced.
 //
 // Options (reference parser c++/src/utils/option_console_parser.h:43-143, same getopt string
 // plus -p and -e):
@@ -13,13 +13,20 @@
 //            4 L1 cons / L2, 5 L0 pen / L1, 6 L0 pen / L2, 7 L1 pen / L1, 8 L1 pen / L2
 //   -s N     constraint s          -g X  penalty gamma        -m N  max iterations
 //   -t X     tolerance             -l N  starting points      -r N  batch size
-//   -u N     OTF (0/1)             -d X  read the CSV as double
+//   -u N     OTF (0/1)             -d X  double precision: rejected (exit 2), see below
 //   -v X     verbose               -x N  distributed grid (accepted, not used: one GPU)
 //   -p P     products: fp32 (default), bf16x3, fp16 (no reference counterpart)
 //   -e       echo the parsed settings and the data shape, then exit without a GPU call
 //            (no reference counterpart)
 // Defaults are the reference OptimizationSettings constructor's
 // (c++/src/class/optimization_settings.h:100-122).
+//
+// -d: the reference runs its whole solve in double then (gpu_console.cu:97-101,
+// multicore_console.cpp:52-56).  This solver's products are FP32, bf16x3 or fp16 MFMA and
+// its iterates FP32, so a double solve does not exist here; -d exits with status 2 and a
+// message instead of returning FP32 results under a "Double precision: 1" label.  The FP32
+// form (-p fp32) with the FP64 explained variance and FP64 support re-solve is the closest
+// this solver offers.
 //
 // Outputs (reference c++/src/utils/file_reader.h:100-146):
 //   <prefix>_x                       "index,value" for every nonzero of x (16 digits)
@@ -181,7 +188,7 @@ void save_statistics(const std::string& prefix, const ConsoleOptions& o, const s
     f << "Starting points: " << st.total_sp << '\n';
     f << "Batch size: " << st.batch << '\n';
     f << "Batching strategy (OTF): " << st.use_otf << '\n';
-    f << "Double precision: " << (o.use_double ? 1 : 0) << '\n';
+    f << "Double precision: " << 0 << '\n';
     f << "tolerance: " << st.tolerance << '\n';
     f << '\n' << "Timing " << '\n';
     f << "Total computational time: " << std::setprecision(16) << s.loop_ms * 1e-3 << " sec" << '\n';
@@ -201,19 +208,23 @@ int main(int argc, char** argv) {
     if (parse(o, argc, argv)) return 1;
     unsigned m = 0, n = 0;
     std::vector<float> B;
-    const bool ok = o.use_double ? read_csv<double>(o.input, m, n, B) : read_csv<float>(o.input, m, n, B);
+    const bool ok = read_csv<float>(o.input, m, n, B);
     if (!ok || m == 0 || n == 0) {
         std::cerr << "cannot read a matrix from " << o.input << "\n";
         return 1;
     }
+    if (o.use_double) {
+        std::cerr << "double precision (-d) is not provided: products run in FP32, bf16x3 or fp16 (-p) with FP32 "
+                     "iterates; run without -d\n";
+        return 2;
+    }
     spca_settings st = o.st;
-    if (o.use_double) st.start = SPCA_START_REFERENCE_F64;
     spca_settings_check(&st, n);  // the reference clamp: s <= n, batches, total rounded up
     if (o.echo) {
         std::cout << "m " << m << " n " << n << " formulation " << st.formulation << " s " << st.constraint
                   << " gamma " << st.penalty << " max_iter " << st.max_iter << " tol " << st.tolerance
-                  << " total_sp " << st.total_sp << " batch " << st.batch << " otf " << st.use_otf << " double "
-                  << (o.use_double ? 1 : 0) << " precision " << st.precision << "\n";
+                  << " total_sp " << st.total_sp << " batch " << st.batch << " otf " << st.use_otf << " double 0"
+                  << " precision " << st.precision << "\n";
         return 0;
     }
     spca_ctx ctx = nullptr;

@@ -681,9 +681,15 @@ Operand z_operand(Workspace& w, int precision) {
 // The second product, A^T*Z, of a solve: the precision's own operands, or, for the
 // L1-variance formulations with exact_sign_products (FP32 and bf16x3), the fixed-point pieces
 // of A against Z = sgn(A x) in int8 on the i8 MFMA: exact integer sums, one rounding per v_j.
-bool uses_sign_products(const spca_settings& st) {
-    return st.exact_sign_products && spca_is_l1_variance(st.formulation) && (st.precision == 0 || st.precision == 1);
+// Its int32 accumulators hold |sum_i (64 q0 + q1) z_i| <= 8160 mp, so it needs mp < 2^18
+// padded rows; taller matrices run the precision's own A^T*Z product.
+constexpr int kFix8MaxRows = 1 << 18;
+bool uses_sign_products(const spca_settings& st, int mp) {
+    return st.exact_sign_products && spca_is_l1_variance(st.formulation) && (st.precision == 0 || st.precision == 1) &&
+           mp < kFix8MaxRows;
 }
+// The step kernel sums |v|/max|v| in 2^-40 fixed point in u64 LDS counters: n < 2^23 columns.
+constexpr int kMaxCols = 1 << 23;
 struct AtzForm {
     int prec;            // launch_gemm_tn precision, or 4: the fixed-point product (launch_gemm_fix8)
     int zprec;           // form prep_z writes Z in
@@ -700,7 +706,7 @@ AtzForm atz_form(const Matrix* M, const spca_settings& st, int bp, int plan_big)
     a.p32 = M->a32_cm;
     a.q = M->q_cm;
     a.qscale = M->qscale;
-    if (uses_sign_products(st)) {
+    if (uses_sign_products(st, M->mp)) {
         a.prec = a.zprec = 4;
         a.hi = a.lo = nullptr;
         a.big = 0;
@@ -721,7 +727,7 @@ AtzForm atz_form(const Matrix* M, const spca_settings& st, int bp, int plan_big)
 // terms it flags ~1e-4 (FP32) to ~2e-4 (bf16x3) of the entries of Gaussian data.
 SignRefine sign_refine(const Matrix* M, const spca_settings& st, Workspace& w) {
     SignRefine r{M->a32_rm, M->rowmax, w.X, M->np, 0.f};
-    if (uses_sign_products(st) && M->a32_rm && M->rowmax) r.eps = st.precision == 0 ? 0x1p-17f : 0x1p-14f;
+    if (uses_sign_products(st, M->mp) && M->a32_rm && M->rowmax) r.eps = st.precision == 0 ? 0x1p-17f : 0x1p-14f;
     return r;
 }
 
@@ -1177,9 +1183,11 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
         return fail(ctx, SPCA_EINVAL, "a sparse (CSC) matrix is solved in FP32 only");
     if (M->sparse && st.resolve_iters > 0)
         return fail(ctx, SPCA_EINVAL, "the FP64 support re-solve needs a dense matrix");
+    if (M->n >= kMaxCols)
+        return fail(ctx, SPCA_EINVAL, "n must be below 2^23 columns (the step kernel's fixed-point sums)");
     unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
     // the fixed-point product, and the FP32 row-major copy for the exact signs of z
-    if (uses_sign_products(st)) need |= SPCA_STORE_FIX8 | SPCA_STORE_FP32;
+    if (!M->sparse && uses_sign_products(st, M->mp)) need |= SPCA_STORE_FIX8 | SPCA_STORE_FP32;
     int rc = M->sparse ? SPCA_OK : add_storage(ctx, M, need);
     if (rc) return rc;
     *nb = spca_settings_check(&st, (unsigned)M->n);
@@ -1896,6 +1904,7 @@ void worker_main(spca_ctx ctx, spca_server* S, int w) {
 void server_stop(spca_ctx ctx) {
     spca_server* S = ctx->server;
     if (!S) return;
+    std::vector<uint64_t> unref;  // matrices held by queued jobs, which no worker will release
     {
         std::lock_guard<std::mutex> lk(S->mu);
         S->stop = true;
@@ -1903,10 +1912,13 @@ void server_stop(spca_ctx ctx) {
             J->status = SPCA_ECANCELLED;
             J->state = SPCA_JOB_CANCELLED;
             J->cancel.store(1);
+            if (J->M) unref.push_back(J->mat);
+            J->M = nullptr;
         }
         S->queue.clear();
         for (auto& kv : S->jobs) kv.second->cancel.store(1);
     }
+    for (uint64_t h : unref) matrix_unref(ctx, h);
     S->cv.notify_all();
     for (auto& t : S->workers) t.join();
     for (auto& E : S->engines) engine_destroy(ctx, E);
@@ -1953,6 +1965,15 @@ int spca_settings_check(spca_settings* st, unsigned n) {
     if (tb * st->batch < st->total_sp) tb++;
     st->total_sp = tb * st->batch;
     return tb;
+}
+
+int spca_product_forms(const spca_settings* st, int m, int n, int* ax_form, int* atz_form) {
+    if (!st || m <= 0 || n <= 0 || m > (1 << 30)) return SPCA_EINVAL;
+    if (!valid_formulation(st->formulation) || st->precision < 0 || st->precision > 2) return SPCA_EINVAL;
+    if (n >= kMaxCols) return SPCA_EINVAL;
+    if (ax_form) *ax_form = st->precision;
+    if (atz_form) *atz_form = uses_sign_products(*st, round_up(m, kPadMN)) ? SPCA_FORM_FIX8 : st->precision;
+    return SPCA_OK;
 }
 
 int spca_is_constrained(int f) { return f >= 0 && f < 4; }
@@ -2296,10 +2317,13 @@ int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, co
             delete J;
             return fail(ctx, SPCA_EINVAL, "bad job settings");
         }
-        const unsigned need = J->st.precision == 0 ? SPCA_STORE_FP32
-                              : J->st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
+        // every storage form the solve reads is built here, under ctx->mu: workers solving
+        // the same matrix side by side must find it complete (they never add storage)
+        unsigned need = J->st.precision == 0 ? SPCA_STORE_FP32
+                        : J->st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
+        if (!M->sparse && uses_sign_products(J->st, M->mp)) need |= SPCA_STORE_FIX8 | SPCA_STORE_FP32;
         (void)hipSetDevice(ctx->device);
-        int rc = add_storage(ctx, M, need);
+        int rc = M->sparse ? SPCA_OK : add_storage(ctx, M, need);
         if (rc) {
             delete J;
             return rc;
@@ -2353,26 +2377,27 @@ int spca_poll(spca_ctx ctx, spca_job job, int* state) {
 int spca_cancel(spca_ctx ctx, spca_job job) {
     if (!ctx || !ctx->server) return SPCA_EINVAL;
     spca_server* S = ctx->server;
-    Job* J = nullptr;
+    // A job taken off the queue here never reaches a worker, so its matrix reference is
+    // dropped here; a running job's worker drops its own.  J is not touched after the lock is
+    // released: a concurrent spca_wait may delete it as soon as it reads CANCELLED.
+    uint64_t unref = 0;
     {
         std::lock_guard<std::mutex> lk(S->mu);
         auto it = S->jobs.find(job);
         if (it == S->jobs.end()) return SPCA_ENOTFOUND;
-        J = it->second;
+        Job* J = it->second;
         J->cancel.store(1);
         for (auto q = S->queue.begin(); q != S->queue.end(); ++q)
             if (*q == J) {
                 S->queue.erase(q);
                 J->status = SPCA_ECANCELLED;
                 J->state = SPCA_JOB_CANCELLED;
+                if (J->M) unref = J->mat;
+                J->M = nullptr;
                 break;
             }
     }
-    if (J->state == SPCA_JOB_CANCELLED && J->M) {
-        // a queued job never reached a worker: drop its matrix reference here
-        if (J->t_start.time_since_epoch().count() == 0) matrix_unref(ctx, J->mat);
-        J->M = nullptr;
-    }
+    if (unref) matrix_unref(ctx, unref);
     S->cv.notify_all();
     return SPCA_OK;
 }

@@ -2293,6 +2293,9 @@ __global__ __launch_bounds__(1024) void otf_converge_kernel(SpState sp, OtfState
         const uint64_t bal = __ballot(term);
         const int before = __popcll(bal & ((1ull << lane) - 1ull));
         if (lane == 0) { wsum[w] = __popcll(bal); wbest[w] = bv; wbesti[w] = bi; wbestg[w] = bg; }
+        // the carried count of earlier chunks, read before the barrier: thread 0 adds this
+        // chunk's total to s_carry after it, while other waves may still be ranking
+        const int carry = s_carry;
         __syncthreads();
         int off = 0, tot = 0;
         for (int k = 0; k < 16; k++) {
@@ -2300,7 +2303,7 @@ __global__ __launch_bounds__(1024) void otf_converge_kernel(SpState sp, OtfState
             tot += wsum[k];
         }
         if (term) {
-            const int rank = s_carry + off + before;
+            const int rank = carry + off + before;
             if (gen0 + rank < total_sp) {
                 o.order[i] = gen0 + rank;
                 o.cur_it[i] = 0;

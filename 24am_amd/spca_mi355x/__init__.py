@@ -37,7 +37,9 @@ EXPORTS = [
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
     "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan", "spca_matrix_generate",
     "spca_matrix_download", "spca_ctx_memory", "spca_solve_components", "spca_matrix_upload_csc",
+    "spca_product_forms",
 ]
+FORM_FP32, FORM_BF16X3, FORM_FP16, FORM_FIX8 = 0, 1, 2, 4
 FUSED_GRAPH = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
 
@@ -147,6 +149,7 @@ def lib():
             "spca_ctx_memory": (I, [P, ctypes.POINTER(U64), ctypes.POINTER(U64), ctypes.POINTER(U64)]),
             "spca_get_plan": (I, [P, I, I, I, I, I, ctypes.POINTER(I), ctypes.POINTER(I),
                                   ctypes.POINTER(ctypes.c_double)]),
+            "spca_product_forms": (I, [ctypes.POINTER(_Settings), I, I, ctypes.POINTER(I), ctypes.POINTER(I)]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -186,6 +189,9 @@ class OptimizationSettings:
         # L1 variance, FP32 and bf16x3: A^T sgn(z) from the exact bf16 split of A (0: the precision's product)
         self.exact_sign_products = 1
         self.exact_ties = 1
+        # reference useDoublePrecision: this solver has no double products, so True is
+        # rejected (SPCA_EINVAL) when the settings reach the library, not run in FP32
+        self.useDoublePrecision = False
         for k, v in kw.items():
             if not hasattr(self, k):
                 raise AttributeError(k)
@@ -207,7 +213,20 @@ class OptimizationSettings:
         self.constraintParameter, self.batchSize, self.totalStartingPoints = c.constraint, c.batch, c.total_sp
         return tb
 
+    def product_forms(self, m, n):
+        """(A*X form, A^T*Z form) a dense m x n solve of these settings runs (FORM_*), decided on
+        the host; raises SpcaError (SPCA_EINVAL) for settings a solve rejects (n >= 2^23)."""
+        c = self._c()
+        ax, atz = ctypes.c_int(), ctypes.c_int()
+        rc = lib().spca_product_forms(ctypes.byref(c), int(m), int(n), ctypes.byref(ax), ctypes.byref(atz))
+        if rc != SPCA_OK:
+            raise SpcaError(rc, f"no solve of {m} x {n} with these settings")
+        return ax.value, atz.value
+
     def _c(self):
+        if self.useDoublePrecision:
+            raise SpcaError(SPCA_EINVAL, "useDoublePrecision: the products run in FP32, bf16x3 or fp16 "
+                                         "(precision); there is no double solve")
         return _Settings(int(self.formulation), int(self.constraintParameter), float(self.penaltyParameter),
                          float(self.tolerance), int(self.maximumIterations), int(self.totalStartingPoints),
                          int(self.batchSize), int(bool(self.useOTF)), int(self.termination), int(self.precision),

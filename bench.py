@@ -53,6 +53,28 @@ def make_problem(seed=2024):
     return A, np.asfortranarray(X0), loadings
 
 
+C1_SHAPE, C1_RANK, C1_S, C1_B = (400, 10304), 20, 16, 64
+
+
+def make_c1_problem(seed=1):
+    """The reference CPU path's configuration: A (FP64, 400 x 10304) = U diag(sigma) V^T +
+    0.05 N(0,1), U (400 x 20) and V (10304 x 20) Gaussian orthonormalised by QR, sigma_k =
+    10 * 0.8^k; then A is clipped to [0, 1] and its columns are centred.  64 starting points of
+    i.i.d. N(0,1), each unit-normalised.  One NumPy PCG64 stream seeded `seed` draws U, V, the
+    noise and the starting points in that order.  Returns (A, X0), both FP64, column major."""
+    m, n = C1_SHAPE
+    rng = np.random.default_rng(seed)
+    U = np.linalg.qr(rng.standard_normal((m, C1_RANK)))[0]
+    V = np.linalg.qr(rng.standard_normal((n, C1_RANK)))[0]
+    sigma = 10.0 * 0.8 ** np.arange(C1_RANK)
+    A = (U * sigma) @ V.T + 0.05 * rng.standard_normal((m, n))
+    A = np.clip(A, 0.0, 1.0)
+    A -= A.mean(axis=0)
+    X0 = rng.standard_normal((n, C1_B))
+    X0 /= np.linalg.norm(X0, axis=0)
+    return np.asfortranarray(A), np.asfortranarray(X0)
+
+
 def calibrate_gamma(A, v1):
     """gamma for each penalised formulation so that the step applied at the strongest
     planted loading keeps ~256 coordinates (midpoint of the 256th and 257th values)."""
@@ -234,14 +256,13 @@ def run_precision_variant(S, ctx, iters, warmup):
     return out
 
 
-def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5):
-    """Request stream: Poisson arrivals at `rate` jobs/s for `seconds`, each job on one of
-    `pool_size` resident seeded matrices (shapes drawn as the uneven jobs), with its own
-    formulation, s and B; tol = 1e-5, at most 200 iterations, device Gaussian starts.  The
-    server admits jobs by HBM footprint under `budget` bytes and runs up to `workers` at once.
-    Latency = submit -> result, measured by the server.  Returns the summary and, per job,
-    the FP64 explained variance of its result (for the cross-precision comparison)."""
-    specs = uneven_specs(pool_size, seed=11)
+SERVE_POOL_SEED = 11
+
+
+def serving_draws(rate, seconds, pool_size, seed=5):
+    """The request stream: Poisson arrival times at `rate` per second over `seconds`, and per
+    request (pool matrix, formulation, s, B) drawn uniformly (s from {16, 64, 256, 1024}, B from
+    {32, 128, 512}); the pool matrices are uneven_specs(pool_size, seed=SERVE_POOL_SEED)."""
     rng = np.random.default_rng(seed)
     t = 0.0
     arrivals = []
@@ -250,9 +271,31 @@ def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5)
         if t > seconds:
             break
         arrivals.append(t)
-    store = {S.PREC_FP32: S.STORE_FP32, S.PREC_BF16X3: S.STORE_BF16X3, S.PREC_FP16: S.STORE_FP16}[precision]
     draws = [(int(rng.integers(0, pool_size)), int(rng.integers(0, 8)), int(rng.choice([16, 64, 256, 1024])),
               int(rng.choice([32, 128, 512]))) for _ in arrivals]
+    return arrivals, draws
+
+
+def serving_settings(S, precision, spec, f, s, B, k):
+    """Settings of request k of the stream on pool matrix `spec`: tol 1e-5, at most 200
+    iterations, device Gaussian starting points keyed by the request index."""
+    s = min(s, spec["n"] - 1)
+    return S.OptimizationSettings(formulation=f, constraintParameter=s,
+                                  penaltyParameter=noise_gamma(f, s, spec["m"], spec["n"]), tolerance=1e-5,
+                                  maximumIterations=200, totalStartingPoints=B, batchSize=B, precision=precision,
+                                  start=S.START_GAUSSIAN, seed=100000 + k, check_every=4)
+
+
+def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5):
+    """Request stream: Poisson arrivals at `rate` jobs/s for `seconds`, each job on one of
+    `pool_size` resident seeded matrices (shapes drawn as the uneven jobs), with its own
+    formulation, s and B; tol = 1e-5, at most 200 iterations, device Gaussian starts.  The
+    server admits jobs by HBM footprint under `budget` bytes and runs up to `workers` at once.
+    Latency = submit -> result, measured by the server.  Returns the summary and, per job,
+    the FP64 explained variance of its result (for the cross-precision comparison)."""
+    specs = uneven_specs(pool_size, seed=SERVE_POOL_SEED)
+    arrivals, draws = serving_draws(rate, seconds, pool_size, seed)
+    store = {S.PREC_FP32: S.STORE_FP32, S.PREC_BF16X3: S.STORE_BF16X3, S.PREC_FP16: S.STORE_FP16}[precision]
     with S.Context(0, hbm_budget_bytes=int(budget)) as ctx:
         t0 = time.time()
         pool = [ctx.generate(sp["m"], sp["n"], sp["seed"], theta=THETA, spike_nnz=SPARSITY, storage=store,
@@ -267,13 +310,8 @@ def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5)
             now = time.time() - t0
             if a > now:
                 time.sleep(a - now)
-            M, sp = pool[pi], specs[pi]
-            sv = min(sv, M.n - 1)
-            st = S.OptimizationSettings(formulation=f, constraintParameter=sv,
-                                        penaltyParameter=noise_gamma(f, sv, M.m, M.n), tolerance=1e-5,
-                                        maximumIterations=200, totalStartingPoints=B, batchSize=B,
-                                        precision=precision, start=S.START_GAUSSIAN, seed=100000 + k,
-                                        check_every=4)
+            M = pool[pi]
+            st = serving_settings(S, precision, specs[pi], f, sv, B, k)
             jobs.append((M.submit(st, per_sp=False), B))
         lat, sp_its, ev, flops = [], 0, [], 0.0
         for (j, B), (pi, _, _, _) in zip(jobs, draws):

@@ -330,6 +330,19 @@ int spca_wait(spca_ctx ctx, spca_job job, float* x_out, spca_stats* stats, spca_
 int spca_resolve_support(spca_ctx ctx, spca_matrix mat, int formulation, const float* x, int max_iter, double tol,
                          double* y_out, double* ev_out, int* iters_out);
 
+/* Product forms a dense solve of these settings on an m x n matrix runs, decided on the host
+ * alone (no device call).  *ax_form: the dense A*X form (the precision; sparse iterates may
+ * still be gathered in FP64 at run time).  *atz_form: the A^T*Z form, SPCA_FORM_FIX8 for the
+ * exact fixed-point A^T sgn(z) (L1 variance, FP32/bf16x3, exact_sign_products, and fewer
+ * than 2^18 rows after padding to 256), else the precision.  Returns SPCA_EINVAL for a bad
+ * formulation or precision, or n >= 2^23 (the limit of the step kernel's fixed-point sums),
+ * which spca_solve rejects too.  No reference counterpart (cuBLAS picks its kernels). */
+#define SPCA_FORM_FP32 0
+#define SPCA_FORM_BF16X3 1
+#define SPCA_FORM_FP16 2
+#define SPCA_FORM_FIX8 4
+int spca_product_forms(const spca_settings* settings, int m, int n, int* ax_form, int* atz_form);
+
 /* ---- reference settings helpers ------------------------------------------- */
 /* reference OptimizationSettings defaults (c++/src/class/optimization_settings.h:101-123) */
 void spca_settings_default(spca_settings* st);

@@ -45,12 +45,36 @@ static unsigned SFX(nnz)(const R* x, int n) {
 
 #define ORACLE_ROW_BLOCK 512
 
+/* Host-BLAS product over the active columns only (timing backend): the active columns of the
+ * right operand are packed, multiplied in one gemm, and unpacked into their own columns, so
+ * per-SP termination costs the active SPs' work alone.  trans: A^T (n x m) instead of A. */
+static void SFX(gemm_packed)(int trans, const R* A, int ld, int rows, int inner, const R* X, int ldx, R* Z,
+                             int ldz, int nb, const unsigned char* active) {
+    int na = 0;
+    for (int j = 0; j < nb; j++) na += active[j] != 0;
+    if (na == 0) return;
+    R* Xc = (R*)malloc(sizeof(R) * (size_t)inner * na);
+    R* Zc = (R*)malloc(sizeof(R) * (size_t)rows * na);
+    for (int j = 0, c = 0; j < nb; j++)
+        if (active[j]) memcpy(Xc + (size_t)inner * c++, X + (long)j * ldx, sizeof(R) * inner);
+    SFX(cblas_gemm)(ORACLE_COLMAJOR, trans ? ORACLE_TRANS : ORACLE_NOTRANS, ORACLE_NOTRANS, rows, na, inner, (R)1,
+                    A, ld, Xc, inner, (R)0, Zc, rows);
+    for (int j = 0, c = 0; j < nb; j++)
+        if (active[j]) memcpy(Z + (long)j * ldz, Zc + (size_t)rows * c++, sizeof(R) * rows);
+    free(Xc);
+    free(Zc);
+}
+
 /* Z[:, j] = A * X[:, j] for columns j with active[j] (active may be NULL). */
 static void SFX(gemm_ax)(const oracle_settings* st, const R* A, int ld, int m, int n,
                          const R* X, int ldx, R* Z, int ldz, int nb, const unsigned char* active) {
     if (st->gemm_backend == 1 && SFX(cblas_gemm) && active == NULL) {
         SFX(cblas_gemm)(ORACLE_COLMAJOR, ORACLE_NOTRANS, ORACLE_NOTRANS, m, nb, n, (R)1, A, ld, X, ldx,
                         (R)0, Z, ldz);
+        return;
+    }
+    if (st->gemm_backend == 1 && SFX(cblas_gemm)) {
+        SFX(gemm_packed)(0, A, ld, m, n, X, ldx, Z, ldz, nb, active);
         return;
     }
     int nblk = (m + ORACLE_ROW_BLOCK - 1) / ORACLE_ROW_BLOCK;
@@ -80,6 +104,10 @@ static void SFX(gemm_atz)(const oracle_settings* st, const R* A, int ld, int m, 
     if (st->gemm_backend == 1 && SFX(cblas_gemm) && active == NULL) {
         SFX(cblas_gemm)(ORACLE_COLMAJOR, ORACLE_TRANS, ORACLE_NOTRANS, n, nb, m, (R)1, A, ld, Z, ldz,
                         (R)0, V, ldv);
+        return;
+    }
+    if (st->gemm_backend == 1 && SFX(cblas_gemm)) {
+        SFX(gemm_packed)(1, A, ld, n, m, Z, ldz, V, ldv, nb, active);
         return;
     }
     int nblk = (n + ORACLE_ROW_BLOCK - 1) / ORACLE_ROW_BLOCK;

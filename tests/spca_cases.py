@@ -70,3 +70,24 @@ def tie_margin(A, X_prev, f, s, gamma, ztol=0.0):
         else:
             out[j] = np.min(np.abs(a - gamma)) / gamma
     return out if zrel is None else np.where(zrel < ztol, 0.0, out)
+
+
+def trajectory_tie_margin(A, X0, f, s, gamma, steps, ztol=0.0, gemm_backend=0, threads=8):
+    """Smallest one-step tie margin (tie_margin) over the FP64 oracle's trajectory from X0, per SP:
+    steps 0 .. steps-1 each start from the oracle's own iterate, advanced one iteration at a time
+    (the iteration depends on the iterate alone, so the chain is the full solve's trajectory).
+    An SP whose solve ends on another support than the oracle's must have passed a step whose
+    margin is within the product's rounding: a documented magnitude tie.  NaN margins (an SP
+    zeroed out by a penalty) do not count as ties.  gemm_backend 1 advances the chain with host
+    BLAS (faster on large matrices; its sums differ from the fixed-order solve's by rounding).
+    Returns (margins, final FP64 iterates)."""
+    A64 = A.astype(np.float64)
+    X = np.asarray(X0, dtype=np.float64)
+    out = np.full(X.shape[1], np.inf)
+    for _ in range(steps):
+        with np.errstate(invalid="ignore", divide="ignore"):
+            out = np.fmin(out, tie_margin(A64, X, f, s, gamma, ztol=ztol))
+        o = O.solve(A64, f, s=s, gamma=gamma, tol=0.0, max_iter=1, total_sp=X.shape[1], X0=X, want_X=True,
+                    lambda_rule=O.LAMBDA_GPU, gemm_backend=gemm_backend, threads=threads)
+        X = o["X"]
+    return out, X

@@ -75,3 +75,43 @@ def test_null_arguments_are_rejected_without_a_device():
     assert L.spca_ctx_destroy(None) == S.SPCA_EINVAL
     assert L.spca_solve(None, 0, None, None, None, None, None, None) == S.SPCA_EINVAL
     assert L.spca_last_error(None) == b"null context"
+
+
+@pytest.mark.parametrize("f", range(8))
+@pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3, S.PREC_FP16])
+def test_product_forms_and_the_fixed_point_row_limit(f, prec):
+    """A^T*Z of the L1-variance formulations (z = sgn(A x)) runs the exact fixed-point product in
+    FP32 and bf16x3 while its int32 sums cannot overflow (padded rows < 2^18); taller matrices,
+    fp16 and the L2-variance formulations run the precision's own product."""
+    st = S.OptimizationSettings(formulation=f, precision=prec)
+    l1v = f in (1, 3, 5, 7)
+    exact = l1v and prec in (S.PREC_FP32, S.PREC_BF16X3)
+    assert st.product_forms(4096, 65536) == (prec, S.FORM_FIX8 if exact else prec)
+    # 261888 rows pad to 2^18 - 256; one more row pads to 2^18, past the accumulators' bound
+    assert st.product_forms(261888, 4096)[1] == (S.FORM_FIX8 if exact else prec)
+    assert st.product_forms(261889, 4096)[1] == prec
+    st.exact_sign_products = 0
+    assert st.product_forms(4096, 65536) == (prec, prec)
+
+
+def test_product_forms_rejects_what_a_solve_rejects():
+    st = S.OptimizationSettings()
+    assert st.product_forms(100, 2 ** 23 - 1) == (0, 0)
+    with pytest.raises(S.SpcaError) as e:
+        st.product_forms(100, 2 ** 23)  # the step kernel's fixed-point sums need n < 2^23
+    assert e.value.code == S.SPCA_EINVAL
+    L = S.lib()
+    c = st._c()
+    assert L.spca_product_forms(None, 10, 10, None, None) == S.SPCA_EINVAL
+    assert L.spca_product_forms(ctypes.byref(c), 0, 10, None, None) == S.SPCA_EINVAL
+    c.formulation = 8
+    assert L.spca_product_forms(ctypes.byref(c), 10, 10, None, None) == S.SPCA_EINVAL
+
+
+def test_double_precision_settings_are_rejected():
+    """The reference's useDoublePrecision (optimization_settings.h:71-99) selects a double solve,
+    which this solver does not have: the settings never reach the library as an FP32 solve."""
+    st = S.OptimizationSettings(useDoublePrecision=True)
+    with pytest.raises(S.SpcaError) as e:
+        st._c()
+    assert e.value.code == S.SPCA_EINVAL

@@ -72,14 +72,25 @@ def test_reference_defaults_and_shape(small_csv):
 
 def test_options_batch_rule_and_rounding(small_csv):
     e = _echo(["-i", small_csv, "-o", "x", "-f", 6, "-g", 0.25, "-m", 7, "-t", 1e-4, "-l", 10, "-r", 16, "-u", 1,
-               "-d", 1, "-p", "bf16x3", "-s", 3])
+               "-p", "bf16x3", "-s", 3])
     # a batch larger than the total raises the total to the batch (option_console_parser.h:148-150)
     assert int(e["total_sp"]) == 16 and int(e["batch"]) == 16
     assert float(e["gamma"]) == 0.25 and int(e["max_iter"]) == 7 and float(e["tol"]) == 1e-4
-    assert int(e["otf"]) == 1 and int(e["double"]) == 1 and int(e["precision"]) == 1 and int(e["s"]) == 3
+    assert int(e["otf"]) == 1 and int(e["double"]) == 0 and int(e["precision"]) == 1 and int(e["s"]) == 3
     # the total is rounded up to whole batches
     e = _echo(["-i", small_csv, "-o", "x", "-f", 2, "-l", 70, "-r", 32])
     assert int(e["total_sp"]) == 96 and int(e["batch"]) == 32
+
+
+@pytest.mark.parametrize("extra", [[], ["-e"]])
+def test_double_precision_is_rejected(small_csv, tmp_path, extra):
+    """-d asks the reference for a double solve (gpu_console.cu:97-101); this solver has no
+    double products, so the console exits with status 2 and says so, before any GPU call and
+    without writing result files."""
+    prefix = str(tmp_path / "res")
+    p = _run(["-i", small_csv, "-o", prefix, "-f", 2, "-d", "double"] + extra, check=False)
+    assert p.returncode == 2 and "double precision" in p.stderr
+    assert not os.path.exists(prefix + "_x") and not os.path.exists(prefix + "_optimizationStatistics")
 
 
 def test_csv_without_trailing_newline(tmp_path):

@@ -284,3 +284,26 @@ def test_cpu_lambda_rule_breaks_ascent():
     b = O.solve(A, 2, s=6, tol=0, max_iter=6, total_sp=8, threads=2, want_X=True)
     assert (a["X"][:, 2] != 0).sum() == 1
     assert b["sp_val"][2] < 0.7 * a["sp_val"][2]
+
+
+@pytest.mark.parametrize("form", [0, 3, 4, 7])
+def test_blas_backend_with_per_sp_termination_matches_loops(form):
+    """The host-BLAS backend packs the columns still iterating under per-SP termination; per SP it
+    gives what the fixed-order loops give (iterations and nonzeros equal, values to rounding)."""
+    rng = np.random.default_rng(5)
+    m, n, B = 60, 300, 24
+    A = rng.standard_normal((m, n))
+    A[:, :5] += 2.0 * rng.standard_normal((m, 1))
+    X0 = rng.standard_normal((n, B))
+    X0 /= np.linalg.norm(X0, axis=0)
+    g = {4: 4.0, 7: 2.0 * np.sqrt(m)}.get(form, 0.0)
+    kw = dict(s=8, gamma=g, tol=1e-7, max_iter=60, total_sp=B, X0=X0, termination=O.TERM_PER_SP, want_X=True,
+              lambda_rule=O.LAMBDA_GPU)
+    a = O.solve(A, form, gemm_backend=0, **kw)
+    b = O.solve(A, form, gemm_backend=1, **kw)
+    assert O.cblas_name(), "no host BLAS"
+    assert len(set(a["sp_iters"])) > 1  # the SPs stop at different iterations
+    np.testing.assert_array_equal(a["sp_iters"], b["sp_iters"])
+    np.testing.assert_array_equal(a["sp_nnz"], b["sp_nnz"])
+    np.testing.assert_allclose(b["sp_val"], a["sp_val"], rtol=1e-12)
+    np.testing.assert_allclose(b["X"], a["X"], atol=1e-12)

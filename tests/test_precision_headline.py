@@ -1,12 +1,12 @@
 """FP32 and bf16x3 against FP64 at the headline shape, all 8 formulations.  Needs an MI355X;
-slow (the FP64 oracle solves on the host, about 10-20 s per formulation on 16 threads).
+slow (the FP64 oracle solves on the host with host BLAS).
 
 4096 x 65536 spiked model (bench.make_problem), s = 256, gamma per penalised formulation as
-the bench calibrates it, tol = 1e-6, at most 200 iterations, on the first 128 of the bench's
+the bench calibrates it, tol = 1e-6, at most 200 iterations, on the first 512 of the bench's
 unit Gaussian starting points.  Per starting point: the explained variance of the GPU's final
 iterate (FP64, on the device) against that of the FP64 oracle's final iterate, and whether the
-supports are identical.  The explained variance is compared over the starting points the
-oracle has converged (its last relative change below tol); the L1 penalty with L2 variance
+supports are identical.  The explained variance is compared over the starting points both
+sides converged (their last relative change below tol); the L1 penalty with L2 variance
 leaves some points still moving after 200 iterations, where the two trajectories are compared
 mid-way and differ by up to ~1e-4 without a change of support.
 
@@ -20,9 +20,14 @@ another fixed point.  Two kinds of tie occur:
     A^T sgn(z) runs in exact fixed point: an FP32 accumulation of it put ~4e-7 of error into
     x, enough to flip such an entry at the next iteration.
 and the entries of z within the dense product's rounding of zero are recomputed in FP64.
-Measured (identical supports): all four L1-variance formulations 128/128 in FP32 and
-bf16x3; L2 variance FP32 127/128 (L0 constraint, L1 penalty) or 128/128, bf16x3 126/128 to
-128/128.  The bars allow two (FP32) and three (bf16x3) starting points below 100%.
+
+Sample: 512 of the bench's 1024 starting points, with per-SP termination on both sides (each SP
+stops once its own relative change is below tol, as the reference OTF driver terminates), so
+the FP64 oracle's host work is the SPs' own iterations rather than the slowest SP's.
+Bars: identical supports on >= 99% of the SPs (at most 5 of 512),
+and the FP64 explained variance within 1e-6 (FP32) / 1e-5 (bf16x3) on every SP that converged
+on the oracle's support.  An SP whose stop lands one iteration apart (its last change within
+rounding of tol) is compared with the oracle run to the GPU's own iteration count.
 """
 import os
 
@@ -34,9 +39,9 @@ import spca_mi355x as S
 
 pytestmark = [pytest.mark.gpu, pytest.mark.slow]
 
-NSP = 128
-# (EV within tol on at least this share, identical supports on at least this share)
-BARS = {S.PREC_FP32: (1e-6, 0.98, 0.98), S.PREC_BF16X3: (1e-5, 0.97, 0.97)}
+NSP = 512
+EV_TOL = {S.PREC_FP32: 1e-6, S.PREC_BF16X3: 1e-5}
+SUPPORT_BAR = 0.99
 
 
 @pytest.fixture(scope="module")
@@ -49,35 +54,47 @@ def headline(ctx):
     M.release()
 
 
+def _ev(A64, X, l2):
+    nrm = np.linalg.norm(X, axis=0)
+    Z = A64 @ (X / np.where(nrm > 0, nrm, 1.0))
+    return np.sum(Z * Z, axis=0) if l2 else np.sum(np.abs(Z), axis=0)
+
+
 @pytest.mark.parametrize("f", range(8))
 def test_headline_precision_against_fp64(headline, f):
     A, A64, X0, gam, M = headline
-    threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+    threads = min(os.cpu_count() or 1, 16)
     gamma = gam.get(f, 0.0)
-    o = O.solve(A64, f, s=256, gamma=gamma, tol=1e-6, max_iter=200, total_sp=NSP, batch=NSP,
-                X0=X0.astype(np.float64), threads=threads, gemm_backend=1, want_X=True, lambda_rule=O.LAMBDA_GPU)
-    Xo = o["X"]
-    nrm = np.linalg.norm(Xo, axis=0)
-    Zo = A64 @ (Xo / np.where(nrm > 0, nrm, 1.0))
     l2 = f in (0, 2, 4, 6)
-    ev_o = np.sum(Zo * Zo, axis=0) if l2 else np.sum(np.abs(Zo), axis=0)
-    conv = o["sp_err"] < 1e-6
-    res = {}
+    common = dict(s=256, gamma=gamma, threads=threads, gemm_backend=1, want_X=True, lambda_rule=O.LAMBDA_GPU)
+    o = O.solve(A64, f, tol=1e-6, max_iter=200, total_sp=NSP, batch=NSP, X0=X0.astype(np.float64),
+                termination=O.TERM_PER_SP, **common)
+    ev_o = _ev(A64, o["X"], l2)
+    conv_o = o["sp_iters"] < 200
+    print(f"\nf={f}: oracle {o['loop_s']:.1f} s, iterations per SP median {np.median(o['sp_iters']):.0f} max "
+          f"{o['sp_iters'].max()}, converged {conv_o.sum()}/{NSP}", flush=True)
     for prec in (S.PREC_FP32, S.PREC_BF16X3):
         st = S.OptimizationSettings(formulation=f, constraintParameter=256, penaltyParameter=gamma, tolerance=1e-6,
-                                    maximumIterations=200, totalStartingPoints=NSP, batchSize=NSP, precision=prec)
+                                    maximumIterations=200, totalStartingPoints=NSP, batchSize=NSP, precision=prec,
+                                    termination=S.TERM_PER_SP)
         stats = S.OptimizationStatistics()
         _, Xg = M.solve(st, X0=X0, stats=stats, want_X=True)
-        rel_all = np.abs(stats.sp_ev_fp64 - ev_o) / ev_o
-        rel = rel_all[conv]
-        same = np.all((Xg != 0) == (Xo != 0), axis=0)
-        res[prec] = (rel, same)
-        print(f"\nf={f} prec={prec}: iterations gpu {stats.it} oracle {o['it']}; median nnz {np.median(o['sp_nnz'])}; "
-              f"oracle converged {conv.sum()}/{NSP}; EV rel (converged) max {rel.max():.2e} median "
-              f"{np.median(rel):.2e}, within {BARS[prec][0]:.0e}: {np.mean(rel <= BARS[prec][0]):.3f}, within 1e-5: "
-              f"{np.mean(rel <= 1e-5):.3f}; EV rel (all) max {rel_all.max():.2e}; identical supports "
-              f"{same.mean():.3f}; off-support SPs {np.nonzero(~same)[0].tolist()}", flush=True)
-    for prec, (rel, same) in res.items():
-        ev_tol, ev_bar, supp_bar = BARS[prec]
-        assert np.mean(rel <= ev_tol) >= ev_bar, (prec, np.sort(rel)[-4:])
-        assert same.mean() >= supp_bar, (prec, np.nonzero(~same)[0])
+        same = np.all((Xg != 0) == (o["X"] != 0), axis=0)
+        conv = conv_o & (stats.iters < 200)
+        ref_ev = ev_o.copy()
+        # an SP that stopped one iteration apart: the oracle's iterate at the GPU's own count
+        late = np.nonzero(conv & same & (stats.iters != o["sp_iters"]))[0]
+        if late.size:
+            for it in np.unique(stats.iters[late]):
+                cols = late[stats.iters[late] == it]
+                r = O.solve(A64, f, tol=0.0, max_iter=int(it), total_sp=cols.size, batch=cols.size,
+                            X0=X0[:, cols].astype(np.float64), **common)
+                ref_ev[cols] = _ev(A64, r["X"], l2)
+        rel = np.abs(stats.sp_ev_fp64 - ref_ev) / ref_ev
+        chk = conv & same
+        print(f"f={f} prec={prec}: identical supports {same.sum()}/{NSP} (off: {np.nonzero(~same)[0].tolist()}); "
+              f"stops one iteration apart {late.size}; EV rel over {chk.sum()} converged on-support SPs max "
+              f"{rel[chk].max():.2e} median {np.median(rel[chk]):.2e}; unconverged GPU/oracle "
+              f"{(stats.iters >= 200).sum()}/{(~conv_o).sum()}", flush=True)
+        assert same.sum() >= int(np.ceil(SUPPORT_BAR * NSP)), (prec, np.nonzero(~same)[0])
+        assert np.all(rel[chk] <= EV_TOL[prec]), (prec, np.sort(rel[chk])[-4:], np.nonzero(chk & (rel > EV_TOL[prec]))[0])
