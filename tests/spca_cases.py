@@ -48,8 +48,10 @@ def tie_margin(A, X_prev, f, s, gamma, ztol=0.0):
     # L1 variance maps z to sgn z: an entry of z within the product's rounding (ztol, relative
     # to the rms of z) of zero may flip sign, which moves every entry of A^T sgn(z); such an
     # SP counts as a tie (margin 0)
+    # (an exact zero of z is no tie: its terms are all zero, so every product form gives 0 too)
     with np.errstate(invalid="ignore", divide="ignore"):
-        zrel = np.min(np.abs(Z), axis=0) / np.sqrt(np.mean(Z * Z, axis=0)) if f not in L2_VAR else None
+        zrel = (np.min(np.where(Z != 0, np.abs(Z), np.inf), axis=0) / np.sqrt(np.mean(Z * Z, axis=0))
+                if f not in L2_VAR else None)
     if f in L2_VAR:
         if f >= 4:
             Z = Z / np.linalg.norm(Z, axis=0)
@@ -91,3 +93,35 @@ def trajectory_tie_margin(A, X0, f, s, gamma, steps, ztol=0.0, gemm_backend=0, t
                     lambda_rule=O.LAMBDA_GPU, gemm_backend=gemm_backend, threads=threads)
         X = o["X"]
     return out, X
+
+
+SOFT = (2, 3, 6, 7)  # formulations whose step is a soft threshold x = sgn(v)(|v| - lam)_+
+
+
+def soft_condition(A, X, f, s, gamma):
+    """Per SP, how much the soft threshold amplifies a relative error of v into x: kappa =
+    |v_S|_2 / |S_lam(v)|_2 over the kept set S, for the step from the (FP64) iterate X.  The
+    step subtracts lam from every kept entry, so where the kept |v_i| sit barely above lam (data
+    whose entries are all alike) an error of eps |v_i| in v becomes an error of about eps kappa in
+    x.  1 for the hard-threshold formulations, whose kept entries are v's own."""
+    out = np.ones(X.shape[1])
+    if f not in SOFT:
+        return out
+    A = A.astype(np.float64)
+    Z = A @ X
+    if f in L2_VAR:
+        if f >= 4:
+            Z = Z / np.linalg.norm(Z, axis=0)
+    else:
+        Z = np.sign(Z)
+    V = A.T @ Z
+    for j in range(V.shape[1]):
+        if f in (2, 3):
+            _, _, lam, _ = O.soft_threshold(V[:, j], s, O.LAMBDA_GPU)
+        else:
+            lam = gamma
+        a = np.abs(V[:, j])
+        kept = a > lam
+        d = np.linalg.norm(a[kept] - lam)
+        out[j] = np.linalg.norm(a[kept]) / d if d > 0 else np.inf
+    return out

@@ -26,7 +26,7 @@ import pytest
 
 import oracle as O
 import spca_mi355x as S
-from spca_cases import explained_variance, trajectory_tie_margin
+from spca_cases import explained_variance, soft_condition, trajectory_tie_margin
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CONSOLE = os.path.join(os.path.dirname(S.LIB_PATH), "spca_console")
@@ -60,7 +60,13 @@ def test_fixtures_are_the_reference_files():
 UT_SP, UT_S, UT_TOL, UT_IT = 1024, 20, 1e-4, 100
 EV_TOL = {S.PREC_FP32: 1e-6, S.PREC_BF16X3: 1e-5}
 VAL_TOL = {S.PREC_FP32: 1e-5, S.PREC_BF16X3: 1e-4}
+# relative rounding of v = A^T z in each product form: FP32 accumulation, bf16x3 split
+V_EPS = {S.PREC_FP32: 1e-7, S.PREC_BF16X3: 1e-6}
+# Tie tolerance: a step whose threshold and nearest competitor are within the product's
+# rounding.  The L1-variance formulations' A^T sgn(z) is the exact fixed-point product (error
+# ~2^-25 relative), so their ties are those within 1e-6; the others use the product's own.
 TIE_TOL = {S.PREC_FP32: 1e-5, S.PREC_BF16X3: 1e-4}
+TIE_TOL_EXACT = 1e-6
 ZTOL = {S.PREC_FP32: 1e-6, S.PREC_BF16X3: 3e-5}
 
 
@@ -93,22 +99,28 @@ def test_gpu_unit_test_cluster(cluster, f, prec):
                 X0=X0.astype(np.float64), want_X=True, lambda_rule=O.LAMBDA_GPU, threads=threads, gemm_backend=1)
     margin, _ = trajectory_tie_margin(A, X0, f, UT_S, 0.0, int(o["it"]), ztol=ZTOL[prec], gemm_backend=1,
                                       threads=threads)
-    tied = margin < TIE_TOL[prec]
+    tied = margin < (TIE_TOL_EXACT if f in (1, 3, 5, 7) else TIE_TOL[prec])
     alive = np.all(np.isfinite(o["X"]), axis=0) & np.any(o["X"] != 0, axis=0)
     ok = alive & ~tied
+    # explained-variance bar per SP: 1e-6 (FP32) / 1e-5 (bf16x3), or the product's rounding of v
+    # amplified by the soft threshold's condition where that is larger (this data's columns
+    # are all alike, so lambda sits just below the kept |v_i|: kappa ~40 for the L1 constraint)
+    kappa = soft_condition(A, o["X"], f, UT_S, 0.0)
+    ev_bar = np.maximum(EV_TOL[prec], kappa * V_EPS[prec])
     supp = np.all((Xg != 0) == (o["X"] != 0), axis=0)
     ev_o = np.full(UT_SP, np.nan)
     ev_o[alive] = explained_variance(A, o["X"][:, alive], f)
     with np.errstate(invalid="ignore", divide="ignore"):
         ev_rel = np.abs(stats.sp_ev_fp64 - ev_o) / ev_o
         vrel = np.abs(stats.values - o["sp_val"]) / np.abs(o["sp_val"])
+    mx = lambda a: float(np.nanmax(a)) if a.size else 0.0
     print(f"  iterations {stats.it}/{o['it']} fval {stats.fval:.9g}/{o['fval']:.9g} alive {alive.sum()} tied "
-          f"{tied.sum()} supports {supp[alive].mean():.4f} EV rel max {np.nanmax(ev_rel[ok]):.2e} value rel max "
-          f"{np.nanmax(vrel[ok]):.2e}", flush=True)
+          f"{tied.sum()} supports {supp[alive].mean():.4f} EV rel max {mx(ev_rel[ok]):.2e} (bar max "
+          f"{mx(ev_bar[ok]):.1e}, kappa max {mx(kappa):.1f}) value rel max {mx(vrel[ok]):.2e}", flush=True)
     assert abs(int(stats.it) - int(o["it"])) <= 1
     assert np.all(supp[ok]), np.nonzero(ok & ~supp)[0]
-    assert np.all(ev_rel[ok] <= EV_TOL[prec]), np.sort(ev_rel[ok])[-4:]
-    assert np.all(vrel[ok] <= VAL_TOL[prec]), np.sort(vrel[ok])[-4:]
+    assert np.all(ev_rel[ok] <= ev_bar[ok]), np.sort(ev_rel[ok] / ev_bar[ok])[-4:]
+    assert np.all(vrel[ok] <= np.maximum(VAL_TOL[prec], kappa[ok] * V_EPS[prec])), np.sort(vrel[ok])[-4:]
     assert abs(stats.fval - o["fval"]) <= VAL_TOL[prec] * abs(o["fval"])
 
 
