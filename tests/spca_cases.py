@@ -33,7 +33,7 @@ def default_gamma(f, m):
 
 def explained_variance(A, X, f):
     """FP64 EV of each normalised column: |A x|_2^2 (L2 variance) or |A x|_1."""
-    A = A.astype(np.float64)
+    A = np.asarray(A, dtype=np.float64)  # no copy when A is FP64 already
     Xn = X.astype(np.float64) / np.linalg.norm(X.astype(np.float64), axis=0)
     Z = A @ Xn
     return np.sum(Z * Z, axis=0) if f in L2_VAR else np.sum(np.abs(Z), axis=0)
@@ -43,7 +43,7 @@ def tie_margin(A, X_prev, f, s, gamma, ztol=0.0):
     """Relative gap between the threshold and the nearest competing value in the last
     step, per SP, computed from the oracle's previous iterate (FP64).  A support that
     differs only where this gap is within rounding is a documented magnitude tie."""
-    A = A.astype(np.float64)
+    A = np.asarray(A, dtype=np.float64)  # no copy when A is FP64 already
     Z = A @ X_prev
     # L1 variance maps z to sgn z: an entry of z within the product's rounding (ztol, relative
     # to the rms of z) of zero may flip sign, which moves every entry of A^T sgn(z); such an
@@ -83,7 +83,7 @@ def trajectory_tie_margin(A, X0, f, s, gamma, steps, ztol=0.0, gemm_backend=0, t
     zeroed out by a penalty) do not count as ties.  gemm_backend 1 advances the chain with host
     BLAS (faster on large matrices; its sums differ from the fixed-order solve's by rounding).
     Returns (margins, final FP64 iterates)."""
-    A64 = A.astype(np.float64)
+    A64 = np.asarray(A, dtype=np.float64)
     X = np.asarray(X0, dtype=np.float64)
     out = np.full(X.shape[1], np.inf)
     for _ in range(steps):
@@ -107,7 +107,7 @@ def soft_condition(A, X, f, s, gamma):
     out = np.ones(X.shape[1])
     if f not in SOFT:
         return out
-    A = A.astype(np.float64)
+    A = np.asarray(A, dtype=np.float64)  # no copy when A is FP64 already
     Z = A @ X
     if f in L2_VAR:
         if f >= 4:
