@@ -570,6 +570,52 @@ int ensure_workspace(spca_ctx ctx, Engine& E, int np, int mp, int bp, int splits
     return ensure_ws(ctx, E.ws, E.stream, np, mp, bp, splits, err);
 }
 
+// The allocation-free solve's memory: consecutive 256-byte aligned blocks of the fixed device
+// array.  With base == nullptr, take() only counts, so the same code sizes a request.
+struct FixedScratch {
+    unsigned char* base = nullptr;
+    size_t cap = 0;
+    size_t used = 0;
+    template <class T>
+    T* take(size_t count) {
+        const size_t off = (used + 255) & ~(size_t)255;
+        used = off + count * sizeof(T);
+        return base && used <= cap ? (T*)(base + off) : nullptr;
+    }
+};
+
+// The workspace of a solve of this shape in fixed scratch: the buffers ensure_ws allocates, minus
+// the 16-bit operand forms the precision and A^T*Z form do not read.  Z and its operand forms
+// are cleared (padded rows must read as zero in A^T*Z); load_batch clears the rest.  Returns false
+// when the scratch is too small (sc.used then holds the bytes the shape needs).
+bool carve_ws(Workspace& w, FixedScratch& sc, int np, int mp, int bp, int splits, int precision, int zprec,
+              hipStream_t s, hipError_t* e) {
+    w = Workspace();
+    w.np = np;
+    w.mp = mp;
+    w.bp = bp;
+    w.splits = splits;
+    const size_t xs = (size_t)np * bp, zs = (size_t)mp * bp;
+    w.X = sc.take<float>(xs);
+    if (precision) w.Xhi = sc.take<uint16_t>(xs);
+    if (precision == 1) w.Xlo = sc.take<uint16_t>(xs);
+    w.Zpart = sc.take<float>(zs * splits);
+    w.Z = sc.take<float>(zs);
+    if (zprec) w.Zhi = sc.take<uint16_t>(zs);
+    if (zprec == 1) w.Zlo = sc.take<uint16_t>(zs);
+    w.V = sc.take<float>(xs);
+    w.fstate = sc.take<float>((size_t)4 * bp);
+    w.istate = sc.take<int>((size_t)3 * bp + state_flag_ints(bp));
+    w.ev = sc.take<double>((size_t)bp);
+    w.corr = sc.take<double>((size_t)4 * bp);
+    *e = hipSuccess;
+    if (!sc.base || sc.used > sc.cap) return false;
+    *e = launch_fill_zero(w.Z, zs * sizeof(float), s);
+    if (*e == hipSuccess && w.Zhi) *e = launch_fill_zero(w.Zhi, zs * sizeof(uint16_t), s);
+    if (*e == hipSuccess && w.Zlo) *e = launch_fill_zero(w.Zlo, zs * sizeof(uint16_t), s);
+    return true;
+}
+
 int engine_init(Engine& E) {
     if (hipStreamCreateWithFlags(&E.stream, hipStreamNonBlocking) != hipSuccess) return SPCA_EDEVICE;
     if (hipHostMalloc((void**)&E.h_done, sizeof(int) * 8, hipHostMallocDefault) != hipSuccess) return SPCA_ENOMEM;
@@ -811,19 +857,20 @@ int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& s
                const float* hX, int shift = 0) {
     const Operand xop = x_operand(w, st.precision);
     const SpState sp = state_of(w);
-    HIP_TRY(ctx, hipMemsetAsync(w.X, 0, sizeof(float) * (size_t)w.np * w.bp, s));
+    // cleared by a kernel: the fixed-scratch solve's buffers keep the previous call's contents
+    HIP_TRY(ctx, launch_fill_zero(w.X, sizeof(float) * (size_t)w.np * w.bp, s));
     if (st.start == SPCA_START_GAUSSIAN)
         HIP_TRY(ctx, launch_gen_starts(w.X, w.np, n, batch, shift, st.seed, s));
     else
         HIP_TRY(ctx, hipMemcpy2DAsync(w.X, sizeof(float) * w.np, hX, sizeof(float) * n, sizeof(float) * n, batch,
                                       hipMemcpyHostToDevice, s));
     if (st.precision) {
-        HIP_TRY(ctx, hipMemsetAsync(w.Xhi, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
-        HIP_TRY(ctx, hipMemsetAsync(w.Xlo, 0, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
+        HIP_TRY(ctx, launch_fill_zero(w.Xhi, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
+        if (st.precision == 1) HIP_TRY(ctx, launch_fill_zero(w.Xlo, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
         HIP_TRY(ctx, launch_x_operand(xop, st.precision, w.np, batch, s));
     }
-    HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
-    HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + state_flag_ints(w.bp)), s));
+    HIP_TRY(ctx, launch_fill_zero(w.fstate, sizeof(float) * 4 * (size_t)w.bp, s));
+    HIP_TRY(ctx, launch_fill_zero(w.istate, sizeof(int) * (3 * (size_t)w.bp + state_flag_ints(w.bp)), s));
     std::vector<int> act(w.bp, 0);
     std::fill(act.begin(), act.begin() + batch, 1);
     HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
@@ -833,17 +880,34 @@ int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& s
 
 // FP64 explained variance of `batch` columns of X (device) into ev (device); the partial
 // sums live in a scratch block that returns to the cache once the kernels are done.
-int run_ev(spca_ctx ctx, hipStream_t s, Matrix* M, const float* X, int ldx, int batch, int formulation, double* ev) {
-    double* scratch = nullptr;
-    uint64_t sb = 0;
+int run_ev(spca_ctx ctx, hipStream_t s, Matrix* M, const float* X, int ldx, int batch, int formulation, double* ev,
+           FixedScratch* fx = nullptr) {
     const int l1 = spca_is_l1_variance(formulation);
-    int rc = dev_alloc(ctx, &scratch, M->sparse ? csr_ev_scratch_doubles(M->m, batch) : ev_scratch_doubles(M->m, M->n, batch),
+    if (fx) {
+        // the fixed array past the workspace, in groups of SPs whose partial sums fit (dense only)
+        hipError_t e = hipSuccess;
+        for (int j0 = 0; j0 < batch && e == hipSuccess;) {
+            int g = batch - j0;
+            while (g > 1 && ev_scratch_doubles(M->m, M->n, g) * sizeof(double) + 256 > fx->cap - fx->used) g = (g + 1) / 2;
+            FixedScratch rest = *fx;
+            double* work = rest.take<double>(ev_scratch_doubles(M->m, M->n, g));
+            if (!work) return fail(ctx, SPCA_EINVAL, "fixed scratch too small for the explained variance of one SP");
+            e = launch_ev_fp64(M->a32_cm, M->mp, M->m, M->n, X + (size_t)j0 * ldx, ldx, g, l1, ev + j0, work, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);  // the next group reuses the same memory
+            j0 += g;
+        }
+        if (e != hipSuccess) return fail(ctx, SPCA_EDEVICE, std::string("explained variance: ") + hipGetErrorString(e));
+        return SPCA_OK;
+    }
+    double* work = nullptr;
+    uint64_t sb = 0;
+    int rc = dev_alloc(ctx, &work, M->sparse ? csr_ev_scratch_doubles(M->m, batch) : ev_scratch_doubles(M->m, M->n, batch),
                        &sb);
     if (rc) return rc;
-    hipError_t e = M->sparse ? launch_csr_ev(sparse_of(M), X, ldx, batch, l1, ev, scratch, s)
-                             : launch_ev_fp64(M->a32_cm, M->mp, M->m, M->n, X, ldx, batch, l1, ev, scratch, s);
+    hipError_t e = M->sparse ? launch_csr_ev(sparse_of(M), X, ldx, batch, l1, ev, work, s)
+                             : launch_ev_fp64(M->a32_cm, M->mp, M->m, M->n, X, ldx, batch, l1, ev, work, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
-    dev_free(ctx, scratch, sb);
+    dev_free(ctx, work, sb);
     if (e != hipSuccess) return fail(ctx, SPCA_EDEVICE, std::string("explained variance: ") + hipGetErrorString(e));
     return SPCA_OK;
 }
@@ -865,7 +929,7 @@ struct Results {
 // final iterates if asked, and the best of the batch (first maximum, as
 // sparse_PCA_solver.h:212-231) against the best so far.
 int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const spca_settings& st, int shift, int batch,
-                  bool want_ev, spca_sp_result* per_sp, float* X_final, Results& R) {
+                  bool want_ev, spca_sp_result* per_sp, float* X_final, Results& R, FixedScratch* fx = nullptr) {
     const SpState sp = state_of(w);
     const int n = M->n;
     std::vector<float> hval(batch), herr(batch);
@@ -882,7 +946,7 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
     // per-SP explained variance when per-SP results are asked for; else only the batch's best
     // SP needs it (for stats), which is computed below once it is known
     if (want_ev && per_sp) {
-        int rc = run_ev(ctx, s, M, w.X, w.np, batch, st.formulation, w.ev);
+        int rc = run_ev(ctx, s, M, w.X, w.np, batch, st.formulation, w.ev, fx);
         if (rc) return rc;
         HIP_TRY(ctx, hipMemcpyAsync(hev.data(), w.ev, sizeof(double) * batch, hipMemcpyDeviceToHost, s));
     }
@@ -918,7 +982,7 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
         HIP_TRY(ctx, hipMemcpyAsync(R.x_best.data(), w.X + (size_t)sel * w.np, sizeof(float) * n,
                                     hipMemcpyDeviceToHost, s));
         if (want_ev && !per_sp) {
-            int rc = run_ev(ctx, s, M, w.X + (size_t)sel * w.np, w.np, 1, st.formulation, w.ev);
+            int rc = run_ev(ctx, s, M, w.X + (size_t)sel * w.np, w.np, 1, st.formulation, w.ev, fx);
             if (rc) return rc;
             HIP_TRY(ctx, hipMemcpyAsync(&hev[sel], w.ev, sizeof(double), hipMemcpyDeviceToHost, s));
             HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -1141,10 +1205,15 @@ int plan_for(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings& st, int bp
 // hipEvent pairs recorded on the solver stream and folded in after each host sync.
 struct PhaseTimer {
     Engine* eng;
+    bool on = true;   // off: no events are created or recorded (the allocation-free solve)
     size_t used = 0;  // events handed out since the last flush (always pairs)
     double ms[3] = {0, 0, 0};
     std::vector<int> phase;
     hipError_t pair(hipEvent_t* a, hipEvent_t* b, int ph) {
+        if (!on) {
+            *a = *b = nullptr;
+            return hipSuccess;
+        }
         while (eng->events.size() < used + 2) {
             hipEvent_t e;
             hipError_t r = hipEventCreate(&e);
@@ -1170,7 +1239,7 @@ struct PhaseTimer {
 // Checks a request and normalises its settings (clamp of s, batch rounding); makes sure the
 // matrix holds the storage the precision needs.  Returns the number of batches in *nb.
 int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const float* X0, const float* x_out,
-                     spca_settings* st_out, int* nb) {
+                     spca_settings* st_out, int* nb, bool may_add_storage = true) {
     spca_settings st = *st_in;
     if (!valid_formulation(st.formulation)) return fail(ctx, SPCA_EINVAL, "formulation must be 0..7");
     if (st.precision < 0 || st.precision > 2) return fail(ctx, SPCA_EINVAL, "precision must be 0..2");
@@ -1188,6 +1257,9 @@ int prepare_settings(spca_ctx ctx, Matrix* M, const spca_settings* st_in, const 
     unsigned need = st.precision == 0 ? SPCA_STORE_FP32 : st.precision == 1 ? SPCA_STORE_BF16X3 : SPCA_STORE_FP16;
     // the fixed-point product, and the FP32 row-major copy for the exact signs of z
     if (!M->sparse && uses_sign_products(st, M->mp)) need |= SPCA_STORE_FIX8 | SPCA_STORE_FP32;
+    if (!may_add_storage && !M->sparse && (M->storage & need) != need)
+        return fail(ctx, SPCA_EINVAL, "the matrix lacks storage form(s) " + std::to_string(need & ~M->storage) +
+                                          " this solve reads; add them (spca_matrix_add_storage) before the solve");
     int rc = M->sparse ? SPCA_OK : add_storage(ctx, M, need);
     if (rc) return rc;
     *nb = spca_settings_check(&st, (unsigned)M->n);
@@ -1206,26 +1278,51 @@ struct Deflation {
     int ncomp = 0;
 };
 
+// a phase-timing event, or nothing when phase timing is off
+hipError_t record(hipEvent_t e, hipStream_t s) { return e ? hipEventRecord(e, s) : hipSuccess; }
+
+// fx (the allocation-free solve): the workspace and the explained-variance partials come from
+// the fixed device array; the plan is the cached or heuristic one (no tuning run), no storage
+// form is added, phase times are not taken (no events are created), and batch mode only.
 int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, const float* X0, float* x_out,
-               spca_stats* stats, spca_sp_result* per_sp, float* X_final, const Deflation* defl = nullptr) {
+               spca_stats* stats, spca_sp_result* per_sp, float* X_final, const Deflation* defl = nullptr,
+               FixedScratch* fx = nullptr) {
     const auto t_wall0 = std::chrono::steady_clock::now();
     spca_settings st;
     int total_batches = 0;
-    int rc = prepare_settings(ctx, M, st_in, X0, x_out, &st, &total_batches);
+    int rc = prepare_settings(ctx, M, st_in, X0, x_out, &st, &total_batches, fx == nullptr);
     if (rc) return rc;
+    if (fx && (st.use_otf || M->sparse || defl))
+        return fail(ctx, SPCA_EINVAL, "the allocation-free solve runs dense matrices in batch mode (no OTF)");
     const int n = M->n, m = M->m;
     const int batch = st.batch;
     const int bp = round_up(batch, kPadB);
     Plan plan;
-    if ((rc = plan_for(ctx, E, M, st, bp, true, X0, &plan))) return rc;
+    if (fx) plan = peek_plan(ctx, st.precision, M->mp, M->np, bp, true);
+    else if ((rc = plan_for(ctx, E, M, st, bp, true, X0, &plan))) return rc;
     const int splits = plan.splits;
-    if ((rc = ensure_workspace(ctx, E, M->np, M->mp, bp, splits))) return rc;
-    Workspace& w = E.ws;
+    const AtzForm atz = atz_form(M, st, bp, plan.big);
+    Workspace fixed_ws;
+    if (fx) {
+        hipError_t e = hipSuccess;
+        FixedScratch probe{nullptr, fx->cap, 0};
+        carve_ws(fixed_ws, probe, M->np, M->mp, bp, splits, st.precision, atz.zprec, E.stream, &e);
+        const size_t ev_need = (ev_scratch_doubles(m, n, 1) + 32) * sizeof(double);
+        if (probe.used + ev_need > fx->cap)
+            return fail(ctx, SPCA_EINVAL, "an allocation-free solve of this shape needs " +
+                                              std::to_string(probe.used + ev_need) + " bytes of scratch; the fixed array holds " +
+                                              std::to_string(fx->cap));
+        fx->used = 0;
+        carve_ws(fixed_ws, *fx, M->np, M->mp, bp, splits, st.precision, atz.zprec, E.stream, &e);
+        if (e != hipSuccess) return fail(ctx, SPCA_EDEVICE, std::string("scratch clear: ") + hipGetErrorString(e));
+    } else if ((rc = ensure_workspace(ctx, E, M->np, M->mp, bp, splits))) {
+        return rc;
+    }
+    Workspace& w = fx ? fixed_ws : E.ws;
     hipStream_t s = E.stream;
     SpState sp = state_of(w);
     const int check_every = st.check_every > 0 ? st.check_every : 1;
 
-    const AtzForm atz = atz_form(M, st, bp, plan.big);
     const SignRefine rf = sign_refine(M, st, w);
     const Operand xop = x_operand(w, st.precision);
     const Operand zop = z_operand(w, atz.zprec);
@@ -1244,7 +1341,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     std::vector<float> hX(st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)n * batch);
     Results R;
     double loop_ms = 0;
-    PhaseTimer pt{&E};
+    PhaseTimer pt{&E, fx == nullptr};
     hipEvent_t* loop_ev = E.loop_ev;
 
     if (st.use_otf) {
@@ -1321,28 +1418,28 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         for (long it = 0; it < cap; it++) {
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
-            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, record(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(xop, st.precision, w.np, defl->idx, defl->val, defl->off, defl->ncomp,
                                                   batch, sp, s));
             HIP_TRY(ctx, launch_ax(M, st, w, splits, plan.big, bp, batch, zsplit, sp, otf_cap > 0, s));
-            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, record(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
-            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_prep_z(w.Zpart, splits, zsplit, M->mp, M->mp, zop, st.formulation, atz.zprec, batch,
                                        sp, rf, s));
-            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, record(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
-            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_atz(atz, M, w, bp, batch, sp, s));
-            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, record(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
-            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, record(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(Operand{w.V, nullptr, nullptr, w.bp}, 0, w.np, defl->idx, defl->val,
                                                   defl->off, defl->ncomp, batch, sp, s));
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
             HIP_TRY(ctx, launch_otf_converge(sp, o, batch, st.tolerance, st.max_iter, total, s));
             HIP_TRY(ctx, launch_otf_refill(sp, o, xop, st.precision, n, w.np, batch, otf_cap, s));
-            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, record(e1, s));
             if ((it + 1) % check_every == 0 || it + 1 == cap) {
                 HIP_TRY(ctx, hipMemcpyAsync(E.h_done, sp.done, sizeof(int) * 2, hipMemcpyDeviceToHost, s));
                 HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -1404,7 +1501,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     } else {
     // union A*X (sparse_products = 2, dense matrix, no deflation): after each iteration the
     // host reads |U| and the next A*X is the FP32 product over U when |U| <= ucap
-    const int ucap = (M->sparse || defl) ? 0 : union_cap(st, n);
+    const int ucap = (M->sparse || defl || fx) ? 0 : union_cap(st, n);
     if (ucap > 0 && (rc = ensure_union(ctx, w, s, round_up(ucap, kGemmBK * splits)))) return rc;
     for (int bt = 0; bt < total_batches; bt++) {
         const int shift = bt * batch;
@@ -1416,7 +1513,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         for (int it = 0; it < st.max_iter; it++) {
             hipEvent_t e0, e1;
             HIP_TRY(ctx, pt.pair(&e0, &e1, 0));
-            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, record(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(xop, st.precision, w.np, defl->idx, defl->val, defl->off, defl->ncomp,
                                                   batch, sp, s));
             int ax_splits = splits;
@@ -1436,18 +1533,18 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
                 // one of the two dense A*X forms runs; the other sees the device flag and returns
                 HIP_TRY(ctx, launch_ax(M, st, w, splits, plan.big, bp, batch, zsplit, sp, true, s));
             }
-            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, record(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
-            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_prep_z(w.Zpart, ax_splits, zsplit, M->mp, M->mp, zop, st.formulation, atz.zprec, batch,
                                        sp, rf, s));
-            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, record(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 1));
-            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_atz(atz, M, w, bp, batch, sp, s));
-            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, record(e1, s));
             HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
-            HIP_TRY(ctx, hipEventRecord(e0, s));
+            HIP_TRY(ctx, record(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(Operand{w.V, nullptr, nullptr, w.bp}, 0, w.np, defl->idx, defl->val,
                                                   defl->off, defl->ncomp, batch, sp, s));
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
@@ -1455,7 +1552,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
                                          st.termination == SPCA_TERM_PER_SP, cap, s));
             if (ucap > 0 && it + 1 < st.max_iter)
                 HIP_TRY(ctx, launch_union_list(w.X, w.np, n, batch, sp, w.uflag, w.uidx, w.ustat, s));
-            HIP_TRY(ctx, hipEventRecord(e1, s));
+            HIP_TRY(ctx, record(e1, s));
             if (ucap > 0 && it + 1 < st.max_iter) {
                 // stat = {|U|, done[0], done[1]}: the host decides the next A*X form
                 HIP_TRY(ctx, hipMemcpyAsync(E.h_done + 4, w.ustat, sizeof(int) * 3, hipMemcpyDeviceToHost, s));
@@ -1481,7 +1578,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             loop_ms += t;
         }
         if ((rc = collect_batch(ctx, s, M, w, st, shift, batch, per_sp != nullptr || stats != nullptr, per_sp, X_final,
-                                R)))
+                                R, fx)))
             return rc;
     }
     }
@@ -1997,6 +2094,9 @@ int spca_ctx_create(int device, uint64_t hbm_budget_bytes, spca_ctx* out) {
         return rc;
     }
     c->stream = c->eng.stream;
+    // the fixed scratch's device address, looked up once per device (spca_solve_noalloc fails
+    // with SPCA_EDEVICE if the lookup did not succeed; nothing else depends on it)
+    (void)scratch_setup(device);
     size_t fr = 0, tot = 0;
     (void)hipMemGetInfo(&fr, &tot);
     c->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)(0.9 * (double)fr);
@@ -2176,6 +2276,22 @@ int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, con
     return solve_and_resolve(ctx, ctx->eng, itm->second, settings, X0, x_out, stats, per_sp, X_final);
 }
 
+
+int spca_solve_noalloc(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, float* x_out,
+                       spca_stats* stats, spca_sp_result* per_sp, float* X_final) {
+    if (!ctx || !settings) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> g(ctx->mu);
+    auto itm = ctx->mats.find(mat);
+    if (itm == ctx->mats.end()) return fail(ctx, SPCA_ENOTFOUND, "unknown matrix handle");
+    (void)hipSetDevice(ctx->device);
+    void* base = scratch_ptr(ctx->device);
+    if (!base) return fail(ctx, SPCA_EDEVICE, "the fixed scratch of this device has no address (lookup failed)");
+    if (settings->resolve_iters > 0)
+        return fail(ctx, SPCA_EINVAL, "the allocation-free solve has no FP64 re-solve (resolve_iters must be 0)");
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    FixedScratch fx{(unsigned char*)base, kScratchBytes, 0};
+    return solve_impl(ctx, ctx->eng, itm->second, settings, X0, x_out, stats, per_sp, X_final, nullptr, &fx);
+}
 
 int spca_solve_components(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, int k,
                           float* X_out, spca_stats* stats) {

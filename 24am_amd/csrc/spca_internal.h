@@ -283,19 +283,19 @@ hipError_t launch_csr_ax(SparseMat A, const float* X, int ldx, float* Z, int ldz
 hipError_t launch_csc_atz(SparseMat A, const float* Z, int ldz, float* V, int ldv, int np, const double* zs, int bp,
                           int batch, SpState sp, hipStream_t st);
 // FP64 explained variance |A_c xhat|_2^2 or |A_c xhat|_1 of `batch` columns of X (ld ldx).
-// scratch: csr_ev_scratch_doubles(m, batch) doubles.
+// work: csr_ev_scratch_doubles(m, batch) doubles.
 size_t csr_ev_scratch_doubles(int m, int batch);
-hipError_t launch_csr_ev(SparseMat A, const float* X, int ldx, int batch, int l1, double* ev, double* scratch,
+hipError_t launch_csr_ev(SparseMat A, const float* X, int ldx, int batch, int l1, double* ev, double* work,
                          hipStream_t st);
 
 // Write the operand forms of X from its FP32 values (for starting points).
 hipError_t launch_x_operand(Operand xop, int precision, int ldx, int batch, hipStream_t st);
 
-// FP64 explained variance of each normalised column of X: |A x|_2^2 or |A x|_1.  scratch:
+// FP64 explained variance of each normalised column of X: |A x|_2^2 or |A x|_1.  work:
 // ev_scratch_doubles(m, n, batch) doubles of device memory.
 size_t ev_scratch_doubles(int m, int n, int batch, int* chunks = nullptr, int* chunk_len = nullptr);
 hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* X, int ldx, int batch,
-                          int l1, double* ev, double* scratch, hipStream_t st);
+                          int l1, double* ev, double* work, hipStream_t st);
 
 // FP64 re-solve on a support (indices idx[0..k)): max_iter iterations of z = map(A_S y),
 // y = A_S^T z / |A_S^T z|, stopping (flag) when the objective changes by <= tol relative.
@@ -304,5 +304,15 @@ hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* 
 hipError_t launch_resolve(const float* Acm, int lda, int m, const int* idx, int k, int l1, double* As, double* y,
                           double* z, double* yp, double* part, double* ev, int* flag, double tol, int max_iter,
                           hipStream_t st);
+
+// Fixed device scratch of the allocation-free solve: kScratchBytes per GPU, in device memory
+// from module load to process end.  scratch_setup(device), with `device` current, looks up the
+// array's device address once per device (hipGetSymbolAddress) and returns that lookup's
+// status; scratch_ptr(device) then gives the address (null if the lookup failed or never ran).
+constexpr size_t kScratchBytes = 128ull << 20;
+hipError_t scratch_setup(int device);
+void* scratch_ptr(int device);
+// p[0 .. bytes) = 0 by a kernel (no memset node where a caller captures); p 16-byte aligned
+hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t st);
 
 }  // namespace spca

@@ -14,6 +14,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <mutex>
+
 #include "spca_internal.h"
 
 namespace spca {
@@ -2632,13 +2635,13 @@ size_t ev_scratch_doubles(int m, int n, int batch, int* chunks_out, int* chunk_l
 }
 
 hipError_t launch_ev_fp64(const float* Acm, int lda, int m, int n, const float* X, int ldx, int batch, int l1,
-                          double* ev, double* scratch, hipStream_t st) {
+                          double* ev, double* work, hipStream_t st) {
     if (batch <= 0) return hipSuccess;
     int chunks = 0, chunk_len = 0;
     ev_scratch_doubles(m, n, batch, &chunks, &chunk_len);
     const int rbs = (m + kEvRows - 1) / kEvRows, ldp = rbs * kEvRows;
     const int rrb = ldp / kEvThreads;  // 256-row blocks of the reduction
-    double* part = scratch;
+    double* part = work;
     double* rpart = part + (size_t)batch * chunks * ldp;
     double* invn = rpart + (size_t)batch * rrb;
     ev_norm_kernel<<<batch, 1024, 0, st>>>(X, ldx, n, invn);
@@ -2669,11 +2672,11 @@ __global__ __launch_bounds__(256) void ev_scalars_kernel(const float* __restrict
         out[batch + b] = s;
     }
 }
-hipError_t launch_csr_ev(SparseMat A, const float* X, int ldx, int batch, int l1, double* ev, double* scratch,
+hipError_t launch_csr_ev(SparseMat A, const float* X, int ldx, int batch, int l1, double* ev, double* work,
                          hipStream_t st) {
     if (batch <= 0) return hipSuccess;
     const int nblk = (A.m + 255) / 256;
-    double* xs = scratch;               // 2 * batch
+    double* xs = work;                  // 2 * batch
     double* invn = xs + 2 * batch;      // batch
     double* part = invn + batch;        // nblk * batch
     ev_scalars_kernel<<<batch, 256, 0, st>>>(X, ldx, A.n, A.mu, xs, batch);
@@ -2764,6 +2767,56 @@ hipError_t launch_resolve(const float* Acm, int lda, int m, const int* idx, int 
     // objective of the final y
     resolve_az_kernel<<<nb, 256, 0, st>>>(As, m, k, y, l1, z, part);
     resolve_sum_kernel<<<1, 256, 0, st>>>(nb, part, ev);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- fixed scratch
+// The workspace of the allocation-free solve (spca_solve_noalloc): one array per GPU, in the
+// code object's device memory from module load to process end.  Host code never passes the
+// array's name to a kernel (in host code it names a host-side stand-in): it takes the device
+// address from hipGetSymbolAddress, once per device with that device current.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices];
+hipError_t g_scratch_rc[kMaxDevices];
+bool g_scratch_done[kMaxDevices];
+}  // namespace
+
+hipError_t scratch_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_done[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch));
+        g_scratch_rc[device] = e;
+        g_scratch_ptr[device] = e == hipSuccess ? ptr : nullptr;
+        g_scratch_done[device] = true;
+    }
+    return g_scratch_rc[device];
+}
+
+void* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    return g_scratch_done[device] ? g_scratch_ptr[device] : nullptr;
+}
+
+// 16-byte stores over the first n16 chunks, then the bytes of the tail (< 16) by one thread
+__global__ void fill_zero_kernel(uint4* __restrict__ p, size_t n16, int tail) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n16; e += (size_t)gridDim.x * blockDim.x)
+        p[e] = make_uint4(0u, 0u, 0u, 0u);
+    if (blockIdx.x == 0 && threadIdx.x == 0)
+        for (int b = 0; b < tail; b++) ((unsigned char*)(p + n16))[b] = 0;
+}
+hipError_t launch_fill_zero(void* p, size_t bytes, hipStream_t st) {
+    if (!bytes) return hipSuccess;
+    if (!p || ((uintptr_t)p & 15)) return hipErrorInvalidValue;
+    const size_t n16 = bytes / 16;
+    const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((n16 + 255) / 256, 4096));
+    fill_zero_kernel<<<grid, 256, 0, st>>>((uint4*)p, n16, (int)(bytes & 15));
     return hipGetLastError();
 }
 

@@ -37,8 +37,9 @@ EXPORTS = [
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
     "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan", "spca_matrix_generate",
     "spca_matrix_download", "spca_ctx_memory", "spca_solve_components", "spca_matrix_upload_csc",
-    "spca_product_forms",
+    "spca_product_forms", "spca_solve_noalloc",
 ]
+NOALLOC_SCRATCH_BYTES = 128 << 20  # the fixed device array of spca_solve_noalloc
 FORM_FP32, FORM_BF16X3, FORM_FP16, FORM_FIX8 = 0, 1, 2, 4
 FUSED_GRAPH = 1
 JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_CANCELLED = 0, 1, 2, 3
@@ -124,6 +125,7 @@ def lib():
             "spca_matrix_release": (I, [P, U64]),
             "spca_matrix_bytes": (I, [P, U64, ctypes.POINTER(U64)]),
             "spca_solve": (I, [P, U64, ctypes.POINTER(_Settings), P, P, ctypes.POINTER(_Stats), P, P]),
+            "spca_solve_noalloc": (I, [P, U64, ctypes.POINTER(_Settings), P, P, ctypes.POINTER(_Stats), P, P]),
             "spca_solve_components": (I, [P, U64, ctypes.POINTER(_Settings), P, I, P, P]),
             "spca_matrix_upload_csc": (I, [P, P, P, P, I, I, P, P, U64, ctypes.POINTER(U64)]),
             "spca_dense_solver_device": (I, [P, U, U, P, P, ctypes.POINTER(_Settings), ctypes.POINTER(_Stats), U, U]),
@@ -478,7 +480,12 @@ class Matrix:
             out.append(S)
         return X, out
 
-    def solve(self, settings, X0=None, stats=None, want_X=False, per_sp=True):
+    def solve_noalloc(self, settings, X0=None, stats=None, want_X=False, per_sp=True):
+        """solve() through spca_solve_noalloc: the device workspace is the library's fixed array
+        (no device allocation); raises SpcaError (SPCA_EINVAL) for shapes that do not fit it."""
+        return self.solve(settings, X0=X0, stats=stats, want_X=want_X, per_sp=per_sp, _entry="spca_solve_noalloc")
+
+    def solve(self, settings, X0=None, stats=None, want_X=False, per_sp=True, _entry="spca_solve"):
         """Returns x (unit norm, length n); fills `stats`; with want_X also returns X (n x total)."""
         c = settings._c()
         total = _total_after_check(c, self.n)
@@ -494,10 +501,10 @@ class Matrix:
         res = (_SpResult * total)() if per_sp else None
         Xf = np.zeros((self.n, total), dtype=np.float32, order="F") if want_X else None
         s = _Stats()
-        _check(self.ctx.ptr, lib().spca_solve(self.ctx.ptr, self.handle, ctypes.byref(c), X0p,
-                                              x.ctypes.data_as(ctypes.c_void_p), ctypes.byref(s),
-                                              ctypes.cast(res, ctypes.c_void_p) if res is not None else None,
-                                              Xf.ctypes.data_as(ctypes.c_void_p) if want_X else None))
+        _check(self.ctx.ptr, getattr(lib(), _entry)(self.ctx.ptr, self.handle, ctypes.byref(c), X0p,
+                                                    x.ctypes.data_as(ctypes.c_void_p), ctypes.byref(s),
+                                                    ctypes.cast(res, ctypes.c_void_p) if res is not None else None,
+                                                    Xf.ctypes.data_as(ctypes.c_void_p) if want_X else None))
         if stats is not None:
             stats._fill(s, res)
         return (x, Xf) if want_X else x

@@ -229,6 +229,29 @@ int spca_matrix_bytes(spca_ctx ctx, spca_matrix mat, uint64_t* bytes);
 int spca_solve(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0,
                float* x_out, spca_stats* stats, spca_sp_result* per_sp, float* X_final);
 
+/* spca_solve without device allocation: every device buffer the solve uses (iterates, K-split
+ * partial products, per-SP state, the FP64 explained-variance partials) lies in one fixed
+ * array of 128 MiB per GPU that the library holds from load to process end (a file-scope
+ * device array; its address is looked up at spca_ctx_create), so the call makes no
+ * hipMalloc/hipFree.  The arguments mean what spca_solve's mean.  Differences:
+ *   - a shape whose buffers exceed the array returns SPCA_EINVAL, saying how many bytes it needs
+ *     (about 8 B (FP32) to 12 B (16-bit forms) per padded n x padded batch entry for the
+ *     iterates, plus 4 B per padded m x padded batch entry for each K split of A*X and for z,
+ *     plus the explained variance's partial sums);
+ *   - the matrix must already hold every storage form the solve reads (SPCA_EINVAL otherwise;
+ *     add them with spca_matrix_add_storage first): the precision's, plus SPCA_STORE_FIX8 and
+ *     SPCA_STORE_FP32 for the L1-variance formulations' exact sign products in FP32/bf16x3;
+ *   - batch mode on dense matrices only (use_otf, CSC matrices and resolve_iters > 0 return
+ *     SPCA_EINVAL; sparse_products = 2 acts as 1); the product plan is the cached or heuristic
+ *     one (no tuning run); the phase times in stats are 0.
+ * Every call on one GPU uses that GPU's one array and leaves it holding its last contents (each
+ * call clears what must start at zero).  Calls on one context are serialised; calls from
+ * different contexts on the same GPU that overlap in time race on the array.
+ * No counterpart in the reference: for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc). */
+int spca_solve_noalloc(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0,
+                       float* x_out, spca_stats* stats, spca_sp_result* per_sp, float* X_final);
+
 /* One job of a fused multi-job solve.  The fields mean what spca_solve's arguments mean;
  * status receives the job's own result code. */
 typedef struct spca_fused_job {
