@@ -72,8 +72,9 @@ struct Workspace {
     uint16_t* Zhi = nullptr;
     uint16_t* Zlo = nullptr;
     float* V = nullptr;
-    float* fstate = nullptr;  // val, fz, err, errmax  (4 x bp floats)
-    int* istate = nullptr;    // iters, nnz, active (bp ints each), done (state_flag_ints(bp))
+    float* fstate = nullptr;  // val, fz, err, errmax, thr  (kStateFloats x bp floats)
+    int* istate = nullptr;    // iters, nnz, active, nkept (bp ints each), done (state_flag_ints(bp))
+    int* kept = nullptr;      // kListMax indices per SP: the nonzeros a list step wrote
     double* ev = nullptr;
     double* corr = nullptr;  // per-SP centring scalars of sparse products (4 x bp)
     // union A*X (sparse_products = 2): index flags and list (np ints each), stat (4 ints),
@@ -502,8 +503,8 @@ int make_generated(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const
 
 void free_ws(spca_ctx ctx, Workspace& w, hipStream_t s) {
     (void)hipStreamSynchronize(s);
-    void* ptrs[] = {w.X,      w.Xhi,   w.Xlo,    w.Zpart, w.Z,  w.Zhi, w.Zlo, w.V,
-                    w.fstate, w.istate, w.ev,    w.corr,  w.uflag, w.uidx, w.ustat, w.Au, w.Xu};
+    void* ptrs[] = {w.X,      w.Xhi,   w.Xlo,    w.Zpart, w.Z,  w.Zhi, w.Zlo, w.V,    w.fstate,
+                    w.istate, w.kept,  w.ev,     w.corr,  w.uflag, w.uidx, w.ustat, w.Au, w.Xu};
     for (void* p : ptrs) raw_free(ctx, p);
     w = Workspace();
 }
@@ -512,8 +513,8 @@ void free_workspace(spca_ctx ctx, Engine& E) { free_ws(ctx, E.ws, E.stream); }
 // device bytes a solve of this shape holds in its workspace
 uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
     const uint64_t xs = (uint64_t)np * bp, zs = (uint64_t)mp * bp;
-    return xs * (4 + 2 + 2 + 4) + zs * splits * 4 + zs * (4 + 2 + 2) + (uint64_t)4 * bp * 4 +
-           ((uint64_t)3 * bp + state_flag_ints(bp)) * 4 + (uint64_t)bp * 8 * 5;
+    return xs * (4 + 2 + 2 + 4) + zs * splits * 4 + zs * (4 + 2 + 2) + (uint64_t)kStateFloats * bp * 4 +
+           ((uint64_t)kStateInts * bp + state_flag_ints(bp)) * 4 + (uint64_t)bp * kListMax * 4 + (uint64_t)bp * 8 * 5;
 }
 
 int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp, int splits,
@@ -530,8 +531,9 @@ int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp,
         (rc = dev_alloc(ctx, &w.Xlo, xs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.Zpart, zs * splits, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.Z, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.Zhi, zs, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.Zlo, zs, &w.bytes, err)) || (rc = dev_alloc(ctx, &w.V, xs, &w.bytes, err)) ||
-        (rc = dev_alloc(ctx, &w.fstate, (size_t)4 * bp, &w.bytes, err)) ||
-        (rc = dev_alloc(ctx, &w.istate, (size_t)3 * bp + state_flag_ints(bp), &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.fstate, (size_t)kStateFloats * bp, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.istate, (size_t)kStateInts * bp + state_flag_ints(bp), &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.kept, (size_t)kListMax * bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.corr, (size_t)4 * bp, &w.bytes, err))) {
         free_ws(ctx, w, s);
@@ -604,8 +606,9 @@ bool carve_ws(Workspace& w, FixedScratch& sc, int np, int mp, int bp, int splits
     if (zprec) w.Zhi = sc.take<uint16_t>(zs);
     if (zprec == 1) w.Zlo = sc.take<uint16_t>(zs);
     w.V = sc.take<float>(xs);
-    w.fstate = sc.take<float>((size_t)4 * bp);
-    w.istate = sc.take<int>((size_t)3 * bp + state_flag_ints(bp));
+    w.fstate = sc.take<float>((size_t)kStateFloats * bp);
+    w.istate = sc.take<int>((size_t)kStateInts * bp + state_flag_ints(bp));
+    w.kept = sc.take<int>((size_t)kListMax * bp);
     w.ev = sc.take<double>((size_t)bp);
     w.corr = sc.take<double>((size_t)4 * bp);
     *e = hipSuccess;
@@ -650,10 +653,13 @@ SpState state_of(Workspace& w) {
     s.fz = w.fstate + w.bp;
     s.err = w.fstate + 2 * w.bp;
     s.errmax = w.fstate + 3 * w.bp;
+    s.thr = w.fstate + 4 * w.bp;
     s.iters = w.istate;
     s.nnz = w.istate + w.bp;
     s.active = w.istate + 2 * w.bp;
-    s.done = w.istate + 3 * w.bp;
+    s.nkept = w.istate + 3 * w.bp;
+    s.done = w.istate + kStateInts * w.bp;
+    s.kept = w.kept;
     return s;
 }
 
@@ -869,8 +875,8 @@ int load_batch(spca_ctx ctx, hipStream_t s, Workspace& w, const spca_settings& s
         if (st.precision == 1) HIP_TRY(ctx, launch_fill_zero(w.Xlo, sizeof(uint16_t) * (size_t)w.np * w.bp, s));
         HIP_TRY(ctx, launch_x_operand(xop, st.precision, w.np, batch, s));
     }
-    HIP_TRY(ctx, launch_fill_zero(w.fstate, sizeof(float) * 4 * (size_t)w.bp, s));
-    HIP_TRY(ctx, launch_fill_zero(w.istate, sizeof(int) * (3 * (size_t)w.bp + state_flag_ints(w.bp)), s));
+    HIP_TRY(ctx, launch_fill_zero(w.fstate, sizeof(float) * kStateFloats * (size_t)w.bp, s));
+    HIP_TRY(ctx, launch_fill_zero(w.istate, sizeof(int) * (kStateInts * (size_t)w.bp + state_flag_ints(w.bp)), s));
     std::vector<int> act(w.bp, 0);
     std::fill(act.begin(), act.begin() + batch, 1);
     HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
@@ -919,6 +925,7 @@ struct Results {
     unsigned total_it = 0;
     unsigned sparse_it = 0;
     unsigned union_it = 0;
+    unsigned list_tries = 0, list_steps = 0;
     unsigned long long sp_its = 0;
     double last_err = 0;
     double ev_best = 0;
@@ -953,6 +960,8 @@ int collect_batch(spca_ctx ctx, hipStream_t s, Matrix* M, Workspace& w, const sp
     HIP_TRY(ctx, hipStreamSynchronize(s));
     R.total_it += (unsigned)done_h[1];
     R.sparse_it += (unsigned)done_h[4];
+    R.list_tries += (unsigned)done_h[2];
+    R.list_steps += (unsigned)done_h[3];
     R.last_err = emax;
     for (int jj = 0; jj < batch; jj++) R.sp_its += (unsigned long long)hit[jj];
     int sel = 0;
@@ -1018,6 +1027,8 @@ void finish_solve(const Results& R, int n, const spca_settings& st, float* x_out
         stats->ev_fp64 = R.ev_best;
         stats->sparse_iterations = R.sparse_it;
         stats->union_iterations = R.union_it;
+        stats->list_tries = R.list_tries;
+        stats->list_steps = R.list_steps;
     }
 }
 
@@ -1336,6 +1347,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
     sa.s = st.constraint;
     sa.gamma = st.penalty;
     sa.batch = batch;
+    sa.sparse_rewrite = defl ? 0 : 1;  // deflation changes x between steps
     set_exact_ties(sa, M, st, w);
 
     std::vector<float> hX(st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)n * batch);
@@ -1405,8 +1417,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             hf[0] = -1.f;  // the_best_solution_value starts at -1
             HIP_TRY(ctx, hipMemcpyAsync(ibuf, hi.data(), sizeof(int) * hi.size(), hipMemcpyHostToDevice, s));
             HIP_TRY(ctx, hipMemcpyAsync(fbuf, hf.data(), sizeof(float) * hf.size(), hipMemcpyHostToDevice, s));
-            HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * 4 * (size_t)w.bp, s));
-            HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (3 * (size_t)w.bp + state_flag_ints(w.bp)), s));
+            HIP_TRY(ctx, hipMemsetAsync(w.fstate, 0, sizeof(float) * kStateFloats * (size_t)w.bp, s));
+            HIP_TRY(ctx, hipMemsetAsync(w.istate, 0, sizeof(int) * (kStateInts * (size_t)w.bp + state_flag_ints(w.bp)), s));
             std::vector<int> act(w.bp, 0);
             std::fill(act.begin(), act.begin() + batch, 1);
             HIP_TRY(ctx, hipMemcpyAsync(sp.active, act.data(), sizeof(int) * w.bp, hipMemcpyHostToDevice, s));
@@ -1483,6 +1495,8 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
         release();
         R.total_it = (unsigned)done_h[1];
         R.sparse_it = (unsigned)done_h[4];
+        R.list_tries = (unsigned)done_h[2];
+        R.list_steps = (unsigned)done_h[3];
         R.best = bestv;
         R.best_sp = gen_h[2];
         R.last_err = emax;
@@ -1788,6 +1802,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         f.sa.s = r.st.constraint;
         f.sa.gamma = r.st.penalty;
         f.sa.batch = r.st.batch;
+        f.sa.sparse_rewrite = 1;
         set_exact_ties(f.sa, M, r.st, r.ws);
         f.mp = M->mp;
         f.np = M->np;

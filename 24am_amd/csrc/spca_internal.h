@@ -59,11 +59,23 @@ struct SpState {
     int* iters;      // iterations performed
     int* nnz;        // nonzeros of the iterate
     int* active;     // 1 while the SP iterates
-    int* done;       // [0]: batch finished, [1]: iterations counted, [2..3]: spare, [4]: iterations
+    int* done;       // [0]: batch finished, [1]: iterations counted, [2]: constrained steps that
+                     // tried the previous threshold's list, [3]: those it decided, [4]: iterations
                      // with a gathered A*X counted, [5..7]: spare, [kSparseFlags + g]: the next
                      // A*X of SPs 256g .. 256g+255 is the sparse gather (their dense tiles stand down)
     float* errmax;   // [0]: batch max error of the last iteration
+    float* thr;      // constrained: the bound of the next step's candidate list, left by the last
+                     // step (0: none)
+    int* nkept;      // constrained: nonzeros the last step wrote through its candidate list, whose
+                     // indices are kept[j * kListMax ..] (-1: the last step wrote the whole column)
+    int* kept;       // kListMax indices per SP
 };
+// per-SP float state: val, fz, err, errmax, thr; int state: iters, nnz, active, nkept
+constexpr int kStateFloats = 5;
+constexpr int kStateInts = 4;
+// Most entries of the step kernel's candidate list (a power of two, for its sort), and so of the
+// nonzeros a step written through the list can leave.
+constexpr int kListMax = 4096;
 
 // Operand buffers of one iterate matrix (n or m rows, padded batch columns).  The FP32
 // values are column major; the 16-bit forms the products read are K-blocked: element i of
@@ -116,6 +128,8 @@ struct StepArgs {
     const float* zf;     // z of this iteration, FP32 (ld mp)
     int mp;
     float band_eps;      // 0: off
+    int sparse_rewrite;  // 1: a step decided on its candidate list rewrites only the previous and new
+                         // nonzeros of x (0 when something else changes x between steps: deflation)
 };
 constexpr int kBandMax = 16;  // most candidates one tie band re-evaluates (more: none)
 
@@ -238,7 +252,8 @@ hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs
                                   int items3, int items4, int any_fix8, int* all_done, hipStream_t st);
 
 // z_j = A x_j over the nonzeros of each active x_j (FP64 accumulation, fixed order) into Z
-// (ld ldz), when sp.done[2] is set; else it returns at once.  A: FP32 column major, ld lda.
+// (ld ldz), for the groups of 256 SPs whose sparse flag (sp.done[kSparseFlags + g]) is set; the
+// other groups' workgroups return at once.  A: FP32 column major, ld lda.
 constexpr int kSparseRows = 1024;  // rows per workgroup of the sparse product
 // union A*X: flags and the ascending list U of the indices nonzero in any iterate whose A*X is
 // dense next iteration, stat = {|U|, done[0], done[1]}; then Au = A[:, U] (row major) and

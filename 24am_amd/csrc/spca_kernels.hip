@@ -1421,16 +1421,40 @@ constexpr int kBins = 2048;
 constexpr double kFix = 1099511627776.0;  // 2^40: n * 2^40 < 2^63 for n < 2^23
 
 constexpr int kCand = 2048;  // candidates of the answer bin resolved in LDS (else the radix fallback)
+// The previous threshold's list: up to kList elements with |v| at or above a bound L the SP's
+// previous step left: the lower edge of the histogram bin (of the list's, or of the column's)
+// holding place s + s/4 + 32 (L0) or survivors + survivors/8 + 64 (L1), or, after a radix
+// fallback, a margin below the threshold it found.
+constexpr int kList = kListMax;
+constexpr int kLBins = 256;   // bins of the list's histogram
+static_assert(kLBins == 256, "the list bins take the top 8 bits of the span");
+constexpr int kLCand = 1024;  // entries of the list's answer bin resolved in LDS (more: the column passes)
+constexpr float kListMarginL0 = 0.9f;
+constexpr float kListMarginL1 = 0.95f;
 
 struct SelShared {
-    uint32_t cnt[kBins];
-    uint64_t s1[kBins];
-    uint64_t s2[kBins];
-    uint64_t cand[kCand];  // (|v| bits << 32) | index, sorted descending
+    union {
+        struct {
+            uint32_t cnt[kBins];
+            uint64_t s1[kBins];
+            uint64_t s2[kBins];
+            uint64_t cand[kCand];  // (|v| bits << 32) | index, sorted descending
+        };
+        struct {
+            // the previous threshold's list: (|v| bits << 32) | index | sign of v in bit 31
+            uint64_t list[kList];
+            uint32_t lcnt[kLBins];   // its histogram over [L, max |v|], fixed-point sums as above
+            uint64_t ls1[kLBins];
+            uint64_t ls2[kLBins];
+            uint64_t lcand[kLCand];  // the list's answer bin, sorted descending
+        };
+    };
     double red[kSelThreads / 64];
     int ires[4];
     uint64_t ures[2];
     int ncand;
+    int nkeep;
+    int nlc;
     double dres[4];
     // exact ties: the re-evaluated band (sorted descending), its value range and keep flags
     int nband;
@@ -1439,6 +1463,89 @@ struct SelShared {
     float band_val[kBandMax];
     int band_keep[kBandMax];
 };
+
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
+    return ((uint64_t)hi << 32) | lo;
+}
+
+// Registers r and r | T of each thread (r with bit T clear holds the lower element index) as one
+// compare-exchange step of a bitonic sort: element i = (w * R + r) * 64 + lane.
+template <int R, int T>
+__device__ __forceinline__ void sort_cx_regs(uint64_t (&x)[R], int k, int w, int lane) {
+    if constexpr (T < R) {
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            if (r & T) continue;
+            const bool desc = ((((w * R + r) * 64 + lane) & k) == 0);
+            const uint64_t a0 = x[r], a1 = x[r | T];
+            const uint64_t hi = a0 > a1 ? a0 : a1, lo = a0 > a1 ? a1 : a0;
+            x[r] = desc ? hi : lo;
+            x[r | T] = desc ? lo : hi;
+        }
+    }
+}
+
+// a[0 .. nc) sorted descending as u64 keys by the whole block (NT threads), nc <= NT * R; a must
+// hold NT * R entries (the tail is padded with 0 keys, which sort last).  A bitonic network over
+// NT * R elements, element i = (w * R + r) * 64 + lane held in register r of lane `lane` of wave
+// w: exchanges across fewer than 64 places are lane shuffles, across 64 .. 64R - 1 places
+// register swaps, and only the widest (64R and more) go through LDS.  Keys are distinct in use
+// (the index is in the low word), so the order is the unique descending one.
+template <int NT, int R>
+__device__ void sort_desc_reg(uint64_t* a, int nc) {
+    constexpr int N = NT * R;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    uint64_t x[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int i = (w * R + r) * 64 + lane;
+        x[r] = i < nc ? a[i] : 0ull;
+    }
+    for (int k = 2; k <= N; k <<= 1)
+        for (int jj = k >> 1; jj > 0; jj >>= 1) {
+            if (jj >= 64 * R) {
+                __syncthreads();  // every read of the previous exchange is done
+#pragma unroll
+                for (int r = 0; r < R; r++) a[(w * R + r) * 64 + lane] = x[r];
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int i = (w * R + r) * 64 + lane;
+                    const uint64_t p = a[i ^ jj];
+                    const bool keep_hi = ((i & jj) == 0) == ((i & k) == 0);
+                    x[r] = keep_hi ? (x[r] > p ? x[r] : p) : (x[r] < p ? x[r] : p);
+                }
+            } else if (jj >= 64) {
+                const int t = jj >> 6;
+                if (t == 1) sort_cx_regs<R, 1>(x, k, w, lane);
+                else if (t == 2) sort_cx_regs<R, 2>(x, k, w, lane);
+                else sort_cx_regs<R, 4>(x, k, w, lane);
+            } else {
+#pragma unroll
+                for (int r = 0; r < R; r++) {
+                    const int i = (w * R + r) * 64 + lane;
+                    const uint64_t p = shfl_xor_u64(x[r], jj);
+                    const bool keep_hi = ((lane & jj) == 0) == ((i & k) == 0);
+                    x[r] = keep_hi ? (x[r] > p ? x[r] : p) : (x[r] < p ? x[r] : p);
+                }
+            }
+        }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) a[(w * R + r) * 64 + lane] = x[r];
+    __syncthreads();
+}
+
+// a[0 .. nc) sorted descending (as u64 keys) in LDS by the whole block; a must hold the next
+// multiple of NT that is NT times a power of two >= nc (padded with zeros, which sort last)
+template <int NT>
+__device__ void bitonic_desc(uint64_t* a, int nc) {
+    if (nc <= NT) sort_desc_reg<NT, 1>(a, nc);
+    else if (nc <= 2 * NT) sort_desc_reg<NT, 2>(a, nc);
+    else if (nc <= 4 * NT) sort_desc_reg<NT, 4>(a, nc);
+    else sort_desc_reg<NT, 8>(a, nc);
+}
 
 __device__ __forceinline__ int lin_bin(float a, float scale) {
     int b = (int)(a * scale);
@@ -1459,15 +1566,16 @@ __device__ __forceinline__ bool pred_r(double c, double S1, double S2, double la
 // find the largest bin whose lower boundary satisfies pred_r.  Writes ires[0] = bin (or
 // -1), ires[1] = count above it, ures = fixed-point sums above it.
 template <int NB, class Boundary>
-__device__ void scan_hist(SelShared& sh, bool l1c, uint32_t rank, uint32_t cA, uint64_t s1A, uint64_t s2A,
-                          double M, double wd, Boundary boundary) {
+__device__ void scan_hist(SelShared& sh, const uint32_t* cnt, const uint64_t* hs1, const uint64_t* hs2, bool l1c,
+                          uint32_t rank, uint32_t cA, uint64_t s1A, uint64_t s2A, double M, double wd,
+                          Boundary boundary) {
     const int lane = threadIdx.x & 63;
     constexpr int PER = NB / 64;
     uint32_t lc = 0;
     uint64_t l1 = 0, l2 = 0;
     for (int b = lane * PER; b < lane * PER + PER; b++) {
-        lc += sh.cnt[b];
-        if (l1c) { l1 += sh.s1[b]; l2 += sh.s2[b]; }
+        lc += cnt[b];
+        if (l1c) { l1 += hs1[b]; l2 += hs2[b]; }
     }
     const uint32_t ic = wave_suffix_u32(lc);
     uint64_t i1 = 0, i2 = 0;
@@ -1478,11 +1586,11 @@ __device__ void scan_hist(SelShared& sh, bool l1c, uint32_t rank, uint32_t cA, u
     uint32_t fc = 0;
     uint64_t f1 = 0, f2 = 0;
     for (int b = lane * PER + PER - 1; b >= lane * PER; b--) {
-        const uint32_t cb = sh.cnt[b];
+        const uint32_t cb = cnt[b];
         if (!l1c) {
             if (found < 0 && c < rank && c + cb >= rank) { found = b; fc = c; }
         } else {
-            const uint64_t b1 = sh.s1[b], b2 = sh.s2[b];
+            const uint64_t b1 = hs1[b], b2 = hs2[b];
             double lam;
             const int forced = boundary(b, lam);  // 1: true, -1: false, 0: evaluate
             const bool p = forced > 0 || (forced == 0 && pred_r((double)(cA + c + cb), (double)(s1A + S1 + b1) * M / kFix,
@@ -1536,104 +1644,37 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     // L1 constraint with s <= 1 (|x|_1 <= |x|_2) admits only 1-sparse x: the top-1 hard
     // threshold, which the lambda scan cannot produce (its quadratic degenerates)
     const bool l1c = (f == 2 || f == 3) && a.s > 1;
+    bool decided = false;  // constrained: the step was resolved on the previous threshold's list
+    int list_cnt = 0;      // entries of that list
     if (f < 4) {
         // ---- constrained: select, then x <- x/|x|_2
-        float mloc = 0.f;
-        sweep([&](int, float v) { mloc = fmaxf(mloc, fabsf(v)); });
-        const float M = block_max<NT>(mloc, sh.red);
-        const float scale = M > 0.f ? (float)kBins / M : 0.f;
-        const double invM = M > 0.f ? 1.0 / (double)M : 0.0;
-        const double Md = (double)M;
-        for (int b = tid; b < kBins; b += NT) {
-            sh.cnt[b] = 0;
-            sh.s1[b] = 0;
-            sh.s2[b] = 0;
-        }
-        __syncthreads();
-        sweep([&](int, float v) {
-            const float av = fabsf(v);
-            const int b = lin_bin(av, scale);
-            atomicAdd(&sh.cnt[b], 1u);
-            if (l1c && av > 0.f) {
-                const double w = (double)av * invM;
-                atomicAdd((unsigned long long*)&sh.s1[b], (unsigned long long)(w * kFix + 0.5));
-                atomicAdd((unsigned long long*)&sh.s2[b], (unsigned long long)(w * w * kFix + 0.5));
-            }
-        });
-        __syncthreads();
         const double wd = (double)(float)sqrt((double)a.s);
-        if (tid < 64)
-            scan_hist<kBins>(sh, l1c, a.s, 0u, 0ull, 0ull, Md, wd, [&](int b, double& l) {
-                l = (double)b * Md / (double)kBins;
-                return 0;
-            });
-        __syncthreads();
-        const int bstar = sh.ires[0];
-        const uint32_t cA_hist = bstar >= 0 ? (uint32_t)sh.ires[1] : 0u;
-        const uint64_t s1A_fix = bstar >= 0 ? sh.ures[0] : 0ull, s2A_fix = bstar >= 0 ? sh.ures[1] : 0ull;
-        const uint32_t cnt_b = bstar >= 0 ? sh.cnt[bstar] : 0u;
         double norm2 = 0;
-        bool fast = bstar >= 0 && cnt_b <= (uint32_t)kCand;
-        if (bstar < 0 && l1c) fast = true;  // no trigger: every element survives, sums over all
-        if (fast) {
-            // ---- compaction: the answer bin's elements into LDS; exact FP64 sums of the bins above
-            if (tid == 0) sh.ncand = 0;
-            __syncthreads();
-            double c_loc = 0, s1_loc = 0, s2_loc = 0;
-            sweep([&](int q, float v) {
-                const int idx = tid + q * NT;
-                if (idx >= a.n) return;
-                const float av = fabsf(v);
-                const int b = lin_bin(av, scale);
-                if (b > bstar) {
-                    c_loc += 1.0;
-                    s1_loc += (double)av;
-                    s2_loc += (double)av * (double)av;
-                } else if (b == bstar) {
-                    const int slot = atomicAdd(&sh.ncand, 1);
-                    if (slot < kCand) sh.cand[slot] = ((uint64_t)__float_as_uint(av) << 32) | (uint32_t)idx;
-                }
-            });
-            const double cAd = block_sum<NT>(c_loc, sh.red);
-            const double S1A = block_sum<NT>(s1_loc, sh.red);
-            const double S2A = block_sum<NT>(s2_loc, sh.red);
-            const int nc = min(sh.ncand, kCand);
-            // bitonic sort of the candidates, descending by (|v|, index): a fixed order
-            int npow = 1;
-            while (npow < nc) npow <<= 1;
-            for (int i = nc + tid; i < npow; i += NT) sh.cand[i] = 0ull;
-            __syncthreads();
-            for (int k = 2; k <= npow; k <<= 1)
-                for (int jj = k >> 1; jj > 0; jj >>= 1) {
-                    for (int i = tid; i < npow; i += NT) {
-                        const int ixj = i ^ jj;
-                        if (ixj > i) {
-                            const uint64_t x0 = sh.cand[i], x1 = sh.cand[ixj];
-                            const bool desc = (i & k) == 0;
-                            if (desc ? (x0 < x1) : (x0 > x1)) {
-                                sh.cand[i] = x1;
-                                sh.cand[ixj] = x0;
-                            }
-                        }
-                    }
-                    __syncthreads();
-                }
+        float next_bound = 0.f;  // the next step's list bound (0: no list)
+        bool decided_column = false;  // the column passes' histogram set next_bound
+        // The threshold from candidates cand[0..nc) (the answer bin of a histogram) sorted
+        // descending by (|v|, index), with the elements above them counted in cA_hist (L0) /
+        // summed in (cAd, S1A, S2A), both exact.  band_inside(lo, hi): every value in [lo, hi] is
+        // a candidate.  list_mode (the histogram is the previous threshold's list's): a clamped
+        // lambda, whose norm needs the whole column, is left to the column passes; returns false
+        // then, block-uniformly.
+        auto resolve = [&](uint64_t* cand, int nc, uint32_t cA_hist, double cAd, double S1A, double S2A,
+                           bool list_mode, bool closed_ok, auto&& band_inside) -> bool {
             if (!l1c) {
                 // ---- L0 constrained: tau = the (s - above)-th largest candidate (counting ties)
                 const uint32_t r = a.s - cA_hist;
-                tau = (r >= 1 && (int)r <= nc) ? __uint_as_float((uint32_t)(sh.cand[r - 1] >> 32)) : 0.f;
+                tau = (r >= 1 && (int)r <= nc) ? __uint_as_float((uint32_t)(cand[r - 1] >> 32)) : 0.f;
                 // exact ties: candidates within the product's rounding of tau could be in either
-                // order; when the band [lo, hi] lies inside this bin (so every such value is a
-                // candidate) and holds 2..kBandMax of them, they are recomputed in FP64 and the
-                // s-th place is decided on the exact values
+                // order; when every value of the band [lo, hi] is a candidate and the band holds
+                // 2..kBandMax of them, they are recomputed in FP64 and the s-th place is decided on
+                // the exact values
                 const float lo = tau * (1.f - 2.f * a.band_eps), hi = tau * (1.f + 2.f * a.band_eps);
-                const bool try_band = a.band_eps > 0.f && tau > 0.f && lin_bin(lo, scale) == bstar &&
-                                      lin_bin(hi, scale) == bstar;
+                const bool try_band = a.band_eps > 0.f && tau > 0.f && band_inside(lo, hi);
                 int i0 = 0, i1 = 0;  // band = sorted candidates [i0, i1)
                 if (try_band) {
                     double above = 0, inband = 0;
                     for (int i = tid; i < nc; i += NT) {
-                        const float av = __uint_as_float((uint32_t)(sh.cand[i] >> 32));
+                        const float av = __uint_as_float((uint32_t)(cand[i] >> 32));
                         above += av > hi ? 1.0 : 0.0;
                         inband += (av <= hi && av >= lo) ? 1.0 : 0.0;
                     }
@@ -1645,7 +1686,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 if (try_band && nb >= 2 && nb <= kBandMax) {
                     const int lane = tid & 63;
                     for (int w = tid >> 6; w < nb; w += NT / 64) {
-                        const int idx = (int)(uint32_t)sh.cand[i0 + w];
+                        const int idx = (int)((uint32_t)cand[i0 + w] & 0x7fffffffu);  // bit 31: list sign
                         const f4* ac = (const f4*)(a.acm + (long)idx * a.mp);
                         const f4* zc = (const f4*)(a.zf + (long)j * a.mp);
                         double d = 0;
@@ -1676,7 +1717,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                             sh.band_val[q + 1] = v;
                             sh.band_idx[q + 1] = ix;
                         }
-                        // the band supplies places i0+1 .. of the bin: r - i0 of them
+                        // the band supplies places i0+1 .. of the candidates: r - i0 of them
                         const float tb = fabsf(sh.band_val[(int)r - 1 - i0]);
                         for (int p = 0; p < nb; p++) sh.band_keep[p] = !(fabsf(sh.band_val[p]) < tb);
                         sh.band_lo = lo;
@@ -1688,76 +1729,288 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 const int nband = sh.nband;
                 double cs = 0;
                 for (int i = tid; i < nc; i += NT) {
-                    const float av = __uint_as_float((uint32_t)(sh.cand[i] >> 32));
+                    const float av = __uint_as_float((uint32_t)(cand[i] >> 32));
                     if (nband ? i < i0 : !(av < tau)) cs += (double)av * (double)av;
                 }
                 if (tid < nband && sh.band_keep[tid]) cs += (double)sh.band_val[tid] * (double)sh.band_val[tid];
                 norm2 = S2A + block_sum<NT>(cs, sh.red);
-            } else {
-                // ---- L1 constrained: the first candidate (descending) where the reference scan
-                // triggers, r(a) >= w over the elements above it; one wave walks 64 at a time
-                if (tid < 64) {
-                    const int lane = tid;
-                    double cc = cAd, c1 = S1A, c2 = S2A;  // carried sums of everything above the chunk
-                    int found = -1;
-                    double fc = 0, f1 = 0, f2 = 0;
-                    for (int c0 = 0; c0 < nc && found < 0; c0 += 64) {
-                        const int i = c0 + lane;
-                        const double av = i < nc ? (double)__uint_as_float((uint32_t)(sh.cand[i] >> 32)) : 0.0;
-                        double p0 = i < nc ? 1.0 : 0.0, p1 = av, p2 = av * av;
-                        // inclusive prefix over lanes 0..lane, fixed shuffle order
-                        for (int o = 1; o < 64; o <<= 1) {
-                            const double t0 = __shfl_up(p0, o, 64), t1 = __shfl_up(p1, o, 64), t2 = __shfl_up(p2, o, 64);
-                            if (lane >= o) { p0 += t0; p1 += t1; p2 += t2; }
-                        }
-                        // sums over the elements before candidate i
-                        const double e0 = cc + p0 - (i < nc ? 1.0 : 0.0), e1 = c1 + p1 - av, e2 = c2 + p2 - av * av;
-                        const bool trig = i < nc && pred_r(e0 + 1.0, e1 + av, e2 + av * av, av, wd);
-                        const uint64_t bal = __ballot(trig);
-                        if (bal) {
-                            const int first = __ffsll((long long)bal) - 1;
-                            found = c0 + first;
-                            fc = __shfl(e0, first, 64);
-                            f1 = __shfl(e1, first, 64);
-                            f2 = __shfl(e2, first, 64);
-                        } else {
-                            cc += __shfl(p0, 63, 64);
-                            c1 += __shfl(p1, 63, 64);
-                            c2 += __shfl(p2, 63, 64);
-                        }
+                return true;
+            }
+            // ---- L1 constrained: the first candidate (descending) where the reference scan
+            // triggers, r(a) >= w over the elements above it; one wave walks 64 at a time
+            if (tid < 64) {
+                const int lane = tid;
+                double cc = cAd, c1 = S1A, c2 = S2A;  // carried sums of everything above the chunk
+                int found = -1;
+                double fc = 0, f1 = 0, f2 = 0;
+                for (int c0 = 0; c0 < nc && found < 0; c0 += 64) {
+                    const int i = c0 + lane;
+                    const double av = i < nc ? (double)__uint_as_float((uint32_t)(cand[i] >> 32)) : 0.0;
+                    double p0 = i < nc ? 1.0 : 0.0, p1 = av, p2 = av * av;
+                    // inclusive prefix over lanes 0..lane, fixed shuffle order
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const double t0 = __shfl_up(p0, o, 64), t1 = __shfl_up(p1, o, 64), t2 = __shfl_up(p2, o, 64);
+                        if (lane >= o) { p0 += t0; p1 += t1; p2 += t2; }
                     }
-                    if (lane == 0) {
-                        if (found < 0) { fc = cc; f1 = c1; f2 = c2; }
-                        sh.dres[0] = fc;
-                        sh.dres[1] = f1;
-                        sh.dres[2] = f2;
+                    // sums over the elements before candidate i
+                    const double e0 = cc + p0 - (i < nc ? 1.0 : 0.0), e1 = c1 + p1 - av, e2 = c2 + p2 - av * av;
+                    const bool trig = i < nc && pred_r(e0 + 1.0, e1 + av, e2 + av * av, av, wd);
+                    const uint64_t bal = __ballot(trig);
+                    if (bal) {
+                        const int first = __ffsll((long long)bal) - 1;
+                        found = c0 + first;
+                        fc = __shfl(e0, first, 64);
+                        f1 = __shfl(e1, first, 64);
+                        f2 = __shfl(e2, first, 64);
+                    } else {
+                        cc += __shfl(p0, 63, 64);
+                        c1 += __shfl(p1, 63, 64);
+                        c2 += __shfl(p2, 63, 64);
+                    }
+                }
+                if (lane == 0) {
+                    if (found < 0) { fc = cc; f1 = c1; f2 = c2; }
+                    sh.dres[0] = fc;
+                    sh.dres[1] = f1;
+                    sh.dres[2] = f2;
+                    sh.ires[3] = found >= 0;
+                    sh.ires[2] = found;
+                }
+            }
+            __syncthreads();
+            const double c = sh.dres[0], S1 = sh.dres[1], S2 = sh.dres[2];
+            const double w2 = wd * wd;
+            const double qa = c * (c - w2);
+            const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
+            const double qc = S1 * S1 - w2 * S2;
+            lam = (float)((-qb - sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa));
+            // no positive root: the constraint is inactive (s >= (|v|_1/|v|_2)^2, e.g. s >= n)
+            // and lambda = 0; the reference scan's quadratic degenerates there (NaN or a
+            // negative root), a branch no fixture pins
+            const bool clamped = !(lam > 0.f);
+            if (clamped) {
+                if (list_mode) return false;
+                lam = 0.f;
+            }
+            if (closed_ok && !clamped) {
+                // every element of the survivor set lies above lam: sum of (a - lam)^2 in closed form
+                const double lf = (double)lam;
+                norm2 = S2 - 2.0 * lf * S1 + c * lf * lf;
+            } else {
+                double acc = 0;
+                sweep([&](int, float v) {
+                    const float t2 = fabsf(v) - lam;
+                    if (t2 > 0.f) acc += (double)t2 * (double)t2;
+                });
+                norm2 = block_sum<NT>(acc, sh.red);
+            }
+            return true;
+        };
+        // ---- the previous threshold's list: from the second iteration on, the SP's threshold
+        // (tau, or lambda) moves little, so one pass collects every element with |v| >= L, the
+        // bound the previous step left a little below its threshold, into LDS (and takes max |v|
+        // on the way).  A histogram of the list then finds the threshold's bin, and only that
+        // bin is sorted.  Exact whenever it decides: the list holds every element >= L, L0 needs
+        // >= s of them (tau >= L then), L1 a trigger boundary at or above L; otherwise the column
+        // passes below run as on a first step, with max |v| known.
+        float Mcol = 0.f;
+        bool have_M = false;
+        const float thr_prev = sp.thr[j];
+        if (thr_prev > 0.f) {
+            const float L = thr_prev;
+            if (tid == 0) sh.ncand = 0;
+            __syncthreads();
+            float mloc = 0.f;
+            sweep([&](int q, float v) {
+                // one LDS atomic per wave for the wave's entries (q is uniform across the wave)
+                const float av = fabsf(v);
+                mloc = fmaxf(mloc, av);
+                const bool in = av >= L && tid + q * NT < a.n;
+                const uint64_t bal = __ballot(in);
+                if (bal) {
+                    const int lane = tid & 63, first = __ffsll((long long)bal) - 1;
+                    int base = 0;
+                    if (lane == first) base = atomicAdd(&sh.ncand, (int)__popcll(bal));
+                    base = __shfl(base, first, 64);
+                    const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
+                    if (in && slot < kList)
+                        sh.list[slot] = ((uint64_t)__float_as_uint(av) << 32) | (uint32_t)(tid + q * NT) |
+                                        (v < 0.f ? 0x80000000u : 0u);
+                }
+            });
+            Mcol = block_max<NT>(mloc, sh.red);  // its barriers also publish the list
+            have_M = true;
+            const int cnt = sh.ncand;
+            list_cnt = cnt;
+            if (cnt > 0 && cnt <= kList && (l1c || cnt >= (int)a.s)) {
+                for (int b = tid; b < kLBins; b += NT) {
+                    sh.lcnt[b] = 0;
+                    sh.ls1[b] = 0;
+                    sh.ls2[b] = 0;
+                }
+                if (tid == 0) sh.nlc = 0;
+                __syncthreads();
+                // bins over the IEEE bits of |v| - bits of L (monotone for non-negative floats):
+                // nearly constant relative width, so the bins near L, where the threshold lies and
+                // the entries crowd, stay small; every bin edge is an exact float
+                const uint32_t Lb = __float_as_uint(L);
+                const uint32_t span = __float_as_uint(Mcol) - Lb;  // Mcol >= L: the list is not empty
+                const int lshift = max(0, 24 - __clz(span));       // span >> lshift < kLBins = 2^8
+                const double invM = 1.0 / (double)Mcol;
+                auto lbin = [&](float av) { return (int)((__float_as_uint(av) - Lb) >> lshift); };  // av >= L
+                auto ledge = [&](int b) { return __uint_as_float(Lb + ((uint32_t)b << lshift)); };
+                for (int e = tid; e < cnt; e += NT) {
+                    const float av = __uint_as_float((uint32_t)(sh.list[e] >> 32));
+                    const int b = lbin(av);
+                    atomicAdd(&sh.lcnt[b], 1u);
+                    if (l1c) {
+                        const double w = (double)av * invM;
+                        atomicAdd((unsigned long long*)&sh.ls1[b], (unsigned long long)(w * kFix + 0.5));
+                        atomicAdd((unsigned long long*)&sh.ls2[b], (unsigned long long)(w * w * kFix + 0.5));
                     }
                 }
                 __syncthreads();
-                const double c = sh.dres[0], S1 = sh.dres[1], S2 = sh.dres[2];
-                const double w2 = wd * wd;
-                const double qa = c * (c - w2);
-                const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
-                const double qc = S1 * S1 - w2 * S2;
-                lam = (float)((-qb - sqrt(qb * qb - 4.0 * qa * qc)) / (2.0 * qa));
-                // no positive root: the constraint is inactive (s >= (|v|_1/|v|_2)^2, e.g. s >= n)
-                // and lambda = 0; the reference scan's quadratic degenerates there (NaN or a
-                // negative root), a branch no fixture pins
-                const bool clamped = !(lam > 0.f);
-                if (clamped) lam = 0.f;
-                if (bstar >= 0 && !clamped) {
-                    // every element of the survivor set lies above lam: sum of (a - lam)^2 in closed form
-                    const double lf = (double)lam;
-                    norm2 = S2 - 2.0 * lf * S1 + c * lf * lf;
-                } else {
-                    double acc = 0;
-                    sweep([&](int, float v) {
-                        const float t2 = fabsf(v) - lam;
-                        if (t2 > 0.f) acc += (double)t2 * (double)t2;
+                if (tid < 64)
+                    scan_hist<kLBins>(sh, sh.lcnt, sh.ls1, sh.ls2, l1c, a.s, 0u, 0ull, 0ull, (double)Mcol, wd,
+                                      [&](int b, double& l) {
+                                          l = (double)ledge(b);
+                                          return 0;
+                                      });
+                __syncthreads();
+                const int lb = sh.ires[0];
+                if (lb >= 0 && sh.lcnt[lb] <= (uint32_t)kLCand) {
+                    const uint32_t cA_hist = (uint32_t)sh.ires[1];
+                    // the answer bin's entries into lcand; exact FP64 sums of the entries above it
+                    double c_loc = 0, s1_loc = 0, s2_loc = 0;
+                    for (int e0 = 0; e0 < cnt; e0 += NT) {  // uniform trip count for the ballot
+                        const int e = e0 + tid;
+                        const uint64_t key = e < cnt ? sh.list[e] : 0ull;
+                        const float av = __uint_as_float((uint32_t)(key >> 32));
+                        const int b = e < cnt ? lbin(av) : -1;
+                        if (b > lb) {
+                            c_loc += 1.0;
+                            s1_loc += (double)av;
+                            s2_loc += (double)av * (double)av;
+                        }
+                        const bool inb = b == lb;
+                        const uint64_t bal = __ballot(inb);
+                        if (bal) {
+                            const int lane = tid & 63, first = __ffsll((long long)bal) - 1;
+                            int base = 0;
+                            if (lane == first) base = atomicAdd(&sh.nlc, (int)__popcll(bal));
+                            base = __shfl(base, first, 64);
+                            if (inb) sh.lcand[base + (int)__popcll(bal & ((1ull << lane) - 1ull))] = key;
+                        }
+                    }
+                    const double cAd = block_sum<NT>(c_loc, sh.red);
+                    const double S1A = block_sum<NT>(s1_loc, sh.red);
+                    const double S2A = block_sum<NT>(s2_loc, sh.red);
+                    const int nc = sh.nlc;  // <= kLCand: the bin's count
+                    bitonic_desc<NT>(sh.lcand, nc);
+                    decided = resolve(sh.lcand, nc, cA_hist, cAd, S1A, S2A, true, true, [&](float lo, float hi) {
+                        return lo >= L && lbin(lo) == lb && lbin(hi) == lb;
                     });
-                    norm2 = block_sum<NT>(acc, sh.red);
+                    if (decided) {
+                        // the next bound: the lower edge of the list bin holding place s + s/4 + 32 (L0)
+                        // or survivors + survivors/8 + 64 (L1); the same L if the list is shorter
+                        const uint32_t keep = !l1c ? a.s : (uint32_t)sh.dres[0];
+                        const uint32_t rank = !l1c ? keep + keep / 4 + 32 : keep + keep / 8 + 64;
+                        __syncthreads();
+                        if (tid < 64)
+                            scan_hist<kLBins>(sh, sh.lcnt, sh.ls1, sh.ls2, false, rank, 0u, 0ull, 0ull, 0.0, 0.0,
+                                              [&](int, double&) { return 0; });
+                        __syncthreads();
+                        const int rb = sh.ires[0];
+                        next_bound = rb >= 0 ? ledge(rb) : L;
+                    }
                 }
             }
+            __syncthreads();  // the column passes below reuse this LDS
+            if (tid == 0) {  // integer counts of list steps tried and decided (order-free)
+                atomicAdd(&sp.done[2], 1);
+                if (decided) atomicAdd(&sp.done[3], 1);
+            }
+        }
+        if (!decided) {
+        float M = Mcol;
+        if (!have_M) {
+            float mloc = 0.f;
+            sweep([&](int, float v) { mloc = fmaxf(mloc, fabsf(v)); });
+            M = block_max<NT>(mloc, sh.red);
+        }
+        const float scale = M > 0.f ? (float)kBins / M : 0.f;
+        const double invM = M > 0.f ? 1.0 / (double)M : 0.0;
+        const double Md = (double)M;
+        for (int b = tid; b < kBins; b += NT) {
+            sh.cnt[b] = 0;
+            sh.s1[b] = 0;
+            sh.s2[b] = 0;
+        }
+        __syncthreads();
+        sweep([&](int, float v) {
+            const float av = fabsf(v);
+            const int b = lin_bin(av, scale);
+            atomicAdd(&sh.cnt[b], 1u);
+            if (l1c && av > 0.f) {
+                const double w = (double)av * invM;
+                atomicAdd((unsigned long long*)&sh.s1[b], (unsigned long long)(w * kFix + 0.5));
+                atomicAdd((unsigned long long*)&sh.s2[b], (unsigned long long)(w * w * kFix + 0.5));
+            }
+        });
+        __syncthreads();
+        if (tid < 64)
+            scan_hist<kBins>(sh, sh.cnt, sh.s1, sh.s2, l1c, a.s, 0u, 0ull, 0ull, Md, wd, [&](int b, double& l) {
+                l = (double)b * Md / (double)kBins;
+                return 0;
+            });
+        __syncthreads();
+        const int bstar = sh.ires[0];
+        const uint32_t cA_hist = bstar >= 0 ? (uint32_t)sh.ires[1] : 0u;
+        const uint64_t s1A_fix = bstar >= 0 ? sh.ures[0] : 0ull, s2A_fix = bstar >= 0 ? sh.ures[1] : 0ull;
+        const uint32_t cnt_b = bstar >= 0 ? sh.cnt[bstar] : 0u;
+        bool fast = bstar >= 0 && cnt_b <= (uint32_t)kCand;
+        if (bstar < 0 && l1c) fast = true;  // no trigger: every element survives, sums over all
+        if (fast) {
+            // ---- compaction: the answer bin's elements into LDS; exact FP64 sums of the bins above
+            if (tid == 0) sh.ncand = 0;
+            __syncthreads();
+            double c_loc = 0, s1_loc = 0, s2_loc = 0;
+            sweep([&](int q, float v) {
+                const int idx = tid + q * NT;
+                if (idx >= a.n) return;
+                const float av = fabsf(v);
+                const int b = lin_bin(av, scale);
+                if (b > bstar) {
+                    c_loc += 1.0;
+                    s1_loc += (double)av;
+                    s2_loc += (double)av * (double)av;
+                } else if (b == bstar) {
+                    const int slot = atomicAdd(&sh.ncand, 1);
+                    if (slot < kCand) sh.cand[slot] = ((uint64_t)__float_as_uint(av) << 32) | (uint32_t)idx;
+                }
+            });
+            const double cAd = block_sum<NT>(c_loc, sh.red);
+            const double S1A = block_sum<NT>(s1_loc, sh.red);
+            const double S2A = block_sum<NT>(s2_loc, sh.red);
+            const int nc = min(sh.ncand, kCand);
+            bitonic_desc<NT>(sh.cand, nc);
+            resolve(sh.cand, nc, cA_hist, cAd, S1A, S2A, false, bstar >= 0, [&](float lo, float hi) {
+                return lin_bin(lo, scale) == bstar && lin_bin(hi, scale) == bstar;
+            });
+            // the next step's list bound: the lower edge of the bin holding place s + s/4 + 32 (L0) or
+            // survivors + survivors/8 + 64 (L1); none for an inactive L1 constraint (every element
+            // survives), a column shorter than that, or a list that would not fit in LDS
+            const uint32_t keep = !l1c ? a.s : (uint32_t)sh.dres[0];
+            const uint32_t rank = !l1c ? keep + keep / 4 + 32 : keep + keep / 8 + 64;
+            __syncthreads();
+            if (tid < 64)
+                scan_hist<kBins>(sh, sh.cnt, sh.s1, sh.s2, false, rank, 0u, 0ull, 0ull, 0.0, 0.0,
+                                 [&](int, double&) { return 0; });
+            __syncthreads();
+            const int rb = sh.ires[0];
+            const bool fits = rb > 0 && (uint32_t)sh.ires[1] + sh.cnt[rb] <= (uint32_t)kList;
+            next_bound = (fits && !(l1c && !(lam > 0.f))) ? (float)((double)rb * Md / (double)kBins) : 0.f;
+            decided_column = true;
         } else {
         // ---- fallback (answer bin larger than the LDS list): radix passes over the column
         uint32_t prefix = 0;  // key of the answer within bin bstar
@@ -1789,7 +2042,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 });
                 __syncthreads();
                 if (tid < 64)
-                    scan_hist<256>(sh, l1c, rank, cA, s1A, s2A, Md, wd, [&](int d, double& l) {
+                    scan_hist<256>(sh, sh.cnt, sh.s1, sh.s2, l1c, rank, cA, s1A, s2A, Md, wd, [&](int d, double& l) {
                         const uint32_t K = (pass == 0 ? 0u : (prefix << (shift + 8))) | ((uint32_t)d << shift);
                         // keys from +inf up (inf, NaN, sign bit set) bound no finite |v|: false.
                         // A boundary at or below the bin's lower edge is true by the choice of bstar.
@@ -1841,6 +2094,9 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         }
         norm2 = block_sum<NT>(norm2, sh.red);
         }
+        }  // !decided
+        if (!decided && !decided_column) next_bound = l1c ? lam * kListMarginL1 : tau * kListMarginL0;
+        if (tid == 0) sp.thr[j] = next_bound;  // the next step's list bound
         const float nf = (float)sqrt(norm2);
         scale_out = 1.0f / nf;
         fval = sp.fz[j];
@@ -1928,15 +2184,87 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     }
     // ---- write the iterate (+ operand forms), count nonzeros, update the SP record
     double nz = 0;
-    {
-        const long base = (long)j * a.ldv;
-        const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + base, (uint32_t)a.n * 4u);
-        // 16-bit forms are K-blocked: element i of column j at ((i/32)*bp + j)*32 + i%32; the
-        // views cover the whole operand (ldv * bp elements) and elements i >= n are skipped
-        const uint32_t opb = a.precision ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
-        const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
-        const __amdgpu_buffer_rsrc_t lr =
-            col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, a.precision == 1 ? opb : 0u);
+    const long xbase = (long)j * a.ldv;
+    const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + xbase, (uint32_t)a.n * 4u);
+    // 16-bit forms are K-blocked: element i of column j at ((i/32)*bp + j)*32 + i%32; the views
+    // cover the whole operand (ldv * bp elements) and elements i >= n are skipped
+    const uint32_t opb = a.precision ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
+    const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
+    const __amdgpu_buffer_rsrc_t lr =
+        col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, a.precision == 1 ? opb : 0u);
+    // x_i (i < n) in FP32 and the operand forms
+    auto put = [&](int i, float x) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, (uint32_t)i * 4u, 0, 0);
+        if (a.precision) {
+            const uint32_t bo = (uint32_t)(blk_index(j, i, xop.bp) * 2);
+            if (a.precision == 1) {
+                const uint16_t h = f2bf(x);
+                __builtin_amdgcn_raw_buffer_store_b16(h, hr, bo, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, bo, 0, 0);
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, bo, 0, 0);
+            }
+        }
+    };
+    if (decided) {
+        // ---- a list step: x's nonzeros are the kept list entries, so only they are written,
+        // after the previous step's nonzeros are cleared: the indices a previous list step kept,
+        // or (after a column step, a refill or with deflation) the whole column
+        int* kept = sp.kept + (long)j * kListMax;
+        const int pk = sp.nkept[j];
+        if (pk >= 0 && a.sparse_rewrite) {
+            for (int e = tid; e < pk; e += NT) put(kept[e], 0.f);
+        } else {
+            for (int q = 0; q < nq; q++)
+                if (tid + q * NT < a.n) put(tid + q * NT, 0.f);
+        }
+        if (tid == 0) sh.nkeep = 0;
+        __syncthreads();  // the clears land before this step's values, which reuse most indices
+        const int nband = !l1c ? sh.nband : 0;
+        const float blo = nband ? sh.band_lo : 0.f, bhi = nband ? sh.band_hi : -1.f;
+        for (int e0 = 0; e0 < list_cnt; e0 += NT) {  // uniform trip count: the ballot below sees every lane
+            const int e = e0 + tid;
+            bool keep = false;
+            float x = 0.f;
+            int i = 0;
+            if (e < list_cnt) {
+                const uint64_t key = sh.list[e];
+                const float av = __uint_as_float((uint32_t)(key >> 32));
+                i = (int)((uint32_t)key & 0x7fffffffu);
+                const float sg = ((uint32_t)key & 0x80000000u) ? -1.f : 1.f;  // the sign of v
+                if (!l1c && av >= blo && av <= bhi) {
+                    // inside the re-evaluated tie band: its decision and exact value
+                    for (int p = 0; p < nband; p++)
+                        if (sh.band_idx[p] == i && sh.band_keep[p]) {
+                            keep = true;
+                            x = sh.band_val[p];
+                        }
+                } else if (!l1c) {
+                    keep = !(av < tau);
+                    x = sg * av;
+                } else {
+                    const float t2 = av - lam;
+                    keep = t2 > 0.f;
+                    x = t2 * sg;
+                }
+                x = x * scale_out;
+            }
+            const uint64_t bal = __ballot(keep);
+            if (bal) {
+                const int lane = tid & 63, first = __ffsll((long long)bal) - 1;
+                int base = 0;
+                if (lane == first) base = atomicAdd(&sh.nkeep, (int)__popcll(bal));
+                base = __shfl(base, first, 64);
+                if (keep) {
+                    kept[base + (int)__popcll(bal & ((1ull << lane) - 1ull))] = i;
+                    put(i, x);
+                }
+            }
+        }
+        __syncthreads();
+        nz = (double)sh.nkeep;  // every kept value is nonzero: |v| >= tau > 0, or |v| - lam > 0
+        if (tid == 0) sp.nkept[j] = sh.nkeep;
+    } else {
         // exact ties (L0 fast path, penalised L2 variance): values inside the re-evaluated band
         // take its decision and value
         const int nband = ((f < 4 && !l1c) || f == 4 || f == 6) ? sh.nband : 0;
@@ -1987,9 +2315,10 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                     }
                 }
         });
+        nz = block_sum<NT>(nz, sh.red);
+        if (f < 4 && tid == 0) sp.nkept[j] = -1;  // the whole column was written
     }
 #undef LDV
-    nz = block_sum<NT>(nz, sh.red);
     if (tid == 0) {
         const float prev = sp.val[j];
         sp.err[j] = fabsf(fval - prev) / prev;  // computeTheError (termination_criteria.h:29-36)
@@ -2362,6 +2691,7 @@ __global__ __launch_bounds__(256) void otf_refill_kernel(SpState sp, OtfState o,
     if (threadIdx.x == 0) {
         sp.val[i] = 0.f;
         sp.err[i] = 0.f;
+        sp.thr[i] = 0.f;  // a new SP has no previous threshold
         sp.nnz[i] = cnt;  // the new iterate's nonzeros, for the sparse A*X decision
         o.refill[i] = 0;
     }

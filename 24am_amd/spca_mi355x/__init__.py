@@ -85,6 +85,7 @@ class _Stats(ctypes.Structure):
         ("ev_resolve_fp64", ctypes.c_double), ("resolve_iterations", ctypes.c_int),
         ("sparse_iterations", ctypes.c_uint),
         ("union_iterations", ctypes.c_uint),
+        ("list_tries", ctypes.c_uint), ("list_steps", ctypes.c_uint),
     ]
 
 
@@ -262,6 +263,8 @@ class OptimizationStatistics:
         self.resolve_iterations = 0
         self.sparse_iterations = 0
         self.union_iterations = 0
+        self.list_tries = 0  # constrained steps that started from the previous threshold's list
+        self.list_steps = 0  # of those, the ones the list decided
 
     def _fill(self, s, per_sp=None):
         self.it, self.fval, self.error = s.it, s.fval, s.error
@@ -273,6 +276,7 @@ class OptimizationStatistics:
         self.ev_resolve_fp64, self.resolve_iterations = s.ev_resolve_fp64, s.resolve_iterations
         self.sparse_iterations = s.sparse_iterations
         self.union_iterations = s.union_iterations
+        self.list_tries, self.list_steps = s.list_tries, s.list_steps
         if per_sp is not None:
             self.values = np.array([r.value for r in per_sp], dtype=np.float32)
             self.iters = np.array([r.iters for r in per_sp], dtype=np.int32)

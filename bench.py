@@ -417,6 +417,7 @@ def main():
                 per[str(f)] = {"sp_it_per_s": sp_it / stats.totalTrueComputationTime,
                                "sparse_iterations": stats.sparse_iterations,
                                "union_iterations": stats.union_iterations,
+                               "list_tries": stats.list_tries, "list_steps": stats.list_steps,
                                "ms_per_step": stats.totalTrueComputationTime * 1e3 / stats.it,
                                "gemm_ax_ms": stats.gemm_ax_ms / stats.it, "gemm_atz_ms": stats.gemm_atz_ms / stats.it,
                                "step_ms": stats.step_ms / stats.it, "fval": stats.fval, "ev_fp64": stats.ev_fp64}
@@ -428,7 +429,8 @@ def main():
                 launches += stats.it
                 print(f"# {p} f{f}: {per[str(f)]['ms_per_step']:.3f} ms/step, "
                       f"{per[str(f)]['sp_it_per_s']:.0f} SP-it/s (ax {per[str(f)]['gemm_ax_ms']:.3f} "
-                      f"atz {per[str(f)]['gemm_atz_ms']:.3f} step {per[str(f)]['step_ms']:.3f} ms)",
+                      f"atz {per[str(f)]['gemm_atz_ms']:.3f} step {per[str(f)]['step_ms']:.3f} ms; list "
+                      f"{stats.list_steps}/{stats.list_tries})",
                       file=sys.stderr, flush=True)
             flops = 2.0 * M_ROWS * N_COLS * BATCH  # per product launch
             ax_avg, atz_avg = ax_ms / launches, atz_ms / launches

@@ -150,6 +150,10 @@ typedef struct spca_stats {
     unsigned sparse_iterations; /* iterations whose A*X was gathered over the iterates' nonzeros */
     unsigned union_iterations;  /* iterations whose A*X was the product over the union of supports
                                    (sparse_products = 2) */
+    unsigned list_tries;        /* constrained SP steps that started from the previous threshold's
+                                   candidate list (every step after an SP's first) */
+    unsigned list_steps;        /* of those, the steps the list decided (the others re-ran the
+                                   column passes of a first step) */
 } spca_stats;
 
 /* per starting point result (reference values / iters / cardinalities) */
