@@ -1697,6 +1697,115 @@ struct FusedRun {
     int splits = 0;
 };
 
+// Loads one job into a fused group of `precision` (-1: the job's own): settings, plan,
+// workspace, starting points.  may_add_storage false: the matrix must already hold the
+// storage the precision reads (the request server builds it at submission).
+int fused_member_init(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, const float* X0,
+                      const float* x_out, int precision, bool may_add_storage, FusedRun& r) {
+    r.M = M;
+    if (!M || !st_in) return fail(ctx, SPCA_EINVAL, "bad matrix or settings");
+    if (M->sparse) return fail(ctx, SPCA_EINVAL, "sparse matrices solve alone");
+    int nb = 0, rc;
+    if ((rc = prepare_settings(ctx, M, st_in, X0, x_out, &r.st, &nb, may_add_storage))) return rc;
+    if (nb != 1 || r.st.use_otf)
+        return fail(ctx, SPCA_EINVAL, "fused jobs run one batch (total_sp <= batch) without OTF");
+    if (precision >= 0 && r.st.precision != precision) return fail(ctx, SPCA_EINVAL, "fused jobs must share one precision");
+    const int bp = round_up(r.st.batch, kPadB);
+    Plan plan;
+    if ((rc = plan_for(ctx, E, M, r.st, bp, false, X0, &plan, &r.ws))) return rc;
+    r.splits = plan.splits;
+    if ((rc = ensure_ws(ctx, r.ws, E.stream, M->np, M->mp, bp, r.splits))) return rc;
+    std::vector<float> hX(r.st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)M->n * r.st.batch);
+    make_starts(r.st, M->n, X0, 0, r.st.batch, hX.data());
+    return load_batch(ctx, E.stream, r.ws, r.st, M->n, r.st.batch, hX.data());
+}
+
+// Work-item offsets of a fused group: each job's tiles and columns follow the previous job's.
+struct FusedTotals {
+    int t1 = 0, t2 = 0, c0 = 0, t3 = 0, t4 = 0, any_fix8 = 0;
+};
+
+// The job-table entry of one member at the current offsets, which it then advances.
+void fused_entry(FusedRun& r, int precision, FusedTotals& T, FusedJob& f) {
+    Matrix* M = r.M;
+    f = FusedJob();
+    f.p1_32 = M->a32_rm;
+    f.p1_hi = precision == 1 ? M->hi_rm : M->h_rm;
+    f.p1_lo = M->lo_rm;
+    const AtzForm atz = atz_form(M, r.st, r.ws.bp, 0);
+    f.p2_32 = M->a32_cm;
+    f.p2_hi = atz.hi;
+    f.p2_lo = atz.lo;
+    f.p2_q = atz.q;
+    f.p2_qscale = atz.qscale;
+    f.p2_prec = atz.prec;
+    f.zprec = atz.zprec;
+    T.any_fix8 |= atz.prec == 4;
+    f.Zpart = r.ws.Zpart;
+    f.V = r.ws.V;
+    f.xop = x_operand(r.ws, precision);
+    f.zop = z_operand(r.ws, atz.zprec);
+    f.rf = sign_refine(M, r.st, r.ws);
+    f.sp = state_of(r.ws);
+    f.sa.formulation = r.st.formulation;
+    f.sa.precision = precision;
+    f.sa.n = M->n;
+    f.sa.ldv = M->np;
+    f.sa.s = r.st.constraint;
+    f.sa.gamma = r.st.penalty;
+    f.sa.batch = r.st.batch;
+    f.sa.sparse_rewrite = 1;
+    set_exact_ties(f.sa, M, r.st, r.ws);
+    f.mp = M->mp;
+    f.np = M->np;
+    f.bp = r.ws.bp;
+    f.splits = r.splits;
+    f.zsplit = (long)M->mp * f.bp;
+    f.tol = r.st.tolerance;
+    f.max_iter = r.st.max_iter;
+    f.per_sp = r.st.termination == SPCA_TERM_PER_SP;
+    f.t1 = T.t1;
+    f.t2 = T.t2;
+    f.c0 = T.c0;
+    f.t3 = T.t3;
+    f.sparse_cap = sparse_cap(r.st, M->n);
+    // FP32 batches of at most 64 SPs: narrow product tiles, as in a solo solve
+    const int nb = (precision == 0 && r.st.batch <= 64) ? (r.st.batch <= 32 ? 32 : 64) : 0;
+    f.nb1 = nb;
+    f.nb2 = f.p2_prec == 0 ? nb : 0;
+    T.t1 += (f.mp / kGemmBM) * (f.nb1 ? 1 : f.bp / kGemmBN) * f.splits;
+    T.t2 += (f.np / kGemmBM) * (f.nb2 ? 1 : f.bp / kGemmBN);
+    T.c0 += r.st.batch;
+    T.t3 += (f.mp + kSparseRows - 1) / kSparseRows * r.st.batch;
+    // small batches: the K-split partials are summed over (row block, SP) items first, as
+    // in a solo solve (the same order, so the same bits)
+    f.t4 = T.t4;
+    f.psum = f.splits > 1 && r.st.batch <= 256;
+    if (f.psum) T.t4 += (f.mp + 255) / 256 * r.st.batch;
+}
+
+// A fused member's results once its state says done: the best SP, the final normalisation,
+// the statistics and (resolve_iters > 0) the FP64 support re-solve.
+// loop_ms(total iterations) gives the loop time the statistics report.
+template <class LoopMs>
+int fused_member_finish(spca_ctx ctx, Engine& E, FusedRun& r, float* x_out, spca_stats* stats, spca_sp_result* per_sp,
+                        LoopMs loop_ms, std::chrono::steady_clock::time_point t_wall0) {
+    Results R;
+    int rc;
+    if ((rc = collect_batch(ctx, E.stream, r.M, r.ws, r.st, 0, r.st.batch, per_sp != nullptr || stats != nullptr, per_sp,
+                            nullptr, R)))
+        return rc;
+    finish_solve(R, r.M->n, r.st, x_out, stats, loop_ms((long)R.total_it), nullptr, t_wall0);
+    if (stats && r.st.resolve_iters > 0) {
+        int its = 0;
+        if ((rc = resolve_impl(ctx, E, r.M, r.st.formulation, x_out, r.st.resolve_iters, r.st.resolve_tol, nullptr,
+                               &stats->ev_resolve_fp64, &its)))
+            return rc;
+        stats->resolve_iterations = its;
+    }
+    return SPCA_OK;
+}
+
 int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigned flags) {
     const auto t_wall0 = std::chrono::steady_clock::now();
     if (!jobs || njobs < 1) return fail(ctx, SPCA_EINVAL, "fused solve needs at least one job");
@@ -1728,109 +1837,21 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
         spca_fused_job& J = jobs[k];
         FusedRun& r = run[k];
         auto it = ctx->mats.find(J.matrix);  // the caller holds ctx->mu
-        if (it != ctx->mats.end()) r.M = it->second;
         if (J.stats) std::memset(J.stats, 0, sizeof(*J.stats));
-        if (!r.M || !J.settings) {
-            J.status = SPCA_EINVAL;
-            return cleanup(fail(ctx, SPCA_EINVAL, "fused job " + std::to_string(k) + ": bad matrix or settings"));
-        }
-        if (r.M->sparse) {
-            J.status = SPCA_EINVAL;
-            return cleanup(fail(ctx, SPCA_EINVAL, "fused job " + std::to_string(k) + ": sparse matrices solve alone"));
-        }
-        int nb = 0;
-        if ((rc = prepare_settings(ctx, r.M, J.settings, J.X0, J.x_out, &r.st, &nb))) {
+        if ((rc = fused_member_init(ctx, E, it != ctx->mats.end() ? it->second : nullptr, J.settings, J.X0, J.x_out,
+                                    precision, true, r))) {
             J.status = rc;
-            return cleanup(rc);
-        }
-        if (nb != 1 || r.st.use_otf) {
-            J.status = SPCA_EINVAL;
-            return cleanup(fail(ctx, SPCA_EINVAL, "fused job " + std::to_string(k) +
-                                                      ": fused jobs run one batch (total_sp <= batch) without OTF"));
-        }
-        if (precision >= 0 && r.st.precision != precision) {
-            J.status = SPCA_EINVAL;
-            return cleanup(fail(ctx, SPCA_EINVAL, "fused jobs must share one precision"));
+            return cleanup(fail(ctx, rc, "fused job " + std::to_string(k) + ": " + tl_err));
         }
         precision = r.st.precision;
         check_every = std::min(check_every, std::max(1, r.st.check_every));
         max_iter = std::max(max_iter, r.st.max_iter);
-        const int bp = round_up(r.st.batch, kPadB);
-        Plan plan;
-        if ((rc = plan_for(ctx, E, r.M, r.st, bp, false, J.X0, &plan, &r.ws))) {
-            J.status = rc;
-            return cleanup(rc);
-        }
-        r.splits = plan.splits;
-        if ((rc = ensure_ws(ctx, r.ws, s, r.M->np, r.M->mp, bp, r.splits))) {
-            J.status = rc;
-            return cleanup(rc);
-        }
-        std::vector<float> hX(r.st.start == SPCA_START_GAUSSIAN ? 0 : (size_t)r.M->n * r.st.batch);
-        make_starts(r.st, r.M->n, J.X0, 0, r.st.batch, hX.data());
-        if ((rc = load_batch(ctx, s, r.ws, r.st, r.M->n, r.st.batch, hX.data()))) return cleanup(rc);
     }
     // ---- job table
     std::vector<FusedJob> tab(njobs);
-    int t1 = 0, t2 = 0, c0 = 0, t3 = 0, t4 = 0, any_fix8 = 0;
-    for (int k = 0; k < njobs; k++) {
-        FusedRun& r = run[k];
-        Matrix* M = r.M;
-        FusedJob& f = tab[k];
-        f.p1_32 = M->a32_rm;
-        f.p1_hi = precision == 1 ? M->hi_rm : M->h_rm;
-        f.p1_lo = M->lo_rm;
-        const AtzForm atz = atz_form(M, r.st, r.ws.bp, 0);
-        f.p2_32 = M->a32_cm;
-        f.p2_hi = atz.hi;
-        f.p2_lo = atz.lo;
-        f.p2_q = atz.q;
-        f.p2_qscale = atz.qscale;
-        f.p2_prec = atz.prec;
-        f.zprec = atz.zprec;
-        any_fix8 |= atz.prec == 4;
-        f.Zpart = r.ws.Zpart;
-        f.V = r.ws.V;
-        f.xop = x_operand(r.ws, precision);
-        f.zop = z_operand(r.ws, atz.zprec);
-        f.rf = sign_refine(M, r.st, r.ws);
-        f.sp = state_of(r.ws);
-        f.sa.formulation = r.st.formulation;
-        f.sa.precision = precision;
-        f.sa.n = M->n;
-        f.sa.ldv = M->np;
-        f.sa.s = r.st.constraint;
-        f.sa.gamma = r.st.penalty;
-        f.sa.batch = r.st.batch;
-        f.sa.sparse_rewrite = 1;
-        set_exact_ties(f.sa, M, r.st, r.ws);
-        f.mp = M->mp;
-        f.np = M->np;
-        f.bp = r.ws.bp;
-        f.splits = r.splits;
-        f.zsplit = (long)M->mp * f.bp;
-        f.tol = r.st.tolerance;
-        f.max_iter = r.st.max_iter;
-        f.per_sp = r.st.termination == SPCA_TERM_PER_SP;
-        f.t1 = t1;
-        f.t2 = t2;
-        f.c0 = c0;
-        f.t3 = t3;
-        f.sparse_cap = sparse_cap(r.st, M->n);
-        // FP32 batches of at most 64 SPs: narrow product tiles, as in a solo solve
-        const int nb = (precision == 0 && r.st.batch <= 64) ? (r.st.batch <= 32 ? 32 : 64) : 0;
-        f.nb1 = nb;
-        f.nb2 = f.p2_prec == 0 ? nb : 0;
-        t1 += (f.mp / kGemmBM) * (f.nb1 ? 1 : f.bp / kGemmBN) * f.splits;
-        t2 += (f.np / kGemmBM) * (f.nb2 ? 1 : f.bp / kGemmBN);
-        c0 += r.st.batch;
-        t3 += (f.mp + kSparseRows - 1) / kSparseRows * r.st.batch;
-        // small batches: the K-split partials are summed over (row block, SP) items first, as
-        // in a solo solve (the same order, so the same bits)
-        f.t4 = t4;
-        f.psum = f.splits > 1 && r.st.batch <= 256;
-        if (f.psum) t4 += (f.mp + 255) / 256 * r.st.batch;
-    }
+    FusedTotals T;
+    for (int k = 0; k < njobs; k++) fused_entry(run[k], precision, T, tab[k]);
+    const int t1 = T.t1, t2 = T.t2, c0 = T.c0, t3 = T.t3, t4 = T.t4, any_fix8 = T.any_fix8;
     if ((rc = dev_alloc(ctx, &d_jobs, (size_t)njobs, &tb)) || (rc = dev_alloc(ctx, &d_done, 4, &tb)))
         return cleanup(rc);
     if (hipMemcpyAsync(d_jobs, tab.data(), sizeof(FusedJob) * njobs, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1881,22 +1902,14 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     for (int k = 0; k < njobs; k++) {
         spca_fused_job& J = jobs[k];
         FusedRun& r = run[k];
-        Results R;
-        if ((rc = collect_batch(ctx, s, r.M, r.ws, r.st, 0, r.st.batch, J.per_sp != nullptr || J.stats != nullptr,
-                                J.per_sp, nullptr, R)))
-            return cleanup(rc);
         // the chunk in which the job finished bounds its loop time
-        const int c = std::min(chunks, std::max(1, (int)((R.total_it + chunk - 1) / chunk)));
-        float t = 0;
-        (void)hipEventElapsedTime(&t, chunk_ev[0], chunk_ev[c]);
-        finish_solve(R, r.M->n, r.st, J.x_out, J.stats, t, nullptr, t_wall0);
-        if (J.stats && r.st.resolve_iters > 0) {
-            int its = 0;
-            if ((rc = resolve_impl(ctx, E, r.M, r.st.formulation, J.x_out, r.st.resolve_iters, r.st.resolve_tol,
-                                   nullptr, &J.stats->ev_resolve_fp64, &its)))
-                return cleanup(rc);
-            J.stats->resolve_iterations = its;
-        }
+        auto loop_ms = [&](long its) {
+            const int c = std::min(chunks, std::max(1, (int)((its + chunk - 1) / chunk)));
+            float t = 0;
+            (void)hipEventElapsedTime(&t, chunk_ev[0], chunk_ev[c]);
+            return (double)t;
+        };
+        if ((rc = fused_member_finish(ctx, E, r, J.x_out, J.stats, J.per_sp, loop_ms, t_wall0))) return cleanup(rc);
         J.status = SPCA_OK;
     }
     return cleanup(SPCA_OK);
@@ -1904,11 +1917,21 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
 
 // ---------------------------------------------------------------- request server
 // Continuous batching: jobs wait in a FIFO queue; each worker thread owns an engine (its
-// own stream and workspace) and takes the next job as soon as it is free and the job's
-// workspace fits the HBM budget.  A job that does not fit waits for a running job to
-// return its memory; it fails with SPCA_EBUDGET only when nothing else is running.  Work
-// from different workers overlaps on the device.  Kernels are deterministic and every job
-// has its own buffers, so a job's result does not depend on what runs beside it.
+// own stream) and takes the next job as soon as it is free.  A short job (dense, one batch,
+// no OTF, 128x128 tiles, an estimated iteration of at most kFuseJobUs) starts a fused group
+// on that worker: the group's jobs iterate together, one launch per phase for all of them
+// (as spca_solve_fused), and at every chunk boundary finished jobs leave (their memory
+// returns at once) and queued short jobs of the group's precision join while the group's
+// estimated iteration stays within kGroupUs and their workspaces fit the HBM budget.  Members
+// iterate in lockstep, so the caps keep a short job's iterations short.  With several
+// workers, worker 0 is the group lane: it takes only short jobs, from anywhere in the queue,
+// and its groups admit them past longer jobs, which the other workers take in order; a
+// single worker admits strictly in order.  Longer jobs (and sparse, several-batch and OTF
+// jobs) are solved alone by the worker that takes them, beside the groups.  A job that does
+// not fit waits for running jobs to return memory; it fails with SPCA_EBUDGET only when
+// nothing else is running.  Kernels are deterministic, every job has its own buffers and a
+// fused job's tiles, plan and summation order do not depend on its group, so a job's result
+// does not depend on what runs beside it.
 struct Job {
     uint64_t id = 0;
     uint64_t mat = 0;
@@ -1925,9 +1948,27 @@ struct Job {
     std::string err;
     std::chrono::steady_clock::time_point t_submit, t_start, t_end;
     int np = 0, mp = 0, bp = 0, splits = 0;
+    bool fusable = false;  // a short job: may run in a fused group
+    double est_us = 0;     // estimated time of one iteration (product flops at a nominal rate)
+    int nsp = 0;           // starting points after the settings check
+    // the result buffers, sized when the job starts (a long queue holds no result memory)
+    void alloc_results() {
+        x.assign(M->n, 0.f);
+        if (want_per_sp) per_sp.resize(nsp);
+    }
 };
 
 }  // namespace
+
+// Fused groups: jobs whose iteration is estimated at most kFuseJobUs, groups whose members'
+// iterations add up to at most kGroupUs (estimates: 4 m n B product flops per iteration at a
+// nominal rate per precision).
+constexpr double kFuseJobUs = 200.0;
+constexpr double kGroupUs = 1000.0;
+inline double est_iteration_us(int precision, int mp, int np, int bp) {
+    const double tflops = precision == 0 ? 140.0 : precision == 1 ? 350.0 : 700.0;
+    return 4.0 * mp * np * bp / (tflops * 1e6);
+}
 
 struct spca_server {
     std::mutex mu;
@@ -1939,6 +1980,8 @@ struct spca_server {
     bool stop = false;
     uint64_t next_id = 1;
     int running = 0;
+    int group_cap = 64;  // most jobs in one fused group; 0: every job is solved alone
+    spca_server_info info{};
 };
 
 namespace {
@@ -1953,19 +1996,241 @@ void matrix_unref(spca_ctx ctx, uint64_t h) {
     }
 }
 
+// The end of a job: its matrix reference is dropped and waiters are woken.
+void retire_job(spca_ctx ctx, spca_server* S, Job* J, int rc) {
+    J->t_end = std::chrono::steady_clock::now();
+    matrix_unref(ctx, J->mat);
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        J->status = rc;
+        J->state = rc == SPCA_ECANCELLED ? SPCA_JOB_CANCELLED : SPCA_JOB_DONE;
+        S->running--;
+    }
+    S->cv.notify_all();
+}
+
+// Jobs a group took but cannot admit go back to the front of the queue, in their order.
+void requeue(spca_server* S, const std::vector<Job*>& back) {
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        for (auto q = back.rbegin(); q != back.rend(); ++q) {
+            (*q)->state = SPCA_JOB_QUEUED;
+            S->queue.push_front(*q);
+            S->running--;
+        }
+    }
+    S->cv.notify_all();
+}
+
+// One member of a running fused group.
+struct Member {
+    Job* J = nullptr;
+    FusedRun r;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;  // before its first chunk; after the latest chunk
+};
+
+// A fused group started by `first` on this worker's stream.  Each chunk is min(check_every)
+// iterations (at most 64) of every member, launched as one fused iteration per step; then the
+// host reads every member's done flag, retires finished and cancelled members and admits
+// queued jobs, which start at iteration 0 beside the others.
+void run_group(spca_ctx ctx, spca_server* S, Engine& E, Job* first, bool lane) {
+    hipStream_t s = E.stream;
+    const int precision = first->st.precision;
+    int cap;
+    {
+        std::lock_guard<std::mutex> lk(S->mu);
+        cap = std::max(1, S->group_cap);
+        S->info.groups++;
+    }
+    free_workspace(ctx, E);  // members hold their own workspaces
+    FusedJob* d_jobs = nullptr;
+    int* d_done = nullptr;
+    int* h_flags = nullptr;
+    uint64_t tb = 0;
+    std::vector<Member*> mem;
+    std::vector<FusedJob> tab;
+    FusedTotals T;
+    bool changed = true;
+    auto drop = [&](Member* m, int rc) {
+        free_ws(ctx, m->r.ws, s);
+        if (m->ev0) (void)hipEventDestroy(m->ev0);
+        if (m->ev1) (void)hipEventDestroy(m->ev1);
+        retire_job(ctx, S, m->J, rc);
+        delete m;
+    };
+    int rc = dev_alloc(ctx, &d_jobs, (size_t)cap, &tb);
+    if (!rc) rc = dev_alloc(ctx, &d_done, 4, &tb);
+    if (!rc && hipHostMalloc((void**)&h_flags, sizeof(int) * cap, hipHostMallocDefault) != hipSuccess) {
+        h_flags = nullptr;
+        rc = fail(ctx, SPCA_EDEVICE, "pinned flags allocation failed");
+    }
+    // admits J (taken off the queue, counted as running); false: no memory for it now while
+    // other members run, so the caller puts it back
+    auto admit = [&](Job* J) -> bool {
+        J->t_start = std::chrono::steady_clock::now();
+        if (J->cancel.load()) {
+            retire_job(ctx, S, J, SPCA_ECANCELLED);
+            return true;
+        }
+        Member* m = new Member();
+        m->J = J;
+        J->alloc_results();
+        const float* X0 = J->X0.empty() ? nullptr : J->X0.data();
+        int e;
+        for (;;) {
+            e = fused_member_init(ctx, E, J->M, &J->st, X0, J->x.data(), precision, false, m->r);
+            if (e != SPCA_EBUDGET) break;
+            free_ws(ctx, m->r.ws, s);
+            if (!mem.empty()) {
+                delete m;
+                return false;
+            }
+            std::unique_lock<std::mutex> lk(S->mu);
+            if (S->running <= 1 || J->cancel.load() || S->stop) break;
+            S->cv.wait_for(lk, std::chrono::milliseconds(50));
+        }
+        if (e == SPCA_OK && (hipEventCreate(&m->ev0) != hipSuccess || hipEventCreate(&m->ev1) != hipSuccess ||
+                             hipEventRecord(m->ev0, s) != hipSuccess))
+            e = fail(ctx, SPCA_EDEVICE, "event creation failed");
+        if (e != SPCA_OK) {
+            J->err = tl_err;
+            drop(m, e);
+            return true;
+        }
+        {
+            std::lock_guard<std::mutex> lk(S->mu);
+            if (!mem.empty()) S->info.joined_running++;
+            S->info.max_members = std::max(S->info.max_members, (int)mem.size() + 1);
+        }
+        mem.push_back(m);
+        changed = true;
+        return true;
+    };
+    if (rc) {
+        first->err = tl_err;
+        retire_job(ctx, S, first, rc);
+    } else {
+        admit(first);
+    }
+    while (!rc && !mem.empty()) {
+        // ---- admission at the chunk boundary.  The group lane takes short jobs of its precision
+        // from anywhere in the queue (other workers take the rest in order); a single worker stops
+        // at the first job the group cannot take, so the group drains and that job runs next.
+        std::vector<Job*> take;
+        {
+            double est = 0;
+            for (Member* m : mem) est += m->J->est_us;
+            std::lock_guard<std::mutex> lk(S->mu);
+            for (auto q = S->queue.begin(); q != S->queue.end() && (int)(mem.size() + take.size()) < cap;) {
+                Job* Q = *q;
+                if (Q->fusable && Q->st.precision == precision && est + Q->est_us <= kGroupUs) {
+                    q = S->queue.erase(q);
+                    Q->state = SPCA_JOB_RUNNING;
+                    S->running++;
+                    take.push_back(Q);
+                    est += Q->est_us;
+                } else if (lane) {
+                    ++q;
+                } else {
+                    break;
+                }
+            }
+        }
+        for (size_t t = 0; t < take.size(); t++)
+            if (!admit(take[t])) {
+                requeue(S, std::vector<Job*>(take.begin() + t, take.end()));
+                break;
+            }
+        if (mem.empty()) break;
+        if (changed) {
+            tab.resize(mem.size());
+            T = FusedTotals();
+            for (size_t k = 0; k < mem.size(); k++) fused_entry(mem[k]->r, precision, T, tab[k]);
+            if (hipMemcpyAsync(d_jobs, tab.data(), sizeof(FusedJob) * tab.size(), hipMemcpyHostToDevice, s) != hipSuccess ||
+                hipMemsetAsync(d_done, 0, sizeof(int) * 4, s) != hipSuccess) {
+                rc = fail(ctx, SPCA_EDEVICE, "fused job table upload failed");
+                break;
+            }
+            changed = false;
+        }
+        // ---- one chunk
+        int chunk = 64;
+        for (Member* m : mem) chunk = std::min(chunk, std::max(1, m->r.st.check_every));
+        hipError_t e = hipSuccess;
+        for (int i = 0; i < chunk && e == hipSuccess; i++)
+            e = launch_fused_iteration(precision, d_jobs, (int)mem.size(), T.t1, T.t2, T.c0, T.t3, T.t4, T.any_fix8,
+                                       d_done, s);
+        for (size_t k = 0; k < mem.size() && e == hipSuccess; k++) {
+            e = hipEventRecord(mem[k]->ev1, s);
+            if (e == hipSuccess) e = hipMemcpyAsync(&h_flags[k], tab[k].sp.done, sizeof(int), hipMemcpyDeviceToHost, s);
+        }
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            rc = fail(ctx, SPCA_EDEVICE, std::string("fused group iteration: ") + hipGetErrorString(e));
+            break;
+        }
+        // ---- retire finished and cancelled members
+        std::vector<Member*> keep;
+        for (size_t k = 0; k < mem.size(); k++) {
+            Member* m = mem[k];
+            Job* J = m->J;
+            if (J->cancel.load()) {
+                drop(m, SPCA_ECANCELLED);
+                changed = true;
+            } else if (h_flags[k]) {
+                float t = 0;
+                (void)hipEventElapsedTime(&t, m->ev0, m->ev1);
+                int r2 = fused_member_finish(ctx, E, m->r, J->x.data(), &J->stats,
+                                             J->want_per_sp ? J->per_sp.data() : nullptr,
+                                             [t](long) { return (double)t; }, J->t_start);
+                if (r2 != SPCA_OK) J->err = tl_err;
+                drop(m, r2);
+                changed = true;
+            } else {
+                keep.push_back(m);
+            }
+        }
+        mem.swap(keep);
+    }
+    if (rc)  // a device failure ends every member
+        for (Member* m : mem) {
+            m->J->err = tl_err;
+            drop(m, rc);
+        }
+    (void)hipStreamSynchronize(s);
+    dev_free(ctx, d_jobs, sizeof(FusedJob) * (uint64_t)cap);
+    dev_free(ctx, d_done, sizeof(int) * 4);
+    if (h_flags) (void)hipHostFree(h_flags);
+}
+
 void worker_main(spca_ctx ctx, spca_server* S, int w) {
     (void)hipSetDevice(ctx->device);
     Engine& E = S->engines[w];
     for (;;) {
         Job* J = nullptr;
+        bool group = false, lane = false;
         {
             std::unique_lock<std::mutex> lk(S->mu);
-            S->cv.wait(lk, [&] { return S->stop || !S->queue.empty(); });
-            if (S->queue.empty()) break;  // stop requested and nothing queued
-            J = S->queue.front();
-            S->queue.pop_front();
+            // the job this worker takes now: the queue's front, or for the group lane its first
+            // short job
+            auto pick = [&]() -> std::deque<Job*>::iterator {
+                lane = w == 0 && S->group_cap > 0 && S->workers.size() > 1;
+                if (!lane) return S->queue.begin();
+                return std::find_if(S->queue.begin(), S->queue.end(), [](const Job* Q) { return Q->fusable; });
+            };
+            S->cv.wait(lk, [&] { return S->stop || pick() != S->queue.end(); });
+            auto it = pick();
+            if (it == S->queue.end()) break;  // stop requested and nothing this worker takes
+            J = *it;
+            S->queue.erase(it);
             J->state = SPCA_JOB_RUNNING;
             S->running++;
+            group = J->fusable && S->group_cap > 0;
+            if (!group) S->info.solo_jobs++;
+        }
+        if (group) {
+            run_group(ctx, S, E, J, lane);
+            continue;
         }
         // admission by HBM footprint
         int rc;
@@ -1984,13 +2249,13 @@ void worker_main(spca_ctx ctx, spca_server* S, int w) {
         if (rc == SPCA_OK && J->cancel.load()) rc = SPCA_ECANCELLED;
         if (rc == SPCA_OK) {
             E.cancel = &J->cancel;
+            J->alloc_results();
             const float* X0 = J->X0.empty() ? nullptr : J->X0.data();
             rc = solve_and_resolve(ctx, E, J->M, &J->st, X0, J->x.data(), &J->stats,
                                    J->want_per_sp ? J->per_sp.data() : nullptr, nullptr);
             E.cancel = nullptr;
             if (rc != SPCA_OK) J->err = tl_err;
         }
-        J->t_end = std::chrono::steady_clock::now();
         // keep the workspace only if the next queued job has the same shape
         bool keep = false;
         {
@@ -2001,14 +2266,7 @@ void worker_main(spca_ctx ctx, spca_server* S, int w) {
             }
         }
         if (!keep) free_workspace(ctx, E);
-        matrix_unref(ctx, J->mat);
-        {
-            std::lock_guard<std::mutex> lk(S->mu);
-            J->status = rc;
-            J->state = rc == SPCA_ECANCELLED ? SPCA_JOB_CANCELLED : SPCA_JOB_DONE;
-            S->running--;
-        }
-        S->cv.notify_all();
+        retire_job(ctx, S, J, rc);
     }
     free_workspace(ctx, E);
 }
@@ -2428,6 +2686,20 @@ int spca_server_stop(spca_ctx ctx) {
     return SPCA_OK;
 }
 
+int spca_server_set_batching(spca_ctx ctx, int max_group_jobs) {
+    if (!ctx || !ctx->server || max_group_jobs < 0 || max_group_jobs > 1024) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->server->mu);
+    ctx->server->group_cap = max_group_jobs;
+    return SPCA_OK;
+}
+
+int spca_server_get_info(spca_ctx ctx, spca_server_info* info) {
+    if (!ctx || !ctx->server || !info) return SPCA_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->server->mu);
+    *info = ctx->server->info;
+    return SPCA_OK;
+}
+
 int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, int want_per_sp,
                 spca_job* out) {
     if (!ctx || !settings || !out) return SPCA_EINVAL;
@@ -2460,13 +2732,19 @@ int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, co
             return rc;
         }
         spca_settings chk = J->st;
-        spca_settings_check(&chk, (unsigned)M->n);
+        const int nbatches = spca_settings_check(&chk, (unsigned)M->n);
+        J->fusable = !M->sparse && nbatches == 1 && !chk.use_otf;
         J->M = M;
         J->mat = mat;
         J->np = M->np;
         J->mp = M->mp;
         J->bp = round_up(chk.batch, kPadB);
-        J->splits = peek_plan(ctx, J->st.precision, M->mp, M->np, J->bp, true).splits;
+        const Plan plan = peek_plan(ctx, J->st.precision, M->mp, M->np, J->bp, true);
+        J->splits = plan.splits;
+        // a fused group runs 128x128 tiles: a shape whose solo plan takes the large tiles is
+        // solved alone, so every job's result is its solo result
+        J->est_us = est_iteration_us(J->st.precision, J->mp, J->np, J->bp);
+        J->fusable = J->fusable && !plan.big && J->est_us <= kFuseJobUs;
         uint64_t need_bytes = workspace_bytes(J->np, J->mp, J->bp, J->splits);
         const int ucap = (M->sparse || chk.use_otf) ? 0 : union_cap(chk, M->n);
         if (ucap > 0) {  // the union A*X buffers (flags, list, Au, Xu)
@@ -2478,9 +2756,8 @@ int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, co
             return fail(ctx, SPCA_EBUDGET, "job workspace exceeds the HBM budget");
         }
         if (J->st.start == SPCA_START_EXPLICIT) J->X0.assign(X0, X0 + (size_t)M->n * chk.total_sp);
-        J->x.assign(M->n, 0.f);
+        J->nsp = chk.total_sp;  // result buffers are allocated when the job starts
         J->want_per_sp = want_per_sp != 0;
-        if (J->want_per_sp) J->per_sp.resize(chk.total_sp);
         M->refs++;
     }
     J->t_submit = std::chrono::steady_clock::now();
@@ -2548,7 +2825,7 @@ int spca_wait(spca_ctx ctx, spca_job job, float* x_out, spca_stats* stats, spca_
     }
     const int rc = J->status;
     if (rc == SPCA_OK) {
-        if (x_out) std::memcpy(x_out, J->x.data(), sizeof(float) * J->x.size());
+        if (x_out && !J->x.empty()) std::memcpy(x_out, J->x.data(), sizeof(float) * J->x.size());
         if (stats) *stats = J->stats;
         if (per_sp && J->want_per_sp) std::memcpy(per_sp, J->per_sp.data(), sizeof(spca_sp_result) * J->per_sp.size());
     } else if (!J->err.empty()) {

@@ -1455,6 +1455,7 @@ struct SelShared {
     int ncand;
     int nkeep;
     int nlc;
+    int wtot[2][kSelThreads / 64];  // per-wave entry counts of the list pass (double-buffered)
     double dres[4];
     // exact ties: the re-evaluated band (sorted descending), its value range and keep flags
     int nband;
@@ -1617,6 +1618,7 @@ __device__ void scan_hist(SelShared& sh, const uint32_t* cnt, const uint64_t* hs
 template <int NT, int DEPTH>
 __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Operand xop, StepArgs a, SpState sp) {
     if (*sp.done || !sp.active[j]) return;
+    static_assert(NT <= kSelThreads, "SelShared holds kSelThreads / 64 waves");
     __shared__ SelShared sh;
     const int tid = threadIdx.x;
     const int nq = (a.n + NT - 1) / NT;
@@ -1818,29 +1820,56 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         const float thr_prev = sp.thr[j];
         if (thr_prev > 0.f) {
             const float L = thr_prev;
-            if (tid == 0) sh.ncand = 0;
-            __syncthreads();
+            // the entries in a fixed order, (chunk of DEPTH rows, thread, row): a block prefix sum
+            // per chunk places them, so every FP64 sum over the list below has a fixed order
             float mloc = 0.f;
-            sweep([&](int q, float v) {
-                // one LDS atomic per wave for the wave's entries (q is uniform across the wave)
-                const float av = fabsf(v);
-                mloc = fmaxf(mloc, av);
-                const bool in = av >= L && tid + q * NT < a.n;
-                const uint64_t bal = __ballot(in);
-                if (bal) {
-                    const int lane = tid & 63, first = __ffsll((long long)bal) - 1;
-                    int base = 0;
-                    if (lane == first) base = atomicAdd(&sh.ncand, (int)__popcll(bal));
-                    base = __shfl(base, first, 64);
-                    const int slot = base + (int)__popcll(bal & ((1ull << lane) - 1ull));
-                    if (in && slot < kList)
-                        sh.list[slot] = ((uint64_t)__float_as_uint(av) << 32) | (uint32_t)(tid + q * NT) |
-                                        (v < 0.f ? 0x80000000u : 0u);
+            int total = 0;  // entries so far, block-uniform
+            {
+                const int lane = tid & 63, w = tid >> 6;
+                int buf = 0;
+                for (int q0 = 0; q0 < nq; q0 += DEPTH, buf ^= 1) {
+                    float vv[DEPTH];
+#pragma unroll
+                    for (int u = 0; u < DEPTH; u++) vv[u] = LDV(q0 + u);
+                    uint32_t in = 0;
+#pragma unroll
+                    for (int u = 0; u < DEPTH; u++) {
+                        const float av = fabsf(vv[u]);
+                        if (q0 + u < nq) mloc = fmaxf(mloc, av);
+                        if (q0 + u < nq && av >= L && tid + (q0 + u) * NT < a.n) in |= 1u << u;
+                    }
+                    const int c = __popc(in);
+                    int incl = c;  // inclusive prefix over the wave's lanes
+                    for (int o = 1; o < 64; o <<= 1) {
+                        const int t = __shfl_up(incl, o, 64);
+                        if (lane >= o) incl += t;
+                    }
+                    // two buffers: a wave reuses one only after the next chunk's barrier, which
+                    // every wave passes after reading it
+                    if (lane == 63) sh.wtot[buf][w] = incl;
+                    __syncthreads();
+                    int before = 0, chunk = 0;
+#pragma unroll
+                    for (int k = 0; k < NT / 64; k++) {
+                        const int t = sh.wtot[buf][k];
+                        before += k < w ? t : 0;
+                        chunk += t;
+                    }
+                    int slot = total + before + incl - c;
+#pragma unroll
+                    for (int u = 0; u < DEPTH; u++)
+                        if ((in >> u) & 1u) {
+                            if (slot < kList)
+                                sh.list[slot] = ((uint64_t)__float_as_uint(fabsf(vv[u])) << 32) |
+                                                (uint32_t)(tid + (q0 + u) * NT) | (vv[u] < 0.f ? 0x80000000u : 0u);
+                            slot++;
+                        }
+                    total += chunk;
                 }
-            });
+            }
             Mcol = block_max<NT>(mloc, sh.red);  // its barriers also publish the list
             have_M = true;
-            const int cnt = sh.ncand;
+            const int cnt = total;
             list_cnt = cnt;
             if (cnt > 0 && cnt <= kList && (l1c || cnt >= (int)a.s)) {
                 for (int b = tid; b < kLBins; b += NT) {

@@ -37,7 +37,7 @@ EXPORTS = [
     "spca_resolve_support", "spca_server_start", "spca_server_stop", "spca_submit", "spca_poll", "spca_cancel",
     "spca_wait", "spca_solve_fused", "spca_set_tuning", "spca_get_plan", "spca_matrix_generate",
     "spca_matrix_download", "spca_ctx_memory", "spca_solve_components", "spca_matrix_upload_csc",
-    "spca_product_forms", "spca_solve_noalloc",
+    "spca_product_forms", "spca_solve_noalloc", "spca_server_set_batching", "spca_server_get_info",
 ]
 NOALLOC_SCRATCH_BYTES = 128 << 20  # the fixed device array of spca_solve_noalloc
 FORM_FP32, FORM_BF16X3, FORM_FP16, FORM_FIX8 = 0, 1, 2, 4
@@ -93,6 +93,11 @@ class _JobTimes(ctypes.Structure):
     _fields_ = [("queue_ms", ctypes.c_double), ("run_ms", ctypes.c_double), ("latency_ms", ctypes.c_double)]
 
 
+class _ServerInfo(ctypes.Structure):
+    _fields_ = [("groups", ctypes.c_int), ("joined_running", ctypes.c_int), ("max_members", ctypes.c_int),
+                ("solo_jobs", ctypes.c_int)]
+
+
 class _SpResult(ctypes.Structure):
     _fields_ = [("value", ctypes.c_float), ("iters", ctypes.c_int), ("nnz", ctypes.c_int),
                 ("converged", ctypes.c_int), ("ev_fp64", ctypes.c_double)]
@@ -141,6 +146,8 @@ def lib():
             "spca_resolve_support": (I, [P, U64, I, P, I, ctypes.c_double, P, P, ctypes.POINTER(I)]),
             "spca_server_start": (I, [P, I]),
             "spca_server_stop": (I, [P]),
+            "spca_server_set_batching": (I, [P, I]),
+            "spca_server_get_info": (I, [P, ctypes.POINTER(_ServerInfo)]),
             "spca_submit": (I, [P, U64, ctypes.POINTER(_Settings), P, I, ctypes.POINTER(U64)]),
             "spca_poll": (I, [P, U64, ctypes.POINTER(I)]),
             "spca_cancel": (I, [P, U64]),
@@ -321,9 +328,18 @@ class Context:
         except Exception:
             pass
 
-    def start_server(self, workers=4):
-        """Start the request server: up to `workers` jobs run at once, each on its own stream."""
+    def start_server(self, workers=4, batching=64):
+        """Start the request server: `workers` worker threads, each on its own stream.  A worker
+        that takes a dense one-batch job runs a fused group that queued jobs of its precision
+        join at chunk boundaries (at most `batching` jobs at once; 0: every job alone)."""
         _check(self._p, lib().spca_server_start(self._p, workers))
+        _check(self._p, lib().spca_server_set_batching(self._p, batching))
+
+    def server_info(self):
+        """Counters of the running server: groups, joined_running, max_members, solo_jobs."""
+        i = _ServerInfo()
+        _check(self._p, lib().spca_server_get_info(self._p, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in _ServerInfo._fields_}
 
     def stop_server(self):
         _check(self._p, lib().spca_server_stop(self._p))

@@ -286,55 +286,124 @@ def serving_settings(S, precision, spec, f, s, B, k):
                                   start=S.START_GAUSSIAN, seed=100000 + k, check_every=4)
 
 
-def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5):
-    """Request stream: Poisson arrivals at `rate` jobs/s for `seconds`, each job on one of
-    `pool_size` resident seeded matrices (shapes drawn as the uneven jobs), with its own
-    formulation, s and B; tol = 1e-5, at most 200 iterations, device Gaussian starts.  The
-    server admits jobs by HBM footprint under `budget` bytes and runs up to `workers` at once.
-    Latency = submit -> result, measured by the server.  Returns the summary and, per job,
-    the FP64 explained variance of its result (for the cross-precision comparison)."""
+def _serve_stream(S, ctx, pool, specs, precision, arrivals, draws, k0, window=None):
+    """Submits request k at arrivals[k] seconds (all at once when arrivals are 0) and waits for
+    the results.  window: stop at `window` seconds, count what finished and what is still queued
+    or running, and cancel the rest.  Returns per-job latencies, SP-iterations, FP64 EVs, flops
+    and the span, plus the backlog at the window's end."""
+    jobs = []
+    t0 = time.time()
+    for k, (a, (pi, f, sv, B)) in enumerate(zip(arrivals, draws)):
+        now = time.time() - t0
+        if a > now:
+            time.sleep(a - now)
+        st = serving_settings(S, precision, specs[pi], f, sv, B, k0 + k)
+        jobs.append((pool[pi].submit(st, per_sp=False), pi, B))
+    backlog = None
+    if window is not None:
+        now = time.time() - t0
+        if window > now:
+            time.sleep(window - now)
+        states = [j.poll() for j, _, _ in jobs]
+        backlog = {"arrived": len(jobs), "done": sum(s_ == S.JOB_DONE for s_ in states),
+                   "running": sum(s_ == S.JOB_RUNNING for s_ in states),
+                   "queued": sum(s_ == S.JOB_QUEUED for s_ in states)}
+        for (j, _, _), s_ in zip(jobs, states):
+            if s_ != S.JOB_DONE:
+                j.cancel()
+    lat, sp_its, ev, flops, done = [], 0, [], 0.0, 0
+    for j, pi, B in jobs:
+        stats = S.OptimizationStatistics()
+        try:
+            _, times = j.wait(stats=stats, per_sp=False)
+        except S.SpcaError as e:
+            if e.code != S.SPCA_ECANCELLED:
+                raise
+            continue
+        done += 1
+        lat.append(times["latency_ms"])
+        sp_its += stats.sp_iterations
+        ev.append(stats.ev_fp64)
+        flops += 4.0 * specs[pi]["m"] * specs[pi]["n"] * B * stats.it
+    span = time.time() - t0 if window is None else window
+    return {"lat": np.array(lat), "sp_its": sp_its, "ev": np.array(ev), "flops": flops, "span": span, "done": done,
+            "backlog": backlog}
+
+
+def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5, batching=64, sustain_seconds=30.0,
+                sustain_load=0.8, overload_seconds=10.0):
+    """Request stream on `pool_size` resident seeded matrices (shapes drawn as the uneven jobs),
+    each request with its own formulation, s and B; tol = 1e-5, at most 200 iterations, device
+    Gaussian starts.  The server admits jobs by HBM footprint under `budget` bytes, with
+    `workers` worker threads and fused groups of at most `batching` jobs.  Three legs:
+    capacity: the `seconds` of Poisson arrivals at `rate` per second submitted at once, timed to
+    the last result (throughput at saturation; the latencies are the drain of the burst);
+    sustained: Poisson arrivals at sustain_load x that capacity for sustain_seconds (p50/p99
+    latency at a load the GPU keeps up with); overload: arrivals at `rate` for
+    overload_seconds, with the backlog counted at the window's end (then cancelled).
+    Latency = submit -> result, measured by the server.  Returns the summary and, per job of
+    the capacity leg, the FP64 explained variance of its result (for the cross-precision
+    comparison)."""
     specs = uneven_specs(pool_size, seed=SERVE_POOL_SEED)
     arrivals, draws = serving_draws(rate, seconds, pool_size, seed)
     store = {S.PREC_FP32: S.STORE_FP32, S.PREC_BF16X3: S.STORE_BF16X3, S.PREC_FP16: S.STORE_FP16}[precision]
+    name = {0: "fp32", 1: "bf16x3", 2: "fp16"}[precision]
     with S.Context(0, hbm_budget_bytes=int(budget)) as ctx:
         t0 = time.time()
         pool = [ctx.generate(sp["m"], sp["n"], sp["seed"], theta=THETA, spike_nnz=SPARSITY, storage=store,
                              key=sp["seed"]) for sp in specs]
         resident = sum(M.nbytes() for M in pool)
-        print(f"# serving {precision}: pool of {pool_size} matrices, {resident / 2**30:.1f} GiB resident "
-              f"(built in {time.time() - t0:.1f}s), {len(arrivals)} arrivals over {seconds}s", file=sys.stderr, flush=True)
-        ctx.start_server(workers)
-        jobs = []
-        t0 = time.time()
-        for k, (a, (pi, f, sv, B)) in enumerate(zip(arrivals, draws)):
-            now = time.time() - t0
-            if a > now:
-                time.sleep(a - now)
-            M = pool[pi]
-            st = serving_settings(S, precision, specs[pi], f, sv, B, k)
-            jobs.append((M.submit(st, per_sp=False), B))
-        lat, sp_its, ev, flops = [], 0, [], 0.0
-        for (j, B), (pi, _, _, _) in zip(jobs, draws):
-            stats = S.OptimizationStatistics()
-            _, times = j.wait(stats=stats, per_sp=False)
-            lat.append(times["latency_ms"])
-            sp_its += stats.sp_iterations
-            ev.append(stats.ev_fp64)
-            flops += 4.0 * specs[pi]["m"] * specs[pi]["n"] * B * stats.it
-        span = time.time() - t0
+        print(f"# serving {name}: pool of {pool_size} matrices, {resident / 2**30:.1f} GiB resident "
+              f"(built in {time.time() - t0:.1f}s), capacity burst of {len(arrivals)} requests", file=sys.stderr,
+              flush=True)
+        ctx.start_server(workers, batching=batching)
+        cap = _serve_stream(S, ctx, pool, specs, precision, [0.0] * len(draws), draws, 0)
+        capacity = cap["done"] / cap["span"]
+        info = ctx.server_info()
+        print(f"# serving {name} capacity: {cap['done']} jobs in {cap['span']:.1f}s ({capacity:.1f} jobs/s, "
+              f"{cap['flops'] / cap['span'] / 1e12:.0f} algorithmic TFLOP/s); server {info}", file=sys.stderr, flush=True)
+        sus = None
+        if sustain_seconds > 0:
+            srate = sustain_load * capacity
+            s_arr, s_draws = serving_draws(srate, sustain_seconds, pool_size, seed + 1)
+            sus = _serve_stream(S, ctx, pool, specs, precision, s_arr, s_draws, 1000000)
+            print(f"# serving {name} sustained at {srate:.1f} jobs/s ({sustain_load:.0%} of capacity) for "
+                  f"{sustain_seconds:.0f}s: {sus['done']} jobs, latency p50 {np.percentile(sus['lat'], 50):.0f} ms p99 "
+                  f"{np.percentile(sus['lat'], 99):.0f} ms", file=sys.stderr, flush=True)
+        over = None
+        if overload_seconds > 0:
+            o_arr, o_draws = serving_draws(rate, overload_seconds, pool_size, seed + 2)
+            over = _serve_stream(S, ctx, pool, specs, precision, o_arr, o_draws, 2000000, window=overload_seconds)
+            b = over["backlog"]
+            print(f"# serving {name} overload at {rate:.0f} jobs/s for {overload_seconds:.0f}s: {b['arrived']} arrived, "
+                  f"{b['done']} done, {b['running']} running, {b['queued']} queued at the window's end "
+                  f"(queue grows {(b['arrived'] - b['done']) / overload_seconds:.0f} jobs/s)", file=sys.stderr, flush=True)
+        info = ctx.server_info()
         ctx.stop_server()
         for M in pool:
             M.release()
-    lat = np.array(lat)
-    res = {"precision": {0: "fp32", 1: "bf16x3", 2: "fp16"}[precision], "arrival_rate": rate, "seconds": seconds,
-           "jobs": len(jobs), "workers": workers, "pool_matrices": pool_size, "resident_GiB": resident / 2**30,
-           "hbm_budget_GB": budget / 1e9, "p50_latency_ms": float(np.percentile(lat, 50)),
-           "p99_latency_ms": float(np.percentile(lat, 99)), "max_latency_ms": float(lat.max()),
-           "jobs_per_s": len(jobs) / span, "sp_it_per_s": sp_its / span, "makespan_s": span, "tol": 1e-5,
-           "algorithmic_tflops": flops / span / 1e12}
-    print(f"# serving {res['precision']}: {len(jobs)} jobs in {span:.1f}s ({res['jobs_per_s']:.1f} jobs/s), latency p50 "
-          f"{res['p50_latency_ms']:.0f} ms p99 {res['p99_latency_ms']:.0f} ms", file=sys.stderr, flush=True)
-    return res, np.array(ev)
+    lat = cap["lat"]
+    res = {"precision": name, "arrival_rate": rate, "seconds": seconds, "jobs": cap["done"], "workers": workers,
+           "batching": batching, "pool_matrices": pool_size, "resident_GiB": resident / 2**30,
+           "hbm_budget_GB": budget / 1e9, "tol": 1e-5,
+           "capacity": {"jobs": cap["done"], "makespan_s": cap["span"], "jobs_per_s": capacity,
+                        "sp_it_per_s": cap["sp_its"] / cap["span"], "algorithmic_tflops": cap["flops"] / cap["span"] / 1e12,
+                        "burst_p50_latency_ms": float(np.percentile(lat, 50)),
+                        "burst_p99_latency_ms": float(np.percentile(lat, 99))},
+           "server": info}
+    if sus is not None:
+        res["sustained"] = {"load": sustain_load, "rate": sustain_load * capacity, "seconds": sustain_seconds,
+                            "jobs": sus["done"], "jobs_per_s": sus["done"] / sus["span"],
+                            "p50_latency_ms": float(np.percentile(sus["lat"], 50)),
+                            "p99_latency_ms": float(np.percentile(sus["lat"], 99)),
+                            "max_latency_ms": float(sus["lat"].max())}
+    if over is not None:
+        b = over["backlog"]
+        # the offered work at `rate`, priced by the capacity leg's mean algorithmic flops per job
+        res["overload"] = dict(b, rate=rate, seconds=overload_seconds,
+                               backlog_growth_jobs_per_s=(b["arrived"] - b["done"]) / overload_seconds,
+                               offered_algorithmic_tflops=cap["flops"] / max(cap["done"], 1) * rate / 1e12)
+    return res, cap["ev"]
 
 
 def cpu_baseline(A, X0, threads):
@@ -374,6 +443,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=1.0, help="request-stream length (0: skip serving)")
     ap.add_argument("--serve-rate", type=float, default=200.0, help="Poisson arrivals per second")
     ap.add_argument("--serve-workers", type=int, default=4)
+    ap.add_argument("--serve-batching", type=int, default=64, help="most jobs per fused group (0: each job alone)")
+    ap.add_argument("--serve-sustain-seconds", type=float, default=30.0, help="sustained-load leg (0: skip)")
+    ap.add_argument("--serve-load", type=float, default=0.8, help="sustained load as a share of measured capacity")
+    ap.add_argument("--serve-overload-seconds", type=float, default=10.0, help="overload leg at --serve-rate (0: skip)")
     ap.add_argument("--serve-pool", type=int, default=24, help="resident seeded matrices the jobs draw from")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -457,7 +530,10 @@ def main():
         evs = {}
         for name, prec in (("fp32", S.PREC_FP32), ("bf16x3", S.PREC_BF16X3)):
             serving[name], evs[name] = run_serving(S, prec, args.serve_rate, args.serve_seconds, args.serve_workers,
-                                                   args.serve_pool, 200e9)
+                                                   args.serve_pool, 200e9, batching=args.serve_batching,
+                                                   sustain_seconds=args.serve_sustain_seconds,
+                                                   sustain_load=args.serve_load,
+                                                   overload_seconds=args.serve_overload_seconds)
         rel = np.abs(evs["bf16x3"] - evs["fp32"]) / np.abs(evs["fp32"])
         serving["bf16x3_vs_fp32_ev"] = {"median_rel": float(np.median(rel)), "max_rel": float(rel.max()),
                                         "share_within_1e-5": float(np.mean(rel <= 1e-5))}
@@ -488,9 +564,9 @@ def main():
     line["uneven_jobs"] = uneven
     line["precision_variant"] = variant
     line["serving"] = serving
-    if serving:
-        line["p99_latency_ms"] = serving["fp32"]["p99_latency_ms"]
-        line["p50_latency_ms"] = serving["fp32"]["p50_latency_ms"]
+    if serving and "sustained" in serving["fp32"]:
+        line["p99_latency_ms"] = serving["fp32"]["sustained"]["p99_latency_ms"]
+        line["p50_latency_ms"] = serving["fp32"]["sustained"]["p50_latency_ms"]
     if "fp32" in results and "bf16x3" in results:
         line["bf16x3_over_fp32"] = results["bf16x3"]["sp_it_per_s"] / results["fp32"]["sp_it_per_s"]
     if "fp32_dense" in results and "bf16x3" in results:

@@ -320,11 +320,19 @@ int spca_solve_components(spca_ctx ctx, spca_matrix mat, const spca_settings* se
                           float* X_out, spca_stats* stats);
 
 /* ---- request server (continuous batching) ---------------------------------- */
-/* No reference counterpart: the reference solves one request per call.  A server runs up to
- * `workers` jobs at once, each on its own HIP stream with its own workspace; queued jobs are
- * admitted first-in first-out as soon as a worker is free and their workspace fits the HBM
- * budget (matrices stay resident and are shared by handle).  Results are independent of
- * what runs beside a job. */
+/* No reference counterpart: the reference solves one request per call.  A server runs
+ * `workers` worker threads, each with its own HIP stream; matrices stay resident and are
+ * shared by handle.  Continuous batching: a short job (dense, one batch with total_sp <=
+ * batch, no OTF, 128x128 product tiles, an estimated iteration of at most 200 us) runs in a
+ * fused group, whose jobs iterate together as spca_solve_fused runs them (one launch per phase
+ * for the group).  After every chunk of min(check_every, 64) iterations, finished and
+ * cancelled jobs leave and return their memory, and queued short jobs of the group's precision
+ * join while the group's estimated iteration stays within 1 ms and their workspaces fit the
+ * HBM budget; they start at iteration 0 beside the others.  With several workers, worker 0 is
+ * the group lane (it takes short jobs from anywhere in the queue); the others take jobs in
+ * order and solve longer ones alone.  A single worker admits strictly in order.  Results are
+ * independent of what runs beside a job: bitwise those of the same job submitted alone, and of
+ * its spca_solve. */
 #define SPCA_JOB_QUEUED 0
 #define SPCA_JOB_RUNNING 1
 #define SPCA_JOB_DONE 2
@@ -336,8 +344,19 @@ typedef struct spca_job_times {
     double latency_ms;        /* submit -> finish */
 } spca_job_times;
 
+/* Counters of a running server. */
+typedef struct spca_server_info {
+    int groups;          /* fused groups started */
+    int joined_running;  /* jobs admitted into a group while other members were iterating */
+    int max_members;     /* most jobs iterating in one group at once */
+    int solo_jobs;       /* jobs solved alone (sparse, several batches, OTF, or batching off) */
+} spca_server_info;
+
 int spca_server_start(spca_ctx ctx, int workers);
 int spca_server_stop(spca_ctx ctx);  /* cancels queued jobs, waits for running ones */
+/* Most jobs per fused group (default 64; 0: every job is solved alone, one per worker). */
+int spca_server_set_batching(spca_ctx ctx, int max_group_jobs);
+int spca_server_get_info(spca_ctx ctx, spca_server_info* info);
 /* X0 is copied at submission; want_per_sp asks for per-SP results in spca_wait. */
 int spca_submit(spca_ctx ctx, spca_matrix mat, const spca_settings* settings, const float* X0, int want_per_sp,
                 spca_job* out);

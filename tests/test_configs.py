@@ -162,9 +162,10 @@ C5_WORKERS = 4
 @pytest.mark.gpu
 @pytest.mark.parametrize("prec", [S.PREC_FP32, S.PREC_BF16X3])
 def test_c5_serving_burst_alone_equals_mix(prec):
-    """The first requests of the c5 stream (200 jobs/s) arrive together: the server runs them side
-    by side under the 200 GB budget, and every result is bitwise the result of the same job solved
-    alone.  The two smallest jobs' answers are then checked against the FP64 oracle."""
+    """The first requests of the c5 stream (200 jobs/s) arrive together: the server runs them as
+    fused groups under the 200 GB budget, and every result is bitwise the result of the same job
+    submitted alone and of its solo solve.  The two smallest jobs' answers are then
+    checked against the FP64 oracle."""
     pool_size = 24
     specs = bench.uneven_specs(pool_size, seed=bench.SERVE_POOL_SEED)
     _, draws = bench.serving_draws(200.0, 1.0, pool_size)
@@ -186,14 +187,24 @@ def test_c5_serving_burst_alone_equals_mix(prec):
             stats = S.OptimizationStatistics()
             x, times = h.wait(stats=stats, per_sp=False)
             mix.append((x, stats))
+        info = ctx.server_info()
+        print(f"  mix done: {info}", flush=True)
+        assert info["max_members"] >= 2  # the burst ran as fused groups
+        # alone: each request submitted to the idle server by itself
+        for k, ((pi, f, sv, B), st, (xm, sm)) in enumerate(zip(draws, sts, mix)):
+            sa = S.OptimizationStatistics()
+            xa, _ = pool[pi].submit(st, per_sp=False).wait(stats=sa, per_sp=False)
+            assert np.array_equal(xa.view(np.uint32), xm.view(np.uint32)), k
+            assert sa.it == sm.it and sa.fval == sm.fval, k
         ctx.stop_server()
-        print("  mix done", flush=True)
+        print("  alone == mix bitwise", flush=True)
+        # fused jobs run their solo tiles, plan and summation order (shapes whose solo plan takes the
+        # large 16-bit tiles are solved alone): the server's result is the solo solve's
         for k, ((pi, f, sv, B), st, (xm, sm)) in enumerate(zip(draws, sts, mix)):
             sa = S.OptimizationStatistics()
             xa = pool[pi].solve(st, stats=sa, per_sp=False)
             assert np.array_equal(xa.view(np.uint32), xm.view(np.uint32)), k
-            assert sa.it == sm.it and sa.fval == sm.fval, k
-        print("  alone == mix bitwise", flush=True)
+        print("  server == solo bitwise", flush=True)
         cost = [specs[pi]["m"] * specs[pi]["n"] * B * sm.it for (pi, _, _, B), (_, sm) in zip(draws, mix)]
         threads = min(os.cpu_count() or 1, 16)
         for k in np.argsort(cost)[:2]:

@@ -16,6 +16,7 @@ A^T sgn(z): such an SP also counts as a tie.
 """
 import json
 import os
+import time
 
 import numpy as np
 import pytest
@@ -199,8 +200,10 @@ def test_planted_support_recovered(small, small_mat):
 
 
 def test_deterministic_reruns(small, small_mat):
+    # every constrained formulation resolves most steps on the previous threshold's list, whose
+    # FP64 sums must not depend on the order waves reach LDS
     A, X0, _ = small
-    for f in (0, 3, 5):
+    for f in (0, 1, 2, 3, 5):
         st = S.OptimizationSettings(formulation=f, constraintParameter=10, penaltyParameter=default_gamma(f, 200),
                                     tolerance=0.0, maximumIterations=20, totalStartingPoints=64, batchSize=64,
                                     precision=S.PREC_BF16X3)
@@ -506,6 +509,61 @@ def test_serving_alone_equals_mix(ctx):
         with pytest.raises(S.SpcaError) as e:
             extra.wait()
         assert e.value.code == S.SPCA_ECANCELLED
+    finally:
+        ctx.stop_server()
+        for M in mats:
+            M.release()
+
+
+def test_serving_jobs_join_a_running_group(ctx):
+    """Slot-level continuous batching: with one worker, jobs submitted while a fused group
+    iterates join it at a chunk boundary (they start at iteration 0 beside the running job),
+    a running member can be cancelled, and every result is bitwise the job's solo result."""
+    mats = []
+    for (m, n, seed) in ((256, 3000, 31), (200, 1000, 32)):
+        A, _ = spiked(m, n, seed=seed)
+        mats.append(ctx.upload(A, storage=S.STORE_FP32, key=200 + seed))
+    long_st = S.OptimizationSettings(formulation=0, constraintParameter=12, tolerance=0.0, maximumIterations=400,
+                                     totalStartingPoints=256, batchSize=256)
+    specs = [(mats[0], long_st, unit_gaussian_starts(mats[0].n, 256, seed=40))]
+    for k, f in enumerate((1, 2, 4, 7)):
+        M = mats[k % 2]
+        B = (32, 64)[k % 2]
+        st = S.OptimizationSettings(formulation=f, constraintParameter=8, penaltyParameter=default_gamma(f, M.m),
+                                    tolerance=1e-5, maximumIterations=60, totalStartingPoints=B, batchSize=B)
+        specs.append((M, st, unit_gaussian_starts(M.n, B, seed=41 + k)))
+    alone = []
+    for M, st, X0 in specs:
+        stats = S.OptimizationStatistics()
+        x = M.solve(st, X0=X0, stats=stats)
+        alone.append((x, stats.values.copy(), stats.fval, stats.it))
+    ctx.start_server(1)
+    try:
+        first = specs[0][0].submit(specs[0][1], X0=specs[0][2])
+        t0 = time.time()
+        while first.poll() != S.JOB_RUNNING and time.time() - t0 < 30:
+            time.sleep(0.001)
+        assert first.poll() == S.JOB_RUNNING
+        later = [M.submit(st, X0=X0) for M, st, X0 in specs[1:]]
+        victim = specs[0][0].submit(long_st, X0=specs[0][2])  # taken by the group, then cancelled
+        t0 = time.time()
+        while victim.poll() != S.JOB_RUNNING and time.time() - t0 < 30:
+            time.sleep(0.001)
+        victim.cancel()
+        for j, (xa, va, fa, ita) in zip([first] + later, alone):
+            stats = S.OptimizationStatistics()
+            x, _ = j.wait(stats=stats)
+            assert stats.it == ita
+            assert np.array_equal(x.view(np.uint32), xa.view(np.uint32))
+            assert np.array_equal(stats.values.view(np.uint32), va.view(np.uint32)) and stats.fval == fa
+        with pytest.raises(S.SpcaError) as e:
+            victim.wait()
+        assert e.value.code == S.SPCA_ECANCELLED
+        info = ctx.server_info()
+        print(f"  server: {info}", flush=True)
+        assert info["groups"] == 1 and info["solo_jobs"] == 0
+        # the four later jobs joined (the victim may be cancelled before it starts iterating)
+        assert info["joined_running"] >= 4 and info["max_members"] >= 3
     finally:
         ctx.stop_server()
         for M in mats:
