@@ -1022,7 +1022,8 @@ void finish_solve(const Results& R, int n, const spca_settings& st, float* x_out
         stats->constraint = st.constraint;
         stats->gemm_ax_ms = phase_ms ? phase_ms[0] : 0.0;
         stats->gemm_atz_ms = phase_ms ? phase_ms[1] : 0.0;
-        stats->step_ms = phase_ms ? phase_ms[2] : 0.0;
+        stats->step_ms = phase_ms ? phase_ms[2] + phase_ms[3] : 0.0;
+        stats->step_kernel_ms = phase_ms ? phase_ms[3] : 0.0;
         stats->sp_iterations = R.sp_its;
         stats->ev_fp64 = R.ev_best;
         stats->sparse_iterations = R.sparse_it;
@@ -1218,7 +1219,7 @@ struct PhaseTimer {
     Engine* eng;
     bool on = true;   // off: no events are created or recorded (the allocation-free solve)
     size_t used = 0;  // events handed out since the last flush (always pairs)
-    double ms[3] = {0, 0, 0};
+    double ms[4] = {0, 0, 0, 0};
     std::vector<int> phase;
     hipError_t pair(hipEvent_t* a, hipEvent_t* b, int ph) {
         if (!on) {
@@ -1444,11 +1445,14 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_atz(atz, M, w, bp, batch, sp, s));
             HIP_TRY(ctx, record(e1, s));
-            HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
-            HIP_TRY(ctx, record(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(Operand{w.V, nullptr, nullptr, w.bp}, 0, w.np, defl->idx, defl->val,
                                                   defl->off, defl->ncomp, batch, sp, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 3));
+            HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
+            HIP_TRY(ctx, record(e1, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
+            HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_otf_converge(sp, o, batch, st.tolerance, st.max_iter, total, s));
             HIP_TRY(ctx, launch_otf_refill(sp, o, xop, st.precision, n, w.np, batch, otf_cap, s));
             HIP_TRY(ctx, record(e1, s));
@@ -1557,11 +1561,14 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
             HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_atz(atz, M, w, bp, batch, sp, s));
             HIP_TRY(ctx, record(e1, s));
-            HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
-            HIP_TRY(ctx, record(e0, s));
             if (defl) HIP_TRY(ctx, launch_deflate(Operand{w.V, nullptr, nullptr, w.bp}, 0, w.np, defl->idx, defl->val,
                                                   defl->off, defl->ncomp, batch, sp, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 3));
+            HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_step(w.V, xop, sa, sp, s));
+            HIP_TRY(ctx, record(e1, s));
+            HIP_TRY(ctx, pt.pair(&e0, &e1, 2));
+            HIP_TRY(ctx, record(e0, s));
             HIP_TRY(ctx, launch_converge(sp, batch, it, st.tolerance, st.max_iter,
                                          st.termination == SPCA_TERM_PER_SP, cap, s));
             if (ucap > 0 && it + 1 < st.max_iter)

@@ -85,7 +85,7 @@ class _Stats(ctypes.Structure):
         ("ev_resolve_fp64", ctypes.c_double), ("resolve_iterations", ctypes.c_int),
         ("sparse_iterations", ctypes.c_uint),
         ("union_iterations", ctypes.c_uint),
-        ("list_tries", ctypes.c_uint), ("list_steps", ctypes.c_uint),
+        ("list_tries", ctypes.c_uint), ("list_steps", ctypes.c_uint), ("step_kernel_ms", ctypes.c_double),
     ]
 
 
@@ -265,6 +265,7 @@ class OptimizationStatistics:
         self.gemm_ax_ms = 0.0
         self.gemm_atz_ms = 0.0
         self.step_ms = 0.0
+        self.step_kernel_ms = 0.0  # of step_ms, the formulation step kernel alone
         self.sp_iterations = 0
         self.ev_resolve_fp64 = None
         self.resolve_iterations = 0
@@ -279,6 +280,7 @@ class OptimizationStatistics:
         self.totalElapsedTime = s.wall_ms / 1e3
         self.n, self.best_sp, self.ev_fp64 = s.n, s.best_sp, s.ev_fp64
         self.gemm_ax_ms, self.gemm_atz_ms, self.step_ms = s.gemm_ax_ms, s.gemm_atz_ms, s.step_ms
+        self.step_kernel_ms = s.step_kernel_ms
         self.sp_iterations = s.sp_iterations
         self.ev_resolve_fp64, self.resolve_iterations = s.ev_resolve_fp64, s.resolve_iterations
         self.sparse_iterations = s.sparse_iterations

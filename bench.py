@@ -26,11 +26,16 @@ M_ROWS, N_COLS, BATCH, SPARSITY = 4096, 65536, 1024, 256
 THETA = (20.0, 15.0, 10.0, 8.0, 6.0)
 PEAK = {"fp32": 157.3, "fp32_dense": 157.3, "bf16x3": 2500.0 / 3.0, "fp16": 2500.0}  # TFLOP/s (bf16x3: bf16 peak / 3)
 HBM_PEAK = 8000.0  # GB/s
-# HBM bytes per launch of the product kernels at this workload, from rocprofv3 --pmc
-# FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs, scaled by the counters' reading of a 1 GiB
-# copy kernel (profiles/gemm_products_spiked4096x65536_B1024_hbm_counters.md).
-MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (6.33 + 0.25) * 2**30, ("fp32", "gemm_ax"): (8.16 + 0.125) * 2**30,
-                    ("bf16x3", "gemm_atz"): (1.50 + 0.26) * 2**30, ("bf16x3", "gemm_ax"): (1.50 + 0.13) * 2**30}
+# HBM bytes (read + written) per real launch at this workload, from rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE runs, each scaled by what the counter reads for the 1 GiB padding copy of A
+# (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0): profiles/gemm_products_spiked4096x65536_B1024_hbm_counters.md
+GIB = 2**30
+MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (6.368 + 0.250) * GIB, ("fp32", "gemm_ax_dense"): (8.116 + 0.125) * GIB,
+                    ("fp32", "gemm_atz_sign_fix8"): (2.613 + 0.255) * GIB, ("fp32", "sparse_ax_gather"): (5.940 + 0.016) * GIB,
+                    ("fp32_dense", "gemm_atz"): (6.368 + 0.250) * GIB, ("fp32_dense", "gemm_ax_dense"): (8.116 + 0.125) * GIB,
+                    ("bf16x3", "gemm_atz"): (1.500 + 0.264) * GIB, ("bf16x3", "gemm_atz_sign_fix8"): (2.285 + 0.255) * GIB,
+                    ("bf16x3", "sparse_ax_gather"): (4.821 + 0.016) * GIB,
+                    ("fp16", "gemm_atz"): (0.750 + 0.261) * GIB}
 
 
 def make_problem(seed=2024):
@@ -172,10 +177,15 @@ def run_otf(S, M, total):
     return out
 
 
-def run_uneven(S, ctx, count, iters, graph=True):
+def run_uneven(S, ctx, count, iters, graph=True, tune=True):
     """The uneven jobs solved two ways on one context: one fused solve (one launch per phase
-    per iteration for all jobs, replayed from a hipGraph) and one solo solve per job.  tol = 0
-    and a fixed iteration count, so both do the same SP-iterations."""
+    per iteration for all jobs, replayed from a hipGraph) and one solo solve per job, first
+    with the heuristic product plans, then (tune) with plans the autotuner measured on each
+    job's shape (mode 1, tuned during an untimed warm-up).  tol = 0 and a fixed iteration count,
+    so both do the same SP-iterations.  The fused path can only save what the solo solves spend
+    outside their products' roofline; `solo_ms_by_job` bounds that saving: jobs whose solo
+    iteration takes >= 1 ms run near their roofline alone, so solo_s / (their solo time) is the
+    most any packing of the others could gain."""
     specs = uneven_specs(count)
     t0 = time.time()
     mats = {}
@@ -194,24 +204,56 @@ def run_uneven(S, ctx, count, iters, graph=True):
     print(f"# uneven: {count} jobs, A resident {a_bytes / 2**30:.1f} GiB, generated in {gen_s:.1f}s",
           file=sys.stderr, flush=True)
     sp_it = sum(sp["B"] * iters for sp in specs)
-    # warm-up (first-touch of every matrix and plan), then the timed fused solve
-    ctx.solve_fused(jobs, graph=graph, per_sp=False)
-    t0 = time.time()
-    fused = ctx.solve_fused(jobs, graph=graph, per_sp=False)
-    fused_s = time.time() - t0
-    t0 = time.time()
-    solo = [M.solve(st, per_sp=False) for (M, st, _) in jobs]
-    solo_s = time.time() - t0
-    same = all(np.array_equal(xf.view(np.uint32), xs.view(np.uint32)) for (xf, _), xs in zip(fused, solo))
+    flops = sum(4.0 * sp["m"] * sp["n"] * sp["B"] * iters for sp in specs)
+
+    def measure(label):
+        # warm-up (first touch of every matrix; with tuning on, the plan search), then timed
+        ctx.solve_fused(jobs, graph=graph, per_sp=False)
+        t0 = time.time()
+        fused = ctx.solve_fused(jobs, graph=graph, per_sp=False)
+        fused_s = time.time() - t0
+        solo, solo_ms = [], []
+        t0 = time.time()
+        for (M, st, _) in jobs:
+            stats = S.OptimizationStatistics()
+            solo.append(M.solve(st, stats=stats, per_sp=False))
+            solo_ms.append(stats.totalTrueComputationTime * 1e3)
+        solo_s = time.time() - t0
+        same = all(np.array_equal(xf.view(np.uint32), xs.view(np.uint32)) for (xf, _), xs in zip(fused, solo))
+        solo_ms = np.array(solo_ms)
+        heavy = solo_ms / iters >= 1.0
+        r = {"fused_s": fused_s, "solo_s": solo_s, "fused_sp_it_per_s": sp_it / fused_s,
+             "solo_sp_it_per_s": sp_it / solo_s, "fused_over_solo": solo_s / fused_s,
+             "fused_tflops": flops / fused_s / 1e12, "bitwise_equal_to_solo": same,
+             "heavy_jobs": int(heavy.sum()), "heavy_share_of_solo": float(solo_ms[heavy].sum() / solo_ms.sum()),
+             "fused_gain_bound": float(solo_s / (solo_ms[heavy].sum() / 1e3)) if heavy.any() else None,
+             "solo_ms_by_job": [round(float(t), 2) for t in solo_ms]}
+        print(f"# uneven {label}: fused {fused_s:.2f}s ({r['fused_sp_it_per_s']:.0f} SP-it/s, {r['fused_tflops']:.1f} "
+              f"TFLOP/s), solo {solo_s:.2f}s ({r['fused_over_solo']:.3f}x), bitwise equal {same}; {r['heavy_jobs']} jobs "
+              f"with solo iterations >= 1 ms take {r['heavy_share_of_solo']:.0%} of the solo time (fused gain bound "
+              f"{r['fused_gain_bound']:.2f}x)", file=sys.stderr, flush=True)
+        return r
+
+    res = {"jobs": count, "iterations": iters, "sp_iterations": sp_it, "A_GiB": a_bytes / 2**30, "graph": graph,
+           "dtype": "f32", "heuristic": measure("heuristic plans")}
+    if tune:
+        import tempfile
+        cache = os.path.join(tempfile.mkdtemp(), "plans.json")
+        heur = [ctx.plan(S.PREC_FP32, sp["m"], sp["n"], sp["B"], fused=True) for sp in specs]
+        ctx.set_tuning(1, cache)
+        t0 = time.time()
+        res["tuned"] = measure("tuned plans")
+        res["tuned"]["tuning_and_warmup_s"] = time.time() - t0 - res["tuned"]["fused_s"] - res["tuned"]["solo_s"]
+        tuned = [ctx.plan(S.PREC_FP32, sp["m"], sp["n"], sp["B"], fused=True) for sp in specs]
+        ctx.set_tuning(0)
+        res["tuned"]["plans_changed"] = int(sum(h[0] != t[0] for h, t in zip(heur, tuned)))
+        res["tuned"]["plans_tuned"] = int(sum(t[2] > 0 for t in tuned))
     for M in mats.values():
         M.release()
-    flops = sum(4.0 * sp["m"] * sp["n"] * sp["B"] * iters for sp in specs)
-    res = {"jobs": count, "iterations": iters, "sp_iterations": sp_it, "A_GiB": a_bytes / 2**30,
-           "fused_s": fused_s, "solo_s": solo_s, "fused_sp_it_per_s": sp_it / fused_s, "solo_sp_it_per_s": sp_it / solo_s,
-           "fused_over_solo": solo_s / fused_s, "fused_tflops": flops / fused_s / 1e12,
-           "bitwise_equal_to_solo": same, "graph": graph, "dtype": "f32"}
-    print(f"# uneven: fused {fused_s:.2f}s ({res['fused_sp_it_per_s']:.0f} SP-it/s, {res['fused_tflops']:.1f} TFLOP/s), "
-          f"solo {solo_s:.2f}s, bitwise equal {same}", file=sys.stderr, flush=True)
+    # headline fields of the fused path (tuned when measured)
+    best = res.get("tuned", res["heuristic"])
+    res.update({k: best[k] for k in ("fused_s", "solo_s", "fused_sp_it_per_s", "solo_sp_it_per_s", "fused_over_solo",
+                                      "fused_tflops", "bitwise_equal_to_solo")})
     return res
 
 
@@ -262,17 +304,19 @@ SERVE_POOL_SEED = 11
 def serving_draws(rate, seconds, pool_size, seed=5):
     """The request stream: Poisson arrival times at `rate` per second over `seconds`, and per
     request (pool matrix, formulation, s, B) drawn uniformly (s from {16, 64, 256, 1024}, B from
-    {32, 128, 512}); the pool matrices are uneven_specs(pool_size, seed=SERVE_POOL_SEED)."""
-    rng = np.random.default_rng(seed)
+    {32, 128, 512}); the pool matrices are uneven_specs(pool_size, seed=SERVE_POOL_SEED).  The
+    job mix comes from its own generator, so request k is the same job at every rate."""
+    rng_t = np.random.default_rng([seed, 1])
+    rng_j = np.random.default_rng([seed, 2])
     t = 0.0
     arrivals = []
     while True:
-        t += rng.exponential(1.0 / rate)
+        t += rng_t.exponential(1.0 / rate)
         if t > seconds:
             break
         arrivals.append(t)
-    draws = [(int(rng.integers(0, pool_size)), int(rng.integers(0, 8)), int(rng.choice([16, 64, 256, 1024])),
-              int(rng.choice([32, 128, 512]))) for _ in arrivals]
+    draws = [(int(rng_j.integers(0, pool_size)), int(rng_j.integers(0, 8)), int(rng_j.choice([16, 64, 256, 1024])),
+              int(rng_j.choice([32, 128, 512]))) for _ in arrivals]
     return arrivals, draws
 
 
@@ -365,7 +409,7 @@ def run_serving(S, precision, rate, seconds, workers, pool_size, budget, seed=5,
         sus = None
         if sustain_seconds > 0:
             srate = sustain_load * capacity
-            s_arr, s_draws = serving_draws(srate, sustain_seconds, pool_size, seed + 1)
+            s_arr, s_draws = serving_draws(srate, sustain_seconds, pool_size, seed)
             sus = _serve_stream(S, ctx, pool, specs, precision, s_arr, s_draws, 1000000)
             print(f"# serving {name} sustained at {srate:.1f} jobs/s ({sustain_load:.0%} of capacity) for "
                   f"{sustain_seconds:.0f}s: {sus['done']} jobs, latency p50 {np.percentile(sus['lat'], 50):.0f} ms p99 "
@@ -421,6 +465,119 @@ def cpu_baseline(A, X0, threads):
             "loop_s": o["loop_s"]}
 
 
+def _avg(per, fs, key):
+    v = [per[str(f)][key] for f in fs if str(f) in per and per[str(f)][key] is not None]
+    return sum(v) / len(v) if v else None
+
+
+def kernel_rooflines(p, per, forms):
+    """Per-kernel roofline lines of one form from its hipEvent phase times (ms per launch):
+    achieved = algorithmic flops (bytes) per launch / the kernel's average launch time.
+    Algorithmic work per launch at m x n = 4096 x 65536, B = 1024:
+      A^T*Z (L2 variance, the FP32 / bf16x3 / fp16 product): 2 m n B flops;
+      A^T sgn(z) (L1 variance; fixed point in fp32 and bf16x3): 2 m n B int8 MACs x 2 (ops);
+      dense A*X (fp32_dense; fp32 f2, whose iterates are dense): 2 m n B flops;
+      gathered A*X (sparse iterates, iterations after the warm-up): s m B gathered A values
+        (4 bytes each; the iterates share columns, which come from L2/MALL, so the gathered
+        rate is reported against HBM only as a reference);
+      step (constrained formulations): read V (4 n B bytes) + write X (4 n B bytes in fp32 and
+        bf16x3 (hi + lo), 2 n B in fp16)."""
+    mnb = float(M_ROWS) * N_COLS * BATCH
+    out = {}
+    l2v, l1v = [f for f in forms if f % 2 == 0], [f for f in forms if f % 2 == 1]
+    atz = _avg(per, l2v, "gemm_atz_ms")
+    if atz:
+        tf = 2 * mnb / (atz / 1e3) / 1e12
+        out["gemm_atz"] = {"bound": "mfma", "ms": atz, "achieved": tf, "peak": PEAK[p], "unit": "TFLOP/s",
+                           "frac": tf / PEAK[p], "work_per_launch": 2 * mnb, "formulations": l2v}
+    atz1 = _avg(per, l1v, "gemm_atz_ms")
+    if atz1 and p in ("fp32", "bf16x3"):
+        tops = 2 * mnb / (atz1 / 1e3) / 1e12
+        # the exact fixed-point product multiplies four int8 pieces of A by the int8 signs: it
+        # issues 4 x 2 m n B int8 ops for the 2 m n B of the product
+        out["gemm_atz_sign_fix8"] = {"bound": "mfma", "ms": atz1, "achieved": tops, "peak": 5000.0, "unit": "TOP/s",
+                                     "frac": tops / 5000.0, "issued_frac": 4 * tops / 5000.0,
+                                     "work_per_launch": 2 * mnb, "formulations": l1v}
+    dense = [f for f in forms if p == "fp32_dense" or (p == "fp32" and f == 2)]
+    ax = _avg(per, dense, "gemm_ax_ms")
+    if ax:
+        tf = 2 * mnb / (ax / 1e3) / 1e12
+        out["gemm_ax_dense"] = {"bound": "mfma", "ms": ax, "achieved": tf, "peak": PEAK[p], "unit": "TFLOP/s",
+                                "frac": tf / PEAK[p], "work_per_launch": 2 * mnb, "formulations": dense}
+    if p != "fp32_dense":
+        l0c = [f for f in forms if f in (0, 1)]
+        g = _avg(per, l0c, "gemm_ax_later_ms")
+        if g:
+            gb = float(SPARSITY) * M_ROWS * BATCH * 4
+            out["sparse_ax_gather"] = {"bound": "l2", "ms": g, "achieved": gb / (g / 1e3) / 1e9, "unit": "GB/s",
+                                       "reference_peak": HBM_PEAK, "bytes_per_launch": gb, "formulations": l0c}
+    cons = [f for f in forms if f < 4]
+    st = _avg(per, cons, "step_kernel_ms")
+    if st:
+        xb = 2 if p == "fp16" else 4
+        by = float(N_COLS) * BATCH * (4 + xb)
+        gbs = by / (st / 1e3) / 1e9
+        out["step"] = {"bound": "hbm", "ms": st, "achieved": gbs, "peak": HBM_PEAK, "unit": "GB/s",
+                       "frac": gbs / HBM_PEAK, "bytes_per_launch": by, "formulations": cons,
+                       "per_formulation_ms": {str(f): per[str(f)]["step_kernel_ms"] for f in cons}}
+    for k, v in out.items():
+        v["traffic"] = MEASURED_TRAFFIC.get((p, k))
+    return out
+
+
+def host_cpus():
+    """The host the CPU baseline runs on: model, logical CPUs, CPUs this process may use."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for ln in fh:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        allowed = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(), "allowed_cpus": allowed,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+def run_c1(S, ctx):
+    """The reference CPU path's configuration (see make_c1_problem) solved on the GPU in FP32
+    and bf16x3 from the same 64 starting points: tol 1e-6, at most 200 iterations."""
+    A, X0 = make_c1_problem()
+    M = ctx.upload(A.astype(np.float32), storage=S.STORE_FP32 | S.STORE_BF16X3)
+    out = {}
+    for name, prec in (("fp32", S.PREC_FP32), ("bf16x3", S.PREC_BF16X3)):
+        st = S.OptimizationSettings(formulation=0, constraintParameter=16, tolerance=1e-6, maximumIterations=200,
+                                    totalStartingPoints=64, batchSize=64, precision=prec)
+        M.solve(st, X0=X0.astype(np.float32), per_sp=False)  # warm-up
+        stats = S.OptimizationStatistics()
+        M.solve(st, X0=X0.astype(np.float32), stats=stats, per_sp=False)
+        out[name] = {"sp_it_per_s": stats.sp_iterations / stats.totalTrueComputationTime, "iterations": stats.it,
+                     "loop_ms": stats.totalTrueComputationTime * 1e3, "fval": stats.fval, "ev_fp64": stats.ev_fp64}
+        print(f"# c1 400x10304 {name}: {stats.it} iterations in {out[name]['loop_ms']:.2f} ms "
+              f"({out[name]['sp_it_per_s']:.0f} SP-it/s), fval {stats.fval:.6f}", file=sys.stderr, flush=True)
+    M.release()
+    return out
+
+
+def cpu_baseline_c1(threads):
+    """The reference CPU path's own configuration in FP64 on the oracle: 400 x 10304 low-rank
+    plus noise (clipped, centred), L0_constrained_L2_PCA, s = 16, 64 starting points, tol 1e-6,
+    at most 200 iterations."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import oracle as O
+    A, X0 = make_c1_problem()
+    o = O.solve(A, 0, s=16, tol=1e-6, max_iter=200, total_sp=64, batch=64, X0=X0, threads=threads, gemm_backend=1,
+                lambda_rule=O.LAMBDA_GPU)
+    return {"value": 64 * o["it"] / o["loop_s"], "unit": "SP-it/s", "cores": threads, "kind": "port",
+            "iterations": o["it"], "loop_s": o["loop_s"], "fval": o["fval"],
+            "sample": "400x10304, B=64, s=16, tol 1e-6, max 200 iterations, L0_constrained_L2_PCA, FP64"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,6 +593,7 @@ def main():
     ap.add_argument("--otf-sps", type=int, default=4096,
                     help="starting points of the batch-vs-OTF comparison (0 skips)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c1", type=int, default=1, help="solve the 400 x 10304 CPU-path configuration on the GPU")
     ap.add_argument("--uneven-jobs", type=int, default=64, help="uneven fused-job workload size (0: skip)")
     ap.add_argument("--uneven-iters", type=int, default=20)
     ap.add_argument("--uneven-graph", type=int, default=1, help="replay the fused iteration from a hipGraph")
@@ -481,19 +639,26 @@ def main():
                                             batchSize=BATCH, precision=pcode[p], check_every=1000,
                                             sparse_products=0 if p == "fp32_dense" else args.sparse_products,
                                             exact_sign_products=0 if p == "fp32_dense" else 1)
+                warm = S.OptimizationStatistics()
                 if args.warmup > 0:
-                    M.solve(st, X0=X0, per_sp=False)
+                    M.solve(st, X0=X0, stats=warm, per_sp=False)
                 st.maximumIterations = args.steps
                 stats = S.OptimizationStatistics()
                 M.solve(st, X0=X0, stats=stats, per_sp=False)
                 sp_it = BATCH * stats.it
+                # the warm-up solve starts from the same X0: its iterations are this solve's first
+                # ones, so the difference is the time of iterations warmup+1 .. steps (gathered A*X
+                # where the iterates are sparse from the second iteration on)
+                later = stats.it - warm.it if args.warmup > 0 and stats.it > warm.it else 0
                 per[str(f)] = {"sp_it_per_s": sp_it / stats.totalTrueComputationTime,
                                "sparse_iterations": stats.sparse_iterations,
                                "union_iterations": stats.union_iterations,
                                "list_tries": stats.list_tries, "list_steps": stats.list_steps,
                                "ms_per_step": stats.totalTrueComputationTime * 1e3 / stats.it,
                                "gemm_ax_ms": stats.gemm_ax_ms / stats.it, "gemm_atz_ms": stats.gemm_atz_ms / stats.it,
-                               "step_ms": stats.step_ms / stats.it, "fval": stats.fval, "ev_fp64": stats.ev_fp64}
+                               "step_ms": stats.step_ms / stats.it, "step_kernel_ms": stats.step_kernel_ms / stats.it,
+                               "gemm_ax_later_ms": (stats.gemm_ax_ms - warm.gemm_ax_ms) / later if later else None,
+                               "fval": stats.fval, "ev_fp64": stats.ev_fp64}
                 tot_sp_it += sp_it
                 tot_ms += stats.totalTrueComputationTime * 1e3
                 ax_ms += stats.gemm_ax_ms
@@ -502,21 +667,15 @@ def main():
                 launches += stats.it
                 print(f"# {p} f{f}: {per[str(f)]['ms_per_step']:.3f} ms/step, "
                       f"{per[str(f)]['sp_it_per_s']:.0f} SP-it/s (ax {per[str(f)]['gemm_ax_ms']:.3f} "
-                      f"atz {per[str(f)]['gemm_atz_ms']:.3f} step {per[str(f)]['step_ms']:.3f} ms; list "
-                      f"{stats.list_steps}/{stats.list_tries})",
+                      f"atz {per[str(f)]['gemm_atz_ms']:.3f} step {per[str(f)]['step_ms']:.3f} "
+                      f"[kernel {per[str(f)]['step_kernel_ms']:.3f}] ms; list {stats.list_steps}/{stats.list_tries})",
                       file=sys.stderr, flush=True)
-            flops = 2.0 * M_ROWS * N_COLS * BATCH  # per product launch
-            ax_avg, atz_avg = ax_ms / launches, atz_ms / launches
-            # The roofline kernel is A^T*Z of the L2-variance formulations (even f): one kernel
-            # in every form (the dominant FP32 MFMA product, the bf16x3 / fp16 product).  The
-            # L1-variance formulations' A^T*sgn(z) runs the exact bf16 split in fp32 and bf16x3.
-            l2f = [f for f in forms if f % 2 == 0] or forms
-            atz_l2 = sum(per[str(f)]["gemm_atz_ms"] for f in l2f) / len(l2f)
-            dom, dom_ms = "gemm_atz", atz_l2
-            results[p] = {"sp_it_per_s": tot_sp_it / (tot_ms / 1e3), "ms_per_step": tot_ms / launches,
-                          "steps": launches, "per_formulation": per,
-                          "kernel_ms_per_step": {"gemm_ax": ax_avg, "gemm_atz": atz_avg, "step": step_ms / launches},
-                          "dominant": dom, "dominant_tflops": flops / (dom_ms / 1e3) / 1e12}
+            # a step = one batched iteration of every formulation (each on its own B = 1024 batch)
+            results[p] = {"sp_it_per_s": tot_sp_it / (tot_ms / 1e3), "ms_per_step": tot_ms / args.steps,
+                          "steps": args.steps, "launches": launches, "per_formulation": per,
+                          "kernel_ms_per_launch": {"gemm_ax": ax_ms / launches, "gemm_atz": atz_ms / launches,
+                                                   "step_epilogue": step_ms / launches},
+                          "kernels": kernel_rooflines(p, per, forms)}
         otf = run_otf(S, M, args.otf_sps) if args.otf_sps > 0 and "fp32" in precs else None
         small = run_small_batches(S, M, X0, [int(b) for b in args.small_batches.split(",") if b], precs,
                                   pcode, args.steps) if args.small_batches else None
@@ -524,6 +683,7 @@ def main():
         uneven = (run_uneven(S, ctx, args.uneven_jobs, args.uneven_iters, graph=bool(args.uneven_graph))
                   if args.uneven_jobs > 0 else None)
         variant = run_precision_variant(S, ctx, args.variant_iters, 1) if args.variant_iters > 0 else None
+        c1 = run_c1(S, ctx) if args.c1 else None
     serving = None
     if args.serve_seconds > 0:
         serving = {}
@@ -539,30 +699,40 @@ def main():
                                         "share_within_1e-5": float(np.mean(rel <= 1e-5))}
     head = results.get("fp32") or next(iter(results.values()))
     hp = "fp32" if "fp32" in results else next(iter(results))
-    roof = {"bound": "mfma", "achieved": head["dominant_tflops"], "peak": PEAK[hp], "unit": "TFLOP/s",
-            "frac": head["dominant_tflops"] / PEAK[hp], "traffic": MEASURED_TRAFFIC.get((hp, head["dominant"])),
-            "kernel": head["dominant"] + " (gemm_tn_f32_kernel, L2-variance formulations)" if hp == "fp32"
-            else head["dominant"], "flops_per_launch": 2.0 * M_ROWS * N_COLS * BATCH}
+    # the dominant kernel: A^T*Z of the L2-variance formulations (gemm_tn_f32_kernel in FP32)
+    dk = head["kernels"]["gemm_atz"]
+    roof = {"bound": "mfma", "achieved": dk["achieved"], "peak": dk["peak"], "unit": "TFLOP/s", "frac": dk["frac"],
+            "traffic": MEASURED_TRAFFIC.get((hp, "gemm_atz")),
+            "kernel": "gemm_atz (gemm_tn_f32_kernel, L2-variance formulations)" if hp == "fp32" else "gemm_atz",
+            "flops_per_launch": dk["work_per_launch"], "ms_per_launch": dk["ms"], "timing": "hipEvents on the solver stream"}
     cpu = None
     if not args.no_cpu:
-        threads = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", "16") or 16))
+        host = host_cpus()
+        threads = max(1, min(host["allowed_cpus"], int(os.environ.get("OMP_NUM_THREADS") or host["allowed_cpus"])))
         cpu = cpu_baseline(A, X0, threads)
+        cpu["host"] = host
+        cpu["c1_fp64"] = cpu_baseline_c1(threads)
     line = {
         "metric": "start-point iterations/sec + p99 SPCA solve latency at fixed expl-var tolerance",
         "value": head["sp_it_per_s"], "unit": "SP-it/s", "n_gpus": 1, "steps": head["steps"],
-        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "warmup": args.warmup, "ms_per_step": head["ms_per_step"], "formulations_per_step": len(forms), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32" if hp in ("fp32", "fp32_dense") else hp, "data": "synthetic",
         "config": {"workload": f"spiked {M_ROWS}x{N_COLS} FP32, B={BATCH}, s={SPARSITY}, tol=0 "
                                f"(fixed {args.steps} iterations), formulations {args.formulations}",
                    "global_batch": BATCH, "precision": hp, "gamma": gam},
         "roofline": roof, "cpu_baseline": cpu,
-        "forms": {p: {k: v for k, v in r.items() if k != "per_formulation"} for p, r in results.items()},
+        "forms": {p: {k: v for k, v in r.items() if k not in ("per_formulation", "kernels")} for p, r in results.items()},
+        "kernels": {p: r["kernels"] for p, r in results.items()},
         "per_formulation": {p: r["per_formulation"] for p, r in results.items()},
     }
     line["small_batches"] = small
     line["otf"] = otf
     line["uneven_jobs"] = uneven
     line["precision_variant"] = variant
+    line["c1_gpu"] = c1
+    if c1 and cpu and cpu.get("c1_fp64"):
+        c1["fp32_over_cpu_fp64"] = c1["fp32"]["sp_it_per_s"] / cpu["c1_fp64"]["value"]
+        c1["fp32_fval_rel_vs_cpu_fp64"] = abs(c1["fp32"]["fval"] - cpu["c1_fp64"]["fval"]) / abs(cpu["c1_fp64"]["fval"])
     line["serving"] = serving
     if serving and "sustained" in serving["fp32"]:
         line["p99_latency_ms"] = serving["fp32"]["sustained"]["p99_latency_ms"]

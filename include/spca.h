@@ -143,7 +143,8 @@ typedef struct spca_stats {
     double ev_fp64;           /* FP64 explained variance of x_out: |A x|_2^2 or |A x|_1 */
     double gemm_ax_ms;        /* summed kernel time of the A*X products (hipEvents) */
     double gemm_atz_ms;       /* summed kernel time of the A^T*Z products */
-    double step_ms;           /* summed time of epilogue + formulation step + reduction kernels */
+    double step_ms;           /* summed time of epilogue + formulation step + reduction kernels
+                                 (deflation projections excluded) */
     unsigned long long sp_iterations; /* sum over SPs of iterations while active */
     double ev_resolve_fp64;   /* explained variance after the FP64 support re-solve (0 if not run) */
     int resolve_iterations;   /* iterations the re-solve ran */
@@ -154,6 +155,7 @@ typedef struct spca_stats {
                                    candidate list (every step after an SP's first) */
     unsigned list_steps;        /* of those, the steps the list decided (the others re-ran the
                                    column passes of a first step) */
+    double step_kernel_ms;      /* of step_ms, the formulation step kernel alone */
 } spca_stats;
 
 /* per starting point result (reference values / iters / cardinalities) */
