@@ -1636,6 +1636,29 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 if (q0 + u < nq) fn(q0 + u, vv[u]);
         }
     };
+    // the iterate's stores: x_i (i < n) in FP32 and the operand forms
+    double nz = 0;
+    const long xbase = (long)j * a.ldv;
+    const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + xbase, (uint32_t)a.n * 4u);
+    // 16-bit forms are K-blocked: element i of column j at ((i/32)*bp + j)*32 + i%32; the views
+    // cover the whole operand (ldv * bp elements) and elements i >= n are skipped
+    const uint32_t opb = a.precision ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
+    const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
+    const __amdgpu_buffer_rsrc_t lr =
+        col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, a.precision == 1 ? opb : 0u);
+    auto put = [&](int i, float x) {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, (uint32_t)i * 4u, 0, 0);
+        if (a.precision) {
+            const uint32_t bo = (uint32_t)(blk_index(j, i, xop.bp) * 2);
+            if (a.precision == 1) {
+                const uint16_t h = f2bf(x);
+                __builtin_amdgcn_raw_buffer_store_b16(h, hr, bo, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, bo, 0, 0);
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, bo, 0, 0);
+            }
+        }
+    };
     const int f = a.formulation;
     float fval;
     // element transform applied when the iterate is written: keep |v| >= tau (L0),
@@ -2130,8 +2153,10 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         scale_out = 1.0f / nf;
         fval = sp.fz[j];
     } else {
-        // ---- penalised (my_cublas_wrapper.h:170-238,636-710), gamma in the iterate's precision
+        // ---- penalised (my_cublas_wrapper.h:170-238,636-710), gamma in the iterate's precision.
+        // The threshold is gamma itself, so one sweep sums the objective and writes the iterate.
         const float g = (float)a.gamma;
+        lam = g;
         double acc = 0, acc_band = 0;
         const bool l0p = (f == 4 || f == 5);
         auto contrib = [&](float v) -> double {
@@ -2142,9 +2167,18 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
             const float t = fabsf(v) - g;
             return t > 0.f ? (double)t * (double)t : 0.0;
         };
+        // the iterate's entry: L0 penalty kept iff v^2 - gamma > 0, so a NaN v (the 0/0 after an
+        // SP was zeroed out) gives 0, as the reference CPU rule (sparse_PCA_thresholding.h:67-102),
+        // equal to the GPU functor otherwise; L1 penalty the soft threshold at gamma
+        auto pen_x = [&](float v) -> float {
+            if (l0p) return (v * v - g) > 0.f ? v : 0.f;
+            const float t2 = fabsf(v) - g;
+            return t2 > 0.f ? t2 * sgnf(v) : 0.f;
+        };
         // exact ties (L2 variance): entries whose |v| lies within 2 eps of the threshold (sqrt(gamma)
         // for the L0 penalty, gamma for the L1 penalty) are listed, recomputed in FP64 below, and
-        // their contributions replaced; more than kBandMax of them: none
+        // their contributions replaced; more than kBandMax of them: none.  The sweep leaves the
+        // band's entries of x unwritten; they are written once the band is decided.
         const float thr = l0p ? sqrtf(g) : g;
         const float lo = thr * (1.f - 2.f * a.band_eps), hi = thr * (1.f + 2.f * a.band_eps);
         const bool band = a.band_eps > 0.f && thr > 0.f;
@@ -2152,16 +2186,26 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
             if (tid == 0) sh.ncand = 0;
             __syncthreads();
         }
+        // Entries inside the band are written by the plain rule here and rewritten below when the
+        // band is decided (the barriers between order the two stores).
+        double nz_band = 0;
         sweep([&](int q, float v) {
-            if (tid + q * NT >= a.n) return;  // the last row's entries past n
-            const double c = contrib(v);
+            const int i = tid + q * NT;
+            const bool in = i < a.n;  // the last row's entries past n (v reads 0 there) are not counted
+            const double c = in ? contrib(v) : 0.0;
             acc += c;
             const float av = fabsf(v);
-            if (band && av >= lo && av <= hi) {
+            const bool inband = band && in && av >= lo && av <= hi;
+            const float x = pen_x(v);
+            if (inband) {
                 acc_band += c;
+                nz_band += x != 0.f ? 1.0 : 0.0;
                 const int slot = atomicAdd(&sh.ncand, 1);
-                if (slot < kBandMax) sh.band_idx[slot] = tid + q * NT;
+                if (slot < kBandMax) sh.band_idx[slot] = i;
+            } else {
+                nz += (in && x != 0.f) ? 1.0 : 0.0;
             }
+            if (in) put(i, x);
         });
         double tot = block_sum<NT>(acc, sh.red);
         if (band) {
@@ -2200,41 +2244,23 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                         tr += contrib(v);
                     }
                     sh.dres[3] = tr;
-                    sh.band_lo = lo;
-                    sh.band_hi = hi;
-                    sh.nband = nb;
                 }
                 __syncthreads();
                 tot = tot - tb + sh.dres[3];
+                // the band's entries by their exact values: kept ones take the recomputed value
+                if (tid < nb) {
+                    const float ve = sh.band_val[tid];
+                    const float x = sh.band_keep[tid] ? (f == 4 ? ve : (fabsf(ve) - g) * sgnf(ve)) : 0.f;
+                    put(sh.band_idx[tid], x);
+                    nz += x != 0.f ? 1.0 : 0.0;
+                }
+            } else {
+                nz += nz_band;  // none re-evaluated: the band's entries keep the plain rule
             }
         }
         fval = l0p ? (float)tot : (float)sqrt(tot);
-        lam = g;
     }
     // ---- write the iterate (+ operand forms), count nonzeros, update the SP record
-    double nz = 0;
-    const long xbase = (long)j * a.ldv;
-    const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + xbase, (uint32_t)a.n * 4u);
-    // 16-bit forms are K-blocked: element i of column j at ((i/32)*bp + j)*32 + i%32; the views
-    // cover the whole operand (ldv * bp elements) and elements i >= n are skipped
-    const uint32_t opb = a.precision ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
-    const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
-    const __amdgpu_buffer_rsrc_t lr =
-        col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, a.precision == 1 ? opb : 0u);
-    // x_i (i < n) in FP32 and the operand forms
-    auto put = [&](int i, float x) {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, (uint32_t)i * 4u, 0, 0);
-        if (a.precision) {
-            const uint32_t bo = (uint32_t)(blk_index(j, i, xop.bp) * 2);
-            if (a.precision == 1) {
-                const uint16_t h = f2bf(x);
-                __builtin_amdgcn_raw_buffer_store_b16(h, hr, bo, 0, 0);
-                __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, bo, 0, 0);
-            } else {
-                __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, bo, 0, 0);
-            }
-        }
-    };
     if (decided) {
         // ---- a list step: x's nonzeros are the kept list entries, so only they are written,
         // after the previous step's nonzeros are cleared: the indices a previous list step kept,
@@ -2293,14 +2319,15 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         __syncthreads();
         nz = (double)sh.nkeep;  // every kept value is nonzero: |v| >= tau > 0, or |v| - lam > 0
         if (tid == 0) sp.nkept[j] = sh.nkeep;
+    } else if (f >= 4) {
+        nz = block_sum<NT>(nz, sh.red);  // the penalised sweep wrote the iterate
     } else {
-        // exact ties (L0 fast path, penalised L2 variance): values inside the re-evaluated band
-        // take its decision and value
-        const int nband = ((f < 4 && !l1c) || f == 4 || f == 6) ? sh.nband : 0;
+        // exact ties (L0 fast path): values inside the re-evaluated band take its decision and value
+        const int nband = !l1c ? sh.nband : 0;
         const float blo = nband ? sh.band_lo : 0.f, bhi = nband ? sh.band_hi : -1.f;
         sweep([&](int q, float v) {
                 float x;
-                if (f < 4 && !l1c) {
+                if (!l1c) {
                     const float av = fabsf(v);
                     if (av >= blo && av <= bhi) {
                         const int i = tid + q * NT;
@@ -2310,39 +2337,14 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                     } else {
                         x = av < tau ? 0.f : v;
                     }
-                }
-                else if (nband && fabsf(v) >= blo && fabsf(v) <= bhi) {
-                    // penalised, inside the re-evaluated band: the exact value decides
-                    const int i = tid + q * NT;
-                    x = 0.f;
-                    for (int p = 0; p < nband; p++)
-                        if (sh.band_idx[p] == i && sh.band_keep[p]) {
-                            const float ve = sh.band_val[p];
-                            x = f == 4 ? ve : (fabsf(ve) - lam) * sgnf(ve);
-                        }
-                } else if (l1c || f == 6 || f == 7) {
+                } else {
                     const float t2 = fabsf(v) - lam;
                     x = t2 > 0.f ? t2 * sgnf(v) : 0.f;
-                } else {
-                    // L0 penalised (lam holds gamma): kept iff v^2 - gamma > 0, so a NaN v (the
-                    // 0/0 after an SP was zeroed out) gives 0, as the reference CPU rule
-                    // (sparse_PCA_thresholding.h:67-102); equal to the GPU functor otherwise
-                    x = (v * v - lam) > 0.f ? v : 0.f;
                 }
-                if (f < 4) x = x * scale_out;
-                __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, tid * 4 + q * NT * 4, 0, 0);
-                nz += (x != 0.f && (tid + q * NT) < a.n) ? 1.0 : 0.0;
+                x = x * scale_out;
                 const int i = tid + q * NT;
-                if (a.precision && i < a.n) {
-                    const uint32_t bo = (uint32_t)(blk_index(j, i, xop.bp) * 2);
-                    if (a.precision == 1) {
-                        const uint16_t h = f2bf(x);
-                        __builtin_amdgcn_raw_buffer_store_b16(h, hr, bo, 0, 0);
-                        __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, bo, 0, 0);
-                    } else {
-                        __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, bo, 0, 0);
-                    }
-                }
+                nz += (x != 0.f && i < a.n) ? 1.0 : 0.0;
+                if (i < a.n) put(i, x);
         });
         nz = block_sum<NT>(nz, sh.red);
         if (f < 4 && tid == 0) sp.nkept[j] = -1;  // the whole column was written
