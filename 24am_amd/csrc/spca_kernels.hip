@@ -79,32 +79,103 @@ __device__ float block_max(float v, double* red) {
     for (int i = 0; i < NT / 64; i++) r = fmaxf(r, (float)red[i]);
     return r;
 }
-__device__ __forceinline__ uint64_t shfl_down_u64(uint64_t v, int o) {
-    uint32_t lo = __shfl_down((uint32_t)v, o, 64);
-    uint32_t hi = __shfl_down((uint32_t)(v >> 32), o, 64);
+// Three sums reduced together (the same tree and order per value as block_sum, so the same
+// bits; the three shuffle chains overlap).  red: 3 * NT / 64 doubles.
+template <int NT>
+__device__ void block_sum3(double& a, double& b, double& c, double* red) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int w = threadIdx.x >> 6;
+    constexpr int W = NT / 64;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[w] = a;
+        red[W + w] = b;
+        red[2 * W + w] = c;
+    }
+    __syncthreads();
+    a = b = c = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        a += red[i];
+        b += red[W + i];
+        c += red[2 * W + i];
+    }
+}
+template <int NT>
+__device__ void block_sum2(double& a, double& b, double* red) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = threadIdx.x >> 6;
+    constexpr int W = NT / 64;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) {
+        red[w] = a;
+        red[W + w] = b;
+    }
+    __syncthreads();
+    a = b = 0;
+#pragma unroll
+    for (int i = 0; i < W; i++) {
+        a += red[i];
+        b += red[W + i];
+    }
+}
+
+// Integer scans over the wave's 64 lanes by DPP: row_shr 1, 2, 4, 8 within each row of 16
+// lanes, then row_bcast:15 (into rows 1 and 3) and row_bcast:31 (into rows 2 and 3).  Integer
+// sums and maxima are exact, so the result is that of any order of evaluation.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint32_t dpp_u32(uint32_t v, uint32_t ident) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)ident, (int)v, CTRL, ROWS, 0xF, false);
+}
+__device__ __forceinline__ uint32_t wave_prefix_u32(uint32_t x) {
+    x += dpp_u32<0x111, 0xF>(x, 0);
+    x += dpp_u32<0x112, 0xF>(x, 0);
+    x += dpp_u32<0x114, 0xF>(x, 0);
+    x += dpp_u32<0x118, 0xF>(x, 0);
+    x += dpp_u32<0x142, 0xA>(x, 0);
+    x += dpp_u32<0x143, 0xC>(x, 0);
+    return x;
+}
+template <int CTRL, int ROWS>
+__device__ __forceinline__ uint64_t dpp_u64(uint64_t v) {
+    const uint32_t lo = dpp_u32<CTRL, ROWS>((uint32_t)v, 0), hi = dpp_u32<CTRL, ROWS>((uint32_t)(v >> 32), 0);
     return ((uint64_t)hi << 32) | lo;
 }
-__device__ __forceinline__ uint32_t shfl_down_u32(uint32_t v, int o) { return __shfl_down(v, o, 64); }
-// inclusive suffix sum over lanes l..63 (lane 63 holds the top)
+__device__ __forceinline__ uint64_t wave_prefix_u64(uint64_t x) {
+    x += dpp_u64<0x111, 0xF>(x);
+    x += dpp_u64<0x112, 0xF>(x);
+    x += dpp_u64<0x114, 0xF>(x);
+    x += dpp_u64<0x118, 0xF>(x);
+    x += dpp_u64<0x142, 0xA>(x);
+    x += dpp_u64<0x143, 0xC>(x);
+    return x;
+}
+// inclusive suffix sum over lanes l..63 (lane 63 holds the top): the wave total minus the
+// prefix below lane l
 __device__ __forceinline__ uint32_t wave_suffix_u32(uint32_t v) {
-    const int l = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t t = shfl_down_u32(v, o);
-        if (l + o < 64) v += t;
-    }
-    return v;
+    const uint32_t p = wave_prefix_u32(v);
+    return (uint32_t)__builtin_amdgcn_readlane((int)p, 63) - p + v;
 }
 __device__ __forceinline__ uint64_t wave_suffix_u64(uint64_t v) {
-    const int l = threadIdx.x & 63;
-    for (int o = 1; o < 64; o <<= 1) {
-        uint64_t t = shfl_down_u64(v, o);
-        if (l + o < 64) v += t;
-    }
-    return v;
+    const uint64_t p = wave_prefix_u64(v);
+    const uint64_t tot = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(p >> 32), 63) << 32) |
+                         (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)p, 63);
+    return tot - p + v;
 }
+// maximum over the wave, for values >= -1
 __device__ __forceinline__ int wave_max_i(int v) {
-    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_down(v, o, 64));
-    return __shfl(v, 0, 64);
+    auto mx = [](int x, uint32_t y) { return max(x, (int)y); };
+    const uint32_t m1 = 0xFFFFFFFFu;  // -1: the identity for values >= -1
+    v = mx(v, dpp_u32<0x111, 0xF>((uint32_t)v, m1));
+    v = mx(v, dpp_u32<0x112, 0xF>((uint32_t)v, m1));
+    v = mx(v, dpp_u32<0x114, 0xF>((uint32_t)v, m1));
+    v = mx(v, dpp_u32<0x118, 0xF>((uint32_t)v, m1));
+    v = mx(v, dpp_u32<0x142, 0xA>((uint32_t)v, m1));
+    v = mx(v, dpp_u32<0x143, 0xC>((uint32_t)v, m1));
+    return __builtin_amdgcn_readlane(v, 63);
 }
 
 // ---------------------------------------------------------------- data preparation
@@ -1449,7 +1520,7 @@ struct SelShared {
             uint64_t lcand[kLCand];  // the list's answer bin, sorted descending
         };
     };
-    double red[kSelThreads / 64];
+    double red[3 * (kSelThreads / 64)];
     int ires[4];
     uint64_t ures[2];
     int ncand;
@@ -1464,6 +1535,15 @@ struct SelShared {
     float band_val[kBandMax];
     int band_keep[kBandMax];
 };
+
+// Workgroup barrier ordering LDS only: a release/acquire fence on the local address space
+// around s_barrier.  Unlike __syncthreads() (whose fence also covers global memory and so
+// waits for every outstanding global load), loads issued before it stay in flight.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
@@ -1615,25 +1695,52 @@ __device__ void scan_hist(SelShared& sh, const uint32_t* cnt, const uint64_t* hs
 // One workgroup of NT threads per starting point.  Each pass streams the column through the
 // bounded view, DEPTH loads in flight per thread; every pass visits a thread's elements in
 // the same order, and reductions have a fixed order, so results are bitwise reproducible.
-template <int NT, int DEPTH>
-__device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Operand xop, StepArgs a, SpState sp) {
+// The step of one SP (column j of V), specialised by kind (step_kind: 0 L0 constraint, 1 L1
+// constraint, 2 L0 penalty, 3 L1 penalty) and iterate precision, so the per-element code of the
+// column sweeps carries no formulation or precision branches.  sh: the calling kernel's LDS.
+template <int NT, int DEPTH, int KIND, int PREC>
+__device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __restrict__ V, Operand xop, StepArgs a,
+                                          SpState sp) {
     if (*sp.done || !sp.active[j]) return;
     static_assert(NT <= kSelThreads, "SelShared holds kSelThreads / 64 waves");
-    __shared__ SelShared sh;
+    static_assert(KIND >= 0 && KIND < 4 && PREC >= 0 && PREC < 3, "step kind and precision");
+    constexpr bool kConstrained = KIND < 2;
     const int tid = threadIdx.x;
     const int nq = (a.n + NT - 1) / NT;
     const __amdgpu_buffer_rsrc_t vr = col_rsrc(V + (long)j * a.ldv, (uint32_t)a.n * 4u);
 #define LDV(q) __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(vr, tid * 4 + (q) * NT * 4, 0, 0))
     // fn(q, v) for this thread's elements q = 0 .. nq-1 (element index tid + q * NT; the last
     // row's entries past n read 0)
+    // Rows are taken DEPTH at a time, alternating between two register buffers: the next DEPTH
+    // rows are loaded (unconditionally: rows past the column read 0 without a memory access)
+    // before the current ones are processed, so the column's loads stay in flight.
     auto sweep = [&](auto&& fn) {
-        for (int q0 = 0; q0 < nq; q0 += DEPTH) {
-            float vv[DEPTH];
+        float ba[DEPTH], bb[DEPTH];
+        auto load = [&](float (&vv)[DEPTH], int q0) {
 #pragma unroll
             for (int u = 0; u < DEPTH; u++) vv[u] = LDV(q0 + u);
+        };
+        auto run = [&](const float (&vv)[DEPTH], int q0) {
 #pragma unroll
             for (int u = 0; u < DEPTH; u++)
                 if (q0 + u < nq) fn(q0 + u, vv[u]);
+        };
+        if constexpr (KIND != 1) {
+            load(ba, 0);
+            for (int q0 = 0; q0 < nq; q0 += 2 * DEPTH) {
+                load(bb, q0 + DEPTH);
+                run(ba, q0);
+                if (q0 + DEPTH >= nq) break;
+                load(ba, q0 + 2 * DEPTH);
+                run(bb, q0 + DEPTH);
+            }
+        } else {
+            // the L1 constraint's passes keep more registers live; with a second buffer the
+            // kernel exceeds 128 VGPRs (four waves per SIMD) and spills: one buffer
+            for (int q0 = 0; q0 < nq; q0 += DEPTH) {
+                load(ba, q0);
+                run(ba, q0);
+            }
         }
     };
     // the iterate's stores: x_i (i < n) in FP32 and the operand forms
@@ -1642,15 +1749,15 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + xbase, (uint32_t)a.n * 4u);
     // 16-bit forms are K-blocked: element i of column j at ((i/32)*bp + j)*32 + i%32; the views
     // cover the whole operand (ldv * bp elements) and elements i >= n are skipped
-    const uint32_t opb = a.precision ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
+    const uint32_t opb = PREC ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
     const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
     const __amdgpu_buffer_rsrc_t lr =
-        col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, a.precision == 1 ? opb : 0u);
+        col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, PREC == 1 ? opb : 0u);
     auto put = [&](int i, float x) {
         __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, (uint32_t)i * 4u, 0, 0);
-        if (a.precision) {
+        if constexpr (PREC != 0) {
             const uint32_t bo = (uint32_t)(blk_index(j, i, xop.bp) * 2);
-            if (a.precision == 1) {
+            if constexpr (PREC == 1) {
                 const uint16_t h = f2bf(x);
                 __builtin_amdgcn_raw_buffer_store_b16(h, hr, bo, 0, 0);
                 __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, bo, 0, 0);
@@ -1659,7 +1766,6 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
             }
         }
     };
-    const int f = a.formulation;
     float fval;
     // element transform applied when the iterate is written: keep |v| >= tau (L0),
     // soft map with lam (L1), penalised maps; then multiply by `scale_out`
@@ -1668,10 +1774,10 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
 
     // L1 constraint with s <= 1 (|x|_1 <= |x|_2) admits only 1-sparse x: the top-1 hard
     // threshold, which the lambda scan cannot produce (its quadratic degenerates)
-    const bool l1c = (f == 2 || f == 3) && a.s > 1;
+    constexpr bool l1c = KIND == 1;
     bool decided = false;  // constrained: the step was resolved on the previous threshold's list
     int list_cnt = 0;      // entries of that list
-    if (f < 4) {
+    if constexpr (kConstrained) {
         // ---- constrained: select, then x <- x/|x|_2
         const double wd = (double)(float)sqrt((double)a.s);
         double norm2 = 0;
@@ -1703,8 +1809,9 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                         above += av > hi ? 1.0 : 0.0;
                         inband += (av <= hi && av >= lo) ? 1.0 : 0.0;
                     }
-                    i0 = (int)block_sum<NT>(above, sh.red);
-                    i1 = i0 + (int)block_sum<NT>(inband, sh.red);
+                    block_sum2<NT>(above, inband, sh.red);
+                    i0 = (int)above;
+                    i1 = i0 + (int)inband;
                 }
                 const int nb = i1 - i0;
                 if (tid == 0) sh.nband = 0;
@@ -1849,36 +1956,39 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
             int total = 0;  // entries so far, block-uniform
             {
                 const int lane = tid & 63, w = tid >> 6;
-                int buf = 0;
-                for (int q0 = 0; q0 < nq; q0 += DEPTH, buf ^= 1) {
-                    float vv[DEPTH];
-#pragma unroll
-                    for (int u = 0; u < DEPTH; u++) vv[u] = LDV(q0 + u);
+                // One chunk of DEPTH rows.  Rows past nq and entries past n read 0 through the
+                // bounded view, and L > 0, so neither is listed and no bound checks are needed.
+                // wtot is double-buffered (buf): a wave reuses one only after the next chunk's
+                // barrier, which every wave passes after reading it.
+                auto chunk = [&](const float (&vv)[DEPTH], int q0, int buf) {
                     uint32_t in = 0;
 #pragma unroll
                     for (int u = 0; u < DEPTH; u++) {
                         const float av = fabsf(vv[u]);
-                        if (q0 + u < nq) mloc = fmaxf(mloc, av);
-                        if (q0 + u < nq && av >= L && tid + (q0 + u) * NT < a.n) in |= 1u << u;
+                        mloc = fmaxf(mloc, av);
+                        in |= (av >= L ? 1u : 0u) << u;
                     }
-                    const int c = __popc(in);
-                    int incl = c;  // inclusive prefix over the wave's lanes
-                    for (int o = 1; o < 64; o <<= 1) {
-                        const int t = __shfl_up(incl, o, 64);
-                        if (lane >= o) incl += t;
+                    const int c = __popc(in);  // at most DEPTH <= 16: five bits
+                    // the wave's exclusive prefix of c and its total, from one ballot per bit of c
+                    int excl = 0, wsum = 0;
+#pragma unroll
+                    for (int bit = 0; bit < 5; bit++) {
+                        const uint64_t m = __ballot((c >> bit) & 1);
+                        excl += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))
+                                << bit;
+                        wsum += __popcll(m) << bit;
                     }
-                    // two buffers: a wave reuses one only after the next chunk's barrier, which
-                    // every wave passes after reading it
-                    if (lane == 63) sh.wtot[buf][w] = incl;
-                    __syncthreads();
-                    int before = 0, chunk = 0;
+                    if (lane == 0) sh.wtot[buf][w] = wsum;
+                    lds_barrier();  // the next chunk's loads stay in flight across it
+                    int before = 0, chunk_tot = 0;
 #pragma unroll
                     for (int k = 0; k < NT / 64; k++) {
                         const int t = sh.wtot[buf][k];
                         before += k < w ? t : 0;
-                        chunk += t;
+                        chunk_tot += t;
                     }
-                    int slot = total + before + incl - c;
+                    int slot = total + before + excl;
 #pragma unroll
                     for (int u = 0; u < DEPTH; u++)
                         if ((in >> u) & 1u) {
@@ -1887,7 +1997,35 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                                                 (uint32_t)(tid + (q0 + u) * NT) | (vv[u] < 0.f ? 0x80000000u : 0u);
                             slot++;
                         }
-                    total += chunk;
+                    total += chunk_tot;
+                };
+                // rows DEPTH at a time, alternating between two register buffers, so the next
+                // chunk's loads are in flight while this one is processed (unconditionally: rows
+                // past the column read 0 without a memory access, and a conditional load would
+                // make the compiler wait for it before the current chunk)
+                auto load_rows = [&](float (&vv)[DEPTH], int q0) {
+#pragma unroll
+                    for (int u = 0; u < DEPTH; u++) vv[u] = LDV(q0 + u);
+                };
+                if constexpr (!l1c) {
+                    float ba[DEPTH], bb[DEPTH];
+                    load_rows(ba, 0);
+                    for (int q0 = 0; q0 < nq; q0 += 2 * DEPTH) {
+                        load_rows(bb, q0 + DEPTH);
+                        chunk(ba, q0, 0);
+                        if (q0 + DEPTH >= nq) break;
+                        load_rows(ba, q0 + 2 * DEPTH);
+                        chunk(bb, q0 + DEPTH, 1);
+                    }
+                } else {
+                    // the L1 constraint keeps more registers live; a second buffer makes the
+                    // kernel spill (measured slower): one buffer
+                    int buf = 0;
+                    for (int q0 = 0; q0 < nq; q0 += DEPTH, buf ^= 1) {
+                        float ba[DEPTH];
+                        load_rows(ba, q0);
+                        chunk(ba, q0, buf);
+                    }
                 }
             }
             Mcol = block_max<NT>(mloc, sh.red);  // its barriers also publish the list
@@ -1954,9 +2092,8 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                             if (inb) sh.lcand[base + (int)__popcll(bal & ((1ull << lane) - 1ull))] = key;
                         }
                     }
-                    const double cAd = block_sum<NT>(c_loc, sh.red);
-                    const double S1A = block_sum<NT>(s1_loc, sh.red);
-                    const double S2A = block_sum<NT>(s2_loc, sh.red);
+                    block_sum3<NT>(c_loc, s1_loc, s2_loc, sh.red);
+                    const double cAd = c_loc, S1A = s1_loc, S2A = s2_loc;
                     const int nc = sh.nlc;  // <= kLCand: the bin's count
                     bitonic_desc<NT>(sh.lcand, nc);
                     decided = resolve(sh.lcand, nc, cA_hist, cAd, S1A, S2A, true, true, [&](float lo, float hi) {
@@ -2041,9 +2178,8 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                     if (slot < kCand) sh.cand[slot] = ((uint64_t)__float_as_uint(av) << 32) | (uint32_t)idx;
                 }
             });
-            const double cAd = block_sum<NT>(c_loc, sh.red);
-            const double S1A = block_sum<NT>(s1_loc, sh.red);
-            const double S2A = block_sum<NT>(s2_loc, sh.red);
+            block_sum3<NT>(c_loc, s1_loc, s2_loc, sh.red);
+            const double cAd = c_loc, S1A = s1_loc, S2A = s2_loc;
             const int nc = min(sh.ncand, kCand);
             bitonic_desc<NT>(sh.cand, nc);
             resolve(sh.cand, nc, cA_hist, cAd, S1A, S2A, false, bstar >= 0, [&](float lo, float hi) {
@@ -2130,9 +2266,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                     S2 += (double)av * (double)av;
                 }
             });
-            c = block_sum<NT>(c, sh.red);
-            S1 = block_sum<NT>(S1, sh.red);
-            S2 = block_sum<NT>(S2, sh.red);
+            block_sum3<NT>(c, S1, S2, sh.red);
             const double w2 = wd * wd;
             const double qa = c * (c - w2);
             const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
@@ -2158,7 +2292,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         const float g = (float)a.gamma;
         lam = g;
         double acc = 0, acc_band = 0;
-        const bool l0p = (f == 4 || f == 5);
+        constexpr bool l0p = KIND == 2;
         auto contrib = [&](float v) -> double {
             if (l0p) {
                 const float t = v * v - g;
@@ -2250,7 +2384,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 // the band's entries by their exact values: kept ones take the recomputed value
                 if (tid < nb) {
                     const float ve = sh.band_val[tid];
-                    const float x = sh.band_keep[tid] ? (f == 4 ? ve : (fabsf(ve) - g) * sgnf(ve)) : 0.f;
+                    const float x = sh.band_keep[tid] ? (l0p ? ve : (fabsf(ve) - g) * sgnf(ve)) : 0.f;
                     put(sh.band_idx[tid], x);
                     nz += x != 0.f ? 1.0 : 0.0;
                 }
@@ -2319,7 +2453,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
         __syncthreads();
         nz = (double)sh.nkeep;  // every kept value is nonzero: |v| >= tau > 0, or |v| - lam > 0
         if (tid == 0) sp.nkept[j] = sh.nkeep;
-    } else if (f >= 4) {
+    } else if constexpr (!kConstrained) {
         nz = block_sum<NT>(nz, sh.red);  // the penalised sweep wrote the iterate
     } else {
         // exact ties (L0 fast path): values inside the re-evaluated band take its decision and value
@@ -2347,7 +2481,7 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
                 if (i < a.n) put(i, x);
         });
         nz = block_sum<NT>(nz, sh.red);
-        if (f < 4 && tid == 0) sp.nkept[j] = -1;  // the whole column was written
+        if (tid == 0) sp.nkept[j] = -1;  // the whole column was written
     }
 #undef LDV
     if (tid == 0) {
@@ -2358,14 +2492,32 @@ __device__ __forceinline__ void step_body(int j, const float* __restrict__ V, Op
     }
 }
 
-template <int NT, int DEPTH>
-__global__ __launch_bounds__(NT) void step_kernel(const float* __restrict__ V, Operand xop, StepArgs a, SpState sp) {
-    step_body<NT, DEPTH>(blockIdx.x, V, xop, a, sp);
+// Step kind of a formulation: 0 L0 constraint (also the L1 constraint with s <= 1, which admits
+// only 1-sparse x: the top-1 hard threshold), 1 L1 constraint, 2 L0 penalty, 3 L1 penalty.
+__host__ __device__ inline int step_kind(int f, unsigned s) {
+    return f >= 6 ? 3 : f >= 4 ? 2 : (f >= 2 && s > 1) ? 1 : 0;
+}
+
+template <int NT, int DEPTH, int KIND, int PREC>
+__global__ __launch_bounds__(NT, 4) void step_kernel(const float* __restrict__ V, Operand xop, StepArgs a, SpState sp) {
+    __shared__ SelShared sh;
+    step_body<NT, DEPTH, KIND, PREC>(sh, blockIdx.x, V, xop, a, sp);
 }
 hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st) {
     if (a.n <= 0 || (long)a.n * 4 > 0x7fffffffL) return hipErrorInvalidValue;
     if (a.precision && (long)a.ldv * xop.bp * 2 > 0xffffffffL) return hipErrorInvalidValue;  // 32-bit store offsets
-    step_kernel<kSelThreads, kSelDepth><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp);
+    if (a.precision < 0 || a.precision > 2) return hipErrorInvalidValue;
+    switch (step_kind(a.formulation, a.s) * 3 + a.precision) {
+#define SPCA_STEP_CASE(K, P)                                                                          \
+    case K * 3 + P:                                                                                   \
+        step_kernel<kSelThreads, kSelDepth, K, P><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp); \
+        break;
+        SPCA_STEP_CASE(0, 0) SPCA_STEP_CASE(0, 1) SPCA_STEP_CASE(0, 2)
+        SPCA_STEP_CASE(1, 0) SPCA_STEP_CASE(1, 1) SPCA_STEP_CASE(1, 2)
+        SPCA_STEP_CASE(2, 0) SPCA_STEP_CASE(2, 1) SPCA_STEP_CASE(2, 2)
+        SPCA_STEP_CASE(3, 0) SPCA_STEP_CASE(3, 1) SPCA_STEP_CASE(3, 2)
+#undef SPCA_STEP_CASE
+    }
     return hipGetLastError();
 }
 
@@ -2559,10 +2711,19 @@ __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __res
                 J.zprec, J.sp, J.rf);
 }
 
-__global__ __launch_bounds__(kSelThreads) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {
+// Jobs of one fused launch share the precision (PREC); each job's step kind is chosen per workgroup.
+template <int PREC>
+__global__ __launch_bounds__(kSelThreads, 4) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {
+    __shared__ SelShared sh;
     const int k = find_job<2>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
-    step_body<kSelThreads, kSelDepth>(blockIdx.x - J.c0, J.V, J.xop, J.sa, J.sp);
+    const int col = blockIdx.x - J.c0;
+    switch (step_kind(J.sa.formulation, J.sa.s)) {
+        case 0: step_body<kSelThreads, kSelDepth, 0, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
+        case 1: step_body<kSelThreads, kSelDepth, 1, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
+        case 2: step_body<kSelThreads, kSelDepth, 2, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
+        default: step_body<kSelThreads, kSelDepth, 3, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
+    }
 }
 
 __global__ __launch_bounds__(256) void fused_converge_kernel(const FusedJob* __restrict__ jobs, int* all_done) {
@@ -2579,7 +2740,7 @@ hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2,
     fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
     fused_gemm_kernel<PREC, 1><<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
     if (any_fix8) fused_fix8_kernel<<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
-    fused_step_kernel<<<cols, kSelThreads, 0, st>>>(jobs, njobs);
+    fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs);
     fused_converge_kernel<<<njobs, 256, 0, st>>>(jobs, all_done);
     return hipGetLastError();
 }
