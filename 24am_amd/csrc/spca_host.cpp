@@ -75,6 +75,7 @@ struct Workspace {
     float* fstate = nullptr;  // val, fz, err, errmax, thr  (kStateFloats x bp floats)
     int* istate = nullptr;    // iters, nnz, active, nkept (bp ints each), done (state_flag_ints(bp))
     int* kept = nullptr;      // kListMax indices per SP: the nonzeros a list step wrote
+    int* gl = nullptr;        // kGatherStride ints per SP: the gathered A*X's nonzero lists (null: scan x)
     double* ev = nullptr;
     double* corr = nullptr;  // per-SP centring scalars of sparse products (4 x bp)
     // union A*X (sparse_products = 2): index flags and list (np ints each), stat (4 ints),
@@ -504,7 +505,7 @@ int make_generated(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const
 void free_ws(spca_ctx ctx, Workspace& w, hipStream_t s) {
     (void)hipStreamSynchronize(s);
     void* ptrs[] = {w.X,      w.Xhi,   w.Xlo,    w.Zpart, w.Z,  w.Zhi, w.Zlo, w.V,    w.fstate,
-                    w.istate, w.kept,  w.ev,     w.corr,  w.uflag, w.uidx, w.ustat, w.Au, w.Xu};
+                    w.istate, w.kept,  w.gl,     w.ev,    w.corr,  w.uflag, w.uidx, w.ustat, w.Au, w.Xu};
     for (void* p : ptrs) raw_free(ctx, p);
     w = Workspace();
 }
@@ -514,7 +515,8 @@ void free_workspace(spca_ctx ctx, Engine& E) { free_ws(ctx, E.ws, E.stream); }
 uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
     const uint64_t xs = (uint64_t)np * bp, zs = (uint64_t)mp * bp;
     return xs * (4 + 2 + 2 + 4) + zs * splits * 4 + zs * (4 + 2 + 2) + (uint64_t)kStateFloats * bp * 4 +
-           ((uint64_t)kStateInts * bp + state_flag_ints(bp)) * 4 + (uint64_t)bp * kListMax * 4 + (uint64_t)bp * 8 * 5;
+           ((uint64_t)kStateInts * bp + state_flag_ints(bp)) * 4 + (uint64_t)bp * kListMax * 4 +
+           (uint64_t)bp * kGatherStride * 4 + (uint64_t)bp * 8 * 5;
 }
 
 int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp, int splits,
@@ -534,6 +536,7 @@ int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp,
         (rc = dev_alloc(ctx, &w.fstate, (size_t)kStateFloats * bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.istate, (size_t)kStateInts * bp + state_flag_ints(bp), &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.kept, (size_t)kListMax * bp, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.gl, (size_t)kGatherStride * bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.corr, (size_t)4 * bp, &w.bytes, err))) {
         free_ws(ctx, w, s);
@@ -816,6 +819,15 @@ hipError_t launch_atz(const AtzForm& a, const Matrix* M, Workspace& w, int bp, i
                           a.big, sp.done, nullptr, s);
 }
 
+// The gathered A*X into partial 0 of Zpart for the groups flagged on the device: over nonzero
+// lists built once per SP when the workspace holds them (the allocation-free solve's does not),
+// else scanning x per row block.  Both sum in ascending index order: the same bits.
+hipError_t launch_gather(const Matrix* M, Workspace& w, int batch, SpState sp, hipStream_t s) {
+    if (w.gl)
+        return launch_sparse_ax_listed(M->a32_cm, M->mp, M->mp, M->n, w.X, w.np, w.Zpart, M->mp, batch, sp, w.gl, s);
+    return launch_sparse_ax(M->a32_cm, M->mp, M->mp, M->n, w.X, w.np, w.Zpart, M->mp, batch, sp, s);
+}
+
 // Zpart = A * X of one iteration: the dense product (K split over `splits`) and, with
 // `gather`, the sparse gather beside it (one of them runs, on the device flag); sparse
 // matrices: the CSR gather with the centring terms, into partial 0.
@@ -836,8 +848,7 @@ hipError_t launch_ax(const Matrix* M, const spca_settings& st, Workspace& w, int
     else
         e = launch_gemm_tn(st.precision, P1_32, P1_hi, M->lo_rm, ld1p, w.X, w.Xhi, w.Xlo, ld1q, w.Zpart, M->mp, zsplit,
                            M->mp, bp, M->np, splits, big, sp.done, gather ? sp.done + kSparseFlags : nullptr, s);
-    if (e == hipSuccess && gather)
-        e = launch_sparse_ax(M->a32_cm, M->mp, M->mp, M->n, w.X, w.np, w.Zpart, M->mp, batch, sp, s);
+    if (e == hipSuccess && gather) e = launch_gather(M, w, batch, sp, s);
     return e;
 }
 
@@ -1544,7 +1555,7 @@ int solve_impl(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* st_in, c
                                                  w.ldu, s));
                 HIP_TRY(ctx, launch_gemm_tn(0, w.Au, nullptr, nullptr, w.ldu, w.Xu, nullptr, nullptr, w.ldu, w.Zpart,
                                             M->mp, zsplit, M->mp, bp, kpad, us, 0, sp.done, sp.done + kSparseFlags, s));
-                HIP_TRY(ctx, launch_sparse_ax(M->a32_cm, M->mp, M->mp, M->n, w.X, w.np, w.Zpart, M->mp, batch, sp, s));
+                HIP_TRY(ctx, launch_gather(M, w, batch, sp, s));
                 ax_splits = us;
                 R.union_it++;
             } else {

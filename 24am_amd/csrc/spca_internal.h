@@ -255,6 +255,12 @@ hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs
 // (ld ldz), for the groups of 256 SPs whose sparse flag (sp.done[kSparseFlags + g]) is set; the
 // other groups' workgroups return at once.  A: FP32 column major, ld lda.
 constexpr int kSparseRows = 1024;  // rows per workgroup of the sparse product
+// Gathered A*X over per-SP nonzero lists (solo solves): each gathered SP's list holds a count
+// (int 0), then up to kGatherList ascending indices (from int kGatherHead) and their values (float
+// bits, kGatherList further on).  An SP with more nonzeros is gathered by scanning x instead.
+constexpr int kGatherList = 1024;
+constexpr int kGatherHead = 64;
+constexpr int kGatherStride = kGatherHead + 2 * kGatherList;
 // union A*X: flags and the ascending list U of the indices nonzero in any iterate whose A*X is
 // dense next iteration, stat = {|U|, done[0], done[1]}; then Au = A[:, U] (row major) and
 // Xu = X[U, :] (row j = SP j), both zero-padded to kpad columns, for the FP32 product
@@ -264,6 +270,8 @@ hipError_t launch_union_gather(const float* Acm, int lda, int mp, const float* X
                                int K, int kpad, float* Au, float* Xu, int ldu, hipStream_t st);
 hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z, int ldz,
                             int batch, SpState sp, hipStream_t st);
+hipError_t launch_sparse_ax_listed(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z,
+                                   int ldz, int batch, SpState sp, int* gl, hipStream_t st);
 
 // Deflation by earlier components (reference SparseDeflationCollection,
 // c++/src/gpower/sparse_PCA_solver_for_CSC.h:56-106): for c = 0 .. ncomp-1 in order, each

@@ -1202,6 +1202,118 @@ hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const floa
     return hipGetLastError();
 }
 
+// The same gather over nonzero lists built once per SP.  nz_list_kernel: one workgroup of 1024
+// threads per gathered SP scans x in 8192-index slabs (ballot compaction, index order (e, wave,
+// lane) = ascending) and writes the count, then up to kGatherList indices and values, to
+// gl + j * kGatherStride.  sparse_ax_listed_kernel: workgroup (row block, SP) stages the list in
+// LDS and walks it with sixteen columns' loads in flight, accumulating in the same ascending
+// index order as sparse_ax_body (so the same bits); an SP with more than kGatherList nonzeros
+// scans x itself as sparse_ax_body does.  Each x is read once instead of once per row block,
+// and no slab barriers sit between a workgroup's column loads.
+__global__ __launch_bounds__(1024) void nz_list_kernel(const float* __restrict__ X, int ldx, int n, SpState sp,
+                                                       int* __restrict__ gl) {
+    const int j = blockIdx.x;
+    if (sp.done[0] || !sp.done[kSparseFlags + (j >> 8)] || !sp.active[j]) return;
+    constexpr int kE = 8, kSlab = 1024 * kE;
+    __shared__ int wcount[kE][16];
+    const __amdgpu_buffer_rsrc_t xr = col_rsrc(X + (long)j * ldx, (uint32_t)n * 4u);  // reads 0 past n
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int* out = gl + (long)j * kGatherStride;
+    int* oidx = out + kGatherHead;
+    int* oval = out + kGatherHead + kGatherList;
+    int total = 0;
+    for (int k0 = 0; k0 < n; k0 += kSlab) {
+        float xv[kE];
+#pragma unroll
+        for (int e = 0; e < kE; e++)
+            xv[e] = __uint_as_float(
+                __builtin_amdgcn_raw_buffer_load_b32(xr, (uint32_t)(k0 + e * 1024 + (int)threadIdx.x) * 4u, 0, 0));
+        uint64_t bal[kE];
+#pragma unroll
+        for (int e = 0; e < kE; e++) {
+            bal[e] = __ballot(xv[e] != 0.f);
+            if (lane == 0) wcount[e][w] = __popcll(bal[e]);
+        }
+        __syncthreads();
+        int cnt = 0;
+#pragma unroll
+        for (int e = 0; e < kE; e++) {
+            int before = cnt;  // nonzeros ahead of (e, w, lane 0)
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                before += i < w ? wcount[e][i] : 0;
+                cnt += wcount[e][i];
+            }
+            if (xv[e] != 0.f) {
+                const int at = total + before + __popcll(bal[e] & ((1ull << lane) - 1ull));
+                if (at < kGatherList) {
+                    oidx[at] = k0 + e * 1024 + (int)threadIdx.x;
+                    oval[at] = __float_as_int(xv[e]);
+                }
+            }
+        }
+        total += cnt;
+        __syncthreads();  // wcount is rewritten by the next slab
+    }
+    if (threadIdx.x == 0) out[0] = total;
+}
+
+__global__ __launch_bounds__(256) void sparse_ax_listed_kernel(const float* __restrict__ A, int lda, int n,
+                                                               const float* __restrict__ X, int ldx,
+                                                               float* __restrict__ Z, int ldz, int mp, SpState sp,
+                                                               const int* __restrict__ gl) {
+    const int rb = blockIdx.x, j = blockIdx.y;
+    if (sp.done[0] || !sp.done[kSparseFlags + (j >> 8)] || !sp.active[j]) return;
+    const int* L = gl + (long)j * kGatherStride;
+    const int cnt = L[0];
+    if (cnt > kGatherList) {  // longer than the list: scan x as the slab gather does
+        sparse_ax_body(rb, j, A, lda, n, X, ldx, Z, ldz, mp, sp);
+        return;
+    }
+    __shared__ int sidx[kGatherList];
+    __shared__ float sval[kGatherList];
+    for (int t = threadIdx.x; t < cnt; t += 256) {
+        sidx[t] = L[kGatherHead + t];
+        sval[t] = __int_as_float(L[kGatherHead + kGatherList + t]);
+    }
+    __syncthreads();
+    const int row0 = rb * kSpRows + 4 * threadIdx.x;
+    if (row0 >= mp) return;  // mp and the row block are multiples of 4 rows
+    double acc[4] = {0, 0, 0, 0};
+    constexpr int kU = 16;  // columns' loads in flight before their FMAs; the sum stays in index order
+    int t = 0;
+    for (; t + kU <= cnt; t += kU) {
+        f4 c[kU];
+#pragma unroll
+        for (int u = 0; u < kU; u++) c[u] = *(const f4*)(A + (long)sidx[t + u] * lda + row0);
+#pragma unroll
+        for (int u = 0; u < kU; u++) {
+            const double v = sval[t + u];
+#pragma unroll
+            for (int r = 0; r < 4; r++) acc[r] += (double)c[u][r] * v;
+        }
+    }
+    for (; t < cnt; t++) {
+        const f4 c = *(const f4*)(A + (long)sidx[t] * lda + row0);
+        const double v = sval[t];
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[r] += (double)c[r] * v;
+    }
+    f4 o;
+#pragma unroll
+    for (int r = 0; r < 4; r++) o[r] = (float)acc[r];
+    *(f4*)(Z + (long)j * ldz + row0) = o;
+}
+hipError_t launch_sparse_ax_listed(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z,
+                                   int ldz, int batch, SpState sp, int* gl, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    if (!gl || (long)n * 4 + 32768 > 0xffffffffL) return hipErrorInvalidValue;  // 32-bit offsets into x
+    nz_list_kernel<<<batch, 1024, 0, st>>>(X, ldx, n, sp, gl);
+    sparse_ax_listed_kernel<<<dim3((mp + kSpRows - 1) / kSpRows, batch), 256, 0, st>>>(Acm, lda, n, X, ldx, Z, ldz,
+                                                                                      mp, sp, gl);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- z epilogue
 __device__ __forceinline__ bool is_constrained(int f) { return f < 4; }
 __device__ __forceinline__ bool is_l1_variance(int f) { return f == 1 || f == 3 || f == 5 || f == 7; }
