@@ -1761,6 +1761,7 @@ void fused_entry(FusedRun& r, int precision, FusedTotals& T, FusedJob& f) {
     T.any_fix8 |= atz.prec == 4;
     f.Zpart = r.ws.Zpart;
     f.V = r.ws.V;
+    f.gl = r.ws.gl;
     f.xop = x_operand(r.ws, precision);
     f.zop = z_operand(r.ws, atz.zprec);
     f.rf = sign_refine(M, r.st, r.ws);

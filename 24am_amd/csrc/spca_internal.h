@@ -163,6 +163,7 @@ struct FusedJob {
     int nb1, nb2;           // FP32 A*X / A^T*Z in the narrow 128 x nb tile (32 or 64; 0: 128 x 128 tiles)
     int t4;                 // first (row block, SP) item of the partial-sum kernel
     int psum;               // 1: the partial-sum kernel adds the K splits into partial 0 before prep_z
+    int* gl;                // the gathered A*X's nonzero lists (kGatherStride ints per SP; null: scan x)
 };
 
 // ---- launchers (spca_kernels.hip); every one enqueues on `st` and returns a hip status
@@ -261,6 +262,9 @@ constexpr int kSparseRows = 1024;  // rows per workgroup of the sparse product
 constexpr int kGatherList = 1024;
 constexpr int kGatherHead = 64;
 constexpr int kGatherStride = kGatherHead + 2 * kGatherList;
+// Solo solves of at most kOrderMaxBatch SPs gather in the order of each SP's largest |x_i|
+// (header int 1 of its list: that index; int 2 of SP i's header: the SP at position i).
+constexpr int kOrderMaxBatch = 4096;
 // union A*X: flags and the ascending list U of the indices nonzero in any iterate whose A*X is
 // dense next iteration, stat = {|U|, done[0], done[1]}; then Au = A[:, U] (row major) and
 // Xu = X[U, :] (row j = SP j), both zero-padded to kpad columns, for the FP32 product

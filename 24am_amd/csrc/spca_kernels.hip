@@ -1210,9 +1210,8 @@ hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const floa
 // index order as sparse_ax_body (so the same bits); an SP with more than kGatherList nonzeros
 // scans x itself as sparse_ax_body does.  Each x is read once instead of once per row block,
 // and no slab barriers sit between a workgroup's column loads.
-__global__ __launch_bounds__(1024) void nz_list_kernel(const float* __restrict__ X, int ldx, int n, SpState sp,
-                                                       int* __restrict__ gl) {
-    const int j = blockIdx.x;
+__device__ __forceinline__ void nz_list_body(int j, const float* __restrict__ X, int ldx, int n, SpState sp,
+                                             int* __restrict__ gl) {
     if (sp.done[0] || !sp.done[kSparseFlags + (j >> 8)] || !sp.active[j]) return;
     constexpr int kE = 8, kSlab = 1024 * kE;
     __shared__ int wcount[kE][16];
@@ -1222,6 +1221,9 @@ __global__ __launch_bounds__(1024) void nz_list_kernel(const float* __restrict__
     int* oidx = out + kGatherHead;
     int* oval = out + kGatherHead + kGatherList;
     int total = 0;
+    // the grouping key of the grouped gather: the index of the largest |x_i| (the lowest such
+    // index on ties), as (|x| bits << 32) | ~index maximised
+    uint64_t best = 0;
     for (int k0 = 0; k0 < n; k0 += kSlab) {
         float xv[kE];
 #pragma unroll
@@ -1245,24 +1247,42 @@ __global__ __launch_bounds__(1024) void nz_list_kernel(const float* __restrict__
                 cnt += wcount[e][i];
             }
             if (xv[e] != 0.f) {
+                const int idx = k0 + e * 1024 + (int)threadIdx.x;
                 const int at = total + before + __popcll(bal[e] & ((1ull << lane) - 1ull));
                 if (at < kGatherList) {
-                    oidx[at] = k0 + e * 1024 + (int)threadIdx.x;
+                    oidx[at] = idx;
                     oval[at] = __float_as_int(xv[e]);
                 }
+                const uint64_t key = ((uint64_t)__float_as_uint(fabsf(xv[e])) << 32) | (uint32_t)~idx;
+                best = key > best ? key : best;
             }
         }
         total += cnt;
         __syncthreads();  // wcount is rewritten by the next slab
     }
-    if (threadIdx.x == 0) out[0] = total;
+    __shared__ uint64_t wbest[16];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const uint32_t lo = __shfl_xor((uint32_t)best, o, 64), hi = __shfl_xor((uint32_t)(best >> 32), o, 64);
+        const uint64_t other = ((uint64_t)hi << 32) | lo;
+        best = other > best ? other : best;
+    }
+    if (lane == 0) wbest[w] = best;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 0; i < 16; i++) best = wbest[i] > best ? wbest[i] : best;
+        out[0] = total;
+        out[1] = best ? (int)~(uint32_t)best : 0x7fffffff;  // no nonzero: after the others
+    }
+}
+__global__ __launch_bounds__(1024) void nz_list_kernel(const float* __restrict__ X, int ldx, int n, SpState sp,
+                                                       int* __restrict__ gl) {
+    nz_list_body(blockIdx.x, X, ldx, n, sp, gl);
 }
 
-__global__ __launch_bounds__(256) void sparse_ax_listed_kernel(const float* __restrict__ A, int lda, int n,
-                                                               const float* __restrict__ X, int ldx,
-                                                               float* __restrict__ Z, int ldz, int mp, SpState sp,
-                                                               const int* __restrict__ gl) {
-    const int rb = blockIdx.x, j = blockIdx.y;
+__device__ __forceinline__ void sparse_ax_listed_body(int rb, int j, const float* __restrict__ A, int lda, int n,
+                                                      const float* __restrict__ X, int ldx, float* __restrict__ Z,
+                                                      int ldz, int mp, SpState sp, const int* __restrict__ gl) {
     if (sp.done[0] || !sp.done[kSparseFlags + (j >> 8)] || !sp.active[j]) return;
     const int* L = gl + (long)j * kGatherStride;
     const int cnt = L[0];
@@ -1304,14 +1324,11 @@ __global__ __launch_bounds__(256) void sparse_ax_listed_kernel(const float* __re
     for (int r = 0; r < 4; r++) o[r] = (float)acc[r];
     *(f4*)(Z + (long)j * ldz + row0) = o;
 }
-hipError_t launch_sparse_ax_listed(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z,
-                                   int ldz, int batch, SpState sp, int* gl, hipStream_t st) {
-    if (batch <= 0) return hipSuccess;
-    if (!gl || (long)n * 4 + 32768 > 0xffffffffL) return hipErrorInvalidValue;  // 32-bit offsets into x
-    nz_list_kernel<<<batch, 1024, 0, st>>>(X, ldx, n, sp, gl);
-    sparse_ax_listed_kernel<<<dim3((mp + kSpRows - 1) / kSpRows, batch), 256, 0, st>>>(Acm, lda, n, X, ldx, Z, ldz,
-                                                                                      mp, sp, gl);
-    return hipGetLastError();
+__global__ __launch_bounds__(256) void sparse_ax_listed_kernel(const float* __restrict__ A, int lda, int n,
+                                                               const float* __restrict__ X, int ldx,
+                                                               float* __restrict__ Z, int ldz, int mp, SpState sp,
+                                                               const int* __restrict__ gl) {
+    sparse_ax_listed_body(blockIdx.x, blockIdx.y, A, lda, n, X, ldx, Z, ldz, mp, sp, gl);
 }
 
 // ---------------------------------------------------------------- z epilogue
@@ -2633,6 +2650,63 @@ hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipS
     return hipGetLastError();
 }
 
+// Support-ordered gather (solo solves of at most kOrderMaxBatch SPs).  nz_list_kernel also
+// keys each SP by the index of its largest |x_i|; order_kernel sorts the gathered SPs by that
+// key, so neighbours in the order mostly share their supports, and the ordered gather runs the
+// listed gather's workgroups as (row block, position in the order), remapped so that each XCD
+// runs one contiguous range of that order.  An XCD's concurrent workgroups then gather mostly
+// the same columns of the same row block, and the repeats are served by its L2 rather than
+// from MALL or HBM.  Every SP's sum is the listed gather's, so the bits are the same.
+
+// position i of the order in SP i's list header (int 2): an SP index, -1 past the gathered SPs
+__global__ __launch_bounds__(1024) void order_kernel(int* __restrict__ gl, int batch, SpState sp) {
+    __shared__ uint64_t a[kOrderMaxBatch];
+    for (int j = threadIdx.x; j < kOrderMaxBatch; j += 1024) {
+        uint64_t k = 0;  // padding: sorts last
+        if (j < batch) {
+            const bool g = !sp.done[0] && sp.done[kSparseFlags + (j >> 8)] && sp.active[j];
+            const uint32_t key = g ? (uint32_t)gl[(long)j * kGatherStride + 1] : 0xfffffffeu;
+            k = ~(((uint64_t)key << 32) | (uint32_t)j);  // descending on ~(key, j): ascending on (key, j)
+        }
+        a[j] = k;
+    }
+    __syncthreads();
+    static_assert(kOrderMaxBatch == 4 * 1024, "the sort's register counts");
+    if (batch <= 1024) sort_desc_reg<1024, 1>(a, batch);
+    else if (batch <= 2048) sort_desc_reg<1024, 2>(a, batch);
+    else sort_desc_reg<1024, 4>(a, batch);
+    for (int i = threadIdx.x; i < batch; i += 1024) {
+        const uint64_t k = ~a[i];
+        gl[(long)i * kGatherStride + 2] = (uint32_t)(k >> 32) == 0xfffffffeu ? -1 : (int)(uint32_t)k;
+    }
+}
+
+__global__ __launch_bounds__(256) void sparse_ax_ordered_kernel(const float* __restrict__ A, int lda, int n,
+                                                                const float* __restrict__ X, int ldx,
+                                                                float* __restrict__ Z, int ldz, int mp, SpState sp,
+                                                                const int* __restrict__ gl, int batch) {
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int rb = t / batch, i = t - rb * batch;
+    const int j = gl[(long)i * kGatherStride + 2];
+    if (j < 0) return;
+    sparse_ax_listed_body(rb, j, A, lda, n, X, ldx, Z, ldz, mp, sp, gl);
+}
+
+hipError_t launch_sparse_ax_listed(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z,
+                                   int ldz, int batch, SpState sp, int* gl, hipStream_t st) {
+    if (batch <= 0) return hipSuccess;
+    if (!gl || (long)n * 4 + 32768 > 0xffffffffL) return hipErrorInvalidValue;  // 32-bit offsets into x
+    nz_list_kernel<<<batch, 1024, 0, st>>>(X, ldx, n, sp, gl);
+    const int rbs = (mp + kSpRows - 1) / kSpRows;
+    if (batch <= kOrderMaxBatch) {
+        order_kernel<<<1, 1024, 0, st>>>(gl, batch, sp);
+        sparse_ax_ordered_kernel<<<rbs * batch, 256, 0, st>>>(Acm, lda, n, X, ldx, Z, ldz, mp, sp, gl, batch);
+    } else {
+        sparse_ax_listed_kernel<<<dim3(rbs, batch), 256, 0, st>>>(Acm, lda, n, X, ldx, Z, ldz, mp, sp, gl);
+    }
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- convergence
 // Batch rule (gpu_sparse_PCA_solver.h:176-186, sparse_PCA_solver.h:203-207): error =
 // max_j err_j with NaN skipped by the '<' update; stop iff it > 0 and error < tol.
@@ -2804,11 +2878,21 @@ __global__ __launch_bounds__(256, 2) void fused_fix8_kernel(const FusedJob* __re
 }
 
 // sparse A*X items: (row block, SP) of each job, row block fastest
+// The gathered SPs' nonzero lists (jobs whose workspace holds them), one workgroup per column
+__global__ __launch_bounds__(1024) void fused_nz_list_kernel(const FusedJob* __restrict__ jobs, int njobs) {
+    const int k = find_job<2>(jobs, njobs, blockIdx.x);
+    const FusedJob& J = jobs[k];
+    if (J.gl) nz_list_body(blockIdx.x - J.c0, J.xop.f32, J.np, J.sa.n, J.sp, J.gl);
+}
 __global__ __launch_bounds__(256) void fused_sparse_ax_kernel(const FusedJob* __restrict__ jobs, int njobs) {
     const int k = find_job<3>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
     const int li = blockIdx.x - J.t3, rbs = (J.mp + kSpRows - 1) / kSpRows;
-    sparse_ax_body(li % rbs, li / rbs, J.p2_32, J.mp, J.sa.n, J.xop.f32, J.np, J.Zpart, J.mp, J.mp, J.sp);
+    if (J.gl)
+        sparse_ax_listed_body(li % rbs, li / rbs, J.p2_32, J.mp, J.sa.n, J.xop.f32, J.np, J.Zpart, J.mp, J.mp, J.sp,
+                              J.gl);
+    else
+        sparse_ax_body(li % rbs, li / rbs, J.p2_32, J.mp, J.sa.n, J.xop.f32, J.np, J.Zpart, J.mp, J.mp, J.sp);
 }
 
 __global__ __launch_bounds__(256) void fused_sum_partials_kernel(const FusedJob* __restrict__ jobs, int njobs) {
@@ -2847,6 +2931,7 @@ template <int PREC>
 hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int items3, int items4,
                         int any_fix8, int* all_done, hipStream_t st) {
     fused_gemm_kernel<PREC, 0><<<tiles1, 256, 0, st>>>(jobs, njobs, tiles1);
+    fused_nz_list_kernel<<<cols, 1024, 0, st>>>(jobs, njobs);
     fused_sparse_ax_kernel<<<items3, 256, 0, st>>>(jobs, njobs);
     if (items4 > 0) fused_sum_partials_kernel<<<items4, 256, 0, st>>>(jobs, njobs);
     fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
