@@ -695,13 +695,14 @@ void reference_starting_point(float* out, unsigned n, unsigned s, bool constrain
 }
 
 // Largest nonzero count per iterate at which A*X is gathered over the nonzeros.  The gather
-// reads ~4 B per nonzero and row; the dense product costs 2n flops per row and SP at the
-// form's MFMA rate, so the break-even density falls as the form gets faster: 1/64 for FP32,
-// 1/128 for bf16x3 (at 4096 x 65536, B = 1024: 0.55 ms gathered at 256 nonzeros per SP
-// against 1.05 ms dense), 1/384 for fp16.
+// reads 4 B per nonzero and row (14-17 TB/s at 4096 x 65536, B = 1024, where SPs share
+// columns); the dense product costs 2n flops per row and SP at the form's MFMA rate.  At that
+// shape the FP32 product takes 4.0 ms and the gather 2.6-3.0 ms at 2400-3100 nonzeros per SP,
+// so FP32 gathers up to n/16.  The 16-bit products take 1.07 ms (bf16x3) and 0.47 ms (fp16)
+// against 0.27-0.51 ms gathered at 256 nonzeros per SP: n/128.
 int sparse_cap(const spca_settings& st, int n) {
     if (!st.sparse_products) return 0;
-    const int div = st.precision == SPCA_PREC_FP32 ? 64 : st.precision == SPCA_PREC_BF16X3 ? 128 : 384;
+    const int div = st.precision == SPCA_PREC_FP32 ? 16 : 128;
     return std::max(1, n / div);
 }
 

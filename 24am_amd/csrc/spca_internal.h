@@ -259,7 +259,7 @@ constexpr int kSparseRows = 1024;  // rows per workgroup of the sparse product
 // Gathered A*X over per-SP nonzero lists (solo solves): each gathered SP's list holds a count
 // (int 0), then up to kGatherList ascending indices (from int kGatherHead) and their values (float
 // bits, kGatherList further on).  An SP with more nonzeros is gathered by scanning x instead.
-constexpr int kGatherList = 1024;
+constexpr int kGatherList = 4096;
 constexpr int kGatherHead = 64;
 constexpr int kGatherStride = kGatherHead + 2 * kGatherList;
 // Solo solves of at most kOrderMaxBatch SPs gather in the order of each SP's largest |x_i|

@@ -1112,14 +1112,15 @@ hipError_t launch_gemm_fix8(const int8_t* P, const float* qscale, long ldp, cons
 // of those groups stand down.
 // The FP64 sum in a fixed order is deterministic and more accurate than any product form.
 constexpr int kSpRows = kSparseRows;  // 4 rows per thread
+// idxs, vals: LDS of the calling kernel, kSlabIndices entries each
+constexpr int kSlabIndices = 2048;
 __device__ __forceinline__ void sparse_ax_body(int rb, int j, const float* __restrict__ A, int lda, int n,
                                                const float* __restrict__ X, int ldx, float* __restrict__ Z, int ldz,
-                                               int mp, SpState sp) {
+                                               int mp, SpState sp, int* idxs, float* vals) {
     if (sp.done[0] || !sp.done[kSparseFlags + (j >> 8)] || !sp.active[j]) return;
     constexpr int kE = 8;                 // x entries per thread per slab, loads in flight
     constexpr int kSlab = 256 * kE;       // indices per slab
-    __shared__ int idxs[kSlab];
-    __shared__ float vals[kSlab];
+    static_assert(kSlab == kSlabIndices, "slab size");
     __shared__ int wcount[kE][4];
     const __amdgpu_buffer_rsrc_t xr = col_rsrc(X + (long)j * ldx, (uint32_t)n * 4u);  // reads 0 past n
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -1192,7 +1193,9 @@ __device__ __forceinline__ void sparse_ax_body(int rb, int j, const float* __res
 __global__ __launch_bounds__(256) void sparse_ax_kernel(const float* __restrict__ A, int lda, int n,
                                                         const float* __restrict__ X, int ldx, float* __restrict__ Z,
                                                         int ldz, int mp, SpState sp) {
-    sparse_ax_body(blockIdx.x, blockIdx.y, A, lda, n, X, ldx, Z, ldz, mp, sp);
+    __shared__ int idxs[kSlabIndices];
+    __shared__ float vals[kSlabIndices];
+    sparse_ax_body(blockIdx.x, blockIdx.y, A, lda, n, X, ldx, Z, ldz, mp, sp, idxs, vals);
 }
 hipError_t launch_sparse_ax(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z, int ldz,
                             int batch, SpState sp, hipStream_t st) {
@@ -1280,18 +1283,19 @@ __global__ __launch_bounds__(1024) void nz_list_kernel(const float* __restrict__
     nz_list_body(blockIdx.x, X, ldx, n, sp, gl);
 }
 
+// sidx, sval: LDS of the calling kernel, kGatherList entries each (shared with the slab gather)
 __device__ __forceinline__ void sparse_ax_listed_body(int rb, int j, const float* __restrict__ A, int lda, int n,
                                                       const float* __restrict__ X, int ldx, float* __restrict__ Z,
-                                                      int ldz, int mp, SpState sp, const int* __restrict__ gl) {
+                                                      int ldz, int mp, SpState sp, const int* __restrict__ gl,
+                                                      int* sidx, float* sval) {
+    static_assert(kGatherList >= kSlabIndices, "the slab gather runs in the list's LDS");
     if (sp.done[0] || !sp.done[kSparseFlags + (j >> 8)] || !sp.active[j]) return;
     const int* L = gl + (long)j * kGatherStride;
     const int cnt = L[0];
     if (cnt > kGatherList) {  // longer than the list: scan x as the slab gather does
-        sparse_ax_body(rb, j, A, lda, n, X, ldx, Z, ldz, mp, sp);
+        sparse_ax_body(rb, j, A, lda, n, X, ldx, Z, ldz, mp, sp, sidx, sval);
         return;
     }
-    __shared__ int sidx[kGatherList];
-    __shared__ float sval[kGatherList];
     for (int t = threadIdx.x; t < cnt; t += 256) {
         sidx[t] = L[kGatherHead + t];
         sval[t] = __int_as_float(L[kGatherHead + kGatherList + t]);
@@ -1328,7 +1332,9 @@ __global__ __launch_bounds__(256) void sparse_ax_listed_kernel(const float* __re
                                                                const float* __restrict__ X, int ldx,
                                                                float* __restrict__ Z, int ldz, int mp, SpState sp,
                                                                const int* __restrict__ gl) {
-    sparse_ax_listed_body(blockIdx.x, blockIdx.y, A, lda, n, X, ldx, Z, ldz, mp, sp, gl);
+    __shared__ int sidx[kGatherList];
+    __shared__ float sval[kGatherList];
+    sparse_ax_listed_body(blockIdx.x, blockIdx.y, A, lda, n, X, ldx, Z, ldz, mp, sp, gl, sidx, sval);
 }
 
 // ---------------------------------------------------------------- z epilogue
@@ -2689,7 +2695,9 @@ __global__ __launch_bounds__(256) void sparse_ax_ordered_kernel(const float* __r
     const int rb = t / batch, i = t - rb * batch;
     const int j = gl[(long)i * kGatherStride + 2];
     if (j < 0) return;
-    sparse_ax_listed_body(rb, j, A, lda, n, X, ldx, Z, ldz, mp, sp, gl);
+    __shared__ int sidx[kGatherList];
+    __shared__ float sval[kGatherList];
+    sparse_ax_listed_body(rb, j, A, lda, n, X, ldx, Z, ldz, mp, sp, gl, sidx, sval);
 }
 
 hipError_t launch_sparse_ax_listed(const float* Acm, int lda, int mp, int n, const float* X, int ldx, float* Z,
@@ -2888,11 +2896,14 @@ __global__ __launch_bounds__(256) void fused_sparse_ax_kernel(const FusedJob* __
     const int k = find_job<3>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
     const int li = blockIdx.x - J.t3, rbs = (J.mp + kSpRows - 1) / kSpRows;
+    __shared__ int sidx[kGatherList];
+    __shared__ float sval[kGatherList];
     if (J.gl)
         sparse_ax_listed_body(li % rbs, li / rbs, J.p2_32, J.mp, J.sa.n, J.xop.f32, J.np, J.Zpart, J.mp, J.mp, J.sp,
-                              J.gl);
+                              J.gl, sidx, sval);
     else
-        sparse_ax_body(li % rbs, li / rbs, J.p2_32, J.mp, J.sa.n, J.xop.f32, J.np, J.Zpart, J.mp, J.mp, J.sp);
+        sparse_ax_body(li % rbs, li / rbs, J.p2_32, J.mp, J.sa.n, J.xop.f32, J.np, J.Zpart, J.mp, J.mp, J.sp, sidx,
+                       sval);
 }
 
 __global__ __launch_bounds__(256) void fused_sum_partials_kernel(const FusedJob* __restrict__ jobs, int njobs) {

@@ -109,8 +109,8 @@ typedef struct spca_settings {
     uint64_t seed;            /* key of SPCA_START_GAUSSIAN starting points */
     int sparse_products;      /* 1 (default): A*X is gathered over the nonzeros of the iterates
                                  (FP64 accumulation) for each group of 256 SPs of a batch once
-                                 every iterating SP of the group has at most n/64 (FP32), n/128
-                                 (bf16x3) or n/384 (fp16) of them; 0: always the dense product
+                                 every iterating SP of the group has at most n/16 (FP32) or
+                                 n/128 (bf16x3, fp16) of them; 0: always the dense product
                                  of `precision`; 2: as 1, and in batch mode (use_otf = 0) on a
                                  dense matrix without deflation the other groups' A*X is the FP32
                                  product over U, the union of their iterates' supports, whenever
