@@ -942,23 +942,28 @@ __device__ __forceinline__ void fix8_stage(const uint8_t* sb, i4 (&accA)[4][4], 
         b[j] = __builtin_bit_cast(i4, v);
         b64[j] = __builtin_bit_cast(i4, v64);
     }
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        i4 a[4];
-        __builtin_amdgcn_sched_barrier(0);  // one piece's fragments at a time
+    // piece p + 1's fragments are read while piece p's MFMAs run (two fragment sets)
+    i4 a[2][4];
+    auto frags = [&](int p, i4 (&d)[4]) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int r = wm * 64 + i * 16 + (lane & 15);
-            a[i] = __builtin_bit_cast(i4, *(const u4*)(sb + p * kFixPart + fix8_lds(r, fc)));
+            d[i] = __builtin_bit_cast(i4, *(const u4*)(sb + p * kFixPart + fix8_lds(r, fc)));
         }
+    };
+    frags(0, a[0]);
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+        if (p < 3) frags(p + 1, a[(p + 1) & 1]);
+        const i4(&cur)[4] = a[p & 1];
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                if (p == 0) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b64[j], accA[i][j], 0, 0, 0);
-                else if (p == 1) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], accA[i][j], 0, 0, 0);
-                else if (p == 2) accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b64[j], accB[i][j], 0, 0, 0);
-                else accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(a[i], b[j], accB[i][j], 0, 0, 0);
+                if (p == 0) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cur[i], b64[j], accA[i][j], 0, 0, 0);
+                else if (p == 1) accA[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cur[i], b[j], accA[i][j], 0, 0, 0);
+                else if (p == 2) accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cur[i], b64[j], accB[i][j], 0, 0, 0);
+                else accB[i][j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(cur[i], b[j], accB[i][j], 0, 0, 0);
             }
     }
 }
