@@ -73,8 +73,11 @@ struct Workspace {
     uint16_t* Zlo = nullptr;
     float* V = nullptr;
     float* fstate = nullptr;  // val, fz, err, errmax, thr  (kStateFloats x bp floats)
-    int* istate = nullptr;    // iters, nnz, active, nkept (bp ints each), done (state_flag_ints(bp))
+    int* istate = nullptr;    // iters, nnz, active, nkept, lmax, lretry (bp ints each),
+                              // done (state_flag_ints(bp))
     int* kept = nullptr;      // kListMax indices per SP: the nonzeros a list step wrote
+    int* lcount = nullptr;                 // step_chunks(np) ints per SP: the chunk scans' entry counts
+    unsigned long long* lstage = nullptr;  // step_chunks(np) * kStageCap entries per SP: their entries
     int* gl = nullptr;        // kGatherStride ints per SP: the gathered A*X's nonzero lists (null: scan x)
     double* ev = nullptr;
     double* corr = nullptr;  // per-SP centring scalars of sparse products (4 x bp)
@@ -505,7 +508,7 @@ int make_generated(spca_ctx ctx, int m, int n, uint64_t seed, int nspikes, const
 void free_ws(spca_ctx ctx, Workspace& w, hipStream_t s) {
     (void)hipStreamSynchronize(s);
     void* ptrs[] = {w.X,      w.Xhi,   w.Xlo,    w.Zpart, w.Z,  w.Zhi, w.Zlo, w.V,    w.fstate,
-                    w.istate, w.kept,  w.gl,     w.ev,    w.corr,  w.uflag, w.uidx, w.ustat, w.Au, w.Xu};
+                    w.istate, w.kept,  w.lcount, w.lstage, w.gl, w.ev,   w.corr,  w.uflag, w.uidx, w.ustat, w.Au, w.Xu};
     for (void* p : ptrs) raw_free(ctx, p);
     w = Workspace();
 }
@@ -516,6 +519,7 @@ uint64_t workspace_bytes(int np, int mp, int bp, int splits) {
     const uint64_t xs = (uint64_t)np * bp, zs = (uint64_t)mp * bp;
     return xs * (4 + 2 + 2 + 4) + zs * splits * 4 + zs * (4 + 2 + 2) + (uint64_t)kStateFloats * bp * 4 +
            ((uint64_t)kStateInts * bp + state_flag_ints(bp)) * 4 + (uint64_t)bp * kListMax * 4 +
+           (uint64_t)bp * step_chunks(np) * (4 + 8 * kStageCap) +
            (uint64_t)bp * kGatherStride * 4 + (uint64_t)bp * 8 * 5;
 }
 
@@ -536,6 +540,8 @@ int ensure_ws(spca_ctx ctx, Workspace& w, hipStream_t s, int np, int mp, int bp,
         (rc = dev_alloc(ctx, &w.fstate, (size_t)kStateFloats * bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.istate, (size_t)kStateInts * bp + state_flag_ints(bp), &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.kept, (size_t)kListMax * bp, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.lcount, (size_t)step_chunks(np) * bp, &w.bytes, err)) ||
+        (rc = dev_alloc(ctx, &w.lstage, (size_t)step_chunks(np) * kStageCap * bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.gl, (size_t)kGatherStride * bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.ev, (size_t)bp, &w.bytes, err)) ||
         (rc = dev_alloc(ctx, &w.corr, (size_t)4 * bp, &w.bytes, err))) {
@@ -612,6 +618,8 @@ bool carve_ws(Workspace& w, FixedScratch& sc, int np, int mp, int bp, int splits
     w.fstate = sc.take<float>((size_t)kStateFloats * bp);
     w.istate = sc.take<int>((size_t)kStateInts * bp + state_flag_ints(bp));
     w.kept = sc.take<int>((size_t)kListMax * bp);
+    w.lcount = sc.take<int>((size_t)step_chunks(np) * bp);
+    w.lstage = sc.take<unsigned long long>((size_t)step_chunks(np) * kStageCap * bp);
     w.ev = sc.take<double>((size_t)bp);
     w.corr = sc.take<double>((size_t)4 * bp);
     *e = hipSuccess;
@@ -661,8 +669,12 @@ SpState state_of(Workspace& w) {
     s.nnz = w.istate + w.bp;
     s.active = w.istate + 2 * w.bp;
     s.nkept = w.istate + 3 * w.bp;
+    s.lmax = (unsigned*)(w.istate + 4 * w.bp);
+    s.lretry = w.istate + 5 * w.bp;
+    s.lcount = w.lcount;
     s.done = w.istate + kStateInts * w.bp;
     s.kept = w.kept;
+    s.lstage = w.lstage;
     return s;
 }
 
@@ -1741,7 +1753,7 @@ int fused_member_init(spca_ctx ctx, Engine& E, Matrix* M, const spca_settings* s
 
 // Work-item offsets of a fused group: each job's tiles and columns follow the previous job's.
 struct FusedTotals {
-    int t1 = 0, t2 = 0, c0 = 0, t3 = 0, t4 = 0, any_fix8 = 0;
+    int t1 = 0, t2 = 0, c0 = 0, t3 = 0, t4 = 0, t5 = 0, any_fix8 = 0;
 };
 
 // The job-table entry of one member at the current offsets, which it then advances.
@@ -1802,6 +1814,9 @@ void fused_entry(FusedRun& r, int precision, FusedTotals& T, FusedJob& f) {
     f.t4 = T.t4;
     f.psum = f.splits > 1 && r.st.batch <= 256;
     if (f.psum) T.t4 += (f.mp + 255) / 256 * r.st.batch;
+    // the chunk scans of constrained steps: (SP, chunk) items
+    f.t5 = T.t5;
+    if (step_kind(r.st.formulation, r.st.constraint) < 2) T.t5 += step_chunks(M->n) * r.st.batch;
 }
 
 // A fused member's results once its state says done: the best SP, the final normalisation,
@@ -1871,7 +1886,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     std::vector<FusedJob> tab(njobs);
     FusedTotals T;
     for (int k = 0; k < njobs; k++) fused_entry(run[k], precision, T, tab[k]);
-    const int t1 = T.t1, t2 = T.t2, c0 = T.c0, t3 = T.t3, t4 = T.t4, any_fix8 = T.any_fix8;
+    const int t1 = T.t1, t2 = T.t2, c0 = T.c0, t3 = T.t3, t4 = T.t4, t5 = T.t5, any_fix8 = T.any_fix8;
     if ((rc = dev_alloc(ctx, &d_jobs, (size_t)njobs, &tb)) || (rc = dev_alloc(ctx, &d_done, 4, &tb)))
         return cleanup(rc);
     if (hipMemcpyAsync(d_jobs, tab.data(), sizeof(FusedJob) * njobs, hipMemcpyHostToDevice, s) != hipSuccess ||
@@ -1881,7 +1896,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
     if (flags & SPCA_FUSED_GRAPH) {
         hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
         for (int i = 0; i < chunk && e == hipSuccess; i++)
-            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, t4, any_fix8, d_done, s);
+            e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, t4, t5, any_fix8, d_done, s);
         hipError_t e2 = hipStreamEndCapture(s, &graph);
         if (e == hipSuccess) e = e2;
         if (e == hipSuccess) e = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
@@ -1905,7 +1920,7 @@ int fused_impl(spca_ctx ctx, Engine& E, spca_fused_job* jobs, int njobs, unsigne
             e = hipGraphLaunch(gexec, s);
         else
             for (int i = 0; i < chunk && e == hipSuccess; i++)
-                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, t4, any_fix8, d_done, s);
+                e = launch_fused_iteration(precision, d_jobs, njobs, t1, t2, c0, t3, t4, t5, any_fix8, d_done, s);
         hipEvent_t ev;
         if (e == hipSuccess && !new_event(&ev)) e = hipErrorOutOfMemory;
         if (e == hipSuccess) e = hipEventRecord(ev, s);
@@ -2178,7 +2193,7 @@ void run_group(spca_ctx ctx, spca_server* S, Engine& E, Job* first, bool lane) {
         for (Member* m : mem) chunk = std::min(chunk, std::max(1, m->r.st.check_every));
         hipError_t e = hipSuccess;
         for (int i = 0; i < chunk && e == hipSuccess; i++)
-            e = launch_fused_iteration(precision, d_jobs, (int)mem.size(), T.t1, T.t2, T.c0, T.t3, T.t4, T.any_fix8,
+            e = launch_fused_iteration(precision, d_jobs, (int)mem.size(), T.t1, T.t2, T.c0, T.t3, T.t4, T.t5, T.any_fix8,
                                        d_done, s);
         for (size_t k = 0; k < mem.size() && e == hipSuccess; k++) {
             e = hipEventRecord(mem[k]->ev1, s);

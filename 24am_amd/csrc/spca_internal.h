@@ -18,11 +18,22 @@ constexpr int kGemmBM = 128;
 constexpr int kGemmBN = 128;
 constexpr int kGemmBK = 32;
 
-// Select/formulation-step kernel: one workgroup of kSelThreads per starting point; at 512
-// threads two workgroups share a CU, so one's sort and scan phases overlap the other's
-// column passes (measured 4-12% faster than 1024 threads on the constrained formulations).
-constexpr int kSelThreads = 512;
+// Select/formulation-step kernel: workgroups of kSelThreads; four share a CU (LDS about 37 KiB
+// each), so the 1024 SPs of a large batch are all resident at once and one's sort and scan
+// phases overlap the others' column reads.  A constrained step whose previous threshold left a
+// list bound has its column scanned first, in chunks of kStepChunk elements, one workgroup per
+// chunk.
+constexpr int kSelThreads = 256;
 constexpr int kSelDepth = 16;  // loads in flight per thread in each of the step kernel's column passes
+constexpr int kStepScanRows = 8;  // 16-byte loads per thread of one chunk scan
+constexpr int kStepChunk = kSelThreads * 4 * kStepScanRows;
+constexpr int kStageCap = 1024;   // staged list entries per chunk (more: the step runs the column passes)
+__host__ __device__ inline int step_chunks(int ldv) { return (ldv + kStepChunk - 1) / kStepChunk; }
+// Step kind of a formulation: 0 L0 constraint (also the L1 constraint with s <= 1, which admits
+// only 1-sparse x: the top-1 hard threshold), 1 L1 constraint, 2 L0 penalty, 3 L1 penalty.
+__host__ __device__ inline int step_kind(int f, unsigned s) {
+    return f >= 6 ? 3 : f >= 4 ? 2 : (f >= 2 && s > 1) ? 1 : 0;
+}
 
 inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 constexpr int cmax(int a, int b) { return a > b ? a : b; }
@@ -69,10 +80,18 @@ struct SpState {
     int* nkept;      // constrained: nonzeros the last step wrote through its candidate list, whose
                      // indices are kept[j * kListMax ..] (-1: the last step wrote the whole column)
     int* kept;       // kListMax indices per SP
+    // the chunk scans of a constrained step: max |v| as float bits (zero between steps; the step
+    // reads and clears it), and per chunk c of SP j (step_chunks(ldv) chunks per SP) its entry
+    // count lcount[j * chunks + c] and its entries lstage[(j * chunks + c) * kStageCap ..]
+    unsigned* lmax;
+    int* lretry;  // 1: the step's first round found the list too long; the second round rescans
+                  // with the higher bound it left in thr
+    int* lcount;
+    unsigned long long* lstage;
 };
-// per-SP float state: val, fz, err, errmax, thr; int state: iters, nnz, active, nkept
+// per-SP float state: val, fz, err, errmax, thr; int state: iters, nnz, active, nkept, lmax, lretry
 constexpr int kStateFloats = 5;
-constexpr int kStateInts = 4;
+constexpr int kStateInts = 6;
 // Most entries of the step kernel's candidate list (a power of two, for its sort), and so of the
 // nonzeros a step written through the list can leave.
 constexpr int kListMax = 4096;
@@ -163,6 +182,7 @@ struct FusedJob {
     int nb1, nb2;           // FP32 A*X / A^T*Z in the narrow 128 x nb tile (32 or 64; 0: 128 x 128 tiles)
     int t4;                 // first (row block, SP) item of the partial-sum kernel
     int psum;               // 1: the partial-sum kernel adds the K splits into partial 0 before prep_z
+    int t5;                 // first (SP, chunk) item of the constrained steps' chunk scans
     int* gl;                // the gathered A*X's nonzero lists (kGatherStride ints per SP; null: scan x)
 };
 
@@ -250,7 +270,7 @@ hipError_t launch_plant(float* A, int m, int mp, const int* idx, const float* co
 // for FP32 jobs of at most 64 SPs); per job the same tile order, K split, sum order and
 // kernel bodies as a solo solve, so a job's FP32 results do not depend on the other jobs.
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
-                                  int items3, int items4, int any_fix8, int* all_done, hipStream_t st);
+                                  int items3, int items4, int items5, int any_fix8, int* all_done, hipStream_t st);
 
 // z_j = A x_j over the nonzeros of each active x_j (FP64 accumulation, fixed order) into Z
 // (ld ldz), for the groups of 256 SPs whose sparse flag (sp.done[kSparseFlags + g]) is set; the

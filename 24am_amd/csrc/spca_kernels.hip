@@ -1620,30 +1620,57 @@ hipError_t launch_union_gather(const float* Acm, int lda, int mp, const float* X
 }
 
 // ---------------------------------------------------------------- formulation step
-// Exact selection without sorting.  A first pass bins |v| linearly over [0, max] (2048
-// bins, LDS histogram), which spreads the LDS atomics; the bin holding the answer is then
-// resolved exactly by four 8-bit radix passes over the IEEE bits of |v| (monotone for
-// non-negative floats), restricted to that bin's elements.  Counts are integers and the
-// sums the lambda search needs are accumulated in 2^-40-scaled fixed point, so every
+// Exact selection without sorting the column.  A first pass bins |v| linearly over [0, max]
+// (kBins bins, LDS histogram); the bin holding the answer is then resolved exactly by sorting
+// its elements, or by four 8-bit radix passes over the IEEE bits of |v| (monotone for
+// non-negative floats) restricted to that bin when it holds too many.  Counts are integers and
+// the sums the lambda search needs are accumulated in 2^-40-scaled fixed point, so every
 // decision is independent of atomic order: results are bitwise reproducible.
 // The column is re-read per pass through a bounded buffer view (coalesced; it stays in
 // L2/MALL), so any n works with one code path.
-constexpr int kBins = 2048;
+constexpr int kBins = 1024;
 constexpr double kFix = 1099511627776.0;  // 2^40: n * 2^40 < 2^63 for n < 2^23
 
 constexpr int kCand = 2048;  // candidates of the answer bin resolved in LDS (else the radix fallback)
 // The previous threshold's list: up to kList elements with |v| at or above a bound L the SP's
 // previous step left: the lower edge of the histogram bin (of the list's, or of the column's)
-// holding place s + s/4 + 32 (L0) or survivors + survivors/8 + 64 (L1), or, after a radix
-// fallback, a margin below the threshold it found.
+// holding place 2s + 64 (L0) or 3/2 survivors + 64 (L1), or, after a radix fallback, a margin
+// below the threshold it found.
 constexpr int kList = kListMax;
 constexpr int kLBins = 256;   // bins of the list's histogram
 static_assert(kLBins == 256, "the list bins take the top 8 bits of the span");
-constexpr int kLCand = 1024;  // entries of the list's answer bin resolved in LDS (more: the column passes)
+constexpr int kLCand = 256;   // entries of the list's answer bin resolved in LDS (more: the column passes)
 constexpr float kListMarginL0 = 0.9f;
 constexpr float kListMarginL1 = 0.95f;
+// The place whose value becomes the next step's list bound, for `keep` elements kept: L0 (keep =
+// s) 2s + 64, capped at 3/4 of the list's room (not below s + s/8 + 64); L1 (keep = the survivors,
+// whose count moves with lambda) keep + keep/8 + 64.  0 (no list: the column passes) past the room.
+__device__ __forceinline__ uint32_t list_rank(uint32_t keep, bool l1c) {
+    const uint32_t tight = keep + keep / 8 + 64;
+    if (tight > (uint32_t)kList) return 0u;
+    return l1c ? tight : max(min(2 * keep + 64, (uint32_t)(3 * kList / 4)), tight);
+}
 
-struct SelShared {
+// LDS of one step workgroup.  Every step kind uses StepSmall (reductions, the exact-tie band);
+// the constrained kinds also the histogram and list arrays of SelShared, under 40 KiB in all, so
+// four 256-thread workgroups share a CU.
+struct StepSmall {
+    double red[3 * (kSelThreads / 64)];
+    int ires[4];
+    uint64_t ures[2];
+    int ncand;
+    int nkeep;
+    int nlc;
+    int wtot[2][kSelThreads / 64];  // per-wave counts (chunk prefixes of the staged list)
+    double dres[4];
+    // exact ties: the re-evaluated band (sorted descending), its value range and keep flags
+    int nband;
+    float band_lo, band_hi;
+    int band_idx[kBandMax];
+    float band_val[kBandMax];
+    int band_keep[kBandMax];
+};
+struct SelShared : StepSmall {
     union {
         struct {
             uint32_t cnt[kBins];
@@ -1660,21 +1687,12 @@ struct SelShared {
             uint64_t lcand[kLCand];  // the list's answer bin, sorted descending
         };
     };
-    double red[3 * (kSelThreads / 64)];
-    int ires[4];
-    uint64_t ures[2];
-    int ncand;
-    int nkeep;
-    int nlc;
-    int wtot[2][kSelThreads / 64];  // per-wave entry counts of the list pass (double-buffered)
-    double dres[4];
-    // exact ties: the re-evaluated band (sorted descending), its value range and keep flags
-    int nband;
-    float band_lo, band_hi;
-    int band_idx[kBandMax];
-    float band_val[kBandMax];
-    int band_keep[kBandMax];
 };
+static_assert(sizeof(SelShared) <= 40 * 1024, "four step workgroups per CU");
+// While the staged list is gathered, ls1 .. lcand hold the chunk counts and places (2 ints per
+// chunk): at most kMaxListChunks chunks (n <= kMaxListChunks * kStepChunk); longer columns take
+// no list.
+constexpr int kMaxListChunks = (int)(2 * sizeof(uint64_t) * kLBins + sizeof(uint64_t) * kLCand) / (2 * (int)sizeof(int));
 
 // Workgroup barrier ordering LDS only: a release/acquire fence on the local address space
 // around s_barrier.  Unlike __syncthreads() (whose fence also covers global memory and so
@@ -1707,7 +1725,7 @@ __device__ __forceinline__ void sort_cx_regs(uint64_t (&x)[R], int k, int w, int
     }
 }
 
-// a[0 .. nc) sorted descending as u64 keys by the whole block (NT threads), nc <= NT * R; a must
+// a[0 .. nc) sorted descending as u64 keys by the whole block (NT threads, R <= 16), nc <= NT * R; a must
 // hold NT * R entries (the tail is padded with 0 keys, which sort last).  A bitonic network over
 // NT * R elements, element i = (w * R + r) * 64 + lane held in register r of lane `lane` of wave
 // w: exchanges across fewer than 64 places are lane shuffles, across 64 .. 64R - 1 places
@@ -1715,6 +1733,7 @@ __device__ __forceinline__ void sort_cx_regs(uint64_t (&x)[R], int k, int w, int
 // (the index is in the low word), so the order is the unique descending one.
 template <int NT, int R>
 __device__ void sort_desc_reg(uint64_t* a, int nc) {
+    static_assert(R == 1 || R == 2 || R == 4 || R == 8 || R == 16, "register exchanges across 1 .. 8 registers");
     constexpr int N = NT * R;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint64_t x[R];
@@ -1738,10 +1757,11 @@ __device__ void sort_desc_reg(uint64_t* a, int nc) {
                     x[r] = keep_hi ? (x[r] > p ? x[r] : p) : (x[r] < p ? x[r] : p);
                 }
             } else if (jj >= 64) {
-                const int t = jj >> 6;
+                const int t = jj >> 6;  // < R <= 16
                 if (t == 1) sort_cx_regs<R, 1>(x, k, w, lane);
                 else if (t == 2) sort_cx_regs<R, 2>(x, k, w, lane);
-                else sort_cx_regs<R, 4>(x, k, w, lane);
+                else if (t == 4) sort_cx_regs<R, 4>(x, k, w, lane);
+                else sort_cx_regs<R, 8>(x, k, w, lane);
             } else {
 #pragma unroll
                 for (int r = 0; r < R; r++) {
@@ -1787,7 +1807,7 @@ __device__ __forceinline__ bool pred_r(double c, double S1, double S2, double la
 // find the largest bin whose lower boundary satisfies pred_r.  Writes ires[0] = bin (or
 // -1), ires[1] = count above it, ures = fixed-point sums above it.
 template <int NB, class Boundary>
-__device__ void scan_hist(SelShared& sh, const uint32_t* cnt, const uint64_t* hs1, const uint64_t* hs2, bool l1c,
+__device__ void scan_hist(StepSmall& sh, const uint32_t* cnt, const uint64_t* hs1, const uint64_t* hs2, bool l1c,
                           uint32_t rank, uint32_t cA, uint64_t s1A, uint64_t s2A, double M, double wd,
                           Boundary boundary) {
     const int lane = threadIdx.x & 63;
@@ -1838,10 +1858,11 @@ __device__ void scan_hist(SelShared& sh, const uint32_t* cnt, const uint64_t* hs
 // The step of one SP (column j of V), specialised by kind (step_kind: 0 L0 constraint, 1 L1
 // constraint, 2 L0 penalty, 3 L1 penalty) and iterate precision, so the per-element code of the
 // column sweeps carries no formulation or precision branches.  sh: the calling kernel's LDS.
-template <int NT, int DEPTH, int KIND, int PREC>
-__device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __restrict__ V, Operand xop, StepArgs a,
-                                          SpState sp) {
+template <int NT, int DEPTH, int KIND, int PREC, class SH>
+__device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict__ V, Operand xop, StepArgs a,
+                                          SpState sp, int round) {
     if (*sp.done || !sp.active[j]) return;
+    if (round && (KIND >= 2 || !sp.lretry[j])) return;  // the second round: only the SPs the first left
     static_assert(NT <= kSelThreads, "SelShared holds kSelThreads / 64 waves");
     static_assert(KIND >= 0 && KIND < 4 && PREC >= 0 && PREC < 3, "step kind and precision");
     constexpr bool kConstrained = KIND < 2;
@@ -1923,14 +1944,16 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
         double norm2 = 0;
         float next_bound = 0.f;  // the next step's list bound (0: no list)
         bool decided_column = false;  // the column passes' histogram set next_bound
-        // The threshold from candidates cand[0..nc) (the answer bin of a histogram) sorted
-        // descending by (|v|, index), with the elements above them counted in cA_hist (L0) /
-        // summed in (cAd, S1A, S2A), both exact.  band_inside(lo, hi): every value in [lo, hi] is
-        // a candidate.  list_mode (the histogram is the previous threshold's list's): a clamped
-        // lambda, whose norm needs the whole column, is left to the column passes; returns false
-        // then, block-uniformly.
+        // The threshold from candidates cand[0..nc) (the answer bin of a histogram, or the whole
+        // previous threshold's list) sorted descending by (|v|, index), with the elements above
+        // them counted in cA_hist (L0) / summed in (cAd, S1A, S2A), both exact.  band_inside(lo,
+        // hi): every value in [lo, hi] is a candidate.  list_mode (the candidates are the list of
+        // every element >= lbound): an L1 trigger below the list (r(lbound) < w over it) or a
+        // clamped lambda, whose norm needs the whole column, is left to the column passes;
+        // returns false then, block-uniformly.  Otherwise the elements just below the candidates
+        // trigger (the histogram scan chose the bin by its lower edge).
         auto resolve = [&](uint64_t* cand, int nc, uint32_t cA_hist, double cAd, double S1A, double S2A,
-                           bool list_mode, bool closed_ok, auto&& band_inside) -> bool {
+                           bool list_mode, bool closed_ok, double lbound, auto&& band_inside) -> bool {
             if (!l1c) {
                 // ---- L0 constrained: tau = the (s - above)-th largest candidate (counting ties)
                 const uint32_t r = a.s - cA_hist;
@@ -2051,6 +2074,9 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
             }
             __syncthreads();
             const double c = sh.dres[0], S1 = sh.dres[1], S2 = sh.dres[2];
+            // no trigger inside the list: the first element below lbound triggers iff r(lbound) >= w
+            // over the list (r grows as the boundary falls); else the survivors reach below it
+            if (list_mode && !sh.ires[3] && !pred_r(c, S1, S2, lbound, wd)) return false;
             const double w2 = wd * wd;
             const double qa = c * (c - w2);
             const double qb = 2.0 * S1 * w2 - 2.0 * c * S1;
@@ -2079,178 +2105,219 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
             return true;
         };
         // ---- the previous threshold's list: from the second iteration on, the SP's threshold
-        // (tau, or lambda) moves little, so one pass collects every element with |v| >= L, the
-        // bound the previous step left a little below its threshold, into LDS (and takes max |v|
-        // on the way).  A histogram of the list then finds the threshold's bin, and only that
-        // bin is sorted.  Exact whenever it decides: the list holds every element >= L, L0 needs
-        // >= s of them (tau >= L then), L1 a trigger boundary at or above L; otherwise the column
-        // passes below run as on a first step, with max |v| known.
+        // (tau, or lambda) moves little, so the elements with |v| >= L, the bound the previous
+        // step left a little below its threshold, were collected by the chunk scans of the previous
+        // launch (step_scan_body), with max |v|.  A histogram of the list finds the threshold's
+        // bin, and only that bin is sorted.  Exact whenever it decides: the list holds every
+        // element >= L, L0 needs >= s of them (tau >= L then), L1 a trigger boundary at or above
+        // L; otherwise the column passes below run as on a first step, with max |v| known.
         float Mcol = 0.f;
         bool have_M = false;
         const float thr_prev = sp.thr[j];
         if (thr_prev > 0.f) {
             const float L = thr_prev;
-            // the entries in a fixed order, (chunk of DEPTH rows, thread, row): a block prefix sum
-            // per chunk places them, so every FP64 sum over the list below has a fixed order
-            float mloc = 0.f;
-            int total = 0;  // entries so far, block-uniform
-            {
-                const int lane = tid & 63, w = tid >> 6;
-                // One chunk of DEPTH rows.  Rows past nq and entries past n read 0 through the
-                // bounded view, and L > 0, so neither is listed and no bound checks are needed.
-                // wtot is double-buffered (buf): a wave reuses one only after the next chunk's
-                // barrier, which every wave passes after reading it.
-                auto chunk = [&](const float (&vv)[DEPTH], int q0, int buf) {
-                    uint32_t in = 0;
+            Mcol = __uint_as_float(sp.lmax[j]);
+            have_M = true;
+            // the staged entries into LDS in chunk order (within a chunk, the scan's order), so the
+            // list's order, and every FP64 sum over it below, is fixed
+            const int nch = (a.n + kStepChunk - 1) / kStepChunk, chunks = step_chunks(a.ldv);
+            const int* lc = sp.lcount + (long)j * chunks;
+            int* rec = (int*)sh.ls1;  // [0, nch): counts; [nch, 2 nch): places (nch <= kMaxListChunks)
+            int over = 0, total = 0;
+            for (int c0 = 0; c0 < nch; c0 += NT) {
+                const int c = c0 + tid;
+                const int k = c < nch ? lc[c] : 0;
+                over |= k > kStageCap;
+                // chunk-order exclusive prefix of the counts: this block of NT chunks' prefix and total
+                const int incl = (int)wave_prefix_u32((uint32_t)k), pre = incl - k;
+                const int wt = __builtin_amdgcn_readlane(incl, 63);
+                const int w = tid >> 6;
+                if ((tid & 63) == 0) sh.wtot[0][w] = wt;
+                __syncthreads();
+                int before = 0, btot = 0;
 #pragma unroll
-                    for (int u = 0; u < DEPTH; u++) {
-                        const float av = fabsf(vv[u]);
-                        mloc = fmaxf(mloc, av);
-                        in |= (av >= L ? 1u : 0u) << u;
+                for (int q = 0; q < NT / 64; q++) {
+                    before += q < w ? sh.wtot[0][q] : 0;
+                    btot += sh.wtot[0][q];
+                }
+                if (c < nch) {
+                    rec[c] = k;
+                    rec[nch + c] = total + before + pre;
+                }
+                total += btot;
+                __syncthreads();  // wtot is reused by the next block of chunks
+            }
+            over = __syncthreads_or(over);  // also publishes rec
+            const int cnt = over ? kList + 1 : total;
+            list_cnt = cnt;
+            const unsigned long long* stg = sp.lstage + (long)j * chunks * kStageCap;
+            if (!l1c && !round && cnt > kList) {
+                // L0, too long (the column's bulk grew past the bound): the staged keys -- every entry
+                // of the chunks that fit their slots, the first kStageCap of any that did not -- by the
+                // level-0 bins over [L, max |v|] (as below), and the second round rescans with the
+                // edge of the bin holding place list_rank(s) as its bound.
+                const uint32_t Lb = __float_as_uint(L);
+                const int shift0 = max(0, 24 - __clz(__float_as_uint(Mcol) - Lb));
+                for (int b = tid; b < kLBins; b += NT) sh.lcnt[b] = 0;
+                __syncthreads();
+                constexpr int P = 16;
+                for (int f0 = 0; f0 < nch * kStageCap; f0 += NT * P) {
+                    unsigned long long r[P];
+#pragma unroll
+                    for (int u = 0; u < P; u++) {
+                        const int f = f0 + u * NT + tid, c = f / kStageCap, e = f - c * kStageCap;
+                        r[u] = (c < nch && e < rec[c]) ? stg[f] : 0ull;
                     }
-                    const int c = __popc(in);  // at most DEPTH <= 16: five bits
-                    // the wave's exclusive prefix of c and its total, from one ballot per bit of c
-                    int excl = 0, wsum = 0;
 #pragma unroll
-                    for (int bit = 0; bit < 5; bit++) {
-                        const uint64_t m = __ballot((c >> bit) & 1);
-                        excl += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
-                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u))
-                                << bit;
-                        wsum += __popcll(m) << bit;
+                    for (int u = 0; u < P; u++) {
+                        const uint32_t kb = (uint32_t)(r[u] >> 32);
+                        if (r[u]) atomicAdd(&sh.lcnt[(kb - Lb) >> shift0], 1u);  // staged keys are >= L
                     }
-                    if (lane == 0) sh.wtot[buf][w] = wsum;
-                    lds_barrier();  // the next chunk's loads stay in flight across it
-                    int before = 0, chunk_tot = 0;
-#pragma unroll
-                    for (int k = 0; k < NT / 64; k++) {
-                        const int t = sh.wtot[buf][k];
-                        before += k < w ? t : 0;
-                        chunk_tot += t;
+                }
+                __syncthreads();
+                const uint32_t want = list_rank(a.s, false);
+                if (tid < 64 && want)
+                    scan_hist<kLBins>(sh, sh.lcnt, sh.ls1, sh.ls2, false, want, 0u, 0ull, 0ull, 0.0, 0.0,
+                                      [&](int, double&) { return 0; });
+                __syncthreads();
+                const int rb = want ? sh.ires[0] : -1;
+                if (rb > 0) {
+                    if (tid == 0) {
+                        sp.thr[j] = __uint_as_float(Lb + ((uint32_t)rb << shift0));
+                        sp.lretry[j] = 1;
+                        sp.lmax[j] = 0u;
                     }
-                    int slot = total + before + excl;
+                    return;  // block-uniform
+                }
+            }
+            if (cnt > 0 && cnt <= kList) {
+                // entry e of chunk c at stage slot c * kStageCap + e; the flat slots are taken 16 per
+                // thread at a time, their loads all issued before the stores
+                const unsigned long long* st = stg;
+                constexpr int P = 16;
+                for (int f0 = 0; f0 < nch * kStageCap; f0 += NT * P) {
+                    unsigned long long r[P];
 #pragma unroll
-                    for (int u = 0; u < DEPTH; u++)
-                        if ((in >> u) & 1u) {
-                            if (slot < kList)
-                                sh.list[slot] = ((uint64_t)__float_as_uint(fabsf(vv[u])) << 32) |
-                                                (uint32_t)(tid + (q0 + u) * NT) | (vv[u] < 0.f ? 0x80000000u : 0u);
-                            slot++;
-                        }
-                    total += chunk_tot;
-                };
-                // rows DEPTH at a time, alternating between two register buffers, so the next
-                // chunk's loads are in flight while this one is processed (unconditionally: rows
-                // past the column read 0 without a memory access, and a conditional load would
-                // make the compiler wait for it before the current chunk)
-                auto load_rows = [&](float (&vv)[DEPTH], int q0) {
-#pragma unroll
-                    for (int u = 0; u < DEPTH; u++) vv[u] = LDV(q0 + u);
-                };
-                if constexpr (!l1c) {
-                    float ba[DEPTH], bb[DEPTH];
-                    load_rows(ba, 0);
-                    for (int q0 = 0; q0 < nq; q0 += 2 * DEPTH) {
-                        load_rows(bb, q0 + DEPTH);
-                        chunk(ba, q0, 0);
-                        if (q0 + DEPTH >= nq) break;
-                        load_rows(ba, q0 + 2 * DEPTH);
-                        chunk(bb, q0 + DEPTH, 1);
+                    for (int u = 0; u < P; u++) {
+                        const int f = f0 + u * NT + tid, c = f / kStageCap, e = f - c * kStageCap;
+                        r[u] = (c < nch && e < rec[c]) ? st[f] : 0ull;
                     }
-                } else {
-                    // the L1 constraint keeps more registers live; a second buffer makes the
-                    // kernel spill (measured slower): one buffer
-                    int buf = 0;
-                    for (int q0 = 0; q0 < nq; q0 += DEPTH, buf ^= 1) {
-                        float ba[DEPTH];
-                        load_rows(ba, q0);
-                        chunk(ba, q0, buf);
+#pragma unroll
+                    for (int u = 0; u < P; u++) {
+                        const int f = f0 + u * NT + tid, c = f / kStageCap, e = f - c * kStageCap;
+                        if (c < nch && e < rec[c]) sh.list[rec[nch + c] + e] = r[u];
                     }
                 }
             }
-            Mcol = block_max<NT>(mloc, sh.red);  // its barriers also publish the list
-            have_M = true;
-            const int cnt = total;
-            list_cnt = cnt;
+            __syncthreads();
+            if (tid == 0) {  // the next scans take the max afresh
+                sp.lmax[j] = 0u;
+                if (round) sp.lretry[j] = 0;
+            }
             if (cnt > 0 && cnt <= kList && (l1c || cnt >= (int)a.s)) {
-                for (int b = tid; b < kLBins; b += NT) {
-                    sh.lcnt[b] = 0;
-                    sh.ls1[b] = 0;
-                    sh.ls2[b] = 0;
-                }
-                if (tid == 0) sh.nlc = 0;
-                __syncthreads();
-                // bins over the IEEE bits of |v| - bits of L (monotone for non-negative floats):
-                // nearly constant relative width, so the bins near L, where the threshold lies and
-                // the entries crowd, stay small; every bin edge is an exact float
-                const uint32_t Lb = __float_as_uint(L);
-                const uint32_t span = __float_as_uint(Mcol) - Lb;  // Mcol >= L: the list is not empty
-                const int lshift = max(0, 24 - __clz(span));       // span >> lshift < kLBins = 2^8
+                // Histogram levels over the IEEE bits of |v| (monotone for non-negative floats),
+                // kLBins bins each: level 0 spans [L, max |v|], each further level the answer bin of
+                // the level before, 8 bits finer, until the answer bin holds at most kLCand entries.
+                // Bins of nearly constant relative width keep the bins near L, where the threshold
+                // lies and the entries crowd, small; every bin edge is an exact float.  Counts and
+                // the fixed-point sums above the range carry from level to level.
                 const double invM = 1.0 / (double)Mcol;
-                auto lbin = [&](float av) { return (int)((__float_as_uint(av) - Lb) >> lshift); };  // av >= L
-                auto ledge = [&](int b) { return __uint_as_float(Lb + ((uint32_t)b << lshift)); };
-                for (int e = tid; e < cnt; e += NT) {
-                    const float av = __uint_as_float((uint32_t)(sh.list[e] >> 32));
-                    const int b = lbin(av);
-                    atomicAdd(&sh.lcnt[b], 1u);
-                    if (l1c) {
-                        const double w = (double)av * invM;
-                        atomicAdd((unsigned long long*)&sh.ls1[b], (unsigned long long)(w * kFix + 0.5));
-                        atomicAdd((unsigned long long*)&sh.ls2[b], (unsigned long long)(w * w * kFix + 0.5));
+                const uint32_t Lb = __float_as_uint(L);
+                const int shift0 = max(0, 24 - __clz(__float_as_uint(Mcol) - Lb));  // span >> shift0 < 2^8
+                auto hist_level = [&](uint32_t base, int shift, uint32_t lo, uint32_t hi, bool sums) {
+                    for (int b = tid; b < kLBins; b += NT) {
+                        sh.lcnt[b] = 0;
+                        sh.ls1[b] = 0;
+                        sh.ls2[b] = 0;
                     }
+                    __syncthreads();
+                    for (int e = tid; e < cnt; e += NT) {
+                        const uint32_t kb = (uint32_t)(sh.list[e] >> 32);
+                        if (kb < lo || kb >= hi) continue;
+                        const int bb = (int)((kb - base) >> shift);
+                        atomicAdd(&sh.lcnt[bb], 1u);
+                        if (sums) {
+                            const double w = (double)__uint_as_float(kb) * invM;
+                            atomicAdd((unsigned long long*)&sh.ls1[bb], (unsigned long long)(w * kFix + 0.5));
+                            atomicAdd((unsigned long long*)&sh.ls2[bb], (unsigned long long)(w * w * kFix + 0.5));
+                        }
+                    }
+                    __syncthreads();
+                };
+                uint32_t base = Lb, lo = Lb, hi = 0xFFFFFFFFu, rank = a.s, cA = 0;
+                uint64_t s1A = 0, s2A = 0;
+                int shift = shift0, lb = -1, level = 0;
+                for (;; level++) {
+                    hist_level(base, shift, lo, hi, l1c);
+                    if (tid < 64)
+                        scan_hist<kLBins>(sh, sh.lcnt, sh.ls1, sh.ls2, l1c, rank, cA, s1A, s2A, (double)Mcol, wd,
+                                          [&](int bb, double& l) {
+                                              l = (double)__uint_as_float(base + ((uint32_t)bb << shift));
+                                              return 0;
+                                          });
+                    __syncthreads();
+                    lb = sh.ires[0];
+                    if (lb < 0) break;
+                    const uint32_t inb = sh.lcnt[lb];
+                    cA += (uint32_t)sh.ires[1];
+                    s1A += sh.ures[0];
+                    s2A += sh.ures[1];
+                    rank -= (uint32_t)sh.ires[1];
+                    lo = base + ((uint32_t)lb << shift);
+                    hi = lo + (1u << shift);
+                    if (inb <= (uint32_t)kLCand || shift == 0) break;
+                    base = lo;
+                    shift = max(0, shift - 8);
+                    __syncthreads();  // every wave has read this level's bin before the next level
                 }
-                __syncthreads();
-                if (tid < 64)
-                    scan_hist<kLBins>(sh, sh.lcnt, sh.ls1, sh.ls2, l1c, a.s, 0u, 0ull, 0ull, (double)Mcol, wd,
-                                      [&](int b, double& l) {
-                                          l = (double)ledge(b);
-                                          return 0;
-                                      });
-                __syncthreads();
-                const int lb = sh.ires[0];
                 if (lb >= 0 && sh.lcnt[lb] <= (uint32_t)kLCand) {
-                    const uint32_t cA_hist = (uint32_t)sh.ires[1];
-                    // the answer bin's entries into lcand; exact FP64 sums of the entries above it
+                    // the answer bin [lo, hi) into lcand; exact FP64 sums of the entries above it
+                    if (tid == 0) sh.nlc = 0;
+                    __syncthreads();
                     double c_loc = 0, s1_loc = 0, s2_loc = 0;
                     for (int e0 = 0; e0 < cnt; e0 += NT) {  // uniform trip count for the ballot
                         const int e = e0 + tid;
                         const uint64_t key = e < cnt ? sh.list[e] : 0ull;
-                        const float av = __uint_as_float((uint32_t)(key >> 32));
-                        const int b = e < cnt ? lbin(av) : -1;
-                        if (b > lb) {
+                        const uint32_t kb = (uint32_t)(key >> 32);
+                        const float av = __uint_as_float(kb);
+                        if (e < cnt && kb >= hi) {
                             c_loc += 1.0;
                             s1_loc += (double)av;
                             s2_loc += (double)av * (double)av;
                         }
-                        const bool inb = b == lb;
+                        const bool inb = e < cnt && kb >= lo && kb < hi;
                         const uint64_t bal = __ballot(inb);
                         if (bal) {
                             const int lane = tid & 63, first = __ffsll((long long)bal) - 1;
-                            int base = 0;
-                            if (lane == first) base = atomicAdd(&sh.nlc, (int)__popcll(bal));
-                            base = __shfl(base, first, 64);
-                            if (inb) sh.lcand[base + (int)__popcll(bal & ((1ull << lane) - 1ull))] = key;
+                            int bs = 0;
+                            if (lane == first) bs = atomicAdd(&sh.nlc, (int)__popcll(bal));
+                            bs = __shfl(bs, first, 64);
+                            if (inb) sh.lcand[bs + (int)__popcll(bal & ((1ull << lane) - 1ull))] = key;
                         }
                     }
                     block_sum3<NT>(c_loc, s1_loc, s2_loc, sh.red);
                     const double cAd = c_loc, S1A = s1_loc, S2A = s2_loc;
                     const int nc = sh.nlc;  // <= kLCand: the bin's count
                     bitonic_desc<NT>(sh.lcand, nc);
-                    decided = resolve(sh.lcand, nc, cA_hist, cAd, S1A, S2A, true, true, [&](float lo, float hi) {
-                        return lo >= L && lbin(lo) == lb && lbin(hi) == lb;
-                    });
+                    decided = resolve(sh.lcand, nc, cA, cAd, S1A, S2A, true, true, (double)__uint_as_float(lo),
+                                      [&](float flo, float fhi) {
+                                          return flo >= L && __float_as_uint(flo) >= lo && __float_as_uint(fhi) < hi;
+                                      });
                     if (decided) {
-                        // the next bound: the lower edge of the list bin holding place s + s/4 + 32 (L0)
-                        // or survivors + survivors/8 + 64 (L1); the same L if the list is shorter
-                        const uint32_t keep = !l1c ? a.s : (uint32_t)sh.dres[0];
-                        const uint32_t rank = !l1c ? keep + keep / 4 + 32 : keep + keep / 8 + 64;
+                        // the next bound: the lower edge of the level-0 bin holding place list_rank
+                        const uint32_t nr = list_rank(!l1c ? a.s : (uint32_t)sh.dres[0], l1c);
                         __syncthreads();
+                        if (nr == 0) {
+                            next_bound = 0.f;
+                        } else {
+                        if (level > 0) hist_level(Lb, shift0, Lb, 0xFFFFFFFFu, false);
                         if (tid < 64)
-                            scan_hist<kLBins>(sh, sh.lcnt, sh.ls1, sh.ls2, false, rank, 0u, 0ull, 0ull, 0.0, 0.0,
+                            scan_hist<kLBins>(sh, sh.lcnt, sh.ls1, sh.ls2, false, nr, 0u, 0ull, 0ull, 0.0, 0.0,
                                               [&](int, double&) { return 0; });
                         __syncthreads();
                         const int rb = sh.ires[0];
-                        next_bound = rb >= 0 ? ledge(rb) : L;
+                        next_bound = rb >= 0 ? __uint_as_float(Lb + ((uint32_t)rb << shift0)) : L;
+                        }
                     }
                 }
             }
@@ -2259,6 +2326,11 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
                 atomicAdd(&sp.done[2], 1);
                 if (decided) atomicAdd(&sp.done[3], 1);
             }
+            // Not decided in the first round with a usable list whose bound was too high (L0: fewer
+            // than s entries; L1: no decision): the scans' counts under L give the highest half-octave
+            // edge L' with enough entries (L0: list_rank(s); L1: twice the list), and the second
+            // round rescans the column with L' instead of running the column passes here.  The
+            // count of entries >= L' is known exactly, so L' is taken only if they fit the list.
         }
         if (!decided) {
         float M = Mcol;
@@ -2322,21 +2394,20 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
             const double cAd = c_loc, S1A = s1_loc, S2A = s2_loc;
             const int nc = min(sh.ncand, kCand);
             bitonic_desc<NT>(sh.cand, nc);
-            resolve(sh.cand, nc, cA_hist, cAd, S1A, S2A, false, bstar >= 0, [&](float lo, float hi) {
+            resolve(sh.cand, nc, cA_hist, cAd, S1A, S2A, false, bstar >= 0, 0.0, [&](float lo, float hi) {
                 return lin_bin(lo, scale) == bstar && lin_bin(hi, scale) == bstar;
             });
-            // the next step's list bound: the lower edge of the bin holding place s + s/4 + 32 (L0) or
-            // survivors + survivors/8 + 64 (L1); none for an inactive L1 constraint (every element
-            // survives), a column shorter than that, or a list that would not fit in LDS
-            const uint32_t keep = !l1c ? a.s : (uint32_t)sh.dres[0];
-            const uint32_t rank = !l1c ? keep + keep / 4 + 32 : keep + keep / 8 + 64;
+            // the next step's list bound: the lower edge of the bin holding place 2s + 64 (L0) or
+            // 3/2 survivors + 64 (L1); none for an inactive L1 constraint (every element survives), a
+            // column shorter than that, or a list that would not fit in LDS
+            const uint32_t rank = list_rank(!l1c ? a.s : (uint32_t)sh.dres[0], l1c);  // 0: no list
             __syncthreads();
             if (tid < 64)
                 scan_hist<kBins>(sh, sh.cnt, sh.s1, sh.s2, false, rank, 0u, 0ull, 0ull, 0.0, 0.0,
                                  [&](int, double&) { return 0; });
             __syncthreads();
             const int rb = sh.ires[0];
-            const bool fits = rb > 0 && (uint32_t)sh.ires[1] + sh.cnt[rb] <= (uint32_t)kList;
+            const bool fits = rank > 0 && rb > 0 && (uint32_t)sh.ires[1] + sh.cnt[rb] <= (uint32_t)kList;
             next_bound = (fits && !(l1c && !(lam > 0.f))) ? (float)((double)rb * Md / (double)kBins) : 0.f;
             decided_column = true;
         } else {
@@ -2422,7 +2493,8 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
         }
         }  // !decided
         if (!decided && !decided_column) next_bound = l1c ? lam * kListMarginL1 : tau * kListMarginL0;
-        if (tid == 0) sp.thr[j] = next_bound;  // the next step's list bound
+        // the next step's list bound (none for columns longer than the chunk records allow)
+        if (tid == 0) sp.thr[j] = a.n <= kMaxListChunks * kStepChunk ? next_bound : 0.f;
         const float nf = (float)sqrt(norm2);
         scale_out = 1.0f / nf;
         fval = sp.fz[j];
@@ -2535,7 +2607,9 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
         fval = l0p ? (float)tot : (float)sqrt(tot);
     }
     // ---- write the iterate (+ operand forms), count nonzeros, update the SP record
-    if (decided) {
+    if constexpr (!kConstrained) {
+        nz = block_sum<NT>(nz, sh.red);  // the penalised sweep wrote the iterate
+    } else if (decided) {
         // ---- a list step: x's nonzeros are the kept list entries, so only they are written,
         // after the previous step's nonzeros are cleared: the indices a previous list step kept,
         // or (after a column step, a refill or with deflation) the whole column
@@ -2593,8 +2667,6 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
         __syncthreads();
         nz = (double)sh.nkeep;  // every kept value is nonzero: |v| >= tau > 0, or |v| - lam > 0
         if (tid == 0) sp.nkept[j] = sh.nkeep;
-    } else if constexpr (!kConstrained) {
-        nz = block_sum<NT>(nz, sh.red);  // the penalised sweep wrote the iterate
     } else {
         // exact ties (L0 fast path): values inside the re-evaluated band take its decision and value
         const int nband = !l1c ? sh.nband : 0;
@@ -2632,25 +2704,130 @@ __device__ __forceinline__ void step_body(SelShared& sh, int j, const float* __r
     }
 }
 
-// Step kind of a formulation: 0 L0 constraint (also the L1 constraint with s <= 1, which admits
-// only 1-sparse x: the top-1 hard threshold), 1 L1 constraint, 2 L0 penalty, 3 L1 penalty.
-__host__ __device__ inline int step_kind(int f, unsigned s) {
-    return f >= 6 ? 3 : f >= 4 ? 2 : (f >= 2 && s > 1) ? 1 : 0;
+
+// One workgroup per SP; the penalised kinds need only StepSmall's LDS, so more of them share a CU.
+template <int NT, int DEPTH, int KIND, int PREC>
+__global__ __launch_bounds__(NT, 4) void step_kernel(const float* __restrict__ V, Operand xop,
+                                                                     StepArgs a, SpState sp, int round) {
+    __shared__ std::conditional_t<(KIND < 2), SelShared, StepSmall> sh;
+    step_body<NT, DEPTH, KIND, PREC>(sh, blockIdx.x, V, xop, a, sp, round);
 }
 
-template <int NT, int DEPTH, int KIND, int PREC>
-__global__ __launch_bounds__(NT, 4) void step_kernel(const float* __restrict__ V, Operand xop, StepArgs a, SpState sp) {
-    __shared__ SelShared sh;
-    step_body<NT, DEPTH, KIND, PREC>(sh, blockIdx.x, V, xop, a, sp);
+// Chunk scans of the constrained steps (kinds 0 and 1): workgroup (SP j, chunk c).  An SP whose
+// previous step left a list bound L has its column read by all its chunk workgroups at once, each
+// over kStepChunk consecutive elements with eight 16-byte loads in flight per thread: the chunk's
+// max |v| (atomicMax on the float bits, which order as the values for |v| >= 0), its count of
+// entries |v| >= L and the entries themselves, in a fixed order (wave, lane, element), in the
+// chunk's own kStageCap slots.  The step runs in the next launch (step_kernel), one workgroup per
+// SP, and reads the chunks in order, so no SP waits for a whole-column pass by one workgroup and
+// no workgroup waits on another.
+__device__ __forceinline__ void step_scan_body(int j, int c, const float* __restrict__ V, const StepArgs& a,
+                                               const SpState& sp, int round) {
+    if (*sp.done || !sp.active[j]) return;
+    if (round && !sp.lretry[j]) return;
+    const float L = sp.thr[j];
+    if (!(L > 0.f)) return;  // no list: the step runs the column passes
+    __shared__ int wtot[2][kSelThreads / 64];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    constexpr int R = kStepScanRows;
+    const int n4 = a.n & ~3, nch = (a.n + kStepChunk - 1) / kStepChunk;
+    const float* col = V + (long)j * a.ldv;
+    // 16-byte loads below n4 (each wholly inside or past the view), the last n - n4 elements by
+    // 4-byte loads; element base + (u * NT + tid) * 4 + k of row u, lane part k
+    const __amdgpu_buffer_rsrc_t r4 = col_rsrc(col, (uint32_t)n4 * 4u);
+    const int base = c * kStepChunk;
+    f4 v[R];
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+        const uint32_t o = (uint32_t)(base + (u * kSelThreads + tid) * 4) * 4u;
+        v[u] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r4, o, 0, 0));
+    }
+    const int tail_i = (c == nch - 1 && tid < a.n - n4) ? n4 + tid : -1;
+    float tv = 0.f;
+    if (tail_i >= 0)
+        tv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(col_rsrc(col, (uint32_t)a.n * 4u),
+                                                                 (uint32_t)tail_i * 4u, 0, 0));
+    float mloc = fabsf(tv);
+    uint32_t in = 0;
+#pragma unroll
+    for (int u = 0; u < R; u++)
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float av = fabsf(v[u][k]);
+            mloc = fmaxf(mloc, av);
+            in |= (av >= L ? 1u : 0u) << (u * 4 + k);
+        }
+    const bool tin = tail_i >= 0 && fabsf(tv) >= L;
+    const int cnt = __popc(in) + (tin ? 1 : 0);  // at most 33: six bits
+    int excl = 0, wsum = 0;
+#pragma unroll
+    for (int bit = 0; bit < 6; bit++) {
+        const uint64_t m = __ballot((cnt >> bit) & 1);
+        excl += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << bit;
+        wsum += __popcll(m) << bit;
+    }
+    const float wmax = wave_max(mloc);
+    if (lane == 0) {
+        wtot[0][w] = wsum;
+        wtot[1][w] = (int)__float_as_uint(wmax);
+    }
+    __syncthreads();
+    int before = 0, tot = 0;
+    uint32_t mx = 0;
+#pragma unroll
+    for (int k = 0; k < kSelThreads / 64; k++) {
+        const int t = wtot[0][k];
+        before += k < w ? t : 0;
+        tot += t;
+        mx = max(mx, (uint32_t)wtot[1][k]);  // non-negative floats order as their bits
+    }
+    const int chunks = step_chunks(a.ldv);
+    if (tid == 0) {
+        if (mx) atomicMax(sp.lmax + j, mx);
+        sp.lcount[(long)j * chunks + c] = tot;  // more than kStageCap: the step runs the column passes
+    }
+    if (!cnt) return;
+    int slot = before + excl;
+    unsigned long long* dst = sp.lstage + ((long)j * chunks + c) * kStageCap;
+    auto stage = [&](float x, int i) {
+        if (slot < kStageCap)
+            dst[slot] = ((uint64_t)__float_as_uint(fabsf(x)) << 32) | (uint32_t)i | (x < 0.f ? 0x80000000u : 0u);
+        slot++;
+    };
+#pragma unroll
+    for (int u = 0; u < R; u++)
+#pragma unroll
+        for (int k = 0; k < 4; k++)
+            if ((in >> (u * 4 + k)) & 1u) stage(v[u][k], base + (u * kSelThreads + tid) * 4 + k);
+    if (tin) stage(tv, tail_i);
+}
+
+__global__ __launch_bounds__(kSelThreads) void step_scan_kernel(const float* __restrict__ V, StepArgs a, SpState sp,
+                                                                int nch, int round) {
+    const int j = blockIdx.x / nch;
+    step_scan_body(j, blockIdx.x - j * nch, V, a, sp, round);
+}
+
+template <int K, int P>
+void launch_step_kind(const float* V, Operand xop, StepArgs a, SpState sp, int nch, hipStream_t st) {
+    // constrained kinds: scans, the step, and a second round of both for the SPs the first round
+    // gave a lower list bound
+    const int rounds = K < 2 ? 2 : 1;
+    for (int r = 0; r < rounds; r++) {
+        if constexpr (K < 2) step_scan_kernel<<<a.batch * nch, kSelThreads, 0, st>>>(V, a, sp, nch, r);
+        step_kernel<kSelThreads, kSelDepth, K, P><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp, r);
+    }
 }
 hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st) {
     if (a.n <= 0 || (long)a.n * 4 > 0x7fffffffL) return hipErrorInvalidValue;
     if (a.precision && (long)a.ldv * xop.bp * 2 > 0xffffffffL) return hipErrorInvalidValue;  // 32-bit store offsets
     if (a.precision < 0 || a.precision > 2) return hipErrorInvalidValue;
+    const int nch = (a.n + kStepChunk - 1) / kStepChunk;
+    if ((long)a.batch * nch > 0x7fffffffL) return hipErrorInvalidValue;
     switch (step_kind(a.formulation, a.s) * 3 + a.precision) {
-#define SPCA_STEP_CASE(K, P)                                                                          \
-    case K * 3 + P:                                                                                   \
-        step_kernel<kSelThreads, kSelDepth, K, P><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp); \
+#define SPCA_STEP_CASE(K, P)                         \
+    case K * 3 + P:                                  \
+        launch_step_kind<K, P>(V, xop, a, sp, nch, st); \
         break;
         SPCA_STEP_CASE(0, 0) SPCA_STEP_CASE(0, 1) SPCA_STEP_CASE(0, 2)
         SPCA_STEP_CASE(1, 0) SPCA_STEP_CASE(1, 1) SPCA_STEP_CASE(1, 2)
@@ -2803,7 +2980,7 @@ hipError_t launch_converge(SpState sp, int batch, int it, double tol, int max_it
 template <int FIELD>
 __device__ __forceinline__ int job_first(const FusedJob* __restrict__ jobs, int k) {
     return FIELD == 0 ? jobs[k].t1 : FIELD == 1 ? jobs[k].t2 : FIELD == 2 ? jobs[k].c0 : FIELD == 3 ? jobs[k].t3
-                                                                                            : jobs[k].t4;
+                                                                           : FIELD == 4 ? jobs[k].t4 : jobs[k].t5;
 }
 template <int FIELD>
 __device__ __forceinline__ int find_job(const FusedJob* __restrict__ jobs, int njobs, int item) {
@@ -2923,18 +3100,27 @@ __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __res
                 J.zprec, J.sp, J.rf);
 }
 
+// The chunk scans of the jobs' constrained steps: item (job, SP, chunk), chunk fastest.
+__global__ __launch_bounds__(kSelThreads) void fused_scan_kernel(const FusedJob* __restrict__ jobs, int njobs,
+                                                                 int round) {
+    const FusedJob& J = jobs[find_job<5>(jobs, njobs, blockIdx.x)];
+    const int nch = (J.sa.n + kStepChunk - 1) / kStepChunk, li = blockIdx.x - J.t5, j = li / nch;
+    step_scan_body(j, li - j * nch, J.V, J.sa, J.sp, round);
+}
+
 // Jobs of one fused launch share the precision (PREC); each job's step kind is chosen per workgroup.
 template <int PREC>
-__global__ __launch_bounds__(kSelThreads, 4) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {
+__global__ __launch_bounds__(kSelThreads, 4) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs,
+                                                                    int round) {
     __shared__ SelShared sh;
     const int k = find_job<2>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
     const int col = blockIdx.x - J.c0;
     switch (step_kind(J.sa.formulation, J.sa.s)) {
-        case 0: step_body<kSelThreads, kSelDepth, 0, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
-        case 1: step_body<kSelThreads, kSelDepth, 1, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
-        case 2: step_body<kSelThreads, kSelDepth, 2, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
-        default: step_body<kSelThreads, kSelDepth, 3, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
+        case 0: step_body<kSelThreads, kSelDepth, 0, PREC>(sh, col, J.V, J.xop, J.sa, J.sp, round); break;
+        case 1: step_body<kSelThreads, kSelDepth, 1, PREC>(sh, col, J.V, J.xop, J.sa, J.sp, round); break;
+        case 2: step_body<kSelThreads, kSelDepth, 2, PREC>(sh, col, J.V, J.xop, J.sa, J.sp, round); break;
+        default: step_body<kSelThreads, kSelDepth, 3, PREC>(sh, col, J.V, J.xop, J.sa, J.sp, round); break;
     }
 }
 
@@ -2945,7 +3131,7 @@ __global__ __launch_bounds__(256) void fused_converge_kernel(const FusedJob* __r
 
 template <int PREC>
 hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols, int items3, int items4,
-                        int any_fix8, int* all_done, hipStream_t st) {
+                        int items5, int any_fix8, int* all_done, hipStream_t st) {
     fused_gemm_kernel<PREC, 0><<<tiles1, 256, 0, st>>>(jobs, njobs, tiles1);
     fused_nz_list_kernel<<<cols, 1024, 0, st>>>(jobs, njobs);
     fused_sparse_ax_kernel<<<items3, 256, 0, st>>>(jobs, njobs);
@@ -2953,19 +3139,25 @@ hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2,
     fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
     fused_gemm_kernel<PREC, 1><<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
     if (any_fix8) fused_fix8_kernel<<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
-    fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs);
+    if (items5 > 0) fused_scan_kernel<<<items5, kSelThreads, 0, st>>>(jobs, njobs, 0);
+    fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs, 0);
+    if (items5 > 0) {  // the second round of the constrained steps (launch_step_kind)
+        fused_scan_kernel<<<items5, kSelThreads, 0, st>>>(jobs, njobs, 1);
+        fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs, 1);
+    }
     fused_converge_kernel<<<njobs, 256, 0, st>>>(jobs, all_done);
     return hipGetLastError();
 }
 
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
-                                  int items3, int items4, int any_fix8, int* all_done, hipStream_t st) {
-    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1 || items3 < 1 || items4 < 0) return hipErrorInvalidValue;
+                                  int items3, int items4, int items5, int any_fix8, int* all_done, hipStream_t st) {
+    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1 || items3 < 1 || items4 < 0 || items5 < 0)
+        return hipErrorInvalidValue;
     if (precision == 0)
-        return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, items3, items4, any_fix8, all_done, st);
+        return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, items3, items4, items5, any_fix8, all_done, st);
     if (precision == 1)
-        return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, items3, items4, any_fix8, all_done, st);
-    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, items3, items4, any_fix8, all_done, st);
+        return fused_launch<1>(jobs, njobs, tiles1, tiles2, cols, items3, items4, items5, any_fix8, all_done, st);
+    return fused_launch<2>(jobs, njobs, tiles1, tiles2, cols, items3, items4, items5, any_fix8, all_done, st);
 }
 
 // ---------------------------------------------------------------- OTF slot refill

@@ -188,6 +188,41 @@ def test_constrained_invariants(small, small_mat):
             np.testing.assert_allclose(np.abs(Xd).sum(0) / np.linalg.norm(Xd, axis=0), np.sqrt(10), rtol=2e-6)
 
 
+@pytest.mark.parametrize("f", [0, 2])
+def test_long_candidate_lists(ctx, f):
+    """A 128 x 4096 spiked matrix whose first column step leaves a list bound low enough that the
+    next step's candidate list holds thousands of entries (sorted whole, 16 keys per thread):
+    the constrained invariants hold after every list step, and the trajectory follows the FP64
+    oracle's supports and explained variance, as in test_trajectory_parity."""
+    A, _ = spiked(128, 4096, seed=3)
+    m, n = A.shape
+    B, s, it = 128, 10, 6
+    X0 = unit_gaussian_starts(n, B, seed=2)
+    M = ctx.upload(A, storage=S.STORE_FP32)
+    try:
+        stats = S.OptimizationStatistics()
+        x, Xg = M.solve(_settings(f, it, S.PREC_FP32, B, s=s, m=m), X0=X0, stats=stats, want_X=True)
+    finally:
+        M.release()
+    print(f"case f={f} list steps {stats.list_steps}/{stats.list_tries}", flush=True)
+    assert stats.list_steps > 0
+    Xd = Xg.astype(np.float64)
+    np.testing.assert_allclose(np.linalg.norm(Xd, axis=0), 1.0, atol=2e-6)
+    if f == 0:
+        assert np.all((Xg != 0).sum(axis=0) == s) and np.all(stats.cardinalities == s)
+    else:
+        np.testing.assert_allclose(np.abs(Xd).sum(0) / np.linalg.norm(Xd, axis=0), np.sqrt(s), rtol=2e-6)
+    o = O.solve(A.astype(np.float64), f, s=s, tol=0.0, max_iter=it, total_sp=B, X0=X0.astype(np.float64),
+                want_X=True, lambda_rule=O.LAMBDA_GPU)
+    margin, _ = trajectory_tie_margin(A, X0, f, s, 0.0, it, ztol=ZTOL[S.PREC_FP32])
+    check = ~(margin < TOL[S.PREC_FP32][2])
+    supp = np.all((Xg != 0) == (o["X"] != 0), axis=0)
+    assert np.all(supp[check]), np.nonzero(check & ~supp)[0]
+    ev_o = explained_variance(A, o["X"], f)
+    ev_rel = np.abs(stats.sp_ev_fp64 - ev_o) / ev_o
+    assert np.all(ev_rel[check] <= 1e-5), np.sort(ev_rel[check])[-5:]
+
+
 def test_planted_support_recovered(small, small_mat):
     A, X0, planted = small
     st = S.OptimizationSettings(formulation=0, constraintParameter=20, tolerance=1e-6, maximumIterations=100,
