@@ -2810,9 +2810,9 @@ __global__ __launch_bounds__(kSelThreads) void step_scan_kernel(const float* __r
 
 template <int K, int P>
 void launch_step_kind(const float* V, Operand xop, StepArgs a, SpState sp, int nch, hipStream_t st) {
-    // constrained kinds: scans, the step, and a second round of both for the SPs the first round
-    // gave a lower list bound
-    const int rounds = K < 2 ? 2 : 1;
+    // constrained kinds: scans and the step; the L0 constraint a second round of both for the SPs
+    // whose lists the first round found too long (with the higher bound it left)
+    const int rounds = K == 0 ? 2 : 1;
     for (int r = 0; r < rounds; r++) {
         if constexpr (K < 2) step_scan_kernel<<<a.batch * nch, kSelThreads, 0, st>>>(V, a, sp, nch, r);
         step_kernel<kSelThreads, kSelDepth, K, P><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp, r);
