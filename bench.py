@@ -30,12 +30,12 @@ HBM_PEAK = 8000.0  # GB/s
 # and WRITE_SIZE runs, each scaled by what the counter reads for the 1 GiB padding copy of A
 # (FETCH_SIZE x 2.0, WRITE_SIZE x 1.0): profiles/gemm_products_spiked4096x65536_B1024_hbm_counters.md
 GIB = 2**30
-MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (6.368 + 0.250) * GIB, ("fp32", "gemm_ax_dense"): (8.116 + 0.125) * GIB,
-                    ("fp32", "gemm_atz_sign_fix8"): (2.613 + 0.255) * GIB, ("fp32", "sparse_ax_gather"): (5.940 + 0.016) * GIB,
-                    ("fp32_dense", "gemm_atz"): (6.368 + 0.250) * GIB, ("fp32_dense", "gemm_ax_dense"): (8.116 + 0.125) * GIB,
-                    ("bf16x3", "gemm_atz"): (1.500 + 0.264) * GIB, ("bf16x3", "gemm_atz_sign_fix8"): (2.285 + 0.255) * GIB,
-                    ("bf16x3", "sparse_ax_gather"): (4.821 + 0.016) * GIB,
-                    ("fp16", "gemm_atz"): (0.750 + 0.261) * GIB}
+MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (6.360 + 0.250) * GIB, ("fp32", "gemm_ax_dense"): (8.170 + 0.125) * GIB,
+                    ("fp32", "gemm_atz_sign_fix8"): (2.347 + 0.255) * GIB, ("fp32", "sparse_ax_gather"): (4.978 + 0.016) * GIB,
+                    ("fp32_dense", "gemm_atz"): (6.360 + 0.250) * GIB, ("fp32_dense", "gemm_ax_dense"): (8.170 + 0.125) * GIB,
+                    ("bf16x3", "gemm_atz"): (1.500 + 0.264) * GIB, ("bf16x3", "gemm_atz_sign_fix8"): (2.231 + 0.255) * GIB,
+                    ("bf16x3", "sparse_ax_gather"): (3.872 + 0.016) * GIB,
+                    ("fp16", "gemm_atz"): (0.750 + 0.261) * GIB, ("fp16", "sparse_ax_gather"): (3.871 + 0.016) * GIB}
 
 
 def make_problem(seed=2024):
@@ -476,7 +476,7 @@ def kernel_rooflines(p, per, forms):
     Algorithmic work per launch at m x n = 4096 x 65536, B = 1024:
       A^T*Z (L2 variance, the FP32 / bf16x3 / fp16 product): 2 m n B flops;
       A^T sgn(z) (L1 variance; fixed point in fp32 and bf16x3): 2 m n B int8 MACs x 2 (ops);
-      dense A*X (fp32_dense; fp32 f2, whose iterates are dense): 2 m n B flops;
+      dense A*X (FP32 solves whose A*X stayed dense, e.g. fp32_dense): 2 m n B flops;
       gathered A*X (sparse iterates, iterations after the warm-up): s m B gathered A values
         (4 bytes each; the iterates share columns, which come from L2/MALL, so the gathered
         rate is reported against HBM only as a reference);
@@ -498,7 +498,9 @@ def kernel_rooflines(p, per, forms):
         out["gemm_atz_sign_fix8"] = {"bound": "mfma", "ms": atz1, "achieved": tops, "peak": 5000.0, "unit": "TOP/s",
                                      "frac": tops / 5000.0, "issued_frac": 4 * tops / 5000.0,
                                      "work_per_launch": 2 * mnb, "formulations": l1v}
-    dense = [f for f in forms if p == "fp32_dense" or (p == "fp32" and f == 2)]
+    # dense A*X lines only from solves whose A*X stayed dense (no gathered or union iterations)
+    dense = [f for f in forms if p in ("fp32", "fp32_dense") and per[str(f)].get("sparse_iterations", 0) == 0
+             and per[str(f)].get("union_iterations", 0) == 0]
     ax = _avg(per, dense, "gemm_ax_ms")
     if ax:
         tf = 2 * mnb / (ax / 1e3) / 1e12
