@@ -1814,9 +1814,9 @@ void fused_entry(FusedRun& r, int precision, FusedTotals& T, FusedJob& f) {
     f.t4 = T.t4;
     f.psum = f.splits > 1 && r.st.batch <= 256;
     if (f.psum) T.t4 += (f.mp + 255) / 256 * r.st.batch;
-    // the chunk scans of constrained steps: (SP, chunk) items
+    // the steps' chunk workgroups: (SP, chunk) items
     f.t5 = T.t5;
-    if (step_kind(r.st.formulation, r.st.constraint) < 2) T.t5 += step_chunks(M->n) * r.st.batch;
+    T.t5 += step_chunks(M->n) * r.st.batch;
 }
 
 // A fused member's results once its state says done: the best SP, the final normalisation,

@@ -182,7 +182,7 @@ struct FusedJob {
     int nb1, nb2;           // FP32 A*X / A^T*Z in the narrow 128 x nb tile (32 or 64; 0: 128 x 128 tiles)
     int t4;                 // first (row block, SP) item of the partial-sum kernel
     int psum;               // 1: the partial-sum kernel adds the K splits into partial 0 before prep_z
-    int t5;                 // first (SP, chunk) item of the constrained steps' chunk scans
+    int t5;                 // first (SP, chunk) item of the steps' chunk workgroups
     int* gl;                // the gathered A*X's nonzero lists (kGatherStride ints per SP; null: scan x)
 };
 

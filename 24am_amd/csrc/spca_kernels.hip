@@ -1858,11 +1858,64 @@ __device__ void scan_hist(StepSmall& sh, const uint32_t* cnt, const uint64_t* hs
 // The step of one SP (column j of V), specialised by kind (step_kind: 0 L0 constraint, 1 L1
 // constraint, 2 L0 penalty, 3 L1 penalty) and iterate precision, so the per-element code of the
 // column sweeps carries no formulation or precision branches.  sh: the calling kernel's LDS.
+// Stores of an iterate's entry x_i (i < n) of SP j: FP32 and the operand forms.  16-bit forms are
+// K-blocked: element i of column j at ((i/32)*bp + j)*32 + i%32; the views cover the whole operand
+// (ldv * bp elements), and elements i >= n are dropped by the FP32 view's bound.
+template <int PREC>
+struct XPut {
+    __amdgpu_buffer_rsrc_t xr, hr, lr;
+    int j, bp;
+    __device__ XPut(const Operand& xop, const StepArgs& a, int j_) : j(j_), bp(xop.bp) {
+        xr = col_rsrc(xop.f32 + (long)j * a.ldv, (uint32_t)a.n * 4u);
+        const uint32_t opb = PREC ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
+        hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
+        lr = col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, PREC == 1 ? opb : 0u);
+    }
+    __device__ __forceinline__ void operator()(int i, float x) const {
+        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, (uint32_t)i * 4u, 0, 0);
+        if constexpr (PREC != 0) {
+            const uint32_t bo = (uint32_t)(blk_index(j, i, bp) * 2);
+            if constexpr (PREC == 1) {
+                const uint16_t h = f2bf(x);
+                __builtin_amdgcn_raw_buffer_store_b16(h, hr, bo, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, bo, 0, 0);
+            } else {
+                __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, bo, 0, 0);
+            }
+        }
+    }
+    // entries i .. i + 3 (i a multiple of 4, i + 3 < n): one 16-byte FP32 store and one 8-byte
+    // store per 16-bit part (the four are consecutive in the K-blocked layout too)
+    __device__ __forceinline__ void put4(int i, f4 x) const {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, x), xr, (uint32_t)i * 4u, 0, 0);
+        if constexpr (PREC != 0) {
+            const uint32_t bo = (uint32_t)(blk_index(j, i, bp) * 2);
+            uint16_t h[4], l[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if constexpr (PREC == 1) {
+                    h[k] = f2bf(x[k]);
+                    l[k] = f2bf(x[k] - bf2f(h[k]));
+                } else {
+                    h[k] = f2h(x[k]);
+                }
+            }
+            typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+            const u2 hv = {(uint32_t)h[0] | ((uint32_t)h[1] << 16), (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
+            __builtin_amdgcn_raw_buffer_store_b64(hv, hr, bo, 0, 0);
+            if constexpr (PREC == 1) {
+                const u2 lv = {(uint32_t)l[0] | ((uint32_t)l[1] << 16), (uint32_t)l[2] | ((uint32_t)l[3] << 16)};
+                __builtin_amdgcn_raw_buffer_store_b64(lv, lr, bo, 0, 0);
+            }
+        }
+    }
+};
+
 template <int NT, int DEPTH, int KIND, int PREC, class SH>
 __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict__ V, Operand xop, StepArgs a,
                                           SpState sp, int round) {
-    if (*sp.done || !sp.active[j]) return;
     if (round && (KIND >= 2 || !sp.lretry[j])) return;  // the second round: only the SPs the first left
+    if (*sp.done || !sp.active[j]) return;
     static_assert(NT <= kSelThreads, "SelShared holds kSelThreads / 64 waves");
     static_assert(KIND >= 0 && KIND < 4 && PREC >= 0 && PREC < 3, "step kind and precision");
     constexpr bool kConstrained = KIND < 2;
@@ -1896,8 +1949,7 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
                 run(bb, q0 + DEPTH);
             }
         } else {
-            // the L1 constraint's passes keep more registers live; with a second buffer the
-            // kernel exceeds 128 VGPRs (four waves per SIMD) and spills: one buffer
+            // the L1 constraint's passes: one buffer (two measured 10-20% slower)
             for (int q0 = 0; q0 < nq; q0 += DEPTH) {
                 load(ba, q0);
                 run(ba, q0);
@@ -1906,27 +1958,7 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
     };
     // the iterate's stores: x_i (i < n) in FP32 and the operand forms
     double nz = 0;
-    const long xbase = (long)j * a.ldv;
-    const __amdgpu_buffer_rsrc_t xr = col_rsrc(xop.f32 + xbase, (uint32_t)a.n * 4u);
-    // 16-bit forms are K-blocked: element i of column j at ((i/32)*bp + j)*32 + i%32; the views
-    // cover the whole operand (ldv * bp elements) and elements i >= n are skipped
-    const uint32_t opb = PREC ? (uint32_t)((long)a.ldv * xop.bp * 2) : 0u;
-    const __amdgpu_buffer_rsrc_t hr = col_rsrc(xop.hi ? (const void*)xop.hi : (const void*)xop.f32, opb);
-    const __amdgpu_buffer_rsrc_t lr =
-        col_rsrc(xop.lo ? (const void*)xop.lo : (const void*)xop.f32, PREC == 1 ? opb : 0u);
-    auto put = [&](int i, float x) {
-        __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(x), xr, (uint32_t)i * 4u, 0, 0);
-        if constexpr (PREC != 0) {
-            const uint32_t bo = (uint32_t)(blk_index(j, i, xop.bp) * 2);
-            if constexpr (PREC == 1) {
-                const uint16_t h = f2bf(x);
-                __builtin_amdgcn_raw_buffer_store_b16(h, hr, bo, 0, 0);
-                __builtin_amdgcn_raw_buffer_store_b16(f2bf(x - bf2f(h)), lr, bo, 0, 0);
-            } else {
-                __builtin_amdgcn_raw_buffer_store_b16(f2h(x), hr, bo, 0, 0);
-            }
-        }
-    };
+    const XPut<PREC> put(xop, a, j);
     float fval;
     // element transform applied when the iterate is written: keep |v| >= tau (L0),
     // soft map with lam (L1), penalised maps; then multiply by `scale_out`
@@ -2500,10 +2532,12 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
         fval = sp.fz[j];
     } else {
         // ---- penalised (my_cublas_wrapper.h:170-238,636-710), gamma in the iterate's precision.
-        // The threshold is gamma itself, so one sweep sums the objective and writes the iterate.
+        // The threshold is gamma itself, so the chunk workgroups of the previous launch
+        // (pen_chunk_body) already wrote x by the plain rule and left per chunk the FP64 sums of
+        // the objective's terms (all, and those of the exact-tie band), the nonzero counts and
+        // the band's indices.  Here the chunks are summed in order and the band is decided.
         const float g = (float)a.gamma;
         lam = g;
-        double acc = 0, acc_band = 0;
         constexpr bool l0p = KIND == 2;
         auto contrib = [&](float v) -> double {
             if (l0p) {
@@ -2513,102 +2547,94 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
             const float t = fabsf(v) - g;
             return t > 0.f ? (double)t * (double)t : 0.0;
         };
-        // the iterate's entry: L0 penalty kept iff v^2 - gamma > 0, so a NaN v (the 0/0 after an
-        // SP was zeroed out) gives 0, as the reference CPU rule (sparse_PCA_thresholding.h:67-102),
-        // equal to the GPU functor otherwise; L1 penalty the soft threshold at gamma
-        auto pen_x = [&](float v) -> float {
-            if (l0p) return (v * v - g) > 0.f ? v : 0.f;
-            const float t2 = fabsf(v) - g;
-            return t2 > 0.f ? t2 * sgnf(v) : 0.f;
-        };
-        // exact ties (L2 variance): entries whose |v| lies within 2 eps of the threshold (sqrt(gamma)
-        // for the L0 penalty, gamma for the L1 penalty) are listed, recomputed in FP64 below, and
-        // their contributions replaced; more than kBandMax of them: none.  The sweep leaves the
-        // band's entries of x unwritten; they are written once the band is decided.
-        const float thr = l0p ? sqrtf(g) : g;
-        const float lo = thr * (1.f - 2.f * a.band_eps), hi = thr * (1.f + 2.f * a.band_eps);
-        const bool band = a.band_eps > 0.f && thr > 0.f;
-        if (band) {
-            if (tid == 0) sh.ncand = 0;
-            __syncthreads();
+        const int nch = (a.n + kStepChunk - 1) / kStepChunk, chunks = step_chunks(a.ldv);
+        const unsigned long long* st = sp.lstage + (long)j * chunks * kStageCap;
+        double tot = 0, tb = 0;
+        int nzi = 0, nzb = 0, nb = 0;
+        if (tid == 0) {  // chunk order: a fixed order for the FP64 sums
+            for (int c = 0; c < nch; c++) {
+                const unsigned long long* r = st + (long)c * kStageCap;
+                tot += __longlong_as_double((long long)r[0]);
+                tb += __longlong_as_double((long long)r[1]);
+                nzi += (int)(r[2] >> 32);
+                nzb += (int)(uint32_t)r[2];
+                nb += (int)r[3];
+            }
+            sh.dres[0] = tot;
+            sh.dres[1] = tb;
+            sh.ires[0] = nzi;
+            sh.ires[1] = nzb;
+            sh.ires[2] = nb;
         }
-        // Entries inside the band are written by the plain rule here and rewritten below when the
-        // band is decided (the barriers between order the two stores).
-        double nz_band = 0;
-        sweep([&](int q, float v) {
-            const int i = tid + q * NT;
-            const bool in = i < a.n;  // the last row's entries past n (v reads 0 there) are not counted
-            const double c = in ? contrib(v) : 0.0;
-            acc += c;
-            const float av = fabsf(v);
-            const bool inband = band && in && av >= lo && av <= hi;
-            const float x = pen_x(v);
-            if (inband) {
-                acc_band += c;
-                nz_band += x != 0.f ? 1.0 : 0.0;
-                const int slot = atomicAdd(&sh.ncand, 1);
-                if (slot < kBandMax) sh.band_idx[slot] = i;
-            } else {
-                nz += (in && x != 0.f) ? 1.0 : 0.0;
+        __syncthreads();
+        tot = sh.dres[0];
+        tb = sh.dres[1];
+        nzi = sh.ires[0];
+        nzb = sh.ires[1];
+        nb = sh.ires[2];
+        nz = (double)nzi;
+        // exact ties (L2 variance): the entries whose |v| lies within 2 eps of the threshold
+        // (sqrt(gamma) for the L0 penalty, gamma for the L1 penalty), 1 .. kBandMax of them in
+        // all, are recomputed in FP64 and their contributions replaced; more: none
+        if (nb >= 1 && nb <= kBandMax) {
+            if (tid == 0) {  // the chunks' band indices, then index order for the replaced sum
+                int k = 0;
+                for (int c = 0; c < nch; c++) {
+                    const unsigned long long* r = st + (long)c * kStageCap;
+                    for (int e = 0; e < (int)r[3]; e++) sh.band_idx[k++] = (int)r[4 + e];
+                }
+                for (int p = 1; p < nb; p++) {
+                    const int ix = sh.band_idx[p];
+                    int q = p - 1;
+                    while (q >= 0 && sh.band_idx[q] > ix) {
+                        sh.band_idx[q + 1] = sh.band_idx[q];
+                        q--;
+                    }
+                    sh.band_idx[q + 1] = ix;
+                }
             }
-            if (in) put(i, x);
-        });
-        double tot = block_sum<NT>(acc, sh.red);
-        if (band) {
-            const double tb = block_sum<NT>(acc_band, sh.red);  // its barriers publish ncand
-            const int nb = sh.ncand;
-            if (nb >= 1 && nb <= kBandMax) {
-                const int lane = tid & 63;
-                if (tid == 0)  // index order, so the replaced sum below has a fixed order
-                    for (int p = 1; p < nb; p++) {
-                        const int ix = sh.band_idx[p];
-                        int q = p - 1;
-                        while (q >= 0 && sh.band_idx[q] > ix) {
-                            sh.band_idx[q + 1] = sh.band_idx[q];
-                            q--;
-                        }
-                        sh.band_idx[q + 1] = ix;
-                    }
-                __syncthreads();
-                for (int w = tid >> 6; w < nb; w += NT / 64) {
-                    const f4* ac = (const f4*)(a.acm + (long)sh.band_idx[w] * a.mp);
-                    const f4* zc = (const f4*)(a.zf + (long)j * a.mp);
-                    double d = 0;
-                    for (int q = lane; q < (a.mp >> 2); q += 64) {
-                        const f4 x = ac[q], y = zc[q];
-                        d += (double)x[0] * y[0] + (double)x[1] * y[1] + (double)x[2] * y[2] + (double)x[3] * y[3];
-                    }
-                    d = wave_sum(d);
-                    if (lane == 0) sh.band_val[w] = (float)d;
+            __syncthreads();
+            const int lane = tid & 63;
+            for (int w = tid >> 6; w < nb; w += NT / 64) {
+                const f4* ac = (const f4*)(a.acm + (long)sh.band_idx[w] * a.mp);
+                const f4* zc = (const f4*)(a.zf + (long)j * a.mp);
+                double d = 0;
+                for (int q = lane; q < (a.mp >> 2); q += 64) {
+                    const f4 x = ac[q], y = zc[q];
+                    d += (double)x[0] * y[0] + (double)x[1] * y[1] + (double)x[2] * y[2] + (double)x[3] * y[3];
                 }
-                __syncthreads();
-                if (tid == 0) {
-                    double tr = 0;
-                    for (int p = 0; p < nb; p++) {
-                        const float v = sh.band_val[p];
-                        sh.band_keep[p] = contrib(v) > 0.0;
-                        tr += contrib(v);
-                    }
-                    sh.dres[3] = tr;
-                }
-                __syncthreads();
-                tot = tot - tb + sh.dres[3];
-                // the band's entries by their exact values: kept ones take the recomputed value
-                if (tid < nb) {
-                    const float ve = sh.band_val[tid];
-                    const float x = sh.band_keep[tid] ? (l0p ? ve : (fabsf(ve) - g) * sgnf(ve)) : 0.f;
-                    put(sh.band_idx[tid], x);
-                    nz += x != 0.f ? 1.0 : 0.0;
-                }
-            } else {
-                nz += nz_band;  // none re-evaluated: the band's entries keep the plain rule
+                d = wave_sum(d);
+                if (lane == 0) sh.band_val[w] = (float)d;
             }
+            __syncthreads();
+            if (tid == 0) {
+                double tr = 0;
+                for (int p = 0; p < nb; p++) {
+                    const float v = sh.band_val[p];
+                    sh.band_keep[p] = contrib(v) > 0.0;
+                    tr += contrib(v);
+                }
+                sh.dres[3] = tr;
+            }
+            __syncthreads();
+            tot = tot - tb + sh.dres[3];
+            // the band's entries by their exact values: kept ones take the recomputed value
+            if (tid < nb) {
+                const float ve = sh.band_val[tid];
+                const float x = sh.band_keep[tid] ? (l0p ? ve : (fabsf(ve) - g) * sgnf(ve)) : 0.f;
+                put(sh.band_idx[tid], x);
+            }
+            int kept_b = 0;
+            for (int p = 0; p < nb; p++) kept_b += sh.band_keep[p] && sh.band_val[p] != 0.f;
+            nz += (double)kept_b;
+        } else {
+            nz += (double)nzb;  // none re-evaluated: the band's entries keep the plain rule
         }
         fval = l0p ? (float)tot : (float)sqrt(tot);
     }
     // ---- write the iterate (+ operand forms), count nonzeros, update the SP record
     if constexpr (!kConstrained) {
-        nz = block_sum<NT>(nz, sh.red);  // the penalised sweep wrote the iterate
+        // the chunk workgroups wrote the iterate; nz is the chunks' count, block-uniform
     } else if (decided) {
         // ---- a list step: x's nonzeros are the kept list entries, so only they are written,
         // after the previous step's nonzeros are cleared: the indices a previous list step kept,
@@ -2723,8 +2749,8 @@ __global__ __launch_bounds__(NT, 4) void step_kernel(const float* __restrict__ V
 // no workgroup waits on another.
 __device__ __forceinline__ void step_scan_body(int j, int c, const float* __restrict__ V, const StepArgs& a,
                                                const SpState& sp, int round) {
-    if (*sp.done || !sp.active[j]) return;
     if (round && !sp.lretry[j]) return;
+    if (*sp.done || !sp.active[j]) return;
     const float L = sp.thr[j];
     if (!(L > 0.f)) return;  // no list: the step runs the column passes
     __shared__ int wtot[2][kSelThreads / 64];
@@ -2802,6 +2828,108 @@ __device__ __forceinline__ void step_scan_body(int j, int c, const float* __rest
     if (tin) stage(tv, tail_i);
 }
 
+// Chunk workgroups of the penalised steps (kinds 2 and 3): workgroup (SP j, chunk c), as the
+// constrained scans.  The threshold is gamma itself, so each chunk writes its entries of x by the
+// plain rule at once and leaves, in its first stage slots: the FP64 sums of the objective's terms
+// over the chunk (all, and those inside the exact-tie band), the nonzero counts (outside and inside
+// the band) and the band's indices (up to kBandMax).  The step (step_kernel) sums the chunks in
+// order and decides the band.  A thread's elements and the block's reduction tree are fixed, so
+// the sums are reproducible.
+template <int KIND, int PREC>
+__device__ __forceinline__ void pen_chunk_body(int j, int c, const float* __restrict__ V, const Operand& xop,
+                                               const StepArgs& a, const SpState& sp) {
+    if (*sp.done || !sp.active[j]) return;
+    __shared__ double red[2 * (kSelThreads / 64)];
+    __shared__ int ib[kSelThreads / 64][2];
+    __shared__ int nband;
+    __shared__ int bidx[kBandMax];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (tid == 0) nband = 0;
+    constexpr int R = kStepScanRows;
+    constexpr bool l0p = KIND == 2;
+    const float g = (float)a.gamma;
+    const float thr = l0p ? sqrtf(g) : g;
+    const float lo = thr * (1.f - 2.f * a.band_eps), hi = thr * (1.f + 2.f * a.band_eps);
+    const bool band = a.band_eps > 0.f && thr > 0.f;
+    const int n4 = a.n & ~3, nch = (a.n + kStepChunk - 1) / kStepChunk;
+    const float* col = V + (long)j * a.ldv;
+    const __amdgpu_buffer_rsrc_t r4 = col_rsrc(col, (uint32_t)n4 * 4u);
+    const int base = c * kStepChunk;
+    f4 v[R];
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+        const uint32_t o = (uint32_t)(base + (u * kSelThreads + tid) * 4) * 4u;
+        v[u] = __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r4, o, 0, 0));
+    }
+    const int tail_i = (c == nch - 1 && tid < a.n - n4) ? n4 + tid : -1;
+    float tv = 0.f;
+    if (tail_i >= 0)
+        tv = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(col_rsrc(col, (uint32_t)a.n * 4u),
+                                                                 (uint32_t)tail_i * 4u, 0, 0));
+    __syncthreads();  // nband cleared
+    const XPut<PREC> put(xop, a, j);
+    double acc = 0, accb = 0;
+    int nzo = 0, nzb = 0;
+    auto one = [&](int i, float x) -> float {  // i < n: the entry's term and band; returns x_i
+        const float t = l0p ? x * x - g : fabsf(x) - g;
+        const double ct = t > 0.f ? (l0p ? (double)t : (double)t * (double)t) : 0.0;
+        // the iterate's entry: L0 penalty kept iff v^2 - gamma > 0, so a NaN v (the 0/0 after an
+        // SP was zeroed out) gives 0, as the reference CPU rule (sparse_PCA_thresholding.h:67-102);
+        // L1 penalty the soft threshold at gamma
+        const float xv = l0p ? ((x * x - g) > 0.f ? x : 0.f) : (t > 0.f ? t * sgnf(x) : 0.f);
+        acc += ct;
+        const float ax = fabsf(x);
+        if (band && ax >= lo && ax <= hi) {
+            accb += ct;
+            nzb += xv != 0.f;
+            const int slot = atomicAdd(&nband, 1);
+            if (slot < kBandMax) bidx[slot] = i;
+        } else {
+            nzo += xv != 0.f;
+        }
+        return xv;
+    };
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+        const int i0 = base + (u * kSelThreads + tid) * 4;
+        if (i0 < n4) {  // the row's four entries are all below n4
+            f4 x;
+#pragma unroll
+            for (int k = 0; k < 4; k++) x[k] = one(i0 + k, v[u][k]);
+            put.put4(i0, x);
+        }
+    }
+    if (tail_i >= 0) put(tail_i, one(tail_i, tv));
+    acc = wave_sum(acc);
+    accb = wave_sum(accb);
+    const int wnz = __builtin_amdgcn_readlane((int)wave_prefix_u32((uint32_t)nzo), 63);
+    const int wnb = __builtin_amdgcn_readlane((int)wave_prefix_u32((uint32_t)nzb), 63);
+    if (lane == 0) {
+        red[w] = acc;
+        red[kSelThreads / 64 + w] = accb;
+        ib[w][0] = wnz;
+        ib[w][1] = wnb;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        double ta = 0, tbb = 0;
+        int za = 0, zb = 0;
+        for (int q = 0; q < kSelThreads / 64; q++) {
+            ta += red[q];
+            tbb += red[kSelThreads / 64 + q];
+            za += ib[q][0];
+            zb += ib[q][1];
+        }
+        unsigned long long* r = sp.lstage + ((long)j * step_chunks(a.ldv) + c) * kStageCap;
+        r[0] = (unsigned long long)__double_as_longlong(ta);
+        r[1] = (unsigned long long)__double_as_longlong(tbb);
+        r[2] = ((unsigned long long)(uint32_t)za << 32) | (uint32_t)zb;
+        const int nb = nband;
+        r[3] = (unsigned long long)nb;  // more than kBandMax in a chunk: the step re-evaluates none
+        for (int e = 0; e < min(nb, kBandMax); e++) r[4 + e] = (unsigned long long)bidx[e];
+    }
+}
+
 __global__ __launch_bounds__(kSelThreads) void step_scan_kernel(const float* __restrict__ V, StepArgs a, SpState sp,
                                                                 int nch, int round) {
     const int j = blockIdx.x / nch;
@@ -2809,12 +2937,21 @@ __global__ __launch_bounds__(kSelThreads) void step_scan_kernel(const float* __r
 }
 
 template <int K, int P>
+__global__ __launch_bounds__(kSelThreads) void pen_chunk_kernel(const float* __restrict__ V, Operand xop, StepArgs a,
+                                                                SpState sp, int nch) {
+    const int j = blockIdx.x / nch;
+    pen_chunk_body<K, P>(j, blockIdx.x - j * nch, V, xop, a, sp);
+}
+
+template <int K, int P>
 void launch_step_kind(const float* V, Operand xop, StepArgs a, SpState sp, int nch, hipStream_t st) {
     // constrained kinds: scans and the step; the L0 constraint a second round of both for the SPs
     // whose lists the first round found too long (with the higher bound it left)
+    // (penalised kinds: the chunk workgroups, then the step that sums them)
     const int rounds = K == 0 ? 2 : 1;
     for (int r = 0; r < rounds; r++) {
         if constexpr (K < 2) step_scan_kernel<<<a.batch * nch, kSelThreads, 0, st>>>(V, a, sp, nch, r);
+        else pen_chunk_kernel<K, P><<<a.batch * nch, kSelThreads, 0, st>>>(V, xop, a, sp, nch);
         step_kernel<kSelThreads, kSelDepth, K, P><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp, r);
     }
 }
@@ -3100,12 +3237,19 @@ __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __res
                 J.zprec, J.sp, J.rf);
 }
 
-// The chunk scans of the jobs' constrained steps: item (job, SP, chunk), chunk fastest.
+// The chunk workgroups of the jobs' steps (constrained scans, penalised chunks): item (job, SP,
+// chunk), chunk fastest.
+template <int PREC>
 __global__ __launch_bounds__(kSelThreads) void fused_scan_kernel(const FusedJob* __restrict__ jobs, int njobs,
                                                                  int round) {
     const FusedJob& J = jobs[find_job<5>(jobs, njobs, blockIdx.x)];
     const int nch = (J.sa.n + kStepChunk - 1) / kStepChunk, li = blockIdx.x - J.t5, j = li / nch;
-    step_scan_body(j, li - j * nch, J.V, J.sa, J.sp, round);
+    switch (step_kind(J.sa.formulation, J.sa.s)) {
+        case 0:
+        case 1: step_scan_body(j, li - j * nch, J.V, J.sa, J.sp, round); break;
+        case 2: if (!round) pen_chunk_body<2, PREC>(j, li - j * nch, J.V, J.xop, J.sa, J.sp); break;
+        default: if (!round) pen_chunk_body<3, PREC>(j, li - j * nch, J.V, J.xop, J.sa, J.sp); break;
+    }
 }
 
 // Jobs of one fused launch share the precision (PREC); each job's step kind is chosen per workgroup.
@@ -3139,19 +3283,18 @@ hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2,
     fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
     fused_gemm_kernel<PREC, 1><<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
     if (any_fix8) fused_fix8_kernel<<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
-    if (items5 > 0) fused_scan_kernel<<<items5, kSelThreads, 0, st>>>(jobs, njobs, 0);
+    fused_scan_kernel<PREC><<<items5, kSelThreads, 0, st>>>(jobs, njobs, 0);
     fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs, 0);
-    if (items5 > 0) {  // the second round of the constrained steps (launch_step_kind)
-        fused_scan_kernel<<<items5, kSelThreads, 0, st>>>(jobs, njobs, 1);
-        fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs, 1);
-    }
+    // the second round of the L0-constrained steps (launch_step_kind)
+    fused_scan_kernel<PREC><<<items5, kSelThreads, 0, st>>>(jobs, njobs, 1);
+    fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs, 1);
     fused_converge_kernel<<<njobs, 256, 0, st>>>(jobs, all_done);
     return hipGetLastError();
 }
 
 hipError_t launch_fused_iteration(int precision, const FusedJob* jobs, int njobs, int tiles1, int tiles2, int cols,
                                   int items3, int items4, int items5, int any_fix8, int* all_done, hipStream_t st) {
-    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1 || items3 < 1 || items4 < 0 || items5 < 0)
+    if (njobs < 1 || tiles1 < 1 || tiles2 < 1 || cols < 1 || items3 < 1 || items4 < 0 || items5 < 1)
         return hipErrorInvalidValue;
     if (precision == 0)
         return fused_launch<0>(jobs, njobs, tiles1, tiles2, cols, items3, items4, items5, any_fix8, all_done, st);

@@ -35,7 +35,9 @@ MEASURED_TRAFFIC = {("fp32", "gemm_atz"): (6.360 + 0.250) * GIB, ("fp32", "gemm_
                     ("fp32_dense", "gemm_atz"): (6.360 + 0.250) * GIB, ("fp32_dense", "gemm_ax_dense"): (8.170 + 0.125) * GIB,
                     ("bf16x3", "gemm_atz"): (1.500 + 0.264) * GIB, ("bf16x3", "gemm_atz_sign_fix8"): (2.231 + 0.255) * GIB,
                     ("bf16x3", "sparse_ax_gather"): (3.872 + 0.016) * GIB,
-                    ("fp16", "gemm_atz"): (0.750 + 0.261) * GIB, ("fp16", "sparse_ax_gather"): (3.871 + 0.016) * GIB}
+                    ("fp16", "gemm_atz"): (0.750 + 0.261) * GIB, ("fp16", "sparse_ax_gather"): (3.871 + 0.016) * GIB,
+                    # the step (scans + step kernels), mean of formulations 0-3 over iterations 3-20
+                    ("fp32", "step"): (0.305 + 0.092) * GIB, ("fp16", "step"): (0.306 + 0.152) * GIB}
 
 
 def make_problem(seed=2024):
