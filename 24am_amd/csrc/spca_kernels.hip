@@ -1661,7 +1661,7 @@ struct StepSmall {
     int ncand;
     int nkeep;
     int nlc;
-    int wtot[2][kSelThreads / 64];  // per-wave counts (chunk prefixes of the staged list)
+    int wtot[kSelThreads / 64];  // per-wave totals (chunk places of the staged list)
     double dres[4];
     // exact ties: the re-evaluated band (sorted descending), its value range and keep flags
     int nband;
@@ -1693,15 +1693,6 @@ static_assert(sizeof(SelShared) <= 40 * 1024, "four step workgroups per CU");
 // chunk): at most kMaxListChunks chunks (n <= kMaxListChunks * kStepChunk); longer columns take
 // no list.
 constexpr int kMaxListChunks = (int)(2 * sizeof(uint64_t) * kLBins + sizeof(uint64_t) * kLCand) / (2 * (int)sizeof(int));
-
-// Workgroup barrier ordering LDS only: a release/acquire fence on the local address space
-// around s_barrier.  Unlike __syncthreads() (whose fence also covers global memory and so
-// waits for every outstanding global load), loads issued before it stay in flight.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
 
 __device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
     const uint32_t lo = __shfl_xor((uint32_t)v, m, 64), hi = __shfl_xor((uint32_t)(v >> 32), m, 64);
@@ -2164,13 +2155,13 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
                 const int incl = (int)wave_prefix_u32((uint32_t)k), pre = incl - k;
                 const int wt = __builtin_amdgcn_readlane(incl, 63);
                 const int w = tid >> 6;
-                if ((tid & 63) == 0) sh.wtot[0][w] = wt;
+                if ((tid & 63) == 0) sh.wtot[w] = wt;
                 __syncthreads();
                 int before = 0, btot = 0;
 #pragma unroll
                 for (int q = 0; q < NT / 64; q++) {
-                    before += q < w ? sh.wtot[0][q] : 0;
-                    btot += sh.wtot[0][q];
+                    before += q < w ? sh.wtot[q] : 0;
+                    btot += sh.wtot[q];
                 }
                 if (c < nch) {
                     rec[c] = k;
