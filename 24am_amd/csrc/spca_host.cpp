@@ -73,7 +73,7 @@ struct Workspace {
     uint16_t* Zlo = nullptr;
     float* V = nullptr;
     float* fstate = nullptr;  // val, fz, err, errmax, thr  (kStateFloats x bp floats)
-    int* istate = nullptr;    // iters, nnz, active, nkept, lmax, lretry (bp ints each),
+    int* istate = nullptr;    // iters, nnz, active, nkept, lmax (bp ints each),
                               // done (state_flag_ints(bp))
     int* kept = nullptr;      // kListMax indices per SP: the nonzeros a list step wrote
     int* lcount = nullptr;                 // step_chunks(np) ints per SP: the chunk scans' entry counts
@@ -670,7 +670,6 @@ SpState state_of(Workspace& w) {
     s.active = w.istate + 2 * w.bp;
     s.nkept = w.istate + 3 * w.bp;
     s.lmax = (unsigned*)(w.istate + 4 * w.bp);
-    s.lretry = w.istate + 5 * w.bp;
     s.lcount = w.lcount;
     s.done = w.istate + kStateInts * w.bp;
     s.kept = w.kept;

@@ -84,14 +84,12 @@ struct SpState {
     // reads and clears it), and per chunk c of SP j (step_chunks(ldv) chunks per SP) its entry
     // count lcount[j * chunks + c] and its entries lstage[(j * chunks + c) * kStageCap ..]
     unsigned* lmax;
-    int* lretry;  // 1: the step's first round found the list too long; the second round rescans
-                  // with the higher bound it left in thr
     int* lcount;
     unsigned long long* lstage;
 };
-// per-SP float state: val, fz, err, errmax, thr; int state: iters, nnz, active, nkept, lmax, lretry
+// per-SP float state: val, fz, err, errmax, thr; int state: iters, nnz, active, nkept, lmax
 constexpr int kStateFloats = 5;
-constexpr int kStateInts = 6;
+constexpr int kStateInts = 5;
 // Most entries of the step kernel's candidate list (a power of two, for its sort), and so of the
 // nonzeros a step written through the list can leave.
 constexpr int kListMax = 4096;

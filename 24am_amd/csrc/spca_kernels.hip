@@ -1902,10 +1902,12 @@ struct XPut {
     }
 };
 
+__device__ __forceinline__ void step_scan_chunk(int j, int c, float L, const float* __restrict__ V,
+                                                const StepArgs& a, const SpState& sp);
+
 template <int NT, int DEPTH, int KIND, int PREC, class SH>
 __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict__ V, Operand xop, StepArgs a,
-                                          SpState sp, int round) {
-    if (round && (KIND >= 2 || !sp.lretry[j])) return;  // the second round: only the SPs the first left
+                                          SpState sp) {
     if (*sp.done || !sp.active[j]) return;
     static_assert(NT <= kSelThreads, "SelShared holds kSelThreads / 64 waves");
     static_assert(KIND >= 0 && KIND < 4 && PREC >= 0 && PREC < 3, "step kind and precision");
@@ -1940,10 +1942,15 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
                 run(bb, q0 + DEPTH);
             }
         } else {
-            // the L1 constraint's passes: one buffer (two measured 10-20% slower)
-            for (int q0 = 0; q0 < nq; q0 += DEPTH) {
-                load(ba, q0);
-                run(ba, q0);
+            // the L1 constraint's passes: one buffer of 2 DEPTH rows, the registers of the two
+            // above (two buffers measured 10-20% slower here)
+            float bc[2 * DEPTH];
+            for (int q0 = 0; q0 < nq; q0 += 2 * DEPTH) {
+#pragma unroll
+                for (int u = 0; u < 2 * DEPTH; u++) bc[u] = LDV(q0 + u);
+#pragma unroll
+                for (int u = 0; u < 2 * DEPTH; u++)
+                    if (q0 + u < nq) fn(q0 + u, bc[u]);
             }
         }
     };
@@ -2138,7 +2145,7 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
         bool have_M = false;
         const float thr_prev = sp.thr[j];
         if (thr_prev > 0.f) {
-            const float L = thr_prev;
+            float L = thr_prev;
             Mcol = __uint_as_float(sp.lmax[j]);
             have_M = true;
             // the staged entries into LDS in chunk order (within a chunk, the scan's order), so the
@@ -2146,6 +2153,10 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
             const int nch = (a.n + kStepChunk - 1) / kStepChunk, chunks = step_chunks(a.ldv);
             const int* lc = sp.lcount + (long)j * chunks;
             int* rec = (int*)sh.ls1;  // [0, nch): counts; [nch, 2 nch): places (nch <= kMaxListChunks)
+            const unsigned long long* stg = sp.lstage + (long)j * chunks * kStageCap;
+            // the chunks' counts and their places in the list (chunk-order exclusive prefix) into
+            // rec; the list's length, or kList + 1 where a chunk overflowed its stage slots
+            auto chunk_counts = [&]() {
             int over = 0, total = 0;
             for (int c0 = 0; c0 < nch; c0 += NT) {
                 const int c = c0 + tid;
@@ -2171,14 +2182,16 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
                 __syncthreads();  // wtot is reused by the next block of chunks
             }
             over = __syncthreads_or(over);  // also publishes rec
-            const int cnt = over ? kList + 1 : total;
-            list_cnt = cnt;
-            const unsigned long long* stg = sp.lstage + (long)j * chunks * kStageCap;
-            if (!l1c && !round && cnt > kList) {
+            return over ? kList + 1 : total;
+            };
+            int cnt = chunk_counts();
+            if (!l1c && cnt > kList) {
                 // L0, too long (the column's bulk grew past the bound): the staged keys -- every entry
                 // of the chunks that fit their slots, the first kStageCap of any that did not -- by the
-                // level-0 bins over [L, max |v|] (as below), and the second round rescans with the
-                // edge of the bin holding place list_rank(s) as its bound.
+                // level-0 bins over [L, max |v|] (as below), and a second round, here in the step's
+                // own workgroup, rescans the column's chunks with the edge of the bin holding place
+                // list_rank(s) as its bound.  Few SPs take it, so this costs less than another
+                // launch of scans and steps for every SP.  max |v| stays (the rescan finds the same).
                 const uint32_t Lb = __float_as_uint(L);
                 const int shift0 = max(0, 24 - __clz(__float_as_uint(Mcol) - Lb));
                 for (int b = tid; b < kLBins; b += NT) sh.lcnt[b] = 0;
@@ -2205,14 +2218,15 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
                 __syncthreads();
                 const int rb = want ? sh.ires[0] : -1;
                 if (rb > 0) {
-                    if (tid == 0) {
-                        sp.thr[j] = __uint_as_float(Lb + ((uint32_t)rb << shift0));
-                        sp.lretry[j] = 1;
-                        sp.lmax[j] = 0u;
+                    L = __uint_as_float(Lb + ((uint32_t)rb << shift0));
+                    for (int c = 0; c < nch; c++) {
+                        step_scan_chunk(j, c, L, V, a, sp);
+                        __syncthreads();  // the scan's LDS totals are reused by the next chunk
                     }
-                    return;  // block-uniform
+                    cnt = chunk_counts();
                 }
             }
+            list_cnt = cnt;
             if (cnt > 0 && cnt <= kList) {
                 // entry e of chunk c at stage slot c * kStageCap + e; the flat slots are taken 16 per
                 // thread at a time, their loads all issued before the stores
@@ -2235,7 +2249,6 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
             __syncthreads();
             if (tid == 0) {  // the next scans take the max afresh
                 sp.lmax[j] = 0u;
-                if (round) sp.lretry[j] = 0;
             }
             if (cnt > 0 && cnt <= kList && (l1c || cnt >= (int)a.s)) {
                 // Histogram levels over the IEEE bits of |v| (monotone for non-negative floats),
@@ -2421,7 +2434,7 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
                 return lin_bin(lo, scale) == bstar && lin_bin(hi, scale) == bstar;
             });
             // the next step's list bound: the lower edge of the bin holding place 2s + 64 (L0) or
-            // 3/2 survivors + 64 (L1); none for an inactive L1 constraint (every element survives), a
+            // 9/8 survivors + 64 (L1); none for an inactive L1 constraint (every element survives), a
             // column shorter than that, or a list that would not fit in LDS
             const uint32_t rank = list_rank(!l1c ? a.s : (uint32_t)sh.dres[0], l1c);  // 0: no list
             __syncthreads();
@@ -2725,9 +2738,9 @@ __device__ __forceinline__ void step_body(SH& sh, int j, const float* __restrict
 // One workgroup per SP; the penalised kinds need only StepSmall's LDS, so more of them share a CU.
 template <int NT, int DEPTH, int KIND, int PREC>
 __global__ __launch_bounds__(NT, 4) void step_kernel(const float* __restrict__ V, Operand xop,
-                                                                     StepArgs a, SpState sp, int round) {
+                                                                     StepArgs a, SpState sp) {
     __shared__ std::conditional_t<(KIND < 2), SelShared, StepSmall> sh;
-    step_body<NT, DEPTH, KIND, PREC>(sh, blockIdx.x, V, xop, a, sp, round);
+    step_body<NT, DEPTH, KIND, PREC>(sh, blockIdx.x, V, xop, a, sp);
 }
 
 // Chunk scans of the constrained steps (kinds 0 and 1): workgroup (SP j, chunk c).  An SP whose
@@ -2738,12 +2751,8 @@ __global__ __launch_bounds__(NT, 4) void step_kernel(const float* __restrict__ V
 // chunk's own kStageCap slots.  The step runs in the next launch (step_kernel), one workgroup per
 // SP, and reads the chunks in order, so no SP waits for a whole-column pass by one workgroup and
 // no workgroup waits on another.
-__device__ __forceinline__ void step_scan_body(int j, int c, const float* __restrict__ V, const StepArgs& a,
-                                               const SpState& sp, int round) {
-    if (round && !sp.lretry[j]) return;
-    if (*sp.done || !sp.active[j]) return;
-    const float L = sp.thr[j];
-    if (!(L > 0.f)) return;  // no list: the step runs the column passes
+__device__ __forceinline__ void step_scan_chunk(int j, int c, float L, const float* __restrict__ V,
+                                                const StepArgs& a, const SpState& sp) {
     __shared__ int wtot[2][kSelThreads / 64];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     constexpr int R = kStepScanRows;
@@ -2817,6 +2826,14 @@ __device__ __forceinline__ void step_scan_body(int j, int c, const float* __rest
         for (int k = 0; k < 4; k++)
             if ((in >> (u * 4 + k)) & 1u) stage(v[u][k], base + (u * kSelThreads + tid) * 4 + k);
     if (tin) stage(tv, tail_i);
+}
+
+__device__ __forceinline__ void step_scan_body(int j, int c, const float* __restrict__ V, const StepArgs& a,
+                                               const SpState& sp) {
+    if (*sp.done || !sp.active[j]) return;
+    const float L = sp.thr[j];
+    if (!(L > 0.f)) return;  // no list: the step runs the column passes
+    step_scan_chunk(j, c, L, V, a, sp);
 }
 
 // Chunk workgroups of the penalised steps (kinds 2 and 3): workgroup (SP j, chunk c), as the
@@ -2922,9 +2939,9 @@ __device__ __forceinline__ void pen_chunk_body(int j, int c, const float* __rest
 }
 
 __global__ __launch_bounds__(kSelThreads) void step_scan_kernel(const float* __restrict__ V, StepArgs a, SpState sp,
-                                                                int nch, int round) {
+                                                                int nch) {
     const int j = blockIdx.x / nch;
-    step_scan_body(j, blockIdx.x - j * nch, V, a, sp, round);
+    step_scan_body(j, blockIdx.x - j * nch, V, a, sp);
 }
 
 template <int K, int P>
@@ -2936,15 +2953,11 @@ __global__ __launch_bounds__(kSelThreads) void pen_chunk_kernel(const float* __r
 
 template <int K, int P>
 void launch_step_kind(const float* V, Operand xop, StepArgs a, SpState sp, int nch, hipStream_t st) {
-    // constrained kinds: scans and the step; the L0 constraint a second round of both for the SPs
-    // whose lists the first round found too long (with the higher bound it left)
-    // (penalised kinds: the chunk workgroups, then the step that sums them)
-    const int rounds = K == 0 ? 2 : 1;
-    for (int r = 0; r < rounds; r++) {
-        if constexpr (K < 2) step_scan_kernel<<<a.batch * nch, kSelThreads, 0, st>>>(V, a, sp, nch, r);
-        else pen_chunk_kernel<K, P><<<a.batch * nch, kSelThreads, 0, st>>>(V, xop, a, sp, nch);
-        step_kernel<kSelThreads, kSelDepth, K, P><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp, r);
-    }
+    // constrained kinds: the scans, then the step (which rescans the column itself for an L0 list
+    // found too long); penalised kinds: the chunk workgroups, then the step that sums them
+    if constexpr (K < 2) step_scan_kernel<<<a.batch * nch, kSelThreads, 0, st>>>(V, a, sp, nch);
+    else pen_chunk_kernel<K, P><<<a.batch * nch, kSelThreads, 0, st>>>(V, xop, a, sp, nch);
+    step_kernel<kSelThreads, kSelDepth, K, P><<<a.batch, kSelThreads, 0, st>>>(V, xop, a, sp);
 }
 hipError_t launch_step(const float* V, Operand xop, StepArgs a, SpState sp, hipStream_t st) {
     if (a.n <= 0 || (long)a.n * 4 > 0x7fffffffL) return hipErrorInvalidValue;
@@ -3231,31 +3244,29 @@ __global__ __launch_bounds__(256) void fused_prep_z_kernel(const FusedJob* __res
 // The chunk workgroups of the jobs' steps (constrained scans, penalised chunks): item (job, SP,
 // chunk), chunk fastest.
 template <int PREC>
-__global__ __launch_bounds__(kSelThreads) void fused_scan_kernel(const FusedJob* __restrict__ jobs, int njobs,
-                                                                 int round) {
+__global__ __launch_bounds__(kSelThreads) void fused_scan_kernel(const FusedJob* __restrict__ jobs, int njobs) {
     const FusedJob& J = jobs[find_job<5>(jobs, njobs, blockIdx.x)];
     const int nch = (J.sa.n + kStepChunk - 1) / kStepChunk, li = blockIdx.x - J.t5, j = li / nch;
     switch (step_kind(J.sa.formulation, J.sa.s)) {
         case 0:
-        case 1: step_scan_body(j, li - j * nch, J.V, J.sa, J.sp, round); break;
-        case 2: if (!round) pen_chunk_body<2, PREC>(j, li - j * nch, J.V, J.xop, J.sa, J.sp); break;
-        default: if (!round) pen_chunk_body<3, PREC>(j, li - j * nch, J.V, J.xop, J.sa, J.sp); break;
+        case 1: step_scan_body(j, li - j * nch, J.V, J.sa, J.sp); break;
+        case 2: pen_chunk_body<2, PREC>(j, li - j * nch, J.V, J.xop, J.sa, J.sp); break;
+        default: pen_chunk_body<3, PREC>(j, li - j * nch, J.V, J.xop, J.sa, J.sp); break;
     }
 }
 
 // Jobs of one fused launch share the precision (PREC); each job's step kind is chosen per workgroup.
 template <int PREC>
-__global__ __launch_bounds__(kSelThreads, 4) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs,
-                                                                    int round) {
+__global__ __launch_bounds__(kSelThreads, 4) void fused_step_kernel(const FusedJob* __restrict__ jobs, int njobs) {
     __shared__ SelShared sh;
     const int k = find_job<2>(jobs, njobs, blockIdx.x);
     const FusedJob& J = jobs[k];
     const int col = blockIdx.x - J.c0;
     switch (step_kind(J.sa.formulation, J.sa.s)) {
-        case 0: step_body<kSelThreads, kSelDepth, 0, PREC>(sh, col, J.V, J.xop, J.sa, J.sp, round); break;
-        case 1: step_body<kSelThreads, kSelDepth, 1, PREC>(sh, col, J.V, J.xop, J.sa, J.sp, round); break;
-        case 2: step_body<kSelThreads, kSelDepth, 2, PREC>(sh, col, J.V, J.xop, J.sa, J.sp, round); break;
-        default: step_body<kSelThreads, kSelDepth, 3, PREC>(sh, col, J.V, J.xop, J.sa, J.sp, round); break;
+        case 0: step_body<kSelThreads, kSelDepth, 0, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
+        case 1: step_body<kSelThreads, kSelDepth, 1, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
+        case 2: step_body<kSelThreads, kSelDepth, 2, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
+        default: step_body<kSelThreads, kSelDepth, 3, PREC>(sh, col, J.V, J.xop, J.sa, J.sp); break;
     }
 }
 
@@ -3274,11 +3285,8 @@ hipError_t fused_launch(const FusedJob* jobs, int njobs, int tiles1, int tiles2,
     fused_prep_z_kernel<<<cols, 256, 0, st>>>(jobs, njobs);
     fused_gemm_kernel<PREC, 1><<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
     if (any_fix8) fused_fix8_kernel<<<tiles2, 256, 0, st>>>(jobs, njobs, tiles2);
-    fused_scan_kernel<PREC><<<items5, kSelThreads, 0, st>>>(jobs, njobs, 0);
-    fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs, 0);
-    // the second round of the L0-constrained steps (launch_step_kind)
-    fused_scan_kernel<PREC><<<items5, kSelThreads, 0, st>>>(jobs, njobs, 1);
-    fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs, 1);
+    fused_scan_kernel<PREC><<<items5, kSelThreads, 0, st>>>(jobs, njobs);
+    fused_step_kernel<PREC><<<cols, kSelThreads, 0, st>>>(jobs, njobs);
     fused_converge_kernel<<<njobs, 256, 0, st>>>(jobs, all_done);
     return hipGetLastError();
 }
