@@ -191,9 +191,11 @@ def test_constrained_invariants(small, small_mat):
 @pytest.mark.parametrize("f", [0, 2])
 def test_long_candidate_lists(ctx, f):
     """A 128 x 4096 spiked matrix whose first column step leaves a list bound low enough that the
-    next step's candidate list holds thousands of entries (sorted whole, 16 keys per thread):
-    the constrained invariants hold after every list step, and the trajectory follows the FP64
-    oracle's supports and explained variance, as in test_trajectory_parity."""
+    next step's list holds thousands of entries, more than a chunk's stage slots: the L0 step
+    (f 0) histograms the staged keys, rescans its column in its own workgroup with a higher
+    bound and decides on that list; the L1 step (f 2) runs the column passes.  The constrained
+    invariants hold after every step, and the trajectory follows the FP64 oracle's supports and
+    explained variance, as in test_trajectory_parity."""
     A, _ = spiked(128, 4096, seed=3)
     m, n = A.shape
     B, s, it = 128, 10, 6
