@@ -190,12 +190,10 @@ def test_constrained_invariants(small, small_mat):
 
 @pytest.mark.parametrize("f", [0, 2])
 def test_long_candidate_lists(ctx, f):
-    """A 128 x 4096 spiked matrix whose first column step leaves a list bound low enough that the
-    next step's list holds thousands of entries, more than a chunk's stage slots: the L0 step
-    (f 0) histograms the staged keys, rescans its column in its own workgroup with a higher
-    bound and decides on that list; the L1 step (f 2) runs the column passes.  The constrained
-    invariants hold after every step, and the trajectory follows the FP64 oracle's supports and
-    explained variance, as in test_trajectory_parity."""
+    """A 128 x 4096 spiked matrix with s = 10, where the list steps from the second iteration on
+    work from the bound a column step left: the constrained invariants hold after every step, and
+    the trajectory follows the FP64 oracle's supports and explained variance, as in
+    test_trajectory_parity."""
     A, _ = spiked(128, 4096, seed=3)
     m, n = A.shape
     B, s, it = 128, 10, 6
@@ -221,6 +219,52 @@ def test_long_candidate_lists(ctx, f):
     supp = np.all((Xg != 0) == (o["X"] != 0), axis=0)
     assert np.all(supp[check]), np.nonzero(check & ~supp)[0]
     ev_o = explained_variance(A, o["X"], f)
+    ev_rel = np.abs(stats.sp_ev_fp64 - ev_o) / ev_o
+    assert np.all(ev_rel[check] <= 1e-5), np.sort(ev_rel[check])[-5:]
+
+
+def _clustered_matrix():
+    """128 x 4096: 20 strong columns, 1300 near-copies of one column aligned with them (|v| within
+    0.2% of each other) and weak noise elsewhere.  A column step puts the list bound at the
+    cluster's bin, so the next L0 list of many SPs holds the whole cluster: more entries than
+    the 4096-element chunk's 1024 stage slots."""
+    rng = np.random.default_rng(7)
+    m, n = 128, 4096
+    A = 0.01 * rng.standard_normal((m, n))
+    strong = np.arange(0, 40, 2)
+    A[:, strong] += rng.standard_normal((m, 20))
+    c = A[:, strong].sum(1)
+    c /= np.linalg.norm(c)
+    cl = np.arange(100, 100 + 3 * 1300, 3)
+    A[:, cl] = np.outer(c, 0.8 * (1 + 0.002 * np.arange(1300) / 1300)) + 0.001 * rng.standard_normal((m, 1300))
+    return A.astype(np.float32)
+
+
+def test_overflowing_l0_lists(ctx):
+    """L0 lists that overflow a chunk's stage slots: the step histograms the staged keys, rescans
+    its column in its own workgroup with a higher bound and resolves on that list (or runs the
+    column passes).  Invariants and the FP64 oracle as in test_long_candidate_lists."""
+    A = _clustered_matrix()
+    m, n = A.shape
+    B, s, it = 64, 10, 6
+    X0 = unit_gaussian_starts(n, B, seed=5)
+    M = ctx.upload(A, storage=S.STORE_FP32)
+    try:
+        stats = S.OptimizationStatistics()
+        x, Xg = M.solve(_settings(0, it, S.PREC_FP32, B, s=s, m=m), X0=X0, stats=stats, want_X=True)
+    finally:
+        M.release()
+    print(f"case overflowing lists: list steps {stats.list_steps}/{stats.list_tries}", flush=True)
+    assert stats.list_tries > 0
+    np.testing.assert_allclose(np.linalg.norm(Xg.astype(np.float64), axis=0), 1.0, atol=2e-6)
+    assert np.all((Xg != 0).sum(axis=0) == s) and np.all(stats.cardinalities == s)
+    o = O.solve(A.astype(np.float64), 0, s=s, tol=0.0, max_iter=it, total_sp=B, X0=X0.astype(np.float64),
+                want_X=True, lambda_rule=O.LAMBDA_GPU)
+    margin, _ = trajectory_tie_margin(A, X0, 0, s, 0.0, it, ztol=ZTOL[S.PREC_FP32])
+    check = ~(margin < TOL[S.PREC_FP32][2])
+    supp = np.all((Xg != 0) == (o["X"] != 0), axis=0)
+    assert np.all(supp[check]), np.nonzero(check & ~supp)[0]
+    ev_o = explained_variance(A, o["X"], 0)
     ev_rel = np.abs(stats.sp_ev_fp64 - ev_o) / ev_o
     assert np.all(ev_rel[check] <= 1e-5), np.sort(ev_rel[check])[-5:]
 
